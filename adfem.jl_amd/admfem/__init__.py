@@ -1,0 +1,12 @@
+"""admfem -- MI355X-native drop-in for AdFem.jl's P1 forward/adjoint path.
+
+The Python mirror of the reference's Julia operator surface
+(src/MFEM/MFEM.jl, MCore.jl, MUtils.jl) over libadmfem.so's C-ABI.
+Every compute call runs the library's HIP kernels; there is no CPU
+fallback -- a missing library raises.
+"""
+from ._lib import AdmfemError, lib  # noqa: F401
+from .mesh import (Mesh, bcedge, bcnode, fem_nodes, gauss_nodes, gauss_weights,  # noqa: F401
+                   get_area, get_ngauss)
+from .system import DirichletSystem, solve_opts  # noqa: F401
+from . import meshgen  # noqa: F401

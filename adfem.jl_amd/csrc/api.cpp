@@ -1,0 +1,562 @@
+// C-ABI of libadmfem: argument checks, handle management, error codes, and
+// the reference-named entry points on the default mesh.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+#include "internal.h"
+
+struct admfem_mesh_s {
+  admfem::Mesh *m;
+};
+struct admfem_system_s {
+  admfem::System *s;
+  admfem::DevBuf<double> u_int, lam_int, g_int, tmp_int;
+};
+
+namespace admfem {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+template <class F>
+static int guarded(F &&f) {
+  try {
+    f();
+    return ADMFEM_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host out of memory");
+    return ADMFEM_ERR_ARG;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return ADMFEM_ERR_ARG;
+  }
+}
+
+static hipStream_t S(void *stream) { return (hipStream_t)stream; }
+
+// The process-wide default mesh behind the reference-named symbols (the
+// reference's `mmesh` singleton, deps/MFEM/Common.cpp:7).
+static std::mutex g_default_mu;
+static admfem_mesh g_default = nullptr;
+
+static Mesh &M(admfem_mesh h) {
+  REQUIRE(h && h->m, ADMFEM_ERR_ARG, "null mesh handle");
+  return *h->m;
+}
+static System &SY(admfem_system h) {
+  REQUIRE(h && h->s, ADMFEM_ERR_ARG, "null system handle");
+  return *h->s;
+}
+
+template <class T>
+static void d2h(T *dst, const T *src, size_t n) {
+  if (n) HIPCHECK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+}  // namespace admfem
+
+using namespace admfem;
+
+extern "C" {
+
+const char *admfem_last_error(void) { return g_err.c_str(); }
+int admfem_version(void) { return 1; }
+
+// ------------------------------------------------------------ B. meshes
+int admfem_mesh_create(const double *coords, int nnode, const int *elems, int nelem,
+                       int order, int degree, admfem_mesh *out) {
+  return guarded([&] {
+    REQUIRE(out, ADMFEM_ERR_ARG, "null out");
+    *out = nullptr;
+    REQUIRE(degree == 1, ADMFEM_ERR_UNSUPPORTED,
+            "only degree 1 (P1) elements are built");
+    Mesh *m = mesh_build(coords, nnode, elems, nelem, order);
+    *out = new admfem_mesh_s{m};
+  });
+}
+
+int admfem_mesh_destroy(admfem_mesh h) {
+  return guarded([&] {
+    if (!h) return;
+    {
+      std::lock_guard<std::mutex> lk(g_default_mu);
+      if (h == g_default) g_default = nullptr;
+    }
+    delete h->m;
+    delete h;
+  });
+}
+
+int admfem_mesh_info(admfem_mesh h, int64_t info[8]) {
+  return guarded([&] {
+    Mesh &m = M(h);
+    mesh_edges(m);
+    info[0] = m.nnode;
+    info[1] = m.nelem;
+    info[2] = m.ngauss;
+    info[3] = m.nnode;
+    info[4] = 3;
+    info[5] = m.nedges;
+    info[6] = (int64_t)m.col.size();
+    info[7] = m.rule.nq;
+  });
+}
+
+int admfem_mesh_gauss(admfem_mesh h, double *x, double *y) {
+  return guarded([&] {
+    Mesh &m = M(h);
+    d2h(x, m.gx.p, m.ngauss);
+    d2h(y, m.gy.p, m.ngauss);
+  });
+}
+
+int admfem_mesh_weights(admfem_mesh h, double *w) {
+  return guarded([&] { d2h(w, M(h).gw.p, M(h).ngauss); });
+}
+
+int admfem_mesh_area(admfem_mesh h, double *a) {
+  return guarded([&] { d2h(a, M(h).area.p, (size_t)M(h).nelem); });
+}
+
+int admfem_mesh_elems(admfem_mesh h, int *elems) {
+  return guarded([&] {
+    Mesh &m = M(h);
+    std::copy(m.elems.begin(), m.elems.end(), elems);
+  });
+}
+
+int admfem_mesh_edges(admfem_mesh h, long long *edges) {
+  return guarded([&] {
+    Mesh &m = M(h);
+    mesh_edges(m);
+    std::copy(m.edges.begin(), m.edges.end(), edges);
+  });
+}
+
+int admfem_mesh_bcnode(admfem_mesh h, long long *out, int *count) {
+  return guarded([&] {
+    Mesh &m = M(h);
+    // boundary edges are used by exactly one element (src/MFEM/MCore.jl:385-406):
+    // count element-edge uses per pattern slot (lo, hi)
+    const int n = m.nnode;
+    std::vector<int> urp(n + 1, 0), ucol;
+    ucol.reserve(m.col.size());
+    for (int u = 0; u < n; u++) {
+      int r = m.perm[u];
+      for (int k = m.rp[r]; k < m.rp[r + 1]; k++) ucol.push_back(m.iperm[m.col[k]]);
+      std::sort(ucol.begin() + urp[u], ucol.end());
+      urp[u + 1] = (int)ucol.size();
+    }
+    std::vector<int> uses(ucol.size(), 0);
+    static const int ev[3][2] = {{0, 1}, {2, 1}, {0, 2}};
+    for (int e = 0; e < m.nelem; e++)
+      for (int j = 0; j < 3; j++) {
+        int a = m.elems[3 * (size_t)e + ev[j][0]], b = m.elems[3 * (size_t)e + ev[j][1]];
+        int lo = std::min(a, b), hi = std::max(a, b);
+        auto it = std::lower_bound(ucol.begin() + urp[lo], ucol.begin() + urp[lo + 1], hi);
+        uses[it - ucol.begin()]++;
+      }
+    std::vector<char> bd(n, 0);
+    for (int lo = 0; lo < n; lo++)
+      for (int k = urp[lo]; k < urp[lo + 1]; k++)
+        if (uses[k] % 2 == 1) {  // set toggling keeps odd counts
+          bd[lo] = 1;
+          bd[ucol[k]] = 1;
+        }
+    int c = 0;
+    for (int u = 0; u < n; u++)
+      if (bd[u]) out[c++] = u + 1;
+    *count = c;
+  });
+}
+
+int admfem_mesh_pattern(admfem_mesh h, int *row_ptr, int *col) {
+  return guarded([&] {
+    Mesh &m = M(h);
+    int pos = 0;
+    row_ptr[0] = 0;
+    for (int u = 0; u < m.nnode; u++) {
+      int r = m.perm[u];
+      int start = pos;
+      for (int k = m.rp[r]; k < m.rp[r + 1]; k++) col[pos++] = m.iperm[m.col[k]];
+      std::sort(col + start, col + pos);
+      row_ptr[u + 1] = pos;
+    }
+  });
+}
+
+// ------------------------------------------------------------ C. operators
+int admfem_fem_laplace_scalar(admfem_mesh h, const double *kappa, long long *indices,
+                              double *vv, void *stream) {
+  return guarded([&] {
+    REQUIRE(kappa && indices && vv, ADMFEM_ERR_ARG, "null pointer");
+    launch_laplace_coo(M(h), kappa, indices, vv, S(stream));
+  });
+}
+
+int admfem_fem_laplace_scalar_grad(admfem_mesh h, const double *grad_vv,
+                                   double *grad_kappa, void *stream) {
+  return guarded([&] {
+    REQUIRE(grad_vv && grad_kappa, ADMFEM_ERR_ARG, "null pointer");
+    launch_laplace_coo_grad(M(h), grad_vv, grad_kappa, S(stream));
+  });
+}
+
+int admfem_fem_mass_matrix(admfem_mesh h, const double *rho, long long *indices,
+                           double *vv, void *stream) {
+  return guarded([&] {
+    REQUIRE(rho && indices && vv, ADMFEM_ERR_ARG, "null pointer");
+    launch_mass_coo(M(h), rho, indices, vv, S(stream));
+  });
+}
+
+int admfem_fem_mass_matrix_grad(admfem_mesh h, const double *grad_vv, double *grad_rho,
+                                void *stream) {
+  return guarded([&] {
+    REQUIRE(grad_vv && grad_rho, ADMFEM_ERR_ARG, "null pointer");
+    launch_mass_coo_grad(M(h), grad_vv, grad_rho, S(stream));
+  });
+}
+
+int admfem_fem_source_scalar(admfem_mesh h, const double *f, double *rhs, void *stream) {
+  return guarded([&] {
+    REQUIRE(f && rhs, ADMFEM_ERR_ARG, "null pointer");
+    launch_source(M(h), f, rhs, false, S(stream));
+  });
+}
+
+int admfem_fem_source_scalar_grad(admfem_mesh h, const double *grad_rhs, double *grad_f,
+                                  void *stream) {
+  return guarded([&] {
+    REQUIRE(grad_rhs && grad_f, ADMFEM_ERR_ARG, "null pointer");
+    launch_source_grad(M(h), grad_rhs, grad_f, S(stream));
+  });
+}
+
+int admfem_impose_dirichlet(const long long *indices, const double *vv, int64_t sN,
+                            const long long *bd, const double *bdval, int bdN,
+                            const double *rhs, int N, long long *oindices, double *ov,
+                            double *orhs, int64_t *out_count, void *stream) {
+  return guarded([&] {
+    REQUIRE(out_count && orhs && rhs && (sN == 0 || (indices && vv && oindices && ov)) &&
+                (bdN == 0 || (bd && bdval)),
+            ADMFEM_ERR_ARG, "null pointer");
+    *out_count = run_impose_dirichlet(indices, vv, sN, bd, bdval, bdN, rhs, N, oindices, ov,
+                                      orhs, S(stream));
+  });
+}
+
+int admfem_impose_dirichlet_grad(const double *grad_ov, const double *grad_orhs,
+                                 const long long *indices, const double *vv, int64_t sN,
+                                 const long long *bd, const double *bdval, int bdN, int N,
+                                 double *grad_vv, double *grad_rhs, double *grad_bdval,
+                                 void *stream) {
+  return guarded([&] {
+    REQUIRE(grad_orhs && grad_rhs && (sN == 0 || (indices && vv && grad_vv)) &&
+                (bdN == 0 || (bd && bdval && grad_bdval)),
+            ADMFEM_ERR_ARG, "null pointer");
+    run_impose_dirichlet_grad(grad_ov, grad_orhs, indices, vv, sN, bd, bdval, bdN, N,
+                              grad_vv, grad_rhs, grad_bdval, S(stream));
+  });
+}
+
+// ------------------------------------------------------------ D. systems
+int admfem_system_create(admfem_mesh h, const long long *bdnode, int nbd,
+                         admfem_system *out) {
+  return guarded([&] {
+    REQUIRE(out, ADMFEM_ERR_ARG, "null out");
+    *out = nullptr;
+    System *s = system_build(&M(h), bdnode, nbd);
+    admfem_system hs = new admfem_system_s();
+    hs->s = s;
+    try {
+      hs->u_int.alloc(s->npad);
+      hs->lam_int.alloc(s->npad);
+      hs->g_int.alloc(s->npad);
+      hs->tmp_int.alloc(s->npad);
+    } catch (...) {
+      system_free(s);
+      delete hs;
+      throw;
+    }
+    *out = hs;
+  });
+}
+
+int admfem_system_destroy(admfem_system h) {
+  return guarded([&] {
+    if (!h) return;
+    system_free(h->s);
+    delete h;
+  });
+}
+
+int admfem_system_info(admfem_system h, int64_t info[4]) {
+  return guarded([&] {
+    System &s = SY(h);
+    info[0] = s.n;
+    info[1] = (int64_t)s.col.size();
+    info[2] = s.nbd_unique;
+    info[3] = (int64_t)(s.device_bytes() + s.mesh->device_bytes() +
+                        4 * sizeof(double) * (size_t)s.npad);
+  });
+}
+
+int admfem_system_pattern(admfem_system h, int *row_ptr, int *col) {
+  return guarded([&] {
+    System &s = SY(h);
+    const Mesh &m = *s.mesh;
+    int pos = 0;
+    row_ptr[0] = 0;
+    for (int u = 0; u < s.n; u++) {
+      int r = m.perm[u];
+      int start = pos;
+      for (int k = s.rp[r]; k < s.rp[r + 1]; k++) col[pos++] = m.iperm[s.col[k]];
+      std::sort(col + start, col + pos);
+      row_ptr[u + 1] = pos;
+    }
+  });
+}
+
+int admfem_system_csr(admfem_system h, int *row_ptr, int *col, double *val, double *rhs) {
+  return guarded([&] {
+    System &s = SY(h);
+    const Mesh &m = *s.mesh;
+    std::vector<double> v(s.col.size()), b(s.n);
+    HIPCHECK(hipDeviceSynchronize());
+    d2h(v.data(), s.val.p, v.size());
+    d2h(b.data(), s.rhs.p, b.size());
+    int pos = 0;
+    row_ptr[0] = 0;
+    std::vector<std::pair<int, double>> row;
+    for (int u = 0; u < s.n; u++) {
+      int r = m.perm[u];
+      row.clear();
+      for (int k = s.rp[r]; k < s.rp[r + 1]; k++) row.push_back({m.iperm[s.col[k]], v[k]});
+      std::sort(row.begin(), row.end(),
+                [](const std::pair<int, double> &a, const std::pair<int, double> &c) {
+                  return a.first < c.first;
+                });
+      for (auto &e : row) {
+        col[pos] = e.first;
+        val[pos] = e.second;
+        pos++;
+      }
+      row_ptr[u + 1] = pos;
+      if (rhs) rhs[u] = b[r];
+    }
+  });
+}
+
+static void assemble_impl(admfem_system h, const admfem_problem *p, hipStream_t st) {
+  System &s = SY(h);
+  REQUIRE(p, ADMFEM_ERR_ARG, "null problem");
+  admfem_problem q = *p;
+  if (p->u_prev) {  // node vector: user -> internal order
+    launch_gather(s.mesh->d_iperm.p, p->u_prev, h->tmp_int.p, s.n, st);
+    q.u_prev = h->tmp_int.p;
+  }
+  launch_assemble(s, q, st);
+  launch_to_sell(s, st);
+}
+
+int admfem_system_assemble(admfem_system h, const admfem_problem *p, void *stream) {
+  return guarded([&] { assemble_impl(h, p, S(stream)); });
+}
+
+int admfem_system_solve(admfem_system h, const double *b, double *x,
+                        const admfem_solve_opts *o, admfem_solve_stats *st, void *stream) {
+  return guarded([&] {
+    System &s = SY(h);
+    REQUIRE(x && o, ADMFEM_ERR_ARG, "null pointer");
+    hipStream_t str = S(stream);
+    const double *b_int = s.rhs.p;
+    if (b) {
+      launch_gather(s.mesh->d_iperm.p, b, h->g_int.p, s.n, str);
+      b_int = h->g_int.p;
+    }
+    admfem_solve_stats tmp{};
+    cg_solve(s, b_int, h->u_int.p, *o, tmp, str);
+    launch_gather(s.mesh->d_perm.p, h->u_int.p, x, s.n, str);
+    HIPCHECK(hipStreamSynchronize(str));
+    if (st) *st = tmp;
+    if (tmp.status != ADMFEM_OK) {
+      set_error("CG did not converge (status " + std::to_string(tmp.status) + ")");
+      throw Error{tmp.status, admfem_last_error()};
+    }
+  });
+}
+
+int admfem_system_kappa_gradient(admfem_system h, const double *u, const double *lam,
+                                 double scale, double *grad_kappa, void *stream) {
+  return guarded([&] {
+    System &s = SY(h);
+    REQUIRE(u && lam && grad_kappa, ADMFEM_ERR_ARG, "null pointer");
+    hipStream_t str = S(stream);
+    launch_gather(s.mesh->d_iperm.p, u, h->u_int.p, s.n, str);
+    launch_gather(s.mesh->d_iperm.p, lam, h->lam_int.p, s.n, str);
+    launch_kappa_gradient(s, h->u_int.p, h->lam_int.p, scale, grad_kappa, str);
+  });
+}
+
+int admfem_system_forward_adjoint(admfem_system h, const admfem_problem *p,
+                                  const int *obs_idx, const double *obs_val, int nobs,
+                                  const admfem_solve_opts *o, double *u, double *lam,
+                                  double *grad_kappa, double *loss,
+                                  admfem_solve_stats st[2], void *stream) {
+  return guarded([&] {
+    System &s = SY(h);
+    REQUIRE(o && (nobs == 0 || (obs_idx && obs_val)) && nobs >= 0, ADMFEM_ERR_ARG,
+            "bad observation arguments");
+    hipStream_t str = S(stream);
+    assemble_impl(h, p, str);
+    admfem_solve_stats s0{}, s1{};
+    cg_solve(s, s.rhs.p, h->u_int.p, *o, s0, str);
+    double L = loss_and_grad(s, h->u_int.p, obs_idx, obs_val, nobs, h->g_int.p, str);
+    cg_solve(s, h->g_int.p, h->lam_int.p, *o, s1, str);
+    if (grad_kappa)
+      launch_kappa_gradient(s, h->u_int.p, h->lam_int.p, p->lap_coef, grad_kappa, str);
+    if (u) launch_gather(s.mesh->d_perm.p, h->u_int.p, u, s.n, str);
+    if (lam) launch_gather(s.mesh->d_perm.p, h->lam_int.p, lam, s.n, str);
+    HIPCHECK(hipStreamSynchronize(str));
+    if (loss) *loss = L;
+    if (st) {
+      st[0] = s0;
+      st[1] = s1;
+    }
+    int bad = s0.status != ADMFEM_OK ? s0.status : s1.status;
+    if (bad != ADMFEM_OK) throw Error{bad, "CG did not converge in the forward or adjoint solve"};
+  });
+}
+
+// ------------------------------------------------- A. reference-named symbols
+admfem_mesh admfem_default_mesh(void) {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  return g_default;
+}
+
+long long *init_nnfem_mesh(double *vertices, int num_vertices, int *element_indices,
+                           int num_elements, int order, int lorder, int degree,
+                           long long *nedges) {
+  (void)lorder;  // line-integral order: boundary-edge integrals are not built
+  long long *ret = nullptr;
+  int rc = guarded([&] {
+    REQUIRE(vertices && element_indices && nedges, ADMFEM_ERR_ARG, "null pointer");
+    REQUIRE(degree == 1, ADMFEM_ERR_UNSUPPORTED, "only degree 1 (P1) elements are built");
+    std::vector<double> c(2 * (size_t)num_vertices);
+    for (int i = 0; i < num_vertices; i++) {
+      c[2 * i] = vertices[3 * i];
+      c[2 * i + 1] = vertices[3 * i + 1];
+    }
+    Mesh *m = mesh_build(c.data(), num_vertices, element_indices, num_elements, order);
+    admfem_mesh h = new admfem_mesh_s{m};
+    {
+      std::lock_guard<std::mutex> lk(g_default_mu);
+      if (g_default) {
+        printf("WARNING: Internal mesh is being overwritten!\n");  // API.cpp:7
+        delete g_default->m;
+        delete g_default;
+      }
+      g_default = h;
+    }
+    mesh_edges(*m);
+    *nedges = m->nedges;
+    ret = (long long *)malloc(sizeof(long long) * std::max<size_t>(1, m->edges.size()));
+    REQUIRE(ret, ADMFEM_ERR_ARG, "host out of memory");
+    std::copy(m->edges.begin(), m->edges.end(), ret);
+  });
+  if (rc != ADMFEM_OK) {
+    fprintf(stderr, "init_nnfem_mesh: %s\n", admfem_last_error());
+    return nullptr;
+  }
+  return ret;
+}
+
+static Mesh *default_or_warn(const char *fn) {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default) {
+    fprintf(stderr, "%s: no mesh (call init_nnfem_mesh first)\n", fn);
+    return nullptr;
+  }
+  return g_default->m;
+}
+
+int mfem_get_ngauss(void) {
+  Mesh *m = default_or_warn("mfem_get_ngauss");
+  return m ? (int)m->ngauss : 0;
+}
+
+void mfem_get_gauss(double *x, double *y) {
+  Mesh *m = default_or_warn("mfem_get_gauss");
+  if (m) guarded([&] {
+      d2h(x, m->gx.p, m->ngauss);
+      d2h(y, m->gy.p, m->ngauss);
+    });
+}
+
+void mfem_get_gauss_weights(double *w) {
+  Mesh *m = default_or_warn("mfem_get_gauss_weights");
+  if (m) guarded([&] { d2h(w, m->gw.p, m->ngauss); });
+}
+
+void mfem_get_area(double *a) {
+  Mesh *m = default_or_warn("mfem_get_area");
+  if (m) guarded([&] { d2h(a, m->area.p, (size_t)m->nelem); });
+}
+
+int mfem_get_elem_ndof(void) { return default_or_warn("mfem_get_elem_ndof") ? 3 : 0; }
+
+int mfem_get_ndof(void) {
+  Mesh *m = default_or_warn("mfem_get_ndof");
+  return m ? m->nnode : 0;
+}
+
+void mfem_get_connectivity(long long *conn) {
+  Mesh *m = default_or_warn("mfem_get_connectivity");
+  if (!m) return;
+  for (size_t k = 0; k < m->elems.size(); k++) conn[k] = m->elems[k] + 1;
+}
+
+void mfem_get_element_to_vertices(long long *elems) {
+  Mesh *m = default_or_warn("mfem_get_element_to_vertices");
+  if (!m) return;
+  for (int i = 0; i < m->nelem; i++)
+    for (int k = 0; k < 3; k++) elems[(size_t)k * m->nelem + i] = m->elems[3 * (size_t)i + k] + 1;
+}
+
+void FemLaplaceScalar_forward_Julia(long long *indices, double *vv, const double *kappa) {
+  Mesh *m = default_or_warn("FemLaplaceScalar_forward_Julia");
+  if (!m) return;
+  int rc = guarded([&] {
+    size_t nt = 9 * (size_t)m->ngauss;
+    DevBuf<double> dk, dv;
+    DevBuf<long long> di;
+    dk.upload(kappa, m->ngauss);
+    dv.alloc(nt);
+    di.alloc(2 * nt);
+    launch_laplace_coo(*m, dk.p, di.p, dv.p, nullptr);
+    d2h(indices, di.p, 2 * nt);
+    d2h(vv, dv.p, nt);
+  });
+  if (rc) fprintf(stderr, "FemLaplaceScalar_forward_Julia: %s\n", admfem_last_error());
+}
+
+void FemSourceScalar_forward_Julia(double *rhs, const double *f) {
+  Mesh *m = default_or_warn("FemSourceScalar_forward_Julia");
+  if (!m) return;
+  int rc = guarded([&] {
+    DevBuf<double> df, dr;
+    df.upload(f, m->ngauss);
+    dr.upload(rhs, m->nnode);
+    launch_source(*m, df.p, dr.p, true, nullptr);
+    d2h(rhs, dr.p, (size_t)m->nnode);
+  });
+  if (rc) fprintf(stderr, "FemSourceScalar_forward_Julia: %s\n", admfem_last_error());
+}
+
+}  // extern "C"

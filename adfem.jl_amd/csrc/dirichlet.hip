@@ -1,0 +1,317 @@
+// ImposeDirichlet and ImposeDirichletGrad on COO triplets
+// (deps/MFEM/ImposeDirichlet/ImposeDirichlet.h:27-93), on the device.
+//
+// Forward: triplets with both ends off the boundary are kept in input order
+// (stable compaction by prefix sum), then (b,b,1.0) for every distinct
+// boundary node in ascending order.  The RHS lifting rhs[i] -= v*bdval[j]
+// (i interior, j boundary) is done per row in triplet order -- the lifted
+// triplets are stably sorted by row and each row's run is summed serially --
+// so orhs is bitwise the reference's.  Duplicate bd entries: the last one
+// wins, as std::map assignment does.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <numeric>
+
+#include "geom.h"
+
+namespace admfem {
+
+namespace {
+
+inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+__global__ void fill_int(int *a, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// bdidx[node] = index into bd/bdval of the entry that wins for node
+__global__ void set_bdidx(const int *__restrict__ keys, const int *__restrict__ win,
+                          int nu, int *__restrict__ bdidx) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nu) bdidx[keys[t]] = win[t];
+}
+
+// keep flag (both ends interior) and lift flag (i interior, j boundary)
+__global__ void classify(const long long *__restrict__ ind, int64_t sN, int N,
+                         const int *__restrict__ bdidx, int *__restrict__ keep,
+                         int *__restrict__ lift, int *__restrict__ err) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sN) return;
+  long long i = ind[2 * k], j = ind[2 * k + 1];
+  if (i < 0 || i >= N || j < 0 || j >= N) {  // never index with a bad id
+    keep[k] = 0;
+    lift[k] = 0;
+    *err = 1;
+    return;
+  }
+  bool bi = bdidx[i] >= 0, bj = bdidx[j] >= 0;
+  keep[k] = (!bi && !bj) ? 1 : 0;
+  lift[k] = (!bi && bj) ? 1 : 0;
+}
+
+__global__ void scatter_kept(const long long *__restrict__ ind,
+                             const double *__restrict__ vv, int64_t sN,
+                             const int *__restrict__ keep,
+                             const int64_t *__restrict__ pos,
+                             long long *__restrict__ oind, double *__restrict__ ov) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sN || !keep[k]) return;
+  int64_t z = pos[k];
+  oind[2 * z] = ind[2 * k];
+  oind[2 * z + 1] = ind[2 * k + 1];
+  ov[z] = vv[k];
+}
+
+// lifted triplet list: key = row (forward) or column (backward), value = k
+__global__ void scatter_lift(const long long *__restrict__ ind, int64_t sN,
+                             const int *__restrict__ lift,
+                             const int64_t *__restrict__ lpos, int use_col,
+                             int *__restrict__ lkey, int64_t *__restrict__ lval) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sN || !lift[k]) return;
+  int64_t z = lpos[k];
+  lkey[z] = (int)ind[2 * k + use_col];
+  lval[z] = k;
+}
+
+__global__ void append_diag(const int *__restrict__ keys, int nu, int64_t base,
+                            long long *__restrict__ oind, double *__restrict__ ov) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nu) return;
+  oind[2 * (base + t)] = keys[t];
+  oind[2 * (base + t) + 1] = keys[t];
+  ov[base + t] = 1.0;
+}
+
+__global__ void copy_rhs(const double *__restrict__ rhs, double *__restrict__ orhs, int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) orhs[i] = rhs[i];
+}
+
+// one thread per run of equal keys in the sorted lift list: serial update in
+// triplet order (the reference's loop order for that row)
+__global__ void lift_rows(const int *__restrict__ skey, const int64_t *__restrict__ sval,
+                          int64_t nl, const long long *__restrict__ ind,
+                          const double *__restrict__ vv,
+                          const int *__restrict__ bdidx,
+                          const double *__restrict__ bdval,
+                          double *__restrict__ orhs) {
+#pragma clang fp contract(off)
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl || (t > 0 && skey[t] == skey[t - 1])) return;
+  int row = skey[t];
+  double acc = orhs[row];
+  for (int64_t u = t; u < nl && skey[u] == row; u++) {
+    int64_t k = sval[u];
+    acc = acc - vv[k] * bdval[bdidx[ind[2 * k + 1]]];
+  }
+  orhs[row] = acc;
+}
+
+__global__ void set_bd_rhs(const int *__restrict__ keys, const int *__restrict__ win,
+                           int nu, const double *__restrict__ bdval,
+                           double *__restrict__ orhs) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nu) orhs[keys[t]] = bdval[win[t]];
+}
+
+// backward: grad_vv, grad_rhs
+__global__ void grad_vv_rhs(const long long *__restrict__ ind, int64_t sN,
+                            const int *__restrict__ keep,
+                            const int64_t *__restrict__ pos,
+                            const int *__restrict__ bdidx,
+                            const double *__restrict__ bdval,
+                            const double *__restrict__ gov,
+                            const double *__restrict__ gorhs,
+                            double *__restrict__ gvv) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sN) return;
+  long long i = ind[2 * k], j = ind[2 * k + 1];
+  bool bi = bdidx[i] >= 0, bj = bdidx[j] >= 0;
+  double g = keep[k] ? gov[pos[k]] : 0.0;
+  if (!bi && bj) g -= bdval[bdidx[j]] * gorhs[i];
+  gvv[k] = g;
+}
+
+__global__ void grad_rhs_k(const int *__restrict__ bdidx, const double *__restrict__ gorhs,
+                           double *__restrict__ grhs, int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) grhs[i] = bdidx[i] >= 0 ? 0.0 : gorhs[i];
+}
+
+__global__ void zero_d(double *a, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+// grad_bdval[win(j)] -= v_k * grad_orhs[i_k] over lifted triplets grouped by
+// column j (serial per column in triplet order), then += grad_orhs[j].
+__global__ void grad_bdval_k(const int *__restrict__ skey, const int64_t *__restrict__ sval,
+                             int64_t nl, const long long *__restrict__ ind,
+                             const double *__restrict__ vv,
+                             const int *__restrict__ bdidx,
+                             const double *__restrict__ gorhs,
+                             double *__restrict__ gbd) {
+#pragma clang fp contract(off)
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl || (t > 0 && skey[t] == skey[t - 1])) return;
+  int j = skey[t];
+  int w = bdidx[j];
+  double acc = gbd[w];
+  for (int64_t u = t; u < nl && skey[u] == j; u++) {
+    int64_t k = sval[u];
+    acc = acc - vv[k] * gorhs[ind[2 * k]];
+  }
+  gbd[w] = acc;
+}
+
+__global__ void grad_bdval_diag(const int *__restrict__ keys, const int *__restrict__ win,
+                                int nu, const double *__restrict__ gorhs,
+                                double *__restrict__ gbd) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nu) gbd[win[t]] += gorhs[keys[t]];
+}
+
+// Host side: distinct boundary nodes (0-based, ascending) and the index of
+// the winning (last) bd entry of each.
+void bd_keys(const long long *bd_dev, int bdN, int N, hipStream_t st,
+             std::vector<int> &keys, std::vector<int> &win) {
+  std::vector<long long> bd(bdN);
+  if (bdN)
+    HIPCHECK(hipMemcpyAsync(bd.data(), bd_dev, sizeof(long long) * bdN,
+                            hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  std::vector<int> idx(bdN);
+  std::iota(idx.begin(), idx.end(), 0);
+  for (int i = 0; i < bdN; i++)
+    REQUIRE(bd[i] >= 1 && bd[i] <= N, ADMFEM_ERR_ARG, "bd entry out of range (1-based)");
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return bd[a] < bd[b]; });
+  keys.clear();
+  win.clear();
+  for (int t = 0; t < bdN; t++) {
+    int key = (int)(bd[idx[t]] - 1);
+    if (!keys.empty() && keys.back() == key) {
+      win.back() = idx[t];  // later entry wins
+    } else {
+      keys.push_back(key);
+      win.push_back(idx[t]);
+    }
+  }
+}
+
+struct DirPrep {
+  DevBuf<int> keys, win, bdidx, keep, lift;
+  DevBuf<int64_t> pos, lpos;
+  int nu = 0;
+  int64_t nkept = 0, nlift = 0;
+};
+
+void exclusive_scan(const int *in, int64_t *out, int64_t n, hipStream_t st) {
+  size_t tmp = 0;
+  HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, st));
+  DevBuf<char> t(std::max<size_t>(tmp, 1));
+  HIPCHECK(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, in, out, (int)n, st));
+}
+
+void prepare(DirPrep &d, const long long *ind, int64_t sN, const long long *bd, int bdN,
+             int N, hipStream_t st) {
+  REQUIRE(sN >= 0 && sN < (int64_t)INT32_MAX, ADMFEM_ERR_ARG, "triplet count out of range");
+  std::vector<int> keys, win;
+  bd_keys(bd, bdN, N, st, keys, win);
+  d.nu = (int)keys.size();
+  d.keys.upload(keys);
+  d.win.upload(win);
+  d.bdidx.alloc(N);
+  fill_int<<<nblk(N, 256), 256, 0, st>>>(d.bdidx.p, N, -1);
+  if (d.nu) set_bdidx<<<nblk(d.nu, 256), 256, 0, st>>>(d.keys.p, d.win.p, d.nu, d.bdidx.p);
+  d.keep.alloc(sN + 1);
+  d.lift.alloc(sN + 1);
+  DevBuf<int> err(1);
+  HIPCHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
+  d.pos.alloc(sN + 1);
+  d.lpos.alloc(sN + 1);
+  HIPCHECK(hipMemsetAsync(d.keep.p, 0, sizeof(int) * (sN + 1), st));
+  HIPCHECK(hipMemsetAsync(d.lift.p, 0, sizeof(int) * (sN + 1), st));
+  if (sN)
+    classify<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, N, d.bdidx.p, d.keep.p, d.lift.p, err.p);
+  HIPCHECK(hipGetLastError());
+  int herr = 0;
+  HIPCHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  exclusive_scan(d.keep.p, d.pos.p, sN + 1, st);
+  exclusive_scan(d.lift.p, d.lpos.p, sN + 1, st);
+  HIPCHECK(hipMemcpyAsync(&d.nkept, d.pos.p + sN, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(&d.nlift, d.lpos.p + sN, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  REQUIRE(!herr, ADMFEM_ERR_ARG, "triplet index out of range [0, N)");
+}
+
+// stably sorted (key, triplet) list of the lifted triplets
+void sorted_lift(const DirPrep &d, const long long *ind, int64_t sN, int use_col,
+                 DevBuf<int> &skey, DevBuf<int64_t> &sval, hipStream_t st) {
+  int64_t nl = d.nlift;
+  DevBuf<int> lkey(std::max<int64_t>(nl, 1));
+  DevBuf<int64_t> lval(std::max<int64_t>(nl, 1));
+  skey.alloc(std::max<int64_t>(nl, 1));
+  sval.alloc(std::max<int64_t>(nl, 1));
+  if (nl == 0) return;
+  scatter_lift<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, d.lift.p, d.lpos.p, use_col, lkey.p,
+                                              lval.p);
+  size_t tmp = 0;
+  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, lkey.p, skey.p, lval.p, sval.p,
+                                              (int)nl, 0, 32, st));
+  DevBuf<char> t(std::max<size_t>(tmp, 1));
+  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, lkey.p, skey.p, lval.p, sval.p,
+                                              (int)nl, 0, 32, st));
+}
+
+}  // namespace
+
+int64_t run_impose_dirichlet(const long long *ind, const double *vv, int64_t sN,
+                             const long long *bd, const double *bdval, int bdN,
+                             const double *rhs, int N, long long *oind, double *ov,
+                             double *orhs, hipStream_t st) {
+  REQUIRE(N > 0 && bdN >= 0, ADMFEM_ERR_ARG, "bad sizes");
+  DirPrep d;
+  prepare(d, ind, sN, bd, bdN, N, st);
+  if (sN) scatter_kept<<<nblk(sN, 256), 256, 0, st>>>(ind, vv, sN, d.keep.p, d.pos.p, oind, ov);
+  if (d.nu) append_diag<<<nblk(d.nu, 256), 256, 0, st>>>(d.keys.p, d.nu, d.nkept, oind, ov);
+  copy_rhs<<<nblk(N, 256), 256, 0, st>>>(rhs, orhs, N);
+  DevBuf<int> skey;
+  DevBuf<int64_t> sval;
+  sorted_lift(d, ind, sN, 0, skey, sval, st);
+  if (d.nlift)
+    lift_rows<<<nblk(d.nlift, 256), 256, 0, st>>>(skey.p, sval.p, d.nlift, ind, vv, d.bdidx.p,
+                                                  bdval, orhs);
+  if (d.nu) set_bd_rhs<<<nblk(d.nu, 256), 256, 0, st>>>(d.keys.p, d.win.p, d.nu, bdval, orhs);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(st));
+  return d.nkept + d.nu;
+}
+
+void run_impose_dirichlet_grad(const double *gov, const double *gorhs,
+                               const long long *ind, const double *vv, int64_t sN,
+                               const long long *bd, const double *bdval, int bdN, int N,
+                               double *gvv, double *grhs, double *gbd, hipStream_t st) {
+  REQUIRE(N > 0 && bdN >= 0, ADMFEM_ERR_ARG, "bad sizes");
+  DirPrep d;
+  prepare(d, ind, sN, bd, bdN, N, st);
+  if (sN)
+    grad_vv_rhs<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, d.keep.p, d.pos.p, d.bdidx.p, bdval,
+                                               gov, gorhs, gvv);
+  grad_rhs_k<<<nblk(N, 256), 256, 0, st>>>(d.bdidx.p, gorhs, grhs, N);
+  if (bdN) zero_d<<<nblk(bdN, 256), 256, 0, st>>>(gbd, bdN);
+  DevBuf<int> skey;
+  DevBuf<int64_t> sval;
+  sorted_lift(d, ind, sN, 1, skey, sval, st);
+  if (d.nlift)
+    grad_bdval_k<<<nblk(d.nlift, 256), 256, 0, st>>>(skey.p, sval.p, d.nlift, ind, vv,
+                                                     d.bdidx.p, gorhs, gbd);
+  if (d.nu)
+    grad_bdval_diag<<<nblk(d.nu, 256), 256, 0, st>>>(d.keys.p, d.win.p, d.nu, gorhs, gbd);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace admfem

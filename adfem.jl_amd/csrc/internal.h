@@ -1,0 +1,189 @@
+// Internal structures of libadmfem (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/admfem.h"
+
+namespace admfem {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+struct Error {
+  int code;
+  std::string msg;
+};
+#define HIPCHECK(expr)                                                        \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess)                                                     \
+      throw ::admfem::Error{ADMFEM_ERR_HIP, std::string(#expr) + ": " +       \
+                                                hipGetErrorString(_e)};       \
+  } while (0)
+#define REQUIRE(cond, code, msg)                                              \
+  do {                                                                        \
+    if (!(cond)) throw ::admfem::Error{code, msg};                            \
+  } while (0)
+
+// Device buffer (RAII over hipMalloc/hipFree).
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T *h, size_t count) {
+    alloc(count);
+    if (count) HIPCHECK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void upload(const std::vector<T> &v) { upload(v.data(), v.size()); }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Packed per-incidence slot word: bits 0-1 local vertex p of the row node in
+// the element, then three 6-bit slots (offset of column conn[e][q] inside the
+// row, q = 0,1,2).  SLOT_LIFT marks a boundary column whose contribution is
+// moved to the right-hand side (Dirichlet lifting).
+constexpr uint32_t SLOT_BITS = 6;
+constexpr uint32_t SLOT_LIFT = 63;
+constexpr int MAX_ROW = 62;
+__host__ __device__ inline uint32_t slot_of(uint32_t w, int q) {
+  return (w >> (2 + SLOT_BITS * q)) & 63u;
+}
+
+constexpr int MAXQ = 3;  // quadrature points per element carried here
+struct QuadRule {
+  int nq;
+  double xi[MAXQ], eta[MAXQ], w[MAXQ];  // reference triangle, weights sum 1/2
+};
+int quad_rule(int order, QuadRule &r);
+
+// --------------------------------------------------------------------- mesh
+struct Mesh {
+  int nnode = 0, nelem = 0, order = 2;
+  QuadRule rule{};
+  int64_t ngauss = 0;
+  // host copies
+  std::vector<double> coords;   // user order, nnode x 2
+  std::vector<int> elems;       // oriented, user node ids, nelem x 3
+  std::vector<int> perm;        // user node id -> internal id
+  std::vector<int> iperm;       // internal id -> user node id
+  std::vector<int> n2e_ptr;     // internal row -> incidences
+  std::vector<int> n2e_elem;    // incident element ids (ascending)
+  std::vector<uint8_t> n2e_p;   // local vertex of the row node
+  std::vector<int> rp;          // pre-BC pattern, internal numbering
+  std::vector<int> col;
+  int64_t nedges = -1;
+  std::vector<long long> edges;  // lazily built, 2*nedges, 1-based user ids
+  // device
+  DevBuf<double> x, y;          // internal order
+  DevBuf<int> conn;             // [3][nelem] internal ids (SoA)
+  DevBuf<int> conn_user;        // [3][nelem] user ids (SoA)
+  DevBuf<int> d_perm, d_iperm;
+  DevBuf<int> d_n2e_ptr, d_n2e_elem;
+  DevBuf<uint32_t> d_slots_full;  // per incidence, slots in the pre-BC row
+  DevBuf<int> d_rp, d_col;
+  DevBuf<double> gx, gy, gw, area;  // geometry (user element order)
+  size_t device_bytes() const;
+};
+
+Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
+                 int order);
+void mesh_edges(Mesh &m);  // fills m.edges/m.nedges (MFEM numbering)
+
+// ------------------------------------------------------------------ system
+struct CGWork;
+struct System {
+  Mesh *mesh = nullptr;
+  int n = 0;       // rows (= nnode)
+  int npad = 0;    // rows padded to a multiple of 64
+  int nbd = 0;     // entries of bdnode as given
+  int nbd_unique = 0;
+  std::vector<int> row_bd;   // internal row -> index into bdval (-1 interior)
+  std::vector<int> rp, col;  // post-BC pattern, internal numbering
+  // device: pattern + assembly maps
+  DevBuf<int> d_rp, d_col, d_row_bd;
+  DevBuf<uint32_t> d_slots;  // per incidence, slots in the post-BC row
+  // assembled FP64 values / rhs (internal order)
+  DevBuf<double> val, rhs;
+  // solver format: SELL-64 off-diagonals + separate diagonal
+  int nslices = 0;
+  std::vector<int> slice_ptr;  // offsets (elements) of each slice
+  DevBuf<int> d_slice_ptr;
+  DevBuf<int> d_sell_col;      // absolute internal column ids
+  DevBuf<double> d_sell_val;   // FP64 values
+  DevBuf<float> d_sell_val32;  // FP32 copy for the mixed-precision solver
+  DevBuf<double> d_diag, d_dinv;
+  DevBuf<float> d_diag32, d_dinv32;
+  int max_block_nnz = 0;  // largest CSR range of a 256-row assembly block
+  CGWork *work = nullptr;
+  size_t device_bytes() const;
+};
+
+System *system_build(Mesh *m, const long long *bdnode, int nbd);
+void system_free(System *s);
+
+// ------------------------------------------------------------- launchers
+// geometry.hip
+void launch_geometry(const Mesh &m, hipStream_t st);
+void launch_laplace_coo(const Mesh &m, const double *kappa, long long *ind,
+                        double *vv, hipStream_t st);
+void launch_laplace_coo_grad(const Mesh &m, const double *gvv, double *gk,
+                             hipStream_t st);
+void launch_mass_coo(const Mesh &m, const double *rho, long long *ind,
+                     double *vv, hipStream_t st);
+void launch_mass_coo_grad(const Mesh &m, const double *gvv, double *gr,
+                          hipStream_t st);
+void launch_source(const Mesh &m, const double *f, double *rhs_user,
+                   bool accumulate, hipStream_t st);
+void launch_source_grad(const Mesh &m, const double *grhs_user, double *gf,
+                        hipStream_t st);
+// assemble.hip
+void launch_assemble(const System &s, const admfem_problem &p, hipStream_t st);
+void launch_to_sell(System &s, hipStream_t st);
+void launch_kappa_gradient(const System &s, const double *u_int,
+                           const double *lam_int, double scale, double *gk,
+                           hipStream_t st);
+// permutation helpers (internal <-> user order)
+void launch_gather(const int *idx, const double *src, double *dst, int n,
+                   hipStream_t st);  // dst[i] = src[idx[i]]
+// dirichlet.hip
+int64_t run_impose_dirichlet(const long long *ind, const double *vv,
+                             int64_t sN, const long long *bd,
+                             const double *bdval, int bdN, const double *rhs,
+                             int N, long long *oind, double *ov, double *orhs,
+                             hipStream_t st);
+void run_impose_dirichlet_grad(const double *gov, const double *gorhs,
+                               const long long *ind, const double *vv,
+                               int64_t sN, const long long *bd,
+                               const double *bdval, int bdN, int N,
+                               double *gvv, double *grhs, double *gbd,
+                               hipStream_t st);
+// cg.hip
+void cg_solve(System &s, const double *b_int, double *x_int,
+              const admfem_solve_opts &o, admfem_solve_stats &stats,
+              hipStream_t st);
+void cg_work_free(CGWork *w);
+// loss gradient: g_int = 0; g[perm[idx]] = 2 (u - obs); returns loss on host
+double loss_and_grad(const System &s, const double *u_int, const int *obs_idx,
+                     const double *obs_val, int nobs, double *g_int,
+                     hipStream_t st);
+
+}  // namespace admfem

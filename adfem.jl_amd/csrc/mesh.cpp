@@ -1,0 +1,435 @@
+// Host-side mesh and system set-up: orientation fix, node reordering,
+// node->element incidence, CSR patterns and the per-incidence slot maps that
+// let the assembly kernels gather without atomics.
+//
+// Restates the topology half of NNFEM_Mesh::init (deps/MFEM/Common.cpp:20-142):
+// orientation (FinalizeTriMesh(1,0,true), Common.cpp:40), the P1 dof map
+// (dof[k] = node[k], Common.cpp:63-67) and the edge table (133-140).  The
+// per-element geometry (area, weights, Gauss points) is computed on the GPU
+// (geometry.hip).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "internal.h"
+
+namespace admfem {
+
+int quad_rule(int order, QuadRule &r) {
+  // Reference-triangle rules (area 1/2).  Order 2 is the symmetric 3-point
+  // rule (a,a),(a,b),(b,a) with a = 1/6, b = 2/3, weight 1/6 each, in MFEM's
+  // listing order (the rule Common.cpp:42-44 takes for the P1 default
+  // order 2, src/MFEM/MFEM.jl:73); orders 0/1 are the centroid rule.
+  if (order == -1) order = 2;
+  if (order == 0 || order == 1) {
+    r.nq = 1;
+    r.xi[0] = r.eta[0] = 1.0 / 3.0;
+    r.w[0] = 0.5;
+    return 1;
+  }
+  if (order == 2) {
+    const double a = 1.0 / 6.0, b = 1.0 - 2.0 * a;
+    r.nq = 3;
+    r.xi[0] = a; r.eta[0] = a;
+    r.xi[1] = a; r.eta[1] = b;
+    r.xi[2] = b; r.eta[2] = a;
+    r.w[0] = r.w[1] = r.w[2] = 1.0 / 6.0;
+    return 3;
+  }
+  return -1;
+}
+
+size_t Mesh::device_bytes() const {
+  return x.bytes() + y.bytes() + conn.bytes() + conn_user.bytes() +
+         d_perm.bytes() + d_iperm.bytes() + d_n2e_ptr.bytes() +
+         d_n2e_elem.bytes() + d_slots_full.bytes() + d_rp.bytes() +
+         d_col.bytes() + gx.bytes() + gy.bytes() + gw.bytes() + area.bytes();
+}
+
+// Reverse Cuthill-McKee order of the node graph (adjacency = pattern rows
+// without the diagonal).  Returns order[k] = node placed at position k.
+static std::vector<int> rcm_order(int n, const std::vector<int> &rp,
+                                  const std::vector<int> &col) {
+  std::vector<int> deg(n);
+  for (int i = 0; i < n; i++) deg[i] = rp[i + 1] - rp[i];
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<char> seen(n, 0);
+  std::vector<int> level(n, -1);
+  std::vector<int> nb;
+  // BFS from s over unvisited nodes, marking `mark`; returns visit order
+  auto bfs = [&](int s, std::vector<char> &mark, std::vector<int> &out,
+                 bool sorted) {
+    size_t head = out.size();
+    out.push_back(s);
+    mark[s] = 1;
+    while (head < out.size()) {
+      int u = out[head++];
+      nb.clear();
+      for (int k = rp[u]; k < rp[u + 1]; k++) {
+        int v = col[k];
+        if (!mark[v]) {
+          mark[v] = 1;
+          nb.push_back(v);
+        }
+      }
+      if (sorted)
+        std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) {
+          return deg[a] < deg[b] || (deg[a] == deg[b] && a < b);
+        });
+      out.insert(out.end(), nb.begin(), nb.end());
+    }
+  };
+  std::vector<int> byDeg(n);
+  std::iota(byDeg.begin(), byDeg.end(), 0);
+  std::stable_sort(byDeg.begin(), byDeg.end(),
+                   [&](int a, int b) { return deg[a] < deg[b]; });
+  std::vector<char> tmpmark(n, 0);
+  std::vector<int> comp;
+  for (int cand : byDeg) {
+    if (seen[cand]) continue;
+    // pseudo-peripheral start: repeat BFS from a min-degree node of the
+    // last level while the eccentricity grows (George & Liu)
+    int start = cand;
+    int ecc = -1;
+    for (int rep = 0; rep < 4; rep++) {
+      comp.clear();
+      bfs(start, tmpmark, comp, false);
+      // levels
+      level[start] = 0;
+      int maxl = 0;
+      for (int u : comp) {
+        for (int k = rp[u]; k < rp[u + 1]; k++) {
+          int v = col[k];
+          if (level[v] < 0) {
+            level[v] = level[u] + 1;
+            maxl = std::max(maxl, level[v]);
+          }
+        }
+      }
+      int best = start;
+      for (int u : comp)
+        if (level[u] == maxl && (best == start || deg[u] < deg[best])) best = u;
+      for (int u : comp) {
+        tmpmark[u] = 0;
+        level[u] = -1;
+      }
+      if (maxl <= ecc) break;
+      ecc = maxl;
+      start = best;
+    }
+    size_t before = order.size();
+    bfs(start, seen, order, true);
+    std::reverse(order.begin() + before, order.end());
+  }
+  return order;
+}
+
+Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
+                 int order) {
+  REQUIRE(nnode > 0 && nelem > 0, ADMFEM_ERR_ARG, "empty mesh");
+  REQUIRE(coords && elems, ADMFEM_ERR_ARG, "null coords/elems");
+  QuadRule rule;
+  REQUIRE(quad_rule(order, rule) > 0, ADMFEM_ERR_UNSUPPORTED,
+          "integration order " + std::to_string(order) +
+              " not built (P1 supports orders 1 and 2)");
+  Mesh *m = new Mesh();
+  try {
+    m->nnode = nnode;
+    m->nelem = nelem;
+    m->order = order == -1 ? 2 : order;
+    m->rule = rule;
+    m->ngauss = (int64_t)rule.nq * nelem;
+    m->coords.assign(coords, coords + 2 * (size_t)nnode);
+    m->elems.assign(elems, elems + 3 * (size_t)nelem);
+    for (size_t k = 0; k < m->elems.size(); k++)
+      REQUIRE(m->elems[k] >= 0 && m->elems[k] < nnode, ADMFEM_ERR_ARG,
+              "element vertex id out of range");
+    // orientation fix (FinalizeTriMesh(1,0,true), Common.cpp:40): swap the
+    // first two vertices of a triangle with negative Jacobian determinant
+    for (int e = 0; e < nelem; e++) {
+      int *v = &m->elems[3 * (size_t)e];
+      double x1 = coords[2 * v[0]], y1 = coords[2 * v[0] + 1];
+      double x2 = coords[2 * v[1]], y2 = coords[2 * v[1] + 1];
+      double x3 = coords[2 * v[2]], y3 = coords[2 * v[2] + 1];
+      double det = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1);
+      if (det < 0.0) std::swap(v[0], v[1]);
+    }
+    // node -> element incidence in user numbering (elements ascending)
+    std::vector<int> uptr(nnode + 1, 0);
+    for (int e = 0; e < nelem; e++)
+      for (int k = 0; k < 3; k++) uptr[m->elems[3 * (size_t)e + k] + 1]++;
+    for (int i = 0; i < nnode; i++) uptr[i + 1] += uptr[i];
+    std::vector<int> uinc(uptr[nnode]);
+    {
+      std::vector<int> pos(uptr.begin(), uptr.end() - 1);
+      for (int e = 0; e < nelem; e++)
+        for (int k = 0; k < 3; k++) uinc[pos[m->elems[3 * (size_t)e + k]]++] = e;
+    }
+    // pre-BC pattern, user numbering: union of incident elements' vertices
+    std::vector<int> urp(nnode + 1, 0), ucol;
+    ucol.reserve((size_t)nnode * 8);
+    std::vector<int> cand;
+    for (int i = 0; i < nnode; i++) {
+      cand.clear();
+      for (int k = uptr[i]; k < uptr[i + 1]; k++) {
+        const int *v = &m->elems[3 * (size_t)uinc[k]];
+        cand.push_back(v[0]);
+        cand.push_back(v[1]);
+        cand.push_back(v[2]);
+      }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      REQUIRE((int)cand.size() <= MAX_ROW, ADMFEM_ERR_UNSUPPORTED,
+              "a node has more than 61 neighbours");
+      ucol.insert(ucol.end(), cand.begin(), cand.end());
+      urp[i + 1] = (int)ucol.size();
+    }
+    // internal numbering: reverse Cuthill-McKee over the node graph
+    {
+      std::vector<int> arp(nnode + 1, 0), acol;
+      acol.reserve(ucol.size());
+      for (int i = 0; i < nnode; i++) {
+        for (int k = urp[i]; k < urp[i + 1]; k++)
+          if (ucol[k] != i) acol.push_back(ucol[k]);
+        arp[i + 1] = (int)acol.size();
+      }
+      m->iperm = rcm_order(nnode, arp, acol);
+      m->perm.assign(nnode, 0);
+      for (int k = 0; k < nnode; k++) m->perm[m->iperm[k]] = k;
+    }
+    const std::vector<int> &P = m->perm, &IP = m->iperm;
+    // internal pattern and incidences
+    m->rp.assign(nnode + 1, 0);
+    m->col.resize(ucol.size());
+    m->n2e_ptr.assign(nnode + 1, 0);
+    m->n2e_elem.resize(uinc.size());
+    m->n2e_p.resize(uinc.size());
+    for (int r = 0; r < nnode; r++) {
+      int u = IP[r];
+      int len = urp[u + 1] - urp[u];
+      m->rp[r + 1] = m->rp[r] + len;
+      int inc = uptr[u + 1] - uptr[u];
+      m->n2e_ptr[r + 1] = m->n2e_ptr[r] + inc;
+    }
+    std::vector<uint32_t> slots(uinc.size());
+    std::vector<int> conn_int(3 * (size_t)nelem), conn_usr(3 * (size_t)nelem);
+    for (int e = 0; e < nelem; e++)
+      for (int k = 0; k < 3; k++) {
+        int u = m->elems[3 * (size_t)e + k];
+        conn_usr[(size_t)k * nelem + e] = u;
+        conn_int[(size_t)k * nelem + e] = P[u];
+      }
+    for (int r = 0; r < nnode; r++) {
+      int u = IP[r];
+      int *c = &m->col[m->rp[r]];
+      int len = 0;
+      for (int k = urp[u]; k < urp[u + 1]; k++) c[len++] = P[ucol[k]];
+      std::sort(c, c + len);
+      int base = m->n2e_ptr[r];
+      for (int k = uptr[u]; k < uptr[u + 1]; k++) {
+        int e = uinc[k];
+        int t = base + (k - uptr[u]);
+        m->n2e_elem[t] = e;
+        int p = -1;
+        uint32_t w = 0;
+        for (int q = 0; q < 3; q++) {
+          int vq = m->elems[3 * (size_t)e + q];
+          if (vq == u && p < 0) p = q;
+          int cq = P[vq];
+          int s = (int)(std::lower_bound(c, c + len, cq) - c);
+          w |= (uint32_t)s << (2 + SLOT_BITS * q);
+        }
+        REQUIRE(p >= 0, ADMFEM_ERR_ARG, "internal incidence error");
+        m->n2e_p[t] = (uint8_t)p;
+        slots[t] = w | (uint32_t)p;
+      }
+    }
+    // device copies
+    std::vector<double> xs(nnode), ys(nnode);
+    for (int r = 0; r < nnode; r++) {
+      xs[r] = coords[2 * (size_t)IP[r]];
+      ys[r] = coords[2 * (size_t)IP[r] + 1];
+    }
+    m->x.upload(xs);
+    m->y.upload(ys);
+    m->conn.upload(conn_int);
+    m->conn_user.upload(conn_usr);
+    m->d_perm.upload(m->perm);
+    m->d_iperm.upload(m->iperm);
+    m->d_n2e_ptr.upload(m->n2e_ptr);
+    m->d_n2e_elem.upload(m->n2e_elem);
+    m->d_slots_full.upload(slots);
+    m->d_rp.upload(m->rp);
+    m->d_col.upload(m->col);
+    m->gx.alloc(m->ngauss);
+    m->gy.alloc(m->ngauss);
+    m->gw.alloc(m->ngauss);
+    m->area.alloc(nelem);
+    launch_geometry(*m, nullptr);
+    HIPCHECK(hipStreamSynchronize(nullptr));
+  } catch (...) {
+    delete m;
+    throw;
+  }
+  return m;
+}
+
+// Edge table in MFEM's numbering: an edge gets the next id the first time it
+// is met walking the oriented elements and their edges (0,1),(1,2),(2,0)
+// (Mesh::GetVertexToVertexTable / DSTable::Push order); output as
+// Common.cpp:133-140: [min+1 ...; max+1 ...].
+void mesh_edges(Mesh &m) {
+  if (m.nedges >= 0) return;
+  const int n = m.nnode;
+  // pattern rows in user numbering give an O(1) slot per (lo, hi) pair
+  std::vector<int> urp(n + 1, 0), ucol;
+  ucol.reserve(m.col.size());
+  for (int u = 0; u < n; u++) {
+    int r = m.perm[u];
+    for (int k = m.rp[r]; k < m.rp[r + 1]; k++) ucol.push_back(m.iperm[m.col[k]]);
+    std::sort(ucol.begin() + urp[u], ucol.end());
+    urp[u + 1] = (int)ucol.size();
+  }
+  std::vector<int> id(ucol.size(), -1);
+  std::vector<int> lo_of, hi_of;
+  static const int ev[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+  for (int e = 0; e < m.nelem; e++)
+    for (int j = 0; j < 3; j++) {
+      int a = m.elems[3 * (size_t)e + ev[j][0]], b = m.elems[3 * (size_t)e + ev[j][1]];
+      int lo = std::min(a, b), hi = std::max(a, b);
+      int s = (int)(std::lower_bound(ucol.begin() + urp[lo], ucol.begin() + urp[lo + 1], hi) -
+                    ucol.begin());
+      if (id[s] < 0) {
+        id[s] = (int)lo_of.size();
+        lo_of.push_back(lo);
+        hi_of.push_back(hi);
+      }
+    }
+  m.nedges = (int64_t)lo_of.size();
+  m.edges.resize(2 * lo_of.size());
+  for (size_t i = 0; i < lo_of.size(); i++) {
+    m.edges[i] = lo_of[i] + 1;
+    m.edges[lo_of.size() + i] = hi_of[i] + 1;
+  }
+}
+
+size_t System::device_bytes() const {
+  return d_rp.bytes() + d_col.bytes() + d_row_bd.bytes() + d_slots.bytes() +
+         val.bytes() + rhs.bytes() + d_slice_ptr.bytes() + d_sell_col.bytes() +
+         d_sell_val.bytes() + d_sell_val32.bytes() + d_diag.bytes() +
+         d_dinv.bytes() + d_diag32.bytes() + d_dinv32.bytes();
+}
+
+System *system_build(Mesh *m, const long long *bdnode, int nbd) {
+  REQUIRE(m, ADMFEM_ERR_ARG, "null mesh");
+  REQUIRE(nbd >= 0 && (nbd == 0 || bdnode), ADMFEM_ERR_ARG, "bad bdnode");
+  System *s = new System();
+  try {
+    const int n = m->nnode;
+    s->mesh = m;
+    s->n = n;
+    s->npad = (n + 63) / 64 * 64;
+    s->nbd = nbd;
+    s->row_bd.assign(n, -1);
+    for (int i = 0; i < nbd; i++) {
+      long long b = bdnode[i];
+      REQUIRE(b >= 1 && b <= n, ADMFEM_ERR_ARG, "bdnode entry out of range (1-based)");
+      s->row_bd[m->perm[b - 1]] = i;  // later entries win (std::map assignment)
+    }
+    for (int r = 0; r < n; r++) s->nbd_unique += s->row_bd[r] >= 0;
+    // post-BC pattern: interior-interior pairs + boundary diagonals
+    // (ImposeDirichlet.h:27-51 keeps triplets with i,j not on the boundary,
+    // then appends (b,b,1))
+    s->rp.assign(n + 1, 0);
+    s->col.reserve(m->col.size());
+    for (int r = 0; r < n; r++) {
+      if (s->row_bd[r] >= 0) {
+        s->col.push_back(r);
+      } else {
+        for (int k = m->rp[r]; k < m->rp[r + 1]; k++)
+          if (s->row_bd[m->col[k]] < 0) s->col.push_back(m->col[k]);
+      }
+      s->rp[r + 1] = (int)s->col.size();
+    }
+    // slot map of the post-BC rows
+    std::vector<uint32_t> slots(m->n2e_elem.size(), 0);
+    for (int r = 0; r < n; r++) {
+      if (s->row_bd[r] >= 0) continue;
+      const int *c = &s->col[s->rp[r]];
+      int len = s->rp[r + 1] - s->rp[r];
+      for (int t = m->n2e_ptr[r]; t < m->n2e_ptr[r + 1]; t++) {
+        int e = m->n2e_elem[t];
+        uint32_t w = m->n2e_p[t];
+        for (int q = 0; q < 3; q++) {
+          int cq = m->perm[m->elems[3 * (size_t)e + q]];
+          uint32_t sl;
+          if (s->row_bd[cq] >= 0) {
+            sl = SLOT_LIFT;
+          } else {
+            sl = (uint32_t)(std::lower_bound(c, c + len, cq) - c);
+          }
+          w |= sl << (2 + SLOT_BITS * q);
+        }
+        slots[t] = w;
+      }
+    }
+    // SELL-64 layout of the off-diagonal entries
+    s->nslices = s->npad / 64;
+    s->slice_ptr.assign(s->nslices + 1, 0);
+    for (int sl = 0; sl < s->nslices; sl++) {
+      int wmax = 0;
+      for (int l = 0; l < 64; l++) {
+        int r = sl * 64 + l;
+        if (r >= n) break;
+        wmax = std::max(wmax, s->rp[r + 1] - s->rp[r] - 1);
+      }
+      s->slice_ptr[sl + 1] = s->slice_ptr[sl] + 64 * wmax;
+    }
+    std::vector<int> sell_col(std::max(1, s->slice_ptr[s->nslices]));
+    for (int sl = 0; sl < s->nslices; sl++) {
+      int off = s->slice_ptr[sl];
+      int wmax = (s->slice_ptr[sl + 1] - off) / 64;
+      for (int l = 0; l < 64; l++) {
+        int r = sl * 64 + l;
+        int kk = 0;
+        if (r < n)
+          for (int k = s->rp[r]; k < s->rp[r + 1]; k++)
+            if (s->col[k] != r) sell_col[off + kk++ * 64 + l] = s->col[k];
+        for (; kk < wmax; kk++) sell_col[off + kk * 64 + l] = r < n ? r : 0;
+      }
+    }
+    s->d_rp.upload(s->rp);
+    s->d_col.upload(s->col);
+    s->d_row_bd.upload(s->row_bd);
+    s->d_slots.upload(slots);
+    s->val.alloc(s->col.size());
+    s->rhs.alloc(s->npad);
+    HIPCHECK(hipMemset(s->rhs.p, 0, s->rhs.bytes()));
+    s->d_slice_ptr.upload(s->slice_ptr);
+    s->d_sell_col.upload(sell_col);
+    s->d_sell_val.alloc(sell_col.size());
+    s->d_sell_val32.alloc(sell_col.size());
+    HIPCHECK(hipMemset(s->d_sell_val.p, 0, s->d_sell_val.bytes()));
+    HIPCHECK(hipMemset(s->d_sell_val32.p, 0, s->d_sell_val32.bytes()));
+    s->d_diag.alloc(s->npad);
+    s->d_dinv.alloc(s->npad);
+    s->d_diag32.alloc(s->npad);
+    s->d_dinv32.alloc(s->npad);
+    for (int r0 = 0; r0 < n; r0 += 256)
+      s->max_block_nnz = std::max(s->max_block_nnz, s->rp[std::min(r0 + 256, n)] - s->rp[r0]);
+  } catch (...) {
+    system_free(s);
+    throw;
+  }
+  return s;
+}
+
+void system_free(System *s) {
+  if (!s) return;
+  if (s->work) cg_work_free(s->work);
+  delete s;
+}
+
+}  // namespace admfem

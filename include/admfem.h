@@ -1,0 +1,271 @@
+/*
+ * libadmfem -- MI355X (gfx950) drop-in for AdFem's P1-triangle forward and
+ * adjoint path.
+ *
+ * Two layers share one implementation:
+ *
+ *  1. Reference-named symbols (section A).  Same names, argument order and
+ *     meaning as the reference library's C-ABI (deps/MFEM/API.cpp and the
+ *     *_forward_Julia functions), on a process-wide default mesh, with HOST
+ *     pointers, so that the reference's Julia `ccall`s bind unchanged.
+ *
+ *  2. Handle-based entry points (sections B-E), DEVICE pointers and a HIP
+ *     stream argument (hipStream_t passed as void*; NULL = default stream).
+ *     They return an admfem status code; admfem_last_error() describes the
+ *     most recent failure on the calling thread.  Section C mirrors the
+ *     reference's TensorFlow custom ops X / XGrad (forward op and its
+ *     registered gradient op) with the same inputs and outputs, for
+ *     ADCME's load_op_and_grad or a ChainRules rrule to bind.
+ *
+ * Indices: node/element ids are 0-based in arrays handed to kernels
+ * (COO indices, elems), and 1-based where the reference's Julia API is
+ * 1-based (bdnode, connectivity getters, the edge table).
+ * Per-Gauss-point arrays are element-major: g = nq*e + k.
+ */
+#ifndef ADMFEM_H
+#define ADMFEM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define ADMFEM_OK 0
+#define ADMFEM_ERR_ARG 1          /* invalid argument / size */
+#define ADMFEM_ERR_HIP 2          /* a HIP runtime call failed */
+#define ADMFEM_ERR_UNSUPPORTED 3  /* element degree / quadrature not built */
+#define ADMFEM_ERR_NOT_CONVERGED 4
+#define ADMFEM_ERR_NAN 5          /* NaN/Inf met in the solve */
+#define ADMFEM_ERR_BUDGET 6       /* request larger than the HBM budget */
+#define ADMFEM_ERR_BREAKDOWN 7    /* CG breakdown (p.Ap <= 0: not SPD) */
+
+const char *admfem_last_error(void);
+int admfem_version(void);
+
+typedef struct admfem_mesh_s *admfem_mesh;
+typedef struct admfem_system_s *admfem_system;
+
+/* ======================================================================
+ * A. Reference-named C-ABI (default mesh, host pointers)
+ * ==================================================================== */
+
+/* deps/MFEM/API.cpp:4-14 (init_nnfem_mesh -> NNFEM_Mesh::init,
+ * deps/MFEM/Common.cpp:20-142).  vertices: 3*num_vertices (x,y,z; z unused),
+ * element_indices: 3*num_elements, 0-based.  Builds the default mesh
+ * (replacing a previous one, with the reference's warning).  Returns a
+ * malloc()ed array of 2*nedges 1-based vertex ids [min...; max...] that the
+ * caller frees (Julia: unsafe_wrap(own=true)), and writes nedges.  Only
+ * degree 1 (P1) and orders 1 and 2 (1- and 3-point rules) are built; other
+ * values return NULL with admfem_last_error() set. */
+long long *init_nnfem_mesh(double *vertices, int num_vertices,
+                           int *element_indices, int num_elements, int order,
+                           int lorder, int degree, long long *nedges);
+/* deps/MFEM/API.cpp:17-19 */
+int mfem_get_ngauss(void);
+/* deps/MFEM/API.cpp:21-24: Gauss point coordinates, ngauss each. */
+void mfem_get_gauss(double *x, double *y);
+/* deps/MFEM/API.cpp:26-34 */
+void mfem_get_gauss_weights(double *w);
+/* deps/MFEM/API.cpp:36-39 */
+void mfem_get_area(double *a);
+/* deps/MFEM/API.cpp:41-43 */
+int mfem_get_elem_ndof(void);
+/* deps/MFEM/API.cpp:45-47 */
+int mfem_get_ndof(void);
+/* deps/MFEM/API.cpp:49-56: 1-based dofs, element-major. */
+void mfem_get_connectivity(long long *conn);
+/* deps/MFEM/API.cpp:58-65: column-major nelem x 3, 1-based, oriented. */
+void mfem_get_element_to_vertices(long long *elems);
+/* deps/MFEM/FemLaplace1/FemLaplaceScalar.h:59-61 (forward of h:3-26):
+ * indices [2*9*ngauss] (i,j pairs, 0-based), vv [9*ngauss], kappa [ngauss]. */
+void FemLaplaceScalar_forward_Julia(long long *indices, double *vv,
+                                    const double *kappa);
+/* deps/MFEM/FemSource1/FemSourceScalar.h:35-37 (forward of h:4-15):
+ * rhs [ndof] is ACCUMULATED into (the caller zeroes it). */
+void FemSourceScalar_forward_Julia(double *rhs, const double *f);
+/* The default mesh as a handle, for sections B-E (NULL if none). */
+admfem_mesh admfem_default_mesh(void);
+
+/* ======================================================================
+ * B. Mesh handles (replace the process-global `mmesh`,
+ *    deps/MFEM/Common.cpp:7, so several meshes can live at once)
+ * ==================================================================== */
+
+/* coords: host nnode x 2 row-major; elems: host nelem x 3 row-major 0-based
+ * (orientation is fixed as NNFEM_Mesh::init does, Common.cpp:40).
+ * order: integration order (-1 -> 2, the reference default for P1,
+ * src/MFEM/MFEM.jl:73); degree must be 1. */
+int admfem_mesh_create(const double *coords, int nnode, const int *elems,
+                       int nelem, int order, int degree, admfem_mesh *out);
+int admfem_mesh_destroy(admfem_mesh m);
+/* info[0..7] = nnode, nelem, ngauss, ndof, elem_ndof, nedges,
+ *              nnz of the assembled (pre-BC) matrix, quadrature points/elem */
+int admfem_mesh_info(admfem_mesh m, int64_t info[8]);
+/* Host-pointer getters, same layouts as section A. */
+int admfem_mesh_gauss(admfem_mesh m, double *x, double *y);
+int admfem_mesh_weights(admfem_mesh m, double *w);
+int admfem_mesh_area(admfem_mesh m, double *a);
+int admfem_mesh_elems(admfem_mesh m, int *elems); /* nelem x 3 row-major 0-based, oriented */
+int admfem_mesh_edges(admfem_mesh m, long long *edges); /* 2*nedges, as init_nnfem_mesh */
+/* Boundary nodes (edges used by one element), ascending, 1-based
+ * (src/MFEM/MCore.jl:385-406, 439-450 bcedge/bcnode; the reference returns a
+ * Set's order, which nothing downstream depends on).  out capacity nnode. */
+int admfem_mesh_bcnode(admfem_mesh m, long long *out, int *count);
+/* Pre-BC CSR pattern (sorted columns, explicit zeros kept), host pointers. */
+int admfem_mesh_pattern(admfem_mesh m, int *row_ptr, int *col);
+
+/* ======================================================================
+ * C. Operators with their gradients (device pointers).  Each mirrors one
+ *    reference TF op "X" and its gradient op "XGrad".
+ * ==================================================================== */
+
+/* FemLaplaceScalar (deps/MFEM/FemLaplace1/FemLaplaceScalar.cpp:14-33,
+ * forward FemLaplaceScalar.h:3-26): kappa [ngauss] -> indices [9*ngauss x 2]
+ * int64, vv [9*ngauss].  Same triplet order as the reference. */
+int admfem_fem_laplace_scalar(admfem_mesh m, const double *kappa,
+                              long long *indices, double *vv, void *stream);
+/* FemLaplaceScalarGrad (FemLaplaceScalar.h:28-54): grad_vv -> grad_kappa. */
+int admfem_fem_laplace_scalar_grad(admfem_mesh m, const double *grad_vv,
+                                   double *grad_kappa, void *stream);
+/* ComputeFemMassMatrixMfem (deps/MFEM/ComputeFemMassMatrix1/
+ * ComputeFemMassMatrixMfem.h:4-29): rho [ngauss] -> indices, vv. */
+int admfem_fem_mass_matrix(admfem_mesh m, const double *rho,
+                           long long *indices, double *vv, void *stream);
+/* ComputeFemMassMatrixMfemGrad (h:31-56): grad_vv -> grad_rho
+ * (overwritten). */
+int admfem_fem_mass_matrix_grad(admfem_mesh m, const double *grad_vv,
+                                double *grad_rho, void *stream);
+/* FemSourceScalar (deps/MFEM/FemSource1/FemSourceScalar.h:4-15):
+ * f [ngauss] -> rhs [ndof] (overwritten, as the op zeroes it first,
+ * FemSourceScalar.cpp:73).  Atomic-free node gather. */
+int admfem_fem_source_scalar(admfem_mesh m, const double *f, double *rhs,
+                             void *stream);
+/* FemSourceScalarGrad (FemSourceScalar.h:17-32): grad_rhs -> grad_f. */
+int admfem_fem_source_scalar_grad(admfem_mesh m, const double *grad_rhs,
+                                  double *grad_f, void *stream);
+/* ImposeDirichlet (deps/MFEM/ImposeDirichlet/ImposeDirichlet.cpp:14-40,
+ * forward ImposeDirichlet.h:27-60).  indices [sN x 2] int64 0-based, vv [sN],
+ * bd [bdN] int64 1-based, rhs [N], bdval [bdN].  Outputs: oindices
+ * [cap x 2], ov [cap], orhs [N], with cap >= sN + bdN; *out_count (host)
+ * receives the output triplet count.  Kept triplets keep their input order
+ * (explicit zeros and duplicates too), then one (b,b,1.0) per distinct
+ * boundary node in ascending order; duplicated bd entries: the last one
+ * wins, as with the reference's std::map assignment.  Synchronises the
+ * stream (the count is data dependent). */
+int admfem_impose_dirichlet(const long long *indices, const double *vv,
+                            int64_t sN, const long long *bd,
+                            const double *bdval, int bdN, const double *rhs,
+                            int N, long long *oindices, double *ov,
+                            double *orhs, int64_t *out_count, void *stream);
+/* ImposeDirichletGrad (ImposeDirichlet.h:63-93): grad_ov [count],
+ * grad_orhs [N] -> grad_vv [sN], grad_rhs [N], grad_bdval [bdN]. */
+int admfem_impose_dirichlet_grad(const double *grad_ov, const double *grad_orhs,
+                                 const long long *indices, const double *vv,
+                                 int64_t sN, const long long *bd,
+                                 const double *bdval, int bdN, int N,
+                                 double *grad_vv, double *grad_rhs,
+                                 double *grad_bdval, void *stream);
+
+/* ======================================================================
+ * D. Systems: mesh + Dirichlet node set, assembled straight into CSR
+ *    (the fused forward path: compute_fem_laplace_matrix1 / mass / source
+ *    -> impose_Dirichlet_boundary_conditions -> K\F -> gradients).
+ * ==================================================================== */
+
+/* bdnode: host, 1-based (as bcnode returns), duplicates allowed (the last
+ * bdval entry of a node wins).  The post-BC CSR pattern is the reference's:
+ * interior-interior element pairs (explicit zeros kept) plus the boundary
+ * diagonals. */
+int admfem_system_create(admfem_mesh m, const long long *bdnode, int nbd,
+                         admfem_system *out);
+int admfem_system_destroy(admfem_system s);
+/* info[0..3] = n, nnz (post-BC), number of distinct boundary nodes,
+ *              bytes of device memory held */
+int admfem_system_info(admfem_system s, int64_t info[4]);
+/* Post-BC CSR pattern, host pointers (row_ptr [n+1], col [nnz]). */
+int admfem_system_pattern(admfem_system s, int *row_ptr, int *col);
+
+/* What to assemble.  A = mass_coef*M(rho) + lap_coef*K(kappa);
+ * rhs = src_coef*F(f) + M(rho)*u_prev (if u_prev) ; Dirichlet values bdval
+ * (per bdnode entry as given to system_create; NULL = homogeneous).
+ * NULL kappa / rho mean all-ones.  Device pointers. */
+typedef struct {
+  const double *kappa;  /* [ngauss] or NULL */
+  const double *rho;    /* [ngauss] or NULL */
+  const double *f;      /* [ngauss] or NULL (no source) */
+  const double *bdval;  /* [nbd] or NULL */
+  const double *u_prev; /* [n] or NULL */
+  double lap_coef;      /* multiplier of K (1 for Poisson, dt for heat) */
+  double mass_coef;     /* multiplier of M (0 for Poisson, 1 for heat) */
+  double src_coef;      /* multiplier of F (1 for Poisson, dt for heat) */
+} admfem_problem;
+
+/* Assembles A (post-BC CSR values, FP64) and rhs into the system's own
+ * device buffers.  No atomics: each row gathers from its incident elements. */
+int admfem_system_assemble(admfem_system s, const admfem_problem *p,
+                           void *stream);
+/* The last assembled system as host CSR in user node order (row_ptr [n+1],
+ * col [nnz] ascending per row, val [nnz]) and rhs [n] (may be NULL):
+ * the matrix and vector impose_Dirichlet_boundary_conditions returns
+ * (src/MFEM/MUtils.jl:201-213), duplicates summed.  Synchronises. */
+int admfem_system_csr(admfem_system s, int *row_ptr, int *col, double *val,
+                      double *rhs);
+
+#define ADMFEM_PREC_FP64 0   /* FP64 Jacobi-PCG */
+#define ADMFEM_PREC_MIXED 1  /* FP32 Jacobi-PCG, FP64 residual replacement */
+typedef struct {
+  int precision;      /* ADMFEM_PREC_* */
+  int max_iter;       /* total inner iterations */
+  double rtol;        /* ||b - A x||_2 <= rtol * ||b||_2 (true residual) */
+  int check_every;    /* iterations between host convergence checks (0: auto) */
+  double replace_delta; /* mixed: residual replacement when the inner residual
+                           fell by this factor (0: auto 1e-2) */
+} admfem_solve_opts;
+
+typedef struct {
+  int iterations;
+  int replacements;   /* mixed: FP64 residual replacements done */
+  int status;         /* ADMFEM_OK / NOT_CONVERGED / NAN / BREAKDOWN */
+  double rel_residual; /* final true residual ||b-Ax||/||b|| in FP64 */
+  double seconds;
+} admfem_solve_stats;
+
+/* Solves A x = b with the system's assembled A.  b, x: device [n]; b NULL =
+ * the assembled rhs.  x is the output (its input value is ignored). */
+int admfem_system_solve(admfem_system s, const double *b, double *x,
+                        const admfem_solve_opts *o, admfem_solve_stats *st,
+                        void *stream);
+
+/* Adjoint gradient, without forming dK: for every element e and Gauss
+ * point k, grad_kappa[g] = -scale * w_g * (grad lam~ . grad u) on e, where
+ * lam~ is lam with the system's boundary entries set to zero.  This equals
+ * the reference chain SparseSolverGrad -> ImposeDirichletGrad
+ * (ImposeDirichlet.h:63-93, which drops the boundary rows' gradient) ->
+ * FemLaplaceScalarGrad (FemLaplaceScalar.h:28-54).  scale = lap_coef. */
+int admfem_system_kappa_gradient(admfem_system s, const double *u,
+                                 const double *lam, double scale,
+                                 double *grad_kappa, void *stream);
+
+/* One forward + adjoint solve for L = sum_k (u[obs_idx[k]] - obs_val[k])^2:
+ * assemble, solve A u = rhs, g = dL/du, solve A lam = g, grad_kappa.
+ * obs_idx: device int32 [nobs], 0-based, distinct.  Outputs u [n], lam [n]
+ * (may be NULL), grad_kappa [ngauss]; *loss (host) receives L.  Stats for
+ * the forward and adjoint solves in st[0], st[1]. */
+int admfem_system_forward_adjoint(admfem_system s, const admfem_problem *p,
+                                  const int *obs_idx, const double *obs_val,
+                                  int nobs, const admfem_solve_opts *o,
+                                  double *u, double *lam, double *grad_kappa,
+                                  double *loss, admfem_solve_stats st[2],
+                                  void *stream);
+
+/* ======================================================================
+ * E. Serving: continuous batching of forward+adjoint requests under an HBM
+ *    budget (no reference counterpart: the reference runs one solve at a
+ *    time on the host).  See admfem_server_* in admfem_server.h.
+ * ==================================================================== */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,392 @@
+/*
+ * CPU oracle for the AdFem P1 forward/adjoint path -- TEST INFRASTRUCTURE ONLY.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+ * it, always as the checker or the timed CPU baseline, never as product code.
+ *
+ * Every loop below restates the reference C++ loop named in its comment, kept
+ * serial and in the reference's iteration order so that the triplet order,
+ * the duplicate handling and the floating-point operation order match it.
+ * The reference's Eigen temporaries become plain scalars; the arithmetic per
+ * term is the same.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Triangle rules on the reference triangle (0,0),(1,0),(0,1), weights summing
+ * to 1/2.  Order 0/1: centroid; order 2: the symmetric 3-point rule with
+ * points (a,a),(a,b),(b,a), a=1/6, b=2/3, weight 1/6 each, in the order MFEM's
+ * IntegrationRules::Get(TRIANGLE, 2) lists them (the rule the reference takes
+ * at deps/MFEM/Common.cpp:42-44 with its default order 2, src/MFEM/MFEM.jl:73). */
+int orc_tri_rule(int order, double *xi, double *eta, double *weight) {
+  if (order == 0 || order == 1) {
+    xi[0] = 1.0 / 3.0; eta[0] = 1.0 / 3.0; weight[0] = 0.5;
+    return 1;
+  }
+  if (order == 2) {
+    const double a = 1.0 / 6.0, b = 1.0 - 2.0 * a, wt = 1.0 / 6.0;
+    xi[0] = a; eta[0] = a;
+    xi[1] = a; eta[1] = b;
+    xi[2] = b; eta[2] = a;
+    weight[0] = weight[1] = weight[2] = wt;
+    return 3;
+  }
+  return -1;
+}
+
+/* deps/MFEM/Common.cpp:9-15; Eigen's norm() is the square root of the
+ * squared norm. */
+static double nrm(double dx, double dy) { return sqrt(dx * dx + dy * dy); }
+static double heron(const double c[3][2]) {
+  double a = nrm(c[0][0] - c[1][0], c[0][1] - c[1][1]);
+  double b = nrm(c[2][0] - c[1][0], c[2][1] - c[1][1]);
+  double cc = nrm(c[0][0] - c[2][0], c[0][1] - c[2][1]);
+  double s = (a + b + cc) / 2.0;
+  return sqrt(s * (s - a) * (s - b) * (s - cc));
+}
+
+/* deps/MFEM/Common.cpp:20-142, P1 branch. */
+int orc_mesh_init(const double *coords, int nnode, int *elems, int nelem,
+                  int order, double *gx, double *gy, double *w, double *area,
+                  double *h, double *hx, double *hy) {
+  (void)nnode;
+  double qx[8], qy[8], qw[8];
+  int nq = orc_tri_rule(order, qx, qy, qw);
+  if (nq < 0) return -1;
+  for (int e = 0; e < nelem; e++) {
+    int *v = elems + 3 * e;
+    /* FinalizeTriMesh(1, 0, true) (Common.cpp:40): CheckElementOrientation
+     * swaps vertices 0 and 1 of a triangle whose Jacobian determinant is
+     * negative. */
+    {
+      double x1 = coords[2 * v[0]], y1 = coords[2 * v[0] + 1];
+      double x2 = coords[2 * v[1]], y2 = coords[2 * v[1] + 1];
+      double x3 = coords[2 * v[2]], y3 = coords[2 * v[2] + 1];
+      double det = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1);
+      if (det < 0.0) { int t = v[0]; v[0] = v[1]; v[1] = t; }
+    }
+    double c[3][2];
+    for (int k = 0; k < 3; k++) {
+      c[k][0] = coords[2 * v[k]];
+      c[k][1] = coords[2 * v[k] + 1];
+    }
+    double A = heron(c); /* Common.cpp:83 */
+    area[e] = A;
+    /* Jacobian of the affine map, columns d/dxi, d/deta; inverse for
+     * CalcPhysDShape: dshape_phys = dshape_ref * J^{-1}. */
+    double J00 = c[1][0] - c[0][0], J01 = c[2][0] - c[0][0];
+    double J10 = c[1][1] - c[0][1], J11 = c[2][1] - c[0][1];
+    double det = J00 * J11 - J01 * J10;
+    double I00 = J11 / det, I01 = -J01 / det, I10 = -J10 / det, I11 = J00 / det;
+    /* reference-element gradients of (1-xi-eta, xi, eta) */
+    const double dref[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+    for (int i = 0; i < nq; i++) {
+      int g = nq * e + i;
+      double hs0 = 1 - qx[i] - qy[i], hs1 = qx[i], hs2 = qy[i]; /* 106-108 */
+      gx[g] = c[0][0] * hs0 + c[1][0] * hs1 + c[2][0] * hs2;     /* 110 */
+      gy[g] = c[0][1] * hs0 + c[1][1] * hs1 + c[2][1] * hs2;     /* 111 */
+      w[g] = qw[i] * A / 0.5;                                    /* 116 */
+      h[3 * g + 0] = hs0; h[3 * g + 1] = hs1; h[3 * g + 2] = hs2;
+      for (int r = 0; r < 3; r++) {
+        hx[3 * g + r] = dref[r][0] * I00 + dref[r][1] * I10;
+        hy[3 * g + r] = dref[r][0] * I01 + dref[r][1] * I11;
+      }
+    }
+  }
+  return nq;
+}
+
+typedef struct { int64_t key; int64_t ord; } keyord;
+static int cmp_keyord(const void *a, const void *b) {
+  const keyord *x = (const keyord *)a, *y = (const keyord *)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->ord < y->ord ? -1 : (x->ord > y->ord);
+}
+static int cmp_i64(const void *a, const void *b) {
+  int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return x < y ? -1 : (x > y);
+}
+
+/* Common.cpp:133-140 on MFEM's edge numbering. */
+int64_t orc_mesh_edges(const int *elems, int nelem, int nnode, int64_t *out) {
+  static const int ev[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+  int64_t M = 3 * (int64_t)nelem;
+  keyord *ko = (keyord *)malloc(sizeof(keyord) * (M ? M : 1));
+  for (int64_t e = 0; e < nelem; e++)
+    for (int j = 0; j < 3; j++) {
+      int64_t a = elems[3 * e + ev[j][0]], b = elems[3 * e + ev[j][1]];
+      int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+      ko[3 * e + j].key = lo * (int64_t)nnode + hi;
+      ko[3 * e + j].ord = 3 * e + j;
+    }
+  qsort(ko, M, sizeof(keyord), cmp_keyord);
+  /* unique keys with their first occurrence, then order by first occurrence */
+  keyord *u = (keyord *)malloc(sizeof(keyord) * (M ? M : 1));
+  int64_t ne = 0;
+  for (int64_t i = 0; i < M; i++)
+    if (i == 0 || ko[i].key != ko[i - 1].key) {
+      u[ne].key = ko[i].ord; /* sort key: first occurrence */
+      u[ne].ord = ko[i].key;
+      ne++;
+    }
+  qsort(u, ne, sizeof(keyord), cmp_keyord);
+  if (out)
+    for (int64_t i = 0; i < ne; i++) {
+      out[i] = u[i].ord / nnode + 1;
+      out[ne + i] = u[i].ord % nnode + 1;
+    }
+  free(ko);
+  free(u);
+  return ne;
+}
+
+/* FemLaplaceScalar.h:3-26 */
+void orc_laplace_forward(const int *elems, int nelem, int nq, const double *w,
+                         const double *hx, const double *hy,
+                         const double *kappa, int64_t *indices, double *vv) {
+  int64_t s = 0, nz = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int k = 0; k < nq; k++) {
+      const double *dx = hx + 3 * s, *dy = hy + 3 * s;
+      double kw = kappa[s], ww = w[s];
+      s++;
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) {
+          /* N = D * D^T * kappa * w, evaluated as Eigen does: the 3x3
+           * product first, then the two scalar factors. */
+          double d = dx[p] * dx[q] + dy[p] * dy[q];
+          indices[2 * nz] = elems[3 * i + p];
+          indices[2 * nz + 1] = elems[3 * i + q];
+          vv[nz] = d * kw * ww;
+          nz++;
+        }
+    }
+}
+
+/* FemLaplaceScalar.h:28-54 */
+void orc_laplace_backward(int nelem, int nq, const double *w, const double *hx,
+                          const double *hy, const double *grad_vv,
+                          double *grad_kappa) {
+  int64_t s = 0, nz = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int k = 0; k < nq; k++) {
+      const double *dx = hx + 3 * s, *dy = hy + 3 * s;
+      double v = 0.0;
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) {
+          double d = (dx[p] * dx[q] + dy[p] * dy[q]) * w[s];
+          v += grad_vv[nz] * d;
+          nz++;
+        }
+      grad_kappa[s] = v;
+      s++;
+    }
+}
+
+/* ComputeFemMassMatrixMfem.h:4-29 */
+void orc_mass_forward(const int *elems, int nelem, int nq, const double *w,
+                      const double *h, const double *rho, int64_t *indices,
+                      double *vv) {
+  int64_t k = 0, k0 = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++) {
+      const double *N = h + 3 * k0;
+      double r = rho[k0], ww = w[k0];
+      k0++;
+      for (int l = 0; l < 3; l++)
+        for (int s = 0; s < 3; s++) {
+          indices[2 * k] = elems[3 * i + l];
+          indices[2 * k + 1] = elems[3 * i + s];
+          vv[k] = N[l] * N[s] * r * ww;
+          k++;
+        }
+    }
+}
+
+/* ComputeFemMassMatrixMfem.h:31-56 */
+void orc_mass_backward(int nelem, int nq, const double *w, const double *h,
+                       const double *grad_vv, double *grad_rho) {
+  int64_t k = 0, k0 = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++) {
+      const double *N = h + 3 * k0;
+      for (int l = 0; l < 3; l++)
+        for (int s = 0; s < 3; s++) {
+          grad_rho[k0] += N[l] * N[s] * w[k0] * grad_vv[k];
+          k++;
+        }
+      k0++;
+    }
+}
+
+/* FemSourceScalar.h:4-15 */
+void orc_source_forward(const int *elems, int nelem, int nq, const double *w,
+                        const double *h, const double *f, double *rhs) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++) {
+      for (int r = 0; r < 3; r++)
+        rhs[elems[3 * i + r]] += f[k] * h[3 * k + r] * w[k];
+      k++;
+    }
+}
+
+/* FemSourceScalar.h:17-32 */
+void orc_source_backward(const int *elems, int nelem, int nq, const double *w,
+                         const double *h, const double *grad_rhs,
+                         double *grad_f) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++) {
+      grad_f[k] = 0.0;
+      for (int r = 0; r < 3; r++)
+        grad_f[k] += h[3 * k + r] * w[k] * grad_rhs[elems[3 * i + r]];
+      k++;
+    }
+}
+
+/* std::map<int64,double> bdMap of the reference as a sorted unique key array
+ * (last assignment wins), with binary-search lookups. */
+typedef struct { int64_t *key; double *val; int64_t *idx; int n; } bdmap;
+static bdmap bdmap_build(const int64_t *bd, const double *bdval, int bdN) {
+  bdmap m;
+  keyord *ko = (keyord *)malloc(sizeof(keyord) * (bdN ? bdN : 1));
+  for (int i = 0; i < bdN; i++) { ko[i].key = bd[i] - 1; ko[i].ord = i; }
+  qsort(ko, bdN, sizeof(keyord), cmp_keyord);
+  m.key = (int64_t *)malloc(sizeof(int64_t) * (bdN ? bdN : 1));
+  m.val = (double *)malloc(sizeof(double) * (bdN ? bdN : 1));
+  m.idx = (int64_t *)malloc(sizeof(int64_t) * (bdN ? bdN : 1));
+  m.n = 0;
+  for (int i = 0; i < bdN; i++) {
+    if (m.n > 0 && m.key[m.n - 1] == ko[i].key) m.n--; /* later one wins */
+    m.key[m.n] = ko[i].key;
+    m.val[m.n] = bdval ? bdval[ko[i].ord] : 0.0;
+    m.idx[m.n] = ko[i].ord;
+    m.n++;
+  }
+  free(ko);
+  return m;
+}
+static int bdmap_find(const bdmap *m, int64_t k) {
+  int lo = 0, hi = m->n - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    if (m->key[mid] == k) return mid;
+    if (m->key[mid] < k) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+static void bdmap_free(bdmap *m) { free(m->key); free(m->val); free(m->idx); }
+
+/* ImposeDirichlet.h:27-60 */
+int64_t orc_dirichlet_forward(const int64_t *indices, const double *vv,
+                              int64_t sN, const int64_t *bd, const double *bdval,
+                              int bdN, const double *rhs, int N,
+                              int64_t *oindices, double *ov, double *orhs) {
+  bdmap m = bdmap_build(bd, bdval, bdN);
+  for (int i = 0; i < N; i++) orhs[i] = rhs[i];
+  int64_t z = 0;
+  for (int64_t k = 0; k < sN; k++) {
+    int64_t i = indices[2 * k], j = indices[2 * k + 1];
+    int bi = bdmap_find(&m, i), bj = bdmap_find(&m, j);
+    if (bi < 0 && bj < 0) {
+      oindices[2 * z] = i; oindices[2 * z + 1] = j; ov[z] = vv[k]; z++;
+    }
+    if (bi < 0 && bj >= 0) orhs[i] = orhs[i] - vv[k] * m.val[bj];
+  }
+  for (int t = 0; t < m.n; t++) {
+    oindices[2 * z] = m.key[t]; oindices[2 * z + 1] = m.key[t]; ov[z] = 1.0; z++;
+    orhs[m.key[t]] = m.val[t];
+  }
+  bdmap_free(&m);
+  return z;
+}
+
+/* ImposeDirichlet.h:63-93 */
+void orc_dirichlet_backward(double *grad_vv_ipt, double *grad_rhs_ipt,
+                            double *grad_bdval, const double *grad_ov,
+                            const double *grad_orhs, const int64_t *indices,
+                            const double *vv, int64_t sN, const int64_t *bd,
+                            const double *bdval, int bdN, int N) {
+  memset(grad_vv_ipt, 0, sizeof(double) * sN);
+  memset(grad_rhs_ipt, 0, sizeof(double) * N);
+  memset(grad_bdval, 0, sizeof(double) * bdN);
+  bdmap m = bdmap_build(bd, bdval, bdN);
+  int64_t z = 0;
+  for (int64_t k = 0; k < sN; k++) {
+    int64_t i = indices[2 * k], j = indices[2 * k + 1];
+    int bi = bdmap_find(&m, i), bj = bdmap_find(&m, j);
+    if (bi < 0 && bj < 0) grad_vv_ipt[k] = grad_ov[z++];
+    if (bi < 0 && bj >= 0) {
+      grad_vv_ipt[k] -= m.val[bj] * grad_orhs[i];
+      grad_bdval[m.idx[bj]] -= vv[k] * grad_orhs[i];
+    }
+  }
+  for (int i = 0; i < N; i++)
+    if (bdmap_find(&m, i) < 0) grad_rhs_ipt[i] = grad_orhs[i];
+  for (int t = 0; t < m.n; t++) grad_bdval[m.idx[t]] += grad_orhs[m.key[t]];
+  bdmap_free(&m);
+}
+
+int64_t orc_coo_to_csr(const int64_t *indices, const double *vv, int64_t sN,
+                       int N, int64_t *row_ptr, int32_t *col, double *val) {
+  int64_t *cnt = (int64_t *)calloc((size_t)N + 1, sizeof(int64_t));
+  for (int64_t k = 0; k < sN; k++) cnt[indices[2 * k] + 1]++;
+  for (int i = 0; i < N; i++) cnt[i + 1] += cnt[i];
+  int64_t *pos = (int64_t *)malloc(sizeof(int64_t) * ((size_t)N + 1));
+  memcpy(pos, cnt, sizeof(int64_t) * ((size_t)N + 1));
+  int64_t *perm = (int64_t *)malloc(sizeof(int64_t) * (sN ? sN : 1));
+  for (int64_t k = 0; k < sN; k++) perm[pos[indices[2 * k]]++] = k; /* stable */
+  int64_t nnz = 0;
+  row_ptr[0] = 0;
+  keyord *buf = NULL;
+  int64_t cap = 0;
+  for (int i = 0; i < N; i++) {
+    int64_t a = cnt[i], b = cnt[i + 1], n = b - a;
+    if (n > cap) { cap = n; buf = (keyord *)realloc(buf, sizeof(keyord) * cap); }
+    for (int64_t t = 0; t < n; t++) {
+      buf[t].key = indices[2 * perm[a + t] + 1];
+      buf[t].ord = perm[a + t];
+    }
+    qsort(buf, n, sizeof(keyord), cmp_keyord); /* by column, then triplet order */
+    for (int64_t t = 0; t < n; t++) {
+      if (t > 0 && buf[t].key == buf[t - 1].key) {
+        val[nnz - 1] += vv[buf[t].ord];
+      } else {
+        col[nnz] = (int32_t)buf[t].key;
+        val[nnz] = vv[buf[t].ord];
+        nnz++;
+      }
+    }
+    row_ptr[i + 1] = nnz;
+  }
+  free(buf); free(perm); free(pos); free(cnt);
+  return nnz;
+}
+
+/* src/MFEM/MCore.jl:385-406 (bcedge: an edge toggled an odd number of times
+ * is a boundary edge) and 439-450 (bcnode: the set of their vertices). */
+int orc_bcnode(const int *elems, int nelem, int nnode, int64_t *out) {
+  static const int ev[3][2] = {{0, 1}, {2, 1}, {0, 2}};
+  int64_t M = 3 * (int64_t)nelem;
+  int64_t *keys = (int64_t *)malloc(sizeof(int64_t) * (M ? M : 1));
+  for (int64_t e = 0; e < nelem; e++)
+    for (int j = 0; j < 3; j++) {
+      int64_t a = elems[3 * e + ev[j][0]], b = elems[3 * e + ev[j][1]];
+      keys[3 * e + j] = (a < b ? a : b) * (int64_t)nnode + (a < b ? b : a);
+    }
+  qsort(keys, M, sizeof(int64_t), cmp_i64);
+  char *isbd = (char *)calloc((size_t)nnode, 1);
+  for (int64_t i = 0; i < M;) {
+    int64_t j = i;
+    while (j < M && keys[j] == keys[i]) j++;
+    if ((j - i) % 2 == 1) { isbd[keys[i] / nnode] = 1; isbd[keys[i] % nnode] = 1; }
+    i = j;
+  }
+  int n = 0;
+  for (int v = 0; v < nnode; v++)
+    if (isbd[v]) out[n++] = v + 1;
+  free(isbd); free(keys);
+  return n;
+}
