@@ -48,6 +48,7 @@ class AdmfemError(RuntimeError):
 _SIGS = {
     "admfem_last_error": (ctypes.c_char_p, []),
     "admfem_version": (c_int, []),
+    "admfem_system_spmv_repeat": (c_int, [c_vp, c_int, c_int, c_vp, c_vp]),
     # A. reference-named
     "init_nnfem_mesh": (c_ll_p, [c_vp, c_int, c_vp, c_int, c_int, c_int, c_int, c_vp]),
     "mfem_get_ngauss": (c_int, []),

@@ -434,6 +434,20 @@ int admfem_system_forward_adjoint(admfem_system h, const admfem_problem *p,
   });
 }
 
+int admfem_system_spmv_repeat(admfem_system h, int precision, int repeat, double *bytes,
+                              void *stream) {
+  return guarded([&] {
+    System &s = SY(h);
+    REQUIRE(repeat >= 0, ADMFEM_ERR_ARG, "bad repeat");
+    spmv_repeat(s, precision, repeat, S(stream));
+    if (bytes) {
+      const double vb = precision == ADMFEM_PREC_MIXED ? 4.0 : 8.0;
+      const double nnz = (double)s.col.size(), n = s.n;
+      *bytes = nnz * (vb + 4.0) + 4.0 * (n + 1) + 2.0 * n * vb;
+    }
+  });
+}
+
 // ------------------------------------------------- A. reference-named symbols
 admfem_mesh admfem_default_mesh(void) {
   std::lock_guard<std::mutex> lk(g_default_mu);

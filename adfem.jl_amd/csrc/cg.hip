@@ -4,19 +4,30 @@
 //
 // Two precisions share the kernels:
 //  * FP64: matrix, vectors and reductions in FP64.
-//  * mixed: matrix and the inner vectors (r, p, q) in FP32; the solution is
-//    accumulated in FP64 every iteration, and whenever the FP32 residual has
-//    fallen by `replace_delta` (or looks converged) it is replaced by the
-//    true FP64 residual b - A x computed with the FP64 matrix ("reliable
-//    updates", van der Vorst & Ye 2000; Clark et al. 2010).  Convergence is
-//    only declared on a true FP64 residual.
+//  * mixed: matrix and the inner vectors (r, z, p, q) stored in FP32 (FP64
+//    arithmetic), the solution correction accumulated in FP32; whenever the
+//    FP32 residual has fallen by `replace_delta` (or looks converged) the
+//    correction is folded into the FP64 solution and the residual is
+//    replaced by the true FP64 residual b - A x, computed with the FP64
+//    matrix ("reliable updates", van der Vorst & Ye 2000; Clark et al.
+//    2010).  Convergence is declared only on a true FP64 residual.
 //
-// Per iteration: spmv (q = A p, p.q), update (x, r, z = D^-1 r, r.z, r.r),
-// [replace], pupdate (p = z + beta p).  Global reductions are deterministic:
-// per-block partials, then the last block to finish (integer ticket) sums
-// them in block order and updates the scalar state.  Every kernel first reads
-// the solver status and returns at once when the solve has finished, so the
-// host can enqueue iterations in chunks and poll the status between chunks.
+// Per iteration:
+//   spmv:    p = z + beta p_old (own row, and on the fly for each gathered
+//            neighbour), q = A p, per-block partials of p.q
+//   scalar:  one block sums the partials in fixed order -> alpha
+//   update:  x += alpha p, r -= alpha q, z = D^-1 r, partials of r.z, r.r
+//   scalar:  -> beta, convergence / replacement decision
+// (mixed, every eighth iteration, when flagged: fold x, replace r, scalar).
+//
+// No atomics and no fences anywhere: on this part a cross-block atomic or a
+// device-scope fence costs tens of ns per block, serialised, which dwarfs a
+// memory-bound SpMV.  Partials are plain stores made visible by the kernel
+// boundary; one single-block kernel turns them into the next scalars, so the
+// wide kernels only read a handful of words and never wait on each other.
+// Every kernel first reads the solver status and returns at once when the
+// solve has finished, so the host enqueues iterations in chunks and polls
+// the status between chunks.
 #include <chrono>
 #include <cmath>
 
@@ -27,18 +38,30 @@ namespace admfem {
 enum { ST_RUN = 0, ST_CONV = 1, ST_MAXIT = 2, ST_NAN = 3, ST_BREAK = 4 };
 
 struct CGScalars {
-  double rz, rz_prev, pq, alpha, beta, rr, bb, tol2, rr_ref, delta2, rr_true;
-  int it, maxit, status, replace, nrepl, mixed;
-  unsigned tick[4];
+  double rtol2, delta2;  // set by the host
+  double bb, tol2;       // ||b||^2, rtol^2 ||b||^2
+  double rz, rz_prev;    // r.z of the current / previous residual
+  double alpha, beta;
+  double rr;             // ||r||^2 of the current residual (recursive)
+  double rr_ref;         // mixed: ||r||^2 at the last replacement
+  int mixed;
+  int status;            // ST_*
+  int it;                // completed iterations
+  int replace;           // mixed: replacement pending
+  int nrepl;
 };
+
+constexpr int NT = 256;   // threads per wide block = 4 slices of 64 rows
+constexpr int RT = 1024;  // threads of the single-block scalar kernels
 
 struct CGWork {
   int npad = 0, nblocks = 0;
-  DevBuf<double> x, r64, p64, q64, g64, lam64;
-  DevBuf<float> r32, p32, q32;
-  DevBuf<double> part;
+  DevBuf<double> x, r64, z64, pa64, pb64, q64;
+  DevBuf<float> xs32, r32, z32, pa32, pb32, q32;
+  DevBuf<double> part;  // 3 x nblocks block partials
   DevBuf<CGScalars> sc;
   CGScalars *h_sc = nullptr;  // pinned
+  std::vector<double> h_part;
   ~CGWork() {
     if (h_sc) (void)hipHostFree(h_sc);
   }
@@ -48,243 +71,293 @@ void cg_work_free(CGWork *w) { delete w; }
 
 namespace {
 
-constexpr int NT = 256;  // threads per block = 4 slices of 64 rows
-
 // Blocks are dispatched round-robin over the 8 XCDs; map them so that each
 // XCD works on one contiguous eighth of the rows (its L2 then holds that
-// eighth's x-gather window instead of all of x).
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-  return (b & 7) * (nb >> 3) + (b >> 3);
+// eighth's gather window).  gridDim.x % 8 == 0.
+__device__ __forceinline__ int my_row() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  return ((b & 7) * (nb >> 3) + (b >> 3)) * NT + threadIdx.x;
 }
 
-// Sum NV values over the grid: per-block partial into part[i*G + block], the
-// last block to arrive sums all partials in block order into tot.  Returns
-// true in the last block only.
+// This block's sums of NV values, stored to part[j * G + blockIdx.x].
 template <int NV>
-__device__ bool grid_reduce(const double (&v)[NV], double *part, unsigned *tick,
-                            double (&tot)[NV]) {
+__device__ __forceinline__ void put_partials(const double (&v)[NV], double *part) {
   __shared__ double sh[NV][NT / 64];
-  __shared__ int last;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int i = 0; i < NV; i++) {
-    double s = wave_sum(v[i]);
-    if (lane == 0) sh[i][wv] = s;
+  for (int j = 0; j < NV; j++) {
+    double t = wave_sum(v[j]);
+    if (lane == 0) sh[j][wv] = t;
   }
   __syncthreads();
-  const int G = gridDim.x;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
-      double s = 0.0;
+    for (int j = 0; j < NV; j++) {
+      double t = 0.0;
 #pragma unroll
-      for (int k = 0; k < NT / 64; k++) s += sh[i][k];
-      part[i * G + blockIdx.x] = s;
+      for (int k = 0; k < NT / 64; k++) t += sh[j][k];
+      part[(size_t)j * gridDim.x + blockIdx.x] = t;
     }
-    __threadfence();
-    unsigned t = atomicAdd(tick, 1u);
-    last = (t == (unsigned)G - 1);
+  }
+}
+
+// Single-block sums of NV partial arrays of length G (fixed order).
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *part, int G, double (&out)[NV]) {
+  __shared__ double sh[NV][RT / 64];
+  double a[NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) a[j] = 0.0;
+  for (int k = threadIdx.x; k < G; k += RT) {
+#pragma unroll
+    for (int j = 0; j < NV; j++) a[j] += part[(size_t)j * G + k];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; j++) {
+    double t = wave_sum(a[j]);
+    if (lane == 0) sh[j][wv] = t;
   }
   __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  __shared__ double sh2[NT / 64];
 #pragma unroll
-  for (int i = 0; i < NV; i++) {
-    double a = 0.0;
-    for (int k = threadIdx.x; k < G; k += NT) a += __builtin_nontemporal_load(&part[i * G + k]);
-    tot[i] = block_sum<NT>(a, sh2);
+  for (int j = 0; j < NV; j++) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < RT / 64; k++) t += sh[j][k];
+    out[j] = t;
   }
-  if (threadIdx.x == 0) *tick = 0u;
-  return true;
 }
 
-// rows [lo, hi) of this block: contiguous chunk of whole slices
-struct RowRange {
-  int lo, hi;
-};
-__device__ __forceinline__ RowRange block_rows(int nslices) {
-  const int nb = gridDim.x;
-  const int lb = xcd_block(blockIdx.x, nb);
-  const int per = (nslices + nb - 1) / nb;
-  int s0 = min(lb * per, nslices), s1 = min(s0 + per, nslices);
-  return {s0 * 64, s1 * 64};
-}
-
-template <typename VT>
-__device__ __forceinline__ VT sell_row(int r, const int *__restrict__ sp,
-                                       const int *__restrict__ sc,
-                                       const VT *__restrict__ sv,
-                                       const VT *__restrict__ diag,
-                                       const VT *__restrict__ p) {
-  const int sl = r >> 6, lane = r & 63;
-  const int off = sp[sl] + lane;
-  const int w = (sp[sl + 1] - sp[sl]) >> 6;
-  VT s = diag[r] * p[r];
-  for (int k = 0; k < w; k++) s += sv[off + 64 * k] * p[sc[off + 64 * k]];
+// One SELL row, FP64 arithmetic: sum_k v_k * f(col_k).  The first 8 entries
+// are unrolled so every column/value load issues before the dependent
+// gathers; wider rows (high-valence nodes) finish in a loop.
+template <typename VT, class F>
+__device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
+                                           const int *__restrict__ sc,
+                                           const VT *__restrict__ sv, F f) {
+  const int sl = i >> 6;
+  const int s0 = sp[sl];
+  const int off = s0 + (i & 63);
+  const int wfull = (sp[sl + 1] - s0) >> 6;
+  const int w = min(wfull, 8);
+  int c[8];
+  VT v[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k < w) {
+      c[k] = sc[off + 64 * k];
+      v[k] = sv[off + 64 * k];
+    }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k < w) s += (double)v[k] * f(c[k]);
+  for (int k = 8; k < wfull; k++) s += (double)sv[off + 64 * k] * f(sc[off + 64 * k]);
   return s;
 }
 
-// x0 = b on identity (boundary) rows, 0 elsewhere; r0 = b - A x0, which is 0
-// on those rows and b on the rest (interior rows of the eliminated system have
-// no boundary columns); z0 = D^-1 r0, p0 = z0.
+// x0 = b on identity (boundary) rows, 0 elsewhere; r0 = b - A x0 (0 on those
+// rows, b on the rest: interior rows of the eliminated system have no
+// boundary columns); z0 = D^-1 r0; p_old = 0 (with beta = 0 the first spmv
+// forms p = z0); partials of r.z, r.r, b.b.
 template <typename VT>
 __global__ void __launch_bounds__(NT)
-    k_init(int nslices, int n, const int *__restrict__ row_bd,
-           const double *__restrict__ b, const VT *__restrict__ dinv,
-           double *__restrict__ x, VT *__restrict__ r, VT *__restrict__ p,
-           double *part, CGScalars *S) {
-  RowRange R = block_rows(nslices);
-  double rz = 0, rr = 0, bb = 0;
-  for (int i = R.lo + threadIdx.x; i < R.hi; i += NT) {
-    double bi = i < n ? b[i] : 0.0;
-    bool ident = i >= n || row_bd[i] >= 0;
-    double ri = ident ? 0.0 : bi;
-    x[i] = ident && i < n ? bi : 0.0;
-    VT rv = (VT)ri;
-    VT z = rv * dinv[i];
+    k_init(int n, int npad, const int *__restrict__ row_bd, const double *__restrict__ b,
+           const VT *__restrict__ dinv, double *__restrict__ x, float *__restrict__ xs,
+           VT *__restrict__ r, VT *__restrict__ z, VT *__restrict__ po, double *part) {
+  const int i = my_row();
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < npad) {
+    const double bi = i < n ? b[i] : 0.0;
+    const bool ident = i >= n || row_bd[i] >= 0;
+    const double ri = ident ? 0.0 : bi;
+    x[i] = (ident && i < n) ? bi : 0.0;
+    if (xs) xs[i] = 0.0f;
+    const VT rv = (VT)ri;
+    const VT zv = (VT)((double)rv * (double)dinv[i]);
     r[i] = rv;
-    p[i] = z;
-    rz += (double)rv * (double)z;
-    rr += ri * ri;
-    bb += bi * bi;
+    z[i] = zv;
+    po[i] = (VT)0;
+    v[0] = (double)rv * (double)zv;
+    v[1] = ri * ri;
+    v[2] = bi * bi;
   }
-  double v[3] = {rz, rr, bb}, t[3];
-  if (grid_reduce<3>(v, part, &S->tick[0], t) && threadIdx.x == 0) {
-    S->rz = t[0];
-    S->rr = t[1];
-    S->rr_ref = t[1];
-    S->rr_true = t[1];
-    S->bb = t[2];
-    S->tol2 *= t[2];  // host stored rtol^2
-    S->it = 0;
-    S->nrepl = 0;
-    S->replace = 0;
-    S->status = (t[1] <= S->tol2) ? ST_CONV : ST_RUN;
-  }
+  put_partials<3>(v, part);
 }
 
-template <typename VT>
-__global__ void __launch_bounds__(NT)
-    k_spmv(int nslices, const int *__restrict__ sp, const int *__restrict__ sc,
-           const VT *__restrict__ sv, const VT *__restrict__ diag,
-           const VT *__restrict__ p, VT *__restrict__ q, double *part,
-           CGScalars *S) {
-  if (S->status != ST_RUN) return;
-  RowRange R = block_rows(nslices);
-  double pq = 0;
-  for (int i = R.lo + threadIdx.x; i < R.hi; i += NT) {
-    VT s = sell_row<VT>(i, sp, sc, sv, diag, p);
-    q[i] = s;
-    pq += (double)p[i] * (double)s;
-  }
-  double v[1] = {pq}, t[1];
-  if (grid_reduce<1>(v, part, &S->tick[1], t) && threadIdx.x == 0) {
-    S->pq = t[0];
-    if (!(t[0] > 0.0)) {
-      S->status = (t[0] != t[0]) ? ST_NAN : ST_BREAK;
-    } else {
-      S->alpha = S->rz / t[0];
-    }
-  }
+__global__ void __launch_bounds__(RT) s_init(int G, const double *part, CGScalars *S) {
+  double t[3];
+  sum_partials<3>(part, G, t);
+  if (threadIdx.x != 0) return;
+  S->rz = t[0];
+  S->rr = t[1];
+  S->rr_ref = t[1];
+  S->bb = t[2];
+  S->tol2 = S->rtol2 * t[2];
+  S->beta = 0.0;
+  S->it = 0;
+  S->nrepl = 0;
+  S->replace = 0;
+  S->status = (!(t[1] == t[1]) || !isfinite(t[1])) ? ST_NAN
+              : (t[1] <= S->tol2)                  ? ST_CONV
+                                                   : ST_RUN;
 }
 
+// p = z + beta p_old; q = A p; partials of p.q
 template <typename VT>
 __global__ void __launch_bounds__(NT)
-    k_update(int nslices, const VT *__restrict__ p, const VT *__restrict__ q,
-             double *__restrict__ x, VT *__restrict__ r,
-             const VT *__restrict__ dinv, double *part, CGScalars *S) {
+    k_spmv(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
+           const VT *__restrict__ sv, const VT *__restrict__ diag, const VT *__restrict__ z,
+           const VT *__restrict__ po, VT *__restrict__ pn, VT *__restrict__ q, double *part,
+           const CGScalars *S) {
   if (S->status != ST_RUN) return;
-  RowRange R = block_rows(nslices);
+  const double beta = S->beta;
+  const int i = my_row();
+  double v[1] = {0.0};
+  if (i < npad) {
+    const double p_i = (double)z[i] + beta * (double)po[i];
+    const double s = (double)diag[i] * p_i +
+                     sell_dot<VT>(i, sp, sc, sv, [&](int c) {
+                       return (double)z[c] + beta * (double)po[c];
+                     });
+    const VT pv = (VT)p_i, qv = (VT)s;
+    pn[i] = pv;
+    q[i] = qv;
+    v[0] = (double)pv * (double)qv;
+  }
+  put_partials<1>(v, part);
+}
+
+__global__ void __launch_bounds__(RT) s_alpha(int G, const double *part, CGScalars *S) {
+  if (S->status != ST_RUN) return;
+  double t[1];
+  sum_partials<1>(part, G, t);
+  if (threadIdx.x != 0) return;
+  if (!(t[0] > 0.0)) {  // not SPD, or NaN
+    S->status = (t[0] != t[0]) ? ST_NAN : ST_BREAK;
+    return;
+  }
+  S->alpha = S->rz / t[0];
+}
+
+// x += alpha p (FP64 x, or the FP32 correction xs in the mixed solve);
+// r -= alpha q; z = D^-1 r; partials of r.z, r.r
+template <typename VT, typename XT>
+__global__ void __launch_bounds__(NT)
+    k_update(int npad, const VT *__restrict__ p, const VT *__restrict__ q,
+             XT *__restrict__ x, VT *__restrict__ r, VT *__restrict__ z,
+             const VT *__restrict__ dinv, double *part, const CGScalars *S) {
+  if (S->status != ST_RUN) return;
   const double alpha = S->alpha;
-  const VT a = (VT)alpha;
-  double rz = 0, rr = 0;
-  for (int i = R.lo + threadIdx.x; i < R.hi; i += NT) {
-    VT pi = p[i];
-    x[i] += alpha * (double)pi;
-    VT ri = r[i] - a * q[i];
-    r[i] = ri;
-    VT z = ri * dinv[i];
-    rz += (double)ri * (double)z;
-    rr += (double)ri * (double)ri;
+  const int i = my_row();
+  double v[2] = {0.0, 0.0};
+  if (i < npad) {
+    const double pi = (double)p[i];
+    x[i] = (XT)((double)x[i] + alpha * pi);
+    const VT rv = (VT)((double)r[i] - alpha * (double)q[i]);
+    const VT zv = (VT)((double)rv * (double)dinv[i]);
+    r[i] = rv;
+    z[i] = zv;
+    v[0] = (double)rv * (double)zv;
+    v[1] = (double)rv * (double)rv;
   }
-  double v[2] = {rz, rr}, t[2];
-  if (grid_reduce<2>(v, part, &S->tick[2], t) && threadIdx.x == 0) {
-    S->rz_prev = S->rz;
-    S->rz = t[0];
-    S->beta = t[0] / S->rz_prev;
-    S->rr = t[1];
-    S->it += 1;
-    if (!(t[1] == t[1]) || !isfinite(t[1])) {
-      S->status = ST_NAN;
-    } else if (S->mixed) {
-      if (t[1] <= S->tol2 || t[1] <= S->delta2 * S->rr_ref) S->replace = 1;
-      if (S->it >= S->maxit && !S->replace) S->status = ST_MAXIT;
-    } else {
-      if (t[1] <= S->tol2) S->status = ST_CONV;
-      else if (S->it >= S->maxit) S->status = ST_MAXIT;
-    }
+  put_partials<2>(v, part);
+}
+
+__global__ void __launch_bounds__(RT) s_beta(int G, const double *part, CGScalars *S,
+                                              int maxit) {
+  if (S->status != ST_RUN) return;
+  double t[2];
+  sum_partials<2>(part, G, t);
+  if (threadIdx.x != 0) return;
+  S->rz_prev = S->rz;
+  S->rz = t[0];
+  S->beta = t[0] / S->rz_prev;
+  S->rr = t[1];
+  S->it += 1;
+  if (!(t[1] == t[1]) || !isfinite(t[1])) {
+    S->status = ST_NAN;
+  } else if (S->mixed) {
+    if (t[1] <= S->tol2 || t[1] <= S->delta2 * S->rr_ref) S->replace = 1;
+    if (S->it >= maxit && !S->replace) S->status = ST_MAXIT;
+  } else {
+    if (t[1] <= S->tol2) S->status = ST_CONV;
+    else if (S->it >= maxit) S->status = ST_MAXIT;
   }
 }
 
-// Mixed precision: r = b - A x in FP64 (FP64 matrix), handed to the FP32
-// inner iteration.
+// mixed, when a replacement is pending (or forced at the end): x += xs
 __global__ void __launch_bounds__(NT)
-    k_replace(int nslices, int n, const int *__restrict__ sp,
-              const int *__restrict__ sc, const double *__restrict__ sv,
-              const double *__restrict__ diag, const double *__restrict__ b,
-              const double *__restrict__ x, float *__restrict__ r32,
-              const float *__restrict__ dinv32, double *part, CGScalars *S) {
+    k_fold(int npad, double *__restrict__ x, float *__restrict__ xs, const CGScalars *S,
+           int force) {
+  if (!force && (S->status != ST_RUN || !S->replace)) return;
+  const int i = my_row();
+  if (i < npad) {
+    x[i] += (double)xs[i];
+    xs[i] = 0.0f;
+  }
+}
+
+// mixed, when a replacement is pending: r = b - A x in FP64 (FP64 matrix),
+// handed to the FP32 iteration with z = D^-1 r; partials of r.z, r.r
+__global__ void __launch_bounds__(NT)
+    k_replace(int n, int npad, const int *__restrict__ sp, const int *__restrict__ sc,
+              const double *__restrict__ sv, const double *__restrict__ diag,
+              const double *__restrict__ b, const double *__restrict__ x,
+              float *__restrict__ r32, float *__restrict__ z32,
+              const float *__restrict__ dinv32, double *part, const CGScalars *S) {
   if (S->status != ST_RUN || !S->replace) return;
-  RowRange R = block_rows(nslices);
-  double rz = 0, rr = 0;
-  for (int i = R.lo + threadIdx.x; i < R.hi; i += NT) {
-    double ri = (i < n ? b[i] : 0.0) - sell_row<double>(i, sp, sc, sv, diag, x);
-    float rf = (float)ri;
+  const int i = my_row();
+  double v[2] = {0.0, 0.0};
+  if (i < npad) {
+    const double ax = diag[i] * x[i] + sell_dot<double>(i, sp, sc, sv, [&](int c) { return x[c]; });
+    const double ri = (i < n ? b[i] : 0.0) - ax;
+    const float rf = (float)ri;
+    const float zf = (float)((double)rf * (double)dinv32[i]);
     r32[i] = rf;
-    float z = rf * dinv32[i];
-    rz += (double)rf * (double)z;
-    rr += ri * ri;
+    z32[i] = zf;
+    v[0] = (double)rf * (double)zf;
+    v[1] = ri * ri;
   }
-  double v[2] = {rz, rr}, t[2];
-  if (grid_reduce<2>(v, part, &S->tick[3], t) && threadIdx.x == 0) {
-    S->rz = t[0];
-    S->beta = t[0] / S->rz_prev;
-    S->rr = t[1];
-    S->rr_ref = t[1];
-    S->rr_true = t[1];
-    S->replace = 0;
-    S->nrepl += 1;
-    if (!(t[1] == t[1]) || !isfinite(t[1])) S->status = ST_NAN;
-    else if (t[1] <= S->tol2) S->status = ST_CONV;
-    else if (S->it >= S->maxit) S->status = ST_MAXIT;
+  put_partials<2>(v, part);
+}
+
+__global__ void __launch_bounds__(RT) s_replace(int G, const double *part, CGScalars *S,
+                                                 int maxit) {
+  if (S->status != ST_RUN || !S->replace) return;
+  double t[2];
+  sum_partials<2>(part, G, t);
+  if (threadIdx.x != 0) return;
+  S->rz = t[0];
+  S->beta = t[0] / S->rz_prev;
+  S->rr = t[1];
+  S->rr_ref = t[1];
+  S->replace = 0;
+  S->nrepl += 1;
+  if (!(t[1] == t[1]) || !isfinite(t[1])) S->status = ST_NAN;
+  else if (t[1] <= S->tol2) S->status = ST_CONV;
+  else if (S->it >= maxit) S->status = ST_MAXIT;
+}
+
+// partials of ||b - A x||^2 in FP64 (final check; summed on the host)
+__global__ void __launch_bounds__(NT)
+    k_true_residual(int n, int npad, const int *__restrict__ sp, const int *__restrict__ sc,
+                    const double *__restrict__ sv, const double *__restrict__ diag,
+                    const double *__restrict__ b, const double *__restrict__ x, double *part) {
+  const int i = my_row();
+  double v[1] = {0.0};
+  if (i < npad) {
+    const double ax = diag[i] * x[i] + sell_dot<double>(i, sp, sc, sv, [&](int c) { return x[c]; });
+    const double ri = (i < n ? b[i] : 0.0) - ax;
+    v[0] = ri * ri;
   }
+  put_partials<1>(v, part);
 }
 
 template <typename VT>
-__global__ void __launch_bounds__(NT)
-    k_pupdate(int nslices, const VT *__restrict__ r, const VT *__restrict__ dinv,
-              VT *__restrict__ p, const CGScalars *S) {
-  if (S->status != ST_RUN) return;
-  RowRange R = block_rows(nslices);
-  const VT beta = (VT)S->beta;
-  for (int i = R.lo + threadIdx.x; i < R.hi; i += NT) p[i] = r[i] * dinv[i] + beta * p[i];
-}
-
-// ||b - A x||^2 in FP64 (final check), into S->rr_true.
-__global__ void __launch_bounds__(NT)
-    k_true_residual(int nslices, int n, const int *__restrict__ sp,
-                    const int *__restrict__ sc, const double *__restrict__ sv,
-                    const double *__restrict__ diag, const double *__restrict__ b,
-                    const double *__restrict__ x, double *part, CGScalars *S) {
-  RowRange R = block_rows(nslices);
-  double rr = 0;
-  for (int i = R.lo + threadIdx.x; i < R.hi; i += NT) {
-    double ri = (i < n ? b[i] : 0.0) - sell_row<double>(i, sp, sc, sv, diag, x);
-    rr += ri * ri;
-  }
-  double v[1] = {rr}, t[1];
-  if (grid_reduce<1>(v, part, &S->tick[0], t) && threadIdx.x == 0) S->rr_true = t[0];
+__global__ void k_fill(VT *a, int n, VT v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 }  // namespace
@@ -295,21 +368,23 @@ static CGWork *ensure_work(System &s) {
   CGWork *w = new CGWork();
   try {
     w->npad = s.npad;
-    // block count: a multiple of 8 (XCD mapping), at most 2048, at least one
-    // slice per block
-    int nb = std::min(2048, (s.nslices + 3) / 4);
-    nb = std::max(8, (nb + 7) / 8 * 8);
+    int nb = (s.npad + NT - 1) / NT;
+    nb = (nb + 7) / 8 * 8;  // XCD mapping needs a multiple of 8
     w->nblocks = nb;
     w->x.alloc(s.npad);
     w->r64.alloc(s.npad);
-    w->p64.alloc(s.npad);
+    w->z64.alloc(s.npad);
+    w->pa64.alloc(s.npad);
+    w->pb64.alloc(s.npad);
     w->q64.alloc(s.npad);
-    w->g64.alloc(s.npad);
-    w->lam64.alloc(s.npad);
+    w->xs32.alloc(s.npad);
     w->r32.alloc(s.npad);
-    w->p32.alloc(s.npad);
+    w->z32.alloc(s.npad);
+    w->pa32.alloc(s.npad);
+    w->pb32.alloc(s.npad);
     w->q32.alloc(s.npad);
-    w->part.alloc(4 * (size_t)nb);
+    w->part.alloc((size_t)3 * nb);
+    w->h_part.resize(nb);
     w->sc.alloc(1);
     HIPCHECK(hipMemset(w->sc.p, 0, sizeof(CGScalars)));
     HIPCHECK(hipHostMalloc((void **)&w->h_sc, sizeof(CGScalars), hipHostMallocDefault));
@@ -321,86 +396,125 @@ static CGWork *ensure_work(System &s) {
   return w;
 }
 
-void cg_solve(System &s, const double *b, double *x_out,
-              const admfem_solve_opts &o, admfem_solve_stats &stats,
-              hipStream_t st) {
+// Instrumentation: the solver's SpMV kernel launched `repeat` times on the
+// system's work vectors (z = 1, p_old = 0, status running): it runs its full
+// body every launch and changes none of its inputs.
+void spmv_repeat(System &s, int precision, int repeat, hipStream_t st) {
+  CGWork *w = ensure_work(s);
+  const int nb = w->nblocks;
+  CGScalars init{};
+  HIPCHECK(hipMemcpyAsync(w->sc.p, &init, sizeof(CGScalars), hipMemcpyHostToDevice, st));
+  const unsigned fb = (unsigned)((s.npad + 255) / 256);
+  if (precision == ADMFEM_PREC_MIXED) {
+    k_fill<float><<<fb, 256, 0, st>>>(w->z32.p, s.npad, 1.0f);
+    k_fill<float><<<fb, 256, 0, st>>>(w->pa32.p, s.npad, 0.0f);
+    for (int k = 0; k < repeat; k++)
+      k_spmv<float><<<nb, NT, 0, st>>>(s.npad, s.d_slice_ptr.p, s.d_sell_col.p,
+                                       s.d_sell_val32.p, s.d_diag32.p, w->z32.p, w->pa32.p,
+                                       w->pb32.p, w->q32.p, w->part.p, w->sc.p);
+  } else {
+    k_fill<double><<<fb, 256, 0, st>>>(w->z64.p, s.npad, 1.0);
+    k_fill<double><<<fb, 256, 0, st>>>(w->pa64.p, s.npad, 0.0);
+    for (int k = 0; k < repeat; k++)
+      k_spmv<double><<<nb, NT, 0, st>>>(s.npad, s.d_slice_ptr.p, s.d_sell_col.p,
+                                        s.d_sell_val.p, s.d_diag.p, w->z64.p, w->pa64.p,
+                                        w->pb64.p, w->q64.p, w->part.p, w->sc.p);
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts &o,
+              admfem_solve_stats &stats, hipStream_t st) {
   auto t0 = std::chrono::steady_clock::now();
   CGWork *w = ensure_work(s);
-  const int nb = w->nblocks, ns = s.nslices;
+  const int nb = w->nblocks, np = s.npad;
   const bool mixed = o.precision == ADMFEM_PREC_MIXED;
   REQUIRE(o.precision == ADMFEM_PREC_FP64 || mixed, ADMFEM_ERR_ARG, "bad precision");
   REQUIRE(o.rtol > 0 && o.max_iter > 0, ADMFEM_ERR_ARG, "bad rtol/max_iter");
   CGScalars init{};
-  init.tol2 = o.rtol * o.rtol;  // scaled by ||b||^2 in k_init
-  init.maxit = o.max_iter;
-  init.mixed = mixed ? 1 : 0;
-  double delta = o.replace_delta > 0 ? o.replace_delta : 1e-2;
+  init.rtol2 = o.rtol * o.rtol;
+  const double delta = o.replace_delta > 0 ? o.replace_delta : 0.1;
   init.delta2 = delta * delta;
+  init.mixed = mixed ? 1 : 0;
   HIPCHECK(hipMemcpyAsync(w->sc.p, &init, sizeof(CGScalars), hipMemcpyHostToDevice, st));
   CGScalars *S = w->sc.p;
+  double *part = w->part.p;
+  const int *sp = s.d_slice_ptr.p, *scol = s.d_sell_col.p;
   if (mixed) {
-    k_init<float><<<nb, NT, 0, st>>>(ns, s.n, s.d_row_bd.p, b, s.d_dinv32.p, w->x.p,
-                                     w->r32.p, w->p32.p, w->part.p, S);
+    k_init<float><<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, s.d_dinv32.p, w->x.p,
+                                     w->xs32.p, w->r32.p, w->z32.p, w->pa32.p, part);
   } else {
-    k_init<double><<<nb, NT, 0, st>>>(ns, s.n, s.d_row_bd.p, b, s.d_dinv.p, w->x.p,
-                                      w->r64.p, w->p64.p, w->part.p, S);
+    k_init<double><<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, s.d_dinv.p, w->x.p, nullptr,
+                                      w->r64.p, w->z64.p, w->pa64.p, part);
   }
+  s_init<<<1, RT, 0, st>>>(nb, part, S);
   HIPCHECK(hipGetLastError());
   int chunk = o.check_every > 0 ? o.check_every : 16;
-  int issued = 0;
-  double rr_last = -1;
-  int it_last = 0;
+  int it = 0;  // iterations enqueued
+  double rr_prev = -1;
+  int it_prev = 0;
   for (;;) {
-    for (int k = 0; k < chunk; k++) {
+    const int stop = std::min(it + chunk, o.max_iter);
+    for (; it < stop; it++) {
+      const bool odd = it & 1;
       if (mixed) {
-        k_spmv<float><<<nb, NT, 0, st>>>(ns, s.d_slice_ptr.p, s.d_sell_col.p,
-                                         s.d_sell_val32.p, s.d_diag32.p, w->p32.p,
-                                         w->q32.p, w->part.p, S);
-        k_update<float><<<nb, NT, 0, st>>>(ns, w->p32.p, w->q32.p, w->x.p, w->r32.p,
-                                           s.d_dinv32.p, w->part.p, S);
-        k_replace<<<nb, NT, 0, st>>>(ns, s.n, s.d_slice_ptr.p, s.d_sell_col.p,
-                                     s.d_sell_val.p, s.d_diag.p, b, w->x.p, w->r32.p,
-                                     s.d_dinv32.p, w->part.p, S);
-        k_pupdate<float><<<nb, NT, 0, st>>>(ns, w->r32.p, s.d_dinv32.p, w->p32.p, S);
+        float *po = odd ? w->pb32.p : w->pa32.p, *pn = odd ? w->pa32.p : w->pb32.p;
+        k_spmv<float><<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val32.p, s.d_diag32.p,
+                                         w->z32.p, po, pn, w->q32.p, part, S);
+        s_alpha<<<1, RT, 0, st>>>(nb, part, S);
+        k_update<float, float><<<nb, NT, 0, st>>>(np, pn, w->q32.p, w->xs32.p, w->r32.p,
+                                                  w->z32.p, s.d_dinv32.p, part, S);
+        s_beta<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
+        if ((it & 7) == 7 || it + 1 == o.max_iter) {  // replacement check every 8
+          k_fold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, S, 0);
+          k_replace<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b,
+                                       w->x.p, w->r32.p, w->z32.p, s.d_dinv32.p, part, S);
+          s_replace<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
+        }
       } else {
-        k_spmv<double><<<nb, NT, 0, st>>>(ns, s.d_slice_ptr.p, s.d_sell_col.p,
-                                          s.d_sell_val.p, s.d_diag.p, w->p64.p,
-                                          w->q64.p, w->part.p, S);
-        k_update<double><<<nb, NT, 0, st>>>(ns, w->p64.p, w->q64.p, w->x.p, w->r64.p,
-                                            s.d_dinv.p, w->part.p, S);
-        k_pupdate<double><<<nb, NT, 0, st>>>(ns, w->r64.p, s.d_dinv.p, w->p64.p, S);
+        double *po = odd ? w->pb64.p : w->pa64.p, *pn = odd ? w->pa64.p : w->pb64.p;
+        k_spmv<double><<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val.p, s.d_diag.p, w->z64.p,
+                                          po, pn, w->q64.p, part, S);
+        s_alpha<<<1, RT, 0, st>>>(nb, part, S);
+        k_update<double, double><<<nb, NT, 0, st>>>(np, pn, w->q64.p, w->x.p, w->r64.p,
+                                                    w->z64.p, s.d_dinv.p, part, S);
+        s_beta<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
       }
     }
     HIPCHECK(hipGetLastError());
-    issued += chunk;
     HIPCHECK(hipMemcpyAsync(w->h_sc, S, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     const CGScalars &h = *w->h_sc;
-    if (h.status != ST_RUN) break;
-    if (issued > o.max_iter + 64) break;  // safety net; kernels stop at maxit
-    // next chunk: estimate the iterations left from the observed rate
+    if (h.status != ST_RUN || it >= o.max_iter) break;
+    // next chunk: the iterations left, estimated from the observed rate
     int next = 16;
-    if (rr_last > 0 && h.rr > 0 && h.it > it_last && h.rr < rr_last) {
-      double rate = std::log(h.rr / rr_last) / (h.it - it_last);
+    if (rr_prev > 0 && h.rr > 0 && h.it > it_prev && h.rr < rr_prev) {
+      double rate = std::log(h.rr / rr_prev) / (h.it - it_prev);
       double left = std::log(h.tol2 / h.rr) / rate;
-      if (std::isfinite(left)) next = (int)std::min(512.0, std::max(8.0, left));
+      if (std::isfinite(left)) next = (int)std::min(1024.0, std::max(8.0, 0.9 * left));
     }
-    rr_last = h.rr;
-    it_last = h.it;
-    chunk = o.check_every > 0 ? o.check_every : next;
+    rr_prev = h.rr;
+    it_prev = h.it;
+    chunk = o.check_every > 0 ? o.check_every : (next + 1) / 2 * 2;
   }
-  // final true residual in FP64
-  k_true_residual<<<nb, NT, 0, st>>>(ns, s.n, s.d_slice_ptr.p, s.d_sell_col.p,
-                                     s.d_sell_val.p, s.d_diag.p, b, w->x.p, w->part.p, S);
+  const bool converged = w->h_sc->status == ST_CONV;
+  if (mixed && !converged)  // stopped without a final replacement
+    k_fold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, S, 1);
+  k_true_residual<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b, w->x.p,
+                                     part);
   HIPCHECK(hipGetLastError());
   if (x_out != w->x.p)
     HIPCHECK(hipMemcpyAsync(x_out, w->x.p, sizeof(double) * s.n, hipMemcpyDeviceToDevice, st));
+  HIPCHECK(hipMemcpyAsync(w->h_part.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                          st));
   HIPCHECK(hipMemcpyAsync(w->h_sc, S, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   const CGScalars &h = *w->h_sc;
+  double rr_true = 0.0;
+  for (int k = 0; k < nb; k++) rr_true += w->h_part[k];
   stats.iterations = h.it;
   stats.replacements = h.nrepl;
-  stats.rel_residual = h.bb > 0 ? std::sqrt(h.rr_true / h.bb) : 0.0;
+  stats.rel_residual = h.bb > 0 ? std::sqrt(rr_true / h.bb) : 0.0;
   switch (h.status) {
     case ST_CONV: stats.status = ADMFEM_OK; break;
     case ST_NAN: stats.status = ADMFEM_ERR_NAN; break;

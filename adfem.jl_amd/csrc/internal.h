@@ -181,6 +181,7 @@ void cg_solve(System &s, const double *b_int, double *x_int,
               const admfem_solve_opts &o, admfem_solve_stats &stats,
               hipStream_t st);
 void cg_work_free(CGWork *w);
+void spmv_repeat(System &s, int precision, int repeat, hipStream_t st);
 // loss gradient: g_int = 0; g[perm[idx]] = 2 (u - obs); returns loss on host
 double loss_and_grad(const System &s, const double *u_int, const int *obs_idx,
                      const double *obs_val, int nobs, double *g_int,

@@ -259,6 +259,14 @@ int admfem_system_forward_adjoint(admfem_system s, const admfem_problem *p,
                                   double *loss, admfem_solve_stats st[2],
                                   void *stream);
 
+/* Instrumentation (no reference counterpart): launches the solver's SpMV
+ * kernel (FP64 or FP32 matrix per `precision`) `repeat` times back to back on
+ * `stream`, on the system's work vectors, for timing the dominant kernel.
+ * *bytes (host, may be NULL) receives its algorithmic bytes per launch, the
+ * canonical CSR figure nnz*(s+4) + 4*(n+1) + 2*n*s with s the value size. */
+int admfem_system_spmv_repeat(admfem_system s, int precision, int repeat,
+                              double *bytes, void *stream);
+
 /* ======================================================================
  * E. Serving: continuous batching of forward+adjoint requests under an HBM
  *    budget (no reference counterpart: the reference runs one solve at a

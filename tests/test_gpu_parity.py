@@ -302,3 +302,41 @@ def test_forward_adjoint_gradient_matches_oracle_chain(prec, tol):
     assert relerr(host(out["u"]), ref["u"]) < tol
     assert relerr(host(out["grad_kappa"]), ref["grad_kappa"]) < tol
     assert out["loss"] == pytest.approx(((ref["u"][obs] - uobs) ** 2).sum(), rel=tol)
+
+
+def fan_mesh(k=12):
+    """Centre node, a ring of k nodes and an outer ring of 2k boundary nodes:
+    the centre row has k interior neighbours (wider than the 8-entry fast
+    path of the SpMV)."""
+    ang1 = 2 * np.pi * np.arange(k) / k
+    ang2 = 2 * np.pi * np.arange(2 * k) / (2 * k)
+    coords = np.concatenate([[[0.0, 0.0]], np.stack([np.cos(ang1), np.sin(ang1)], 1),
+                             2.0 * np.stack([np.cos(ang2), np.sin(ang2)], 1)])
+    r1 = lambda i: 1 + (i % k)
+    r2 = lambda j: 1 + k + (j % (2 * k))
+    el = [[0, r1(i), r1(i + 1)] for i in range(k)]
+    for i in range(k):
+        el += [[r1(i), r2(2 * i), r2(2 * i + 1)], [r1(i), r2(2 * i + 1), r1(i + 1)],
+               [r1(i + 1), r2(2 * i + 1), r2(2 * i + 2)]]
+    return coords, np.array(el)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("mixed", 1e-8)])
+def test_wide_rows_fan_mesh(prec, tol):
+    c, e = fan_mesh(12)
+    gm = admfem.Mesh(c, e, zero_based=True)
+    om = O.OracleMesh(c, e)
+    rng = np.random.default_rng(12)
+    kap = rng.random(om.ngauss) + 0.5
+    f = rng.uniform(-1, 1, om.ngauss)
+    bd = om.bcnode()
+    assert bd.size == 24
+    S = admfem.DirichletSystem(gm, bd).assemble(kappa=kap, f=f)
+    A0, r0 = _oracle_system(om, kap, f, bd, np.zeros(bd.size))
+    A, r = S.csr()
+    assert np.diff(A.indptr).max() == 13  # centre: itself + 12 ring nodes
+    assert relerr(A.toarray(), A0.toarray()) < 1e-13
+    u, st = S.solve(precision=prec, rtol=1e-13 if prec == "fp64" else 1e-11)
+    u0 = O.DirectSolver(A0).solve(r0)
+    assert st["status"] == 0
+    assert relerr(host(u), u0) < tol
