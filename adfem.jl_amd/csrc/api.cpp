@@ -14,6 +14,9 @@ struct admfem_system_s {
   admfem::System *s;
   admfem::DevBuf<double> u_int, lam_int, g_int, tmp_int;
 };
+struct admfem_server_s {
+  admfem::Server *sv;
+};
 
 namespace admfem {
 
@@ -445,6 +448,61 @@ int admfem_system_spmv_repeat(admfem_system h, int precision, int repeat, double
       const double nnz = (double)s.col.size(), n = s.n;
       *bytes = nnz * (vb + 4.0) + 4.0 * (n + 1) + 2.0 * n * vb;
     }
+  });
+}
+
+// ------------------------------------------------------------ E. serving
+int admfem_server_create(const admfem_server_opts *o, admfem_server *out) {
+  return guarded([&] {
+    REQUIRE(o && out, ADMFEM_ERR_ARG, "null pointer");
+    *out = nullptr;
+    Server *sv = server_create(*o);
+    *out = new admfem_server_s{sv};
+  });
+}
+
+int admfem_server_destroy(admfem_server h) {
+  return guarded([&] {
+    if (!h) return;
+    server_free(h->sv);
+    delete h;
+  });
+}
+
+static Server &SV(admfem_server h) {
+  REQUIRE(h && h->sv, ADMFEM_ERR_ARG, "null server handle");
+  return *h->sv;
+}
+
+int admfem_server_add_topology(admfem_server h, admfem_system s, double jitter, int *topo_id) {
+  return guarded([&] {
+    REQUIRE(topo_id && jitter >= 0, ADMFEM_ERR_ARG, "bad argument");
+    *topo_id = server_add_topology(&SV(h), &SY(s), jitter);
+  });
+}
+
+int admfem_server_submit(admfem_server h, const admfem_request *r, int64_t *req_id) {
+  return guarded([&] {
+    REQUIRE(r && req_id, ADMFEM_ERR_ARG, "null pointer");
+    *req_id = server_submit(&SV(h), *r);
+  });
+}
+
+int admfem_server_run(admfem_server h, double timeout_s) {
+  return guarded([&] { server_run(&SV(h), timeout_s); });
+}
+
+int admfem_server_result(admfem_server h, int64_t id, admfem_result *out) {
+  return guarded([&] {
+    REQUIRE(out, ADMFEM_ERR_ARG, "null pointer");
+    server_result(&SV(h), id, *out);
+  });
+}
+
+int admfem_server_stats(admfem_server h, double stats[8]) {
+  return guarded([&] {
+    REQUIRE(stats, ADMFEM_ERR_ARG, "null pointer");
+    server_stats(&SV(h), stats);
   });
 }
 

@@ -211,33 +211,67 @@ inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 
 }  // namespace
 
-void launch_assemble(const System &s, const admfem_problem &p, hipStream_t st) {
+void launch_assemble_to(const System &s, const double *x, const double *y,
+                        const admfem_problem &p, double *val, double *rhs,
+                        hipStream_t st) {
   const Mesh &m = *s.mesh;
   // the largest per-block CSR range decides the dynamic LDS size
   size_t lds = (size_t)std::max(s.max_block_nnz, 1) * sizeof(double);
   REQUIRE(lds <= 160 * 1024, ADMFEM_ERR_UNSUPPORTED, "row block too dense for LDS");
   assemble_kernel<<<nblk(s.n, AS_NT), AS_NT, lds, st>>>(
-      s.n, m.nelem, m.rule, m.conn.p, m.x.p, m.y.p, m.d_n2e_ptr.p,
-      m.d_n2e_elem.p, s.d_slots.p, s.d_rp.p, s.d_row_bd.p, p, s.val.p,
-      s.rhs.p);
+      s.n, m.nelem, m.rule, m.conn.p, x, y, m.d_n2e_ptr.p, m.d_n2e_elem.p,
+      s.d_slots.p, s.d_rp.p, s.d_row_bd.p, p, val, rhs);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_assemble(const System &s, const admfem_problem &p, hipStream_t st) {
+  launch_assemble_to(s, s.mesh->x.p, s.mesh->y.p, p, s.val.p, s.rhs.p, st);
+}
+
+void launch_to_sell_to(const System &s, const double *val, double *sv, float *sv32,
+                       double *diag, double *dinv, float *diag32, float *dinv32,
+                       hipStream_t st) {
+  to_sell_kernel<<<nblk(s.npad, 256), 256, 0, st>>>(s.n, s.d_rp.p, s.d_col.p, val,
+                                                    s.d_slice_ptr.p, sv, sv32, diag, dinv,
+                                                    diag32, dinv32, s.npad);
   HIPCHECK(hipGetLastError());
 }
 
 void launch_to_sell(System &s, hipStream_t st) {
-  to_sell_kernel<<<nblk(s.npad, 256), 256, 0, st>>>(
-      s.n, s.d_rp.p, s.d_col.p, s.val.p, s.d_slice_ptr.p, s.d_sell_val.p,
-      s.d_sell_val32.p, s.d_diag.p, s.d_dinv.p, s.d_diag32.p, s.d_dinv32.p,
-      s.npad);
+  launch_to_sell_to(s, s.val.p, s.d_sell_val.p, s.d_sell_val32.p, s.d_diag.p, s.d_dinv.p,
+                    s.d_diag32.p, s.d_dinv32.p, st);
+}
+
+void launch_kappa_gradient_to(const System &s, const double *x, const double *y,
+                              const double *u_int, const double *lam_int, double scale,
+                              double *gk, hipStream_t st) {
+  const Mesh &m = *s.mesh;
+  kappa_grad_kernel<<<nblk(m.nelem, 256), 256, 0, st>>>(
+      m.nelem, m.rule, m.conn.p, x, y, s.d_row_bd.p, u_int, lam_int, scale, gk);
   HIPCHECK(hipGetLastError());
 }
 
 void launch_kappa_gradient(const System &s, const double *u_int,
                            const double *lam_int, double scale, double *gk,
                            hipStream_t st) {
-  const Mesh &m = *s.mesh;
-  kappa_grad_kernel<<<nblk(m.nelem, 256), 256, 0, st>>>(
-      m.nelem, m.rule, m.conn.p, m.x.p, m.y.p, s.d_row_bd.p, u_int, lam_int,
-      scale, gk);
+  launch_kappa_gradient_to(s, s.mesh->x.p, s.mesh->y.p, u_int, lam_int, scale, gk, st);
+}
+
+// Loss and its gradient without a host round trip: g (npad, internal order)
+// is cleared with a kernel, loss2 receives {L, bad-index flag}.
+namespace {
+__global__ void fill_zero(double *a, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+}  // namespace
+
+void launch_loss(const System &s, const double *u_int, const int *obs_idx,
+                 const double *obs_val, int nobs, double *g_int, double *loss2,
+                 hipStream_t st) {
+  fill_zero<<<nblk(s.npad, 256), 256, 0, st>>>(g_int, s.npad);
+  loss_kernel<<<1, 1024, 0, st>>>(s.n, s.mesh->d_perm.p, obs_idx, obs_val, nobs, u_int, g_int,
+                                  loss2);
   HIPCHECK(hipGetLastError());
 }
 

@@ -1,0 +1,49 @@
+// Device-side descriptors of the batched solver and the serving engine.
+#pragma once
+#include "internal.h"
+
+namespace admfem {
+
+constexpr int BATCH_NT = 256;  // rows per block of the batched kernels
+
+// One system of the batch: its matrix (topology arrays shared with every
+// system of the same mesh) and its vectors.  Node vectors are in the
+// topology's internal order, padded to npad.
+struct BSys {
+  const int *sp, *sc;       // SELL slice offsets / columns (topology)
+  const int *row_bd;        // boundary index per row (topology)
+  const double *sv64, *diag64;
+  const float *sv32, *diag32, *dinv32;
+  const double *b;          // right-hand side
+  double *x[2];             // FP64 solution, two buffers (swapped on replacement)
+  float *xs[2];             // FP32 correction since the last replacement
+  float *r, *z, *p[2], *q;
+  double *part;             // 3 * nblk block partials
+  int n, npad, nblk;
+};
+
+// CG state of one system.
+struct BScal {
+  double rtol2, delta2, bb, tol2, rz, rz_prev, alpha, beta, rr, rr_ref;
+  int status, it, maxit, replace, nrepl, sel;
+};
+
+// A Gaussian random field of random Fourier features:
+// g(x) = scale * sum_j cos(wx_j x + wy_j y + phi_j).
+struct SynField {
+  int nmodes;
+  double scale;
+  double wx[32], wy[32], phi[32];
+};
+
+void batch_init(const BSys *D, BScal *S, int sys, int nblk, double rtol2, double delta2,
+                int maxit, hipStream_t st);
+void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, const BSys *D,
+                      BScal *S, int iters, hipStream_t st);
+void batch_final(const BSys *D, const BScal *S, int sys, int npad, double *out,
+                 hipStream_t st);
+void synth_inputs(const System &s, const double *base_x, const double *base_y, uint64_t seed,
+                  double amp, const SynField &kf, const SynField *uf, double *x, double *y,
+                  double *kappa, double *f, double *u_prev_int, hipStream_t st);
+
+}  // namespace admfem

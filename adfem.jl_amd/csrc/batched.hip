@@ -1,0 +1,431 @@
+// Batched mixed-precision Jacobi-PCG over many independent systems at once,
+// for the serving engine (server.cpp), plus on-device synthetic inputs for
+// the serving workload.
+//
+// One launch serves every active system: a block map assigns each block of
+// the grid to (system, block-within-system); per-system descriptors hold the
+// system's SELL matrix and vectors, per-system scalars its CG state.  Each
+// system's reductions run over its own partials in a fixed order, and its
+// replacement schedule depends only on its own iteration count, so a system
+// produces bit-identical results alone or in any mix.
+//
+// Per iteration (all active systems):
+//   bk_spmv   p = z + beta p_old, q = A p (FP32 storage, FP64 arithmetic)
+//   bs_alpha  one block per system: alpha
+//   bk_update xs += alpha p, r -= alpha q, z = D^-1 r
+//   bs_beta   beta, convergence / replacement decision
+//   bk_replace (systems flagged): x' = x + xs, r = b - A64 x' in FP64, xs' = 0
+//   bs_replace true-residual convergence test, swap the x / xs buffers
+// No atomics, no fences (see cg.hip).
+#include <cmath>
+
+#include "batched.h"
+#include "geom.h"
+
+namespace admfem {
+
+namespace {
+
+constexpr int NT = BATCH_NT;
+constexpr int ST_RUN = 0, ST_CONV = 1, ST_MAXIT = 2, ST_NAN = 3, ST_BREAK = 4;
+
+__device__ __forceinline__ void put_partials2(double a, double b, double *part, int G,
+                                              int lb) {
+  __shared__ double sh[2][NT / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  __syncthreads();  // sh may still be read by a previous call
+  if (lane == 0) {
+    sh[0][wv] = a;
+    sh[1][wv] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < NT / 64; k++) {
+      s0 += sh[0][k];
+      s1 += sh[1][k];
+    }
+    part[lb] = s0;
+    part[G + lb] = s1;
+  }
+}
+
+__device__ __forceinline__ void sum2(const double *part, int G, double &o0, double &o1) {
+  __shared__ double sh[2][NT / 64];
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < G; k += NT) {
+    a += part[k];
+    b += part[G + k];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  __syncthreads();  // sh may still be read by a previous call
+  if (lane == 0) {
+    sh[0][wv] = a;
+    sh[1][wv] = b;
+  }
+  __syncthreads();
+  o0 = 0.0;
+  o1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < NT / 64; k++) {
+    o0 += sh[0][k];
+    o1 += sh[1][k];
+  }
+}
+
+template <typename VT, class F>
+__device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
+                                           const int *__restrict__ sc,
+                                           const VT *__restrict__ sv, F f) {
+  const int sl = i >> 6;
+  const int s0 = sp[sl];
+  const int off = s0 + (i & 63);
+  const int wfull = (sp[sl + 1] - s0) >> 6;
+  const int w = min(wfull, 8);
+  int c[8];
+  VT v[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k < w) {
+      c[k] = sc[off + 64 * k];
+      v[k] = sv[off + 64 * k];
+    }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k < w) s += (double)v[k] * f(c[k]);
+  for (int k = 8; k < wfull; k++) s += (double)sv[off + 64 * k] * f(sc[off + 64 * k]);
+  return s;
+}
+
+// --------------------------------------------------------------- CG kernels
+// init of ONE system (grid = its nblk): x0 = b on identity rows; r0, z0;
+// p_old = 0; partials of r.z, r.r; and b.b into part[2G..3G)
+__global__ void __launch_bounds__(NT)
+    bk_init(const BSys *__restrict__ D, int sys) {
+  const BSys &d = D[sys];
+  const int i = blockIdx.x * NT + threadIdx.x;
+  double rz = 0, rr = 0, bb = 0;
+  if (i < d.npad) {
+    const double bi = i < d.n ? d.b[i] : 0.0;
+    const bool ident = i >= d.n || d.row_bd[i] >= 0;
+    const double ri = ident ? 0.0 : bi;
+    d.x[0][i] = (ident && i < d.n) ? bi : 0.0;
+    d.xs[0][i] = 0.0f;
+    const float rv = (float)ri;
+    const float zv = (float)((double)rv * (double)d.dinv32[i]);
+    d.r[i] = rv;
+    d.z[i] = zv;
+    d.p[0][i] = 0.0f;
+    rz = (double)rv * (double)zv;
+    rr = ri * ri;
+    bb = bi * bi;
+  }
+  put_partials2(rz, rr, d.part, d.nblk, blockIdx.x);
+  __syncthreads();
+  put_partials2(bb, 0.0, d.part + 2 * d.nblk, d.nblk, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(NT) bs_init(const BSys *__restrict__ D, BScal *S, int sys,
+                                              double rtol2, double delta2, int maxit) {
+  const BSys &d = D[sys];
+  double rz, rr, bb, unused;
+  sum2(d.part, d.nblk, rz, rr);
+  sum2(d.part + 2 * d.nblk, d.nblk, bb, unused);
+  if (threadIdx.x != 0) return;
+  BScal &s = S[sys];
+  s.rtol2 = rtol2;
+  s.delta2 = delta2;
+  s.maxit = maxit;
+  s.bb = bb;
+  s.tol2 = rtol2 * bb;
+  s.rz = rz;
+  s.rz_prev = rz;
+  s.rr = rr;
+  s.rr_ref = rr;
+  s.alpha = 0.0;
+  s.beta = 0.0;
+  s.it = 0;
+  s.replace = 0;
+  s.nrepl = 0;
+  s.sel = 0;
+  s.status = (!(rr == rr) || !isfinite(rr)) ? ST_NAN : (rr <= s.tol2 ? ST_CONV : ST_RUN);
+}
+
+__global__ void __launch_bounds__(NT)
+    bk_spmv(const int2 *__restrict__ map, const BSys *__restrict__ D, const BScal *S) {
+  const int2 m = map[blockIdx.x];
+  if (m.x < 0) return;
+  const BScal &s = S[m.x];
+  if (s.status != ST_RUN) return;
+  const BSys &d = D[m.x];
+  const int i = m.y * NT + threadIdx.x;
+  const double beta = s.beta;
+  const float *__restrict__ po = d.p[s.it & 1];
+  float *__restrict__ pn = d.p[(s.it + 1) & 1];
+  const float *__restrict__ z = d.z;
+  double pq = 0;
+  if (i < d.npad) {
+    const double p_i = (double)z[i] + beta * (double)po[i];
+    const double a = (double)d.diag32[i] * p_i +
+                     sell_dot<float>(i, d.sp, d.sc, d.sv32, [&](int c) {
+                       return (double)z[c] + beta * (double)po[c];
+                     });
+    const float pv = (float)p_i, qv = (float)a;
+    pn[i] = pv;
+    d.q[i] = qv;
+    pq = (double)pv * (double)qv;
+  }
+  put_partials2(pq, 0.0, d.part, d.nblk, m.y);
+}
+
+__global__ void __launch_bounds__(NT)
+    bs_alpha(const int *__restrict__ list, const BSys *__restrict__ D, BScal *S) {
+  const int sys = list[blockIdx.x];
+  BScal &s = S[sys];
+  if (s.status != ST_RUN) return;
+  double pq, unused;
+  sum2(D[sys].part, D[sys].nblk, pq, unused);
+  if (threadIdx.x != 0) return;
+  if (!(pq > 0.0)) {
+    s.status = (pq != pq) ? ST_NAN : ST_BREAK;
+    return;
+  }
+  s.alpha = s.rz / pq;
+}
+
+__global__ void __launch_bounds__(NT)
+    bk_update(const int2 *__restrict__ map, const BSys *__restrict__ D, const BScal *S) {
+  const int2 m = map[blockIdx.x];
+  if (m.x < 0) return;
+  const BScal &s = S[m.x];
+  if (s.status != ST_RUN) return;
+  const BSys &d = D[m.x];
+  const int i = m.y * NT + threadIdx.x;
+  const double alpha = s.alpha;
+  const float *__restrict__ p = d.p[(s.it + 1) & 1];
+  float *__restrict__ xs = d.xs[s.sel];
+  double rz = 0, rr = 0;
+  if (i < d.npad) {
+    xs[i] = (float)((double)xs[i] + alpha * (double)p[i]);
+    const float rv = (float)((double)d.r[i] - alpha * (double)d.q[i]);
+    const float zv = (float)((double)rv * (double)d.dinv32[i]);
+    d.r[i] = rv;
+    d.z[i] = zv;
+    rz = (double)rv * (double)zv;
+    rr = (double)rv * (double)rv;
+  }
+  put_partials2(rz, rr, d.part, d.nblk, m.y);
+}
+
+__global__ void __launch_bounds__(NT)
+    bs_beta(const int *__restrict__ list, const BSys *__restrict__ D, BScal *S) {
+  const int sys = list[blockIdx.x];
+  BScal &s = S[sys];
+  if (s.status != ST_RUN) return;
+  double rz, rr;
+  sum2(D[sys].part, D[sys].nblk, rz, rr);
+  if (threadIdx.x != 0) return;
+  s.rz_prev = s.rz;
+  s.rz = rz;
+  s.beta = rz / s.rz_prev;
+  s.rr = rr;
+  s.it += 1;
+  if (!(rr == rr) || !isfinite(rr)) {
+    s.status = ST_NAN;
+    return;
+  }
+  // replacement: when the residual looks converged, or (checked every 8th
+  // iteration of this system) when it fell by delta since the last one
+  if (rr <= s.tol2 || ((s.it & 7) == 0 && rr <= s.delta2 * s.rr_ref)) s.replace = 1;
+  else if (s.it >= s.maxit) s.status = ST_MAXIT;
+}
+
+// flagged systems: x' = x + xs into the other buffer, r = b - A64 x', xs' = 0
+__global__ void __launch_bounds__(NT)
+    bk_replace(const int2 *__restrict__ map, const BSys *__restrict__ D, const BScal *S) {
+  const int2 m = map[blockIdx.x];
+  if (m.x < 0) return;
+  const BScal &s = S[m.x];
+  if (s.status != ST_RUN || !s.replace) return;
+  const BSys &d = D[m.x];
+  const int i = m.y * NT + threadIdx.x;
+  const double *__restrict__ x = d.x[s.sel];
+  const float *__restrict__ xs = d.xs[s.sel];
+  double rz = 0, rr = 0;
+  if (i < d.npad) {
+    const double xi = x[i] + (double)xs[i];
+    const double ax = d.diag64[i] * xi + sell_dot<double>(i, d.sp, d.sc, d.sv64, [&](int c) {
+                        return x[c] + (double)xs[c];
+                      });
+    d.x[s.sel ^ 1][i] = xi;
+    d.xs[s.sel ^ 1][i] = 0.0f;
+    const double ri = (i < d.n ? d.b[i] : 0.0) - ax;
+    const float rf = (float)ri;
+    const float zf = (float)((double)rf * (double)d.dinv32[i]);
+    d.r[i] = rf;
+    d.z[i] = zf;
+    rz = (double)rf * (double)zf;
+    rr = ri * ri;
+  }
+  put_partials2(rz, rr, d.part, d.nblk, m.y);
+}
+
+__global__ void __launch_bounds__(NT)
+    bs_replace(const int *__restrict__ list, const BSys *__restrict__ D, BScal *S) {
+  const int sys = list[blockIdx.x];
+  BScal &s = S[sys];
+  if (s.status != ST_RUN || !s.replace) return;
+  double rz, rr;
+  sum2(D[sys].part, D[sys].nblk, rz, rr);
+  if (threadIdx.x != 0) return;
+  s.rz = rz;
+  s.beta = rz / s.rz_prev;
+  s.rr = rr;
+  s.rr_ref = rr;
+  s.replace = 0;
+  s.nrepl += 1;
+  s.sel ^= 1;
+  if (!(rr == rr) || !isfinite(rr)) s.status = ST_NAN;
+  else if (rr <= s.tol2) s.status = ST_CONV;
+  else if (s.it >= s.maxit) s.status = ST_MAXIT;
+}
+
+// solution of a finished system: out = x + xs (xs is zero after a final
+// replacement)
+__global__ void bk_final(const BSys *__restrict__ D, const BScal *S, int sys,
+                         double *__restrict__ out) {
+  const BSys &d = D[sys];
+  const int sel = S[sys].sel;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.npad) out[i] = d.x[sel][i] + (double)d.xs[sel][i];
+}
+
+// --------------------------------------------------------------- synthesis
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {  // SplitMix64 finaliser
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double unif(uint64_t seed, uint64_t k) {  // [0, 1)
+  return (double)(mix64(seed ^ mix64(k)) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// coordinates (internal order): base + U[-a, a]^2 on non-boundary nodes
+__global__ void syn_coords(int n, const double *__restrict__ bx, const double *__restrict__ by,
+                           const int *__restrict__ row_bd, const int *__restrict__ iperm,
+                           uint64_t seed, double amp, double *__restrict__ x,
+                           double *__restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double dx = 0, dy = 0;
+  if (row_bd[r] < 0) {
+    const uint64_t u = (uint64_t)iperm[r];  // jitter keyed by the user node id
+    dx = amp * (2.0 * unif(seed, 2 * u) - 1.0);
+    dy = amp * (2.0 * unif(seed, 2 * u + 1) - 1.0);
+  }
+  x[r] = bx[r] + dx;
+  y[r] = by[r] + dy;
+}
+
+// per Gauss point: kappa = exp(GRF) and f ~ U[-1, 1]
+__global__ void syn_gauss(int nelem, QuadRule rule, const int *__restrict__ conn,
+                          const double *__restrict__ x, const double *__restrict__ y,
+                          SynField F, uint64_t fseed, double *__restrict__ kappa,
+                          double *__restrict__ f) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  double ex[3], ey[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int v = conn[(size_t)k * nelem + e];
+    ex[k] = x[v];
+    ey[k] = y[v];
+  }
+  for (int k = 0; k < rule.nq; k++) {
+    const double h0 = 1 - rule.xi[k] - rule.eta[k], h1 = rule.xi[k], h2 = rule.eta[k];
+    const double gx = ex[0] * h0 + ex[1] * h1 + ex[2] * h2;
+    const double gy = ey[0] * h0 + ey[1] * h1 + ey[2] * h2;
+    double g = 0.0;
+    for (int j = 0; j < F.nmodes; j++) g += cos(F.wx[j] * gx + F.wy[j] * gy + F.phi[j]);
+    const size_t gi = (size_t)rule.nq * e + k;
+    kappa[gi] = exp(F.scale * g);
+    f[gi] = 2.0 * unif(fseed, gi) - 1.0;
+  }
+}
+
+// heat initial state (internal order): a GRF at the nodes
+__global__ void syn_nodal(int n, const double *__restrict__ x, const double *__restrict__ y,
+                          SynField F, double *__restrict__ u) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double g = 0.0;
+  for (int j = 0; j < F.nmodes; j++) g += cos(F.wx[j] * x[r] + F.wy[j] * y[r] + F.phi[j]);
+  u[r] = F.scale * g;
+}
+
+// user-order interleaved coordinates -> internal-order x, y
+__global__ void perm_coords(int n, const int *__restrict__ iperm, const double *__restrict__ c,
+                            double *__restrict__ x, double *__restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int u = iperm[r];
+  x[r] = c[2 * (size_t)u];
+  y[r] = c[2 * (size_t)u + 1];
+}
+
+inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+void launch_perm_coords(int n, const int *iperm, const double *coords_user, double *x,
+                        double *y, hipStream_t st) {
+  perm_coords<<<nb(n, 256), 256, 0, st>>>(n, iperm, coords_user, x, y);
+  HIPCHECK(hipGetLastError());
+}
+
+void batch_init(const BSys *D, BScal *S, int sys, int nblk, double rtol2, double delta2,
+                int maxit, hipStream_t st) {
+  bk_init<<<nblk, NT, 0, st>>>(D, sys);
+  bs_init<<<1, NT, 0, st>>>(D, S, sys, rtol2, delta2, maxit);
+  HIPCHECK(hipGetLastError());
+}
+
+void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, const BSys *D,
+                      BScal *S, int iters, hipStream_t st) {
+  for (int k = 0; k < iters; k++) {
+    bk_spmv<<<nmap, NT, 0, st>>>(map, D, S);
+    bs_alpha<<<nlist, NT, 0, st>>>(list, D, S);
+    bk_update<<<nmap, NT, 0, st>>>(map, D, S);
+    bs_beta<<<nlist, NT, 0, st>>>(list, D, S);
+    bk_replace<<<nmap, NT, 0, st>>>(map, D, S);
+    bs_replace<<<nlist, NT, 0, st>>>(list, D, S);
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void batch_final(const BSys *D, const BScal *S, int sys, int npad, double *out,
+                 hipStream_t st) {
+  bk_final<<<nb(npad, 256), 256, 0, st>>>(D, S, sys, out);
+  HIPCHECK(hipGetLastError());
+}
+
+void synth_inputs(const System &s, const double *base_x, const double *base_y, uint64_t seed,
+                  double amp, const SynField &kf, const SynField *uf, double *x, double *y,
+                  double *kappa, double *f, double *u_prev_int, hipStream_t st) {
+  const Mesh &m = *s.mesh;
+  syn_coords<<<nb(s.n, 256), 256, 0, st>>>(s.n, base_x, base_y, s.d_row_bd.p, m.d_iperm.p,
+                                            seed, amp, x, y);
+  syn_gauss<<<nb(m.nelem, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p, x, y, kf,
+                                               seed ^ 0x5A5A5A5A5A5A5A5Aull, kappa, f);
+  if (uf && u_prev_int) syn_nodal<<<nb(s.n, 256), 256, 0, st>>>(s.n, x, y, *uf, u_prev_int);
+  HIPCHECK(hipGetLastError());
+}
+
+}  // namespace admfem

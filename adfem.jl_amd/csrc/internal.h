@@ -157,10 +157,22 @@ void launch_source_grad(const Mesh &m, const double *grhs_user, double *gf,
                         hipStream_t st);
 // assemble.hip
 void launch_assemble(const System &s, const admfem_problem &p, hipStream_t st);
+void launch_assemble_to(const System &s, const double *x, const double *y,
+                        const admfem_problem &p, double *val, double *rhs,
+                        hipStream_t st);
 void launch_to_sell(System &s, hipStream_t st);
+void launch_to_sell_to(const System &s, const double *val, double *sv, float *sv32,
+                       double *diag, double *dinv, float *diag32, float *dinv32,
+                       hipStream_t st);
 void launch_kappa_gradient(const System &s, const double *u_int,
                            const double *lam_int, double scale, double *gk,
                            hipStream_t st);
+void launch_kappa_gradient_to(const System &s, const double *x, const double *y,
+                              const double *u_int, const double *lam_int, double scale,
+                              double *gk, hipStream_t st);
+void launch_loss(const System &s, const double *u_int, const int *obs_idx,
+                 const double *obs_val, int nobs, double *g_int, double *loss2,
+                 hipStream_t st);
 // permutation helpers (internal <-> user order)
 void launch_gather(const int *idx, const double *src, double *dst, int n,
                    hipStream_t st);  // dst[i] = src[idx[i]]
@@ -182,6 +194,15 @@ void cg_solve(System &s, const double *b_int, double *x_int,
               hipStream_t st);
 void cg_work_free(CGWork *w);
 void spmv_repeat(System &s, int precision, int repeat, hipStream_t st);
+// server.cpp
+struct Server;
+Server *server_create(const admfem_server_opts &o);
+void server_free(Server *sv);
+int server_add_topology(Server *sv, System *s, double jitter);
+int64_t server_submit(Server *sv, const admfem_request &r);
+void server_run(Server *sv, double timeout);
+void server_result(Server *sv, int64_t id, admfem_result &out);
+void server_stats(Server *sv, double *stats);
 // loss gradient: g_int = 0; g[perm[idx]] = 2 (u - obs); returns loss on host
 double loss_and_grad(const System &s, const double *u_int, const int *obs_idx,
                      const double *obs_val, int nobs, double *g_int,

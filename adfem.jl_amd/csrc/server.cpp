@@ -1,0 +1,649 @@
+// Serving engine: continuous batching of forward+adjoint requests on one GPU.
+//
+// A request runs the reference's pipeline for its own coefficients
+// (assemble -> Dirichlet -> u = K^-1 F -> L -> lambda = K^-1 dL/du ->
+// dL/dkappa).  The engine admits arrived requests while their device memory
+// fits the HBM budget (an arena carved once at creation, so admission and
+// release never call hipMalloc/hipFree), advances every admitted system with
+// one batched launch per CG step (batched.hip), and moves systems that
+// converge to their next phase at the chunk boundaries, where finished
+// requests release their memory to the waiting queue.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <random>
+#include <thread>
+#include <unordered_set>
+
+#include "batched.h"
+
+namespace admfem {
+
+namespace {
+
+constexpr size_t ALIGN = 256;
+inline size_t up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
+
+// Best-fit allocator over one device allocation, coalescing on free.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0, peak = 0;
+  std::map<size_t, size_t> free_;  // offset -> size
+  void init(size_t bytes) {
+    cap = up(bytes);
+    HIPCHECK(hipMalloc(&base, cap));
+    free_.clear();
+    free_[0] = cap;
+  }
+  ~Arena() {
+    if (base) (void)hipFree(base);
+  }
+  bool alloc(size_t bytes, size_t &off) {
+    bytes = up(bytes);
+    auto best = free_.end();
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= bytes && (best == free_.end() || it->second < best->second)) best = it;
+    if (best == free_.end()) return false;
+    off = best->first;
+    size_t rest = best->second - bytes;
+    free_.erase(best);
+    if (rest) free_[off + bytes] = rest;
+    used += bytes;
+    peak = std::max(peak, used);
+    return true;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = up(bytes);
+    used -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct Topo {
+  System *sys;
+  double jitter;
+  int nblk;                  // batched blocks of 256 rows
+  size_t bytes;              // device bytes of one request on it
+  std::vector<int> interior; // user ids of interior nodes
+};
+
+enum { Q_WAIT = 0, Q_FWD = 1, Q_ADJ = 2, Q_DONE = 3 };
+
+struct Req {
+  admfem_request r;
+  admfem_result res{};
+  int state = Q_WAIT;
+  int slot = -1;
+  size_t off = 0;
+  // device buffers (inside the arena)
+  double *x, *y, *kappa, *f, *uprev, *val, *b, *sv64, *diag64, *dinv64, *xa, *xb, *u64, *lam64,
+      *grad, *obsv, *loss2, *part, *stage;
+  float *sv32, *diag32, *dinv32, *xsa, *xsb, *r32, *z32, *pa, *pb, *q32;
+  int *obsi;
+  int nobs = 0;
+};
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+// small kernels of the engine (batched.hip)
+void launch_perm_coords(int n, const int *iperm, const double *coords_user, double *x,
+                        double *y, hipStream_t st);
+
+struct Server {
+  admfem_server_opts o{};
+  hipStream_t st = nullptr;
+  Arena arena;
+  std::vector<Topo> topos;
+  std::vector<Req> reqs;
+  int max_slots = 0;
+  std::vector<int> free_slots;
+  DevBuf<BSys> d_desc;
+  DevBuf<BScal> d_scal;
+  BScal *h_scal = nullptr;       // pinned mirror
+  int2 *h_map = nullptr;         // pinned staging for the block map
+  int *h_list = nullptr;
+  size_t map_cap = 0, list_cap = 0;
+  DevBuf<int2> d_map;
+  DevBuf<int> d_list;
+  bool started = false;
+  std::chrono::steady_clock::time_point t0;
+  // stats
+  int64_t steps = 0, iters = 0, batch_sum = 0;
+  int peak_active = 0;
+  double wall = 0;
+
+  ~Server() {
+    if (h_scal) (void)hipHostFree(h_scal);
+    if (h_map) (void)hipHostFree(h_map);
+    if (h_list) (void)hipHostFree(h_list);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  // ------------------------------------------------------------ layout
+  static size_t layout(const System &s, Req *q, char *base, int nobs) {
+    const Mesh &m = *s.mesh;
+    const size_t np = s.npad, ng = m.ngauss, nz = s.col.size(),
+                 ns = std::max<size_t>(1, s.slice_ptr.back());
+    const size_t nblk = (np + BATCH_NT - 1) / BATCH_NT;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      size_t o = off;
+      off += up(bytes);
+      return base ? (void *)(base + o) : nullptr;
+    };
+    auto D = [&](size_t n) { return (double *)take(n * sizeof(double)); };
+    auto F = [&](size_t n) { return (float *)take(n * sizeof(float)); };
+    double *x = D(np), *y = D(np), *kappa = D(ng), *f = D(ng), *uprev = D(np), *val = D(nz),
+           *b = D(np), *sv64 = D(ns), *diag64 = D(np), *dinv64 = D(np), *xa = D(np),
+           *xb = D(np), *u64 = D(np), *lam64 = D(np), *grad = D(ng), *obsv = D(nobs + 1),
+           *loss2 = D(2), *part = D(3 * nblk), *stage = D(2 * np);
+    float *sv32 = F(ns), *diag32 = F(np), *dinv32 = F(np), *xsa = F(np), *xsb = F(np),
+          *r32 = F(np), *z32 = F(np), *pa = F(np), *pb = F(np), *q32 = F(np);
+    int *obsi = (int *)take((nobs + 1) * sizeof(int));
+    if (q) {
+      q->x = x; q->y = y; q->kappa = kappa; q->f = f; q->uprev = uprev; q->val = val;
+      q->b = b; q->sv64 = sv64; q->diag64 = diag64; q->dinv64 = dinv64; q->xa = xa;
+      q->xb = xb; q->u64 = u64; q->lam64 = lam64; q->grad = grad; q->obsv = obsv;
+      q->loss2 = loss2; q->part = part; q->stage = stage; q->sv32 = sv32; q->diag32 = diag32;
+      q->dinv32 = dinv32; q->xsa = xsa; q->xsb = xsb; q->r32 = r32; q->z32 = z32;
+      q->pa = pa; q->pb = pb; q->q32 = q32; q->obsi = obsi;
+    }
+    return off;
+  }
+
+  int nobs_of(const Req &q) const {
+    if (q.r.source == ADMFEM_SRC_SYNTH)
+      return std::max<int>(1, (int)topos[q.r.topo].interior.size() / 100);
+    return q.r.nobs;
+  }
+
+  // ------------------------------------------------------------ phases
+  void write_desc(Req &q, const double *b) {
+    const System &s = *topos[q.r.topo].sys;
+    BSys d{};
+    d.sp = s.d_slice_ptr.p;
+    d.sc = s.d_sell_col.p;
+    d.row_bd = s.d_row_bd.p;
+    d.sv64 = q.sv64;
+    d.diag64 = q.diag64;
+    d.sv32 = q.sv32;
+    d.diag32 = q.diag32;
+    d.dinv32 = q.dinv32;
+    d.b = b;
+    d.x[0] = q.xa;
+    d.x[1] = q.xb;
+    d.xs[0] = q.xsa;
+    d.xs[1] = q.xsb;
+    d.r = q.r32;
+    d.z = q.z32;
+    d.p[0] = q.pa;
+    d.p[1] = q.pb;
+    d.q = q.q32;
+    d.part = q.part;
+    d.n = s.n;
+    d.npad = s.npad;
+    d.nblk = topos[q.r.topo].nblk;
+    HIPCHECK(hipMemcpyAsync(d_desc.p + q.slot, &d, sizeof(BSys), hipMemcpyHostToDevice, st));
+  }
+
+  void inputs(Req &q) {
+    const Topo &T = topos[q.r.topo];
+    const System &s = *T.sys;
+    const Mesh &m = *s.mesh;
+    const admfem_request &r = q.r;
+    const bool heat = r.kind == 1;
+    if (r.source == ADMFEM_SRC_SYNTH) {
+      std::mt19937_64 rng(r.seed);
+      std::normal_distribution<double> nrm(0.0, 1.0 / 0.1);
+      std::uniform_real_distribution<double> ph(0.0, 2.0 * M_PI);
+      SynField kf{}, uf{};
+      kf.nmodes = uf.nmodes = 32;
+      kf.scale = uf.scale = std::sqrt(2.0 / 32.0);
+      for (int j = 0; j < 32; j++) {
+        kf.wx[j] = nrm(rng);
+        kf.wy[j] = nrm(rng);
+        kf.phi[j] = ph(rng);
+      }
+      for (int j = 0; j < 32; j++) {
+        uf.wx[j] = nrm(rng);
+        uf.wy[j] = nrm(rng);
+        uf.phi[j] = ph(rng);
+      }
+      synth_inputs(s, m.x.p, m.y.p, r.seed, T.jitter, kf, heat ? &uf : nullptr, q.x, q.y,
+                   q.kappa, q.f, q.uprev, st);
+      // observations: 1% of the interior nodes, obs_val = 0
+      std::vector<int> idx;
+      std::unordered_set<int> seen;
+      std::uniform_int_distribution<size_t> pick(0, T.interior.size() - 1);
+      while ((int)idx.size() < q.nobs) {
+        int u = T.interior[pick(rng)];
+        if (seen.insert(u).second) idx.push_back(u);
+      }
+      std::vector<double> zero(q.nobs, 0.0);
+      HIPCHECK(hipMemcpyAsync(q.obsi, idx.data(), sizeof(int) * q.nobs, hipMemcpyHostToDevice, st));
+      HIPCHECK(hipMemcpyAsync(q.obsv, zero.data(), sizeof(double) * q.nobs, hipMemcpyHostToDevice,
+                              st));
+      HIPCHECK(hipStreamSynchronize(st));  // host staging vectors go out of scope
+      return;
+    }
+    const hipMemcpyKind kind =
+        r.source == ADMFEM_SRC_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    REQUIRE(r.kappa && r.f && (!heat || r.u_prev) && (r.nobs == 0 || (r.obs_idx && r.obs_val)),
+            ADMFEM_ERR_ARG, "request inputs missing");
+    // user-order inputs are staged, then permuted to the internal order
+    if (r.coords) {
+      HIPCHECK(hipMemcpyAsync(q.stage, r.coords, sizeof(double) * 2 * s.n, kind, st));
+      launch_perm_coords(s.n, m.d_iperm.p, q.stage, q.x, q.y, st);
+    } else {
+      HIPCHECK(hipMemcpyAsync(q.x, m.x.p, sizeof(double) * s.n, hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipMemcpyAsync(q.y, m.y.p, sizeof(double) * s.n, hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHECK(hipMemcpyAsync(q.kappa, r.kappa, sizeof(double) * m.ngauss, kind, st));
+    HIPCHECK(hipMemcpyAsync(q.f, r.f, sizeof(double) * m.ngauss, kind, st));
+    if (heat) {
+      HIPCHECK(hipMemcpyAsync(q.stage, r.u_prev, sizeof(double) * s.n, kind, st));
+      launch_gather(m.d_iperm.p, q.stage, q.uprev, s.n, st);
+    }
+    if (r.nobs) {
+      HIPCHECK(hipMemcpyAsync(q.obsi, r.obs_idx, sizeof(int) * r.nobs, kind, st));
+      HIPCHECK(hipMemcpyAsync(q.obsv, r.obs_val, sizeof(double) * r.nobs, kind, st));
+    }
+    if (r.source == ADMFEM_SRC_HOST) HIPCHECK(hipStreamSynchronize(st));
+  }
+
+  admfem_problem problem(const Req &q) const {
+    admfem_problem p{};
+    const bool heat = q.r.kind == 1;
+    p.kappa = q.kappa;
+    p.f = q.f;
+    p.u_prev = heat ? q.uprev : nullptr;
+    p.lap_coef = heat ? q.r.dt : 1.0;
+    p.mass_coef = heat ? 1.0 : 0.0;
+    p.src_coef = heat ? q.r.dt : 1.0;
+    return p;
+  }
+
+  void assemble(Req &q) {
+    const System &s = *topos[q.r.topo].sys;
+    inputs(q);
+    admfem_problem p = problem(q);
+    launch_assemble_to(s, q.x, q.y, p, q.val, q.b, st);
+    launch_to_sell_to(s, q.val, q.sv64, q.sv32, q.diag64, q.dinv64, q.diag32, q.dinv32, st);
+  }
+
+  void start_cg(Req &q) {
+    const double d = o.replace_delta > 0 ? o.replace_delta : 0.1;
+    batch_init(d_desc.p, d_scal.p, q.slot, topos[q.r.topo].nblk, o.rtol * o.rtol, d * d,
+               o.max_iter, st);
+  }
+
+  bool admit(Req &q, double t) {
+    const Topo &T = topos[q.r.topo];
+    if (free_slots.empty()) return false;
+    q.nobs = nobs_of(q);
+    size_t bytes = layout(*T.sys, nullptr, nullptr, q.nobs);
+    if (bytes > arena.cap) {
+      q.state = Q_DONE;
+      q.res.done = 1;
+      q.res.status = ADMFEM_ERR_BUDGET;
+      q.res.t_done = t;
+      return true;
+    }
+    size_t off;
+    if (!arena.alloc(bytes, off)) return false;
+    q.off = off;
+    q.res.device_bytes = (double)bytes;
+    layout(*T.sys, &q, arena.base + off, q.nobs);
+    q.slot = free_slots.back();
+    free_slots.pop_back();
+    q.res.t_admit = t;
+    assemble(q);
+    write_desc(q, q.b);
+    start_cg(q);
+    q.state = Q_FWD;
+    return true;
+  }
+
+  void release(Req &q) {
+    arena.release(q.off, layout(*topos[q.r.topo].sys, nullptr, nullptr, q.nobs));
+    free_slots.push_back(q.slot);
+    q.slot = -1;
+  }
+
+  // forward solve finished: u, loss, dL/du; start the adjoint solve
+  void to_adjoint(Req &q, const BScal &sc) {
+    const System &s = *topos[q.r.topo].sys;
+    q.res.iters_fwd = sc.it;
+    batch_final(d_desc.p, d_scal.p, q.slot, s.npad, q.u64, st);
+    launch_loss(s, q.u64, q.obsi, q.obsv, q.nobs, q.b, q.loss2, st);  // g overwrites b
+    start_cg(q);
+    q.state = Q_ADJ;
+  }
+
+  void finish(Req &q, const BScal &sc, double t) {
+    const System &s = *topos[q.r.topo].sys;
+    const Mesh &m = *s.mesh;
+    q.res.iters_adj = sc.it;
+    batch_final(d_desc.p, d_scal.p, q.slot, s.npad, q.lam64, st);
+    admfem_problem p = problem(q);
+    launch_kappa_gradient_to(s, q.x, q.y, q.u64, q.lam64, p.lap_coef, q.grad, st);
+    if (q.r.u_out) {
+      launch_gather(m.d_perm.p, q.u64, q.stage, s.n, st);  // user order
+      HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
+                              st));
+    }
+    if (q.r.grad_out)
+      HIPCHECK(hipMemcpyAsync(q.r.grad_out, q.grad, sizeof(double) * m.ngauss,
+                              hipMemcpyDeviceToHost, st));
+    double l2[2];
+    HIPCHECK(hipMemcpyAsync(l2, q.loss2, sizeof(l2), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    q.res.loss = l2[0];
+    if (l2[1] != 0.0 && q.res.status == ADMFEM_OK) q.res.status = ADMFEM_ERR_ARG;
+    q.res.t_done = now_s(t0);
+    (void)t;
+    q.res.done = 1;
+    q.state = Q_DONE;
+    release(q);
+  }
+
+  void fail(Req &q, int code) {
+    HIPCHECK(hipStreamSynchronize(st));
+    q.res.status = code;
+    q.res.done = 1;
+    q.res.t_done = now_s(t0);
+    q.state = Q_DONE;
+    release(q);
+  }
+
+  static int code_of(int status) {
+    switch (status) {
+      case 1: return ADMFEM_OK;
+      case 3: return ADMFEM_ERR_NAN;
+      case 4: return ADMFEM_ERR_BREAKDOWN;
+      default: return ADMFEM_ERR_NOT_CONVERGED;
+    }
+  }
+
+  // ------------------------------------------------------------ batched run
+  void ensure_map(size_t nmap, size_t nlist) {
+    if (nmap > map_cap) {
+      if (h_map) HIPCHECK(hipHostFree(h_map));
+      map_cap = std::max(nmap, 2 * map_cap);
+      HIPCHECK(hipHostMalloc((void **)&h_map, sizeof(int2) * map_cap, hipHostMallocDefault));
+      d_map.alloc(map_cap);
+    }
+    if (nlist > list_cap) {
+      if (h_list) HIPCHECK(hipHostFree(h_list));
+      list_cap = std::max(nlist, 2 * list_cap);
+      HIPCHECK(hipHostMalloc((void **)&h_list, sizeof(int) * list_cap, hipHostMallocDefault));
+      d_list.alloc(list_cap);
+    }
+  }
+
+  void run_batched(double timeout) {
+    const int chunk = o.chunk > 0 ? o.chunk : 16;
+    std::vector<int> order(reqs.size());
+    for (size_t i = 0; i < reqs.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return reqs[a].r.arrival < reqs[b].r.arrival;
+    });
+    size_t next = 0;  // first request (in arrival order) not yet admitted
+    while (next < order.size() && reqs[order[next]].state != Q_WAIT) next++;
+    std::vector<int> active;
+    for (size_t i = 0; i < reqs.size(); i++)
+      if (reqs[i].state == Q_FWD || reqs[i].state == Q_ADJ) active.push_back((int)i);
+    for (;;) {
+      double t = now_s(t0);
+      if (t > timeout) break;
+      // admit arrived requests, FIFO, while memory and slots allow
+      while (next < order.size() && (o.max_active <= 0 || (int)active.size() < o.max_active)) {
+        Req &q = reqs[order[next]];
+        if (q.r.arrival > t) break;
+        if (!admit(q, t)) break;
+        if (q.state != Q_DONE) active.push_back(order[next]);
+        next++;
+      }
+      peak_active = std::max(peak_active, (int)active.size());
+      if (active.empty()) {
+        if (next >= order.size()) break;
+        double wait = reqs[order[next]].r.arrival - now_s(t0);
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+        continue;
+      }
+      // block map over the active systems
+      size_t nmap = 0;
+      for (int i : active) nmap += topos[reqs[i].r.topo].nblk;
+      ensure_map(nmap, active.size());
+      size_t k = 0;
+      for (size_t a = 0; a < active.size(); a++) {
+        const Req &q = reqs[active[a]];
+        for (int b = 0; b < topos[q.r.topo].nblk; b++) h_map[k++] = make_int2(q.slot, b);
+        h_list[a] = q.slot;
+      }
+      HIPCHECK(hipMemcpyAsync(d_map.p, h_map, sizeof(int2) * nmap, hipMemcpyHostToDevice, st));
+      HIPCHECK(hipMemcpyAsync(d_list.p, h_list, sizeof(int) * active.size(),
+                              hipMemcpyHostToDevice, st));
+      batch_iterations(d_map.p, (int)nmap, d_list.p, (int)active.size(), d_desc.p, d_scal.p,
+                       chunk, st);
+      steps += 1;
+      iters += chunk;
+      batch_sum += (int64_t)active.size();
+      HIPCHECK(hipMemcpyAsync(h_scal, d_scal.p, sizeof(BScal) * max_slots,
+                              hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      // phase changes
+      std::vector<int> still;
+      for (int i : active) {
+        Req &q = reqs[i];
+        const BScal sc = h_scal[q.slot];
+        if (sc.status == 0) {
+          still.push_back(i);
+          continue;
+        }
+        if (sc.status != 1) {
+          if (q.state == Q_FWD) q.res.iters_fwd = sc.it;
+          else q.res.iters_adj = sc.it;
+          fail(q, code_of(sc.status));
+          continue;
+        }
+        if (q.state == Q_FWD) {
+          q.res.relres_fwd = sc.bb > 0 ? std::sqrt(sc.rr / sc.bb) : 0.0;
+          to_adjoint(q, sc);
+          still.push_back(i);
+        } else {
+          q.res.relres_adj = sc.bb > 0 ? std::sqrt(sc.rr / sc.bb) : 0.0;
+          finish(q, sc, now_s(t0));
+        }
+      }
+      active.swap(still);
+    }
+    wall = now_s(t0);
+  }
+
+  // ------------------------------------------------------------ sequential
+  void run_seq(double timeout) {
+    std::vector<int> order(reqs.size());
+    for (size_t i = 0; i < reqs.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return reqs[a].r.arrival < reqs[b].r.arrival;
+    });
+    for (int i : order) {
+      Req &q = reqs[i];
+      if (q.state != Q_WAIT) continue;
+      double t = now_s(t0);
+      if (t > timeout) break;
+      if (q.r.arrival > t)
+        std::this_thread::sleep_for(std::chrono::duration<double>(q.r.arrival - t));
+      t = now_s(t0);
+      System &s = *topos[q.r.topo].sys;
+      q.nobs = nobs_of(q);
+      size_t bytes = layout(s, nullptr, nullptr, q.nobs);
+      size_t off;
+      if (bytes > arena.cap || !arena.alloc(bytes, off)) {
+        q.res.status = ADMFEM_ERR_BUDGET;
+        q.res.done = 1;
+        q.state = Q_DONE;
+        continue;
+      }
+      q.off = off;
+      q.res.device_bytes = (double)bytes;
+      layout(s, &q, arena.base + off, q.nobs);
+      q.slot = 0;
+      q.res.t_admit = t;
+      peak_active = std::max(peak_active, 1);
+      // FP64 single-system path on the topology's own solver buffers
+      inputs(q);
+      admfem_problem p = problem(q);
+      launch_assemble_to(s, q.x, q.y, p, s.val.p, s.rhs.p, st);
+      launch_to_sell(s, st);
+      admfem_solve_opts so{ADMFEM_PREC_FP64, o.max_iter, o.rtol, 0, 0.0};
+      admfem_solve_stats s0{}, s1{};
+      cg_solve(s, s.rhs.p, q.u64, so, s0, st);
+      launch_loss(s, q.u64, q.obsi, q.obsv, q.nobs, q.b, q.loss2, st);
+      cg_solve(s, q.b, q.lam64, so, s1, st);
+      q.res.iters_fwd = s0.iterations;
+      q.res.iters_adj = s1.iterations;
+      q.res.relres_fwd = s0.rel_residual;
+      q.res.relres_adj = s1.rel_residual;
+      iters += s0.iterations + s1.iterations;
+      steps += 1;
+      batch_sum += 1;
+      q.res.status = s0.status != ADMFEM_OK ? s0.status : s1.status;
+      if (q.res.status != ADMFEM_OK) {
+        fail_seq(q);
+        continue;
+      }
+      const Mesh &m = *s.mesh;
+      launch_kappa_gradient_to(s, q.x, q.y, q.u64, q.lam64, p.lap_coef, q.grad, st);
+      if (q.r.u_out) {
+        launch_gather(m.d_perm.p, q.u64, q.stage, s.n, st);
+        HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
+                                st));
+      }
+      if (q.r.grad_out)
+        HIPCHECK(hipMemcpyAsync(q.r.grad_out, q.grad, sizeof(double) * m.ngauss,
+                                hipMemcpyDeviceToHost, st));
+      double l2[2];
+      HIPCHECK(hipMemcpyAsync(l2, q.loss2, sizeof(l2), hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      q.res.loss = l2[0];
+      q.res.t_done = now_s(t0);
+      q.res.done = 1;
+      q.state = Q_DONE;
+      arena.release(q.off, bytes);
+    }
+    wall = now_s(t0);
+  }
+
+  void fail_seq(Req &q) {
+    HIPCHECK(hipStreamSynchronize(st));
+    q.res.done = 1;
+    q.res.t_done = now_s(t0);
+    q.state = Q_DONE;
+    arena.release(q.off, layout(*topos[q.r.topo].sys, nullptr, nullptr, q.nobs));
+  }
+};
+
+// ---------------------------------------------------------------- C-ABI glue
+Server *server_create(const admfem_server_opts &o) {
+  REQUIRE(o.hbm_budget_bytes > 0, ADMFEM_ERR_ARG, "budget must be positive");
+  REQUIRE(o.rtol > 0 && o.max_iter > 0, ADMFEM_ERR_ARG, "bad rtol / max_iter");
+  REQUIRE(o.mode == ADMFEM_MODE_BATCHED || o.mode == ADMFEM_MODE_SEQ_FP64, ADMFEM_ERR_ARG,
+          "bad mode");
+  Server *sv = new Server();
+  try {
+    sv->o = o;
+    HIPCHECK(hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking));
+    sv->arena.init((size_t)o.hbm_budget_bytes);
+    sv->max_slots = o.max_active > 0 ? o.max_active : 1024;
+    for (int i = sv->max_slots - 1; i >= 0; i--) sv->free_slots.push_back(i);
+    sv->d_desc.alloc(sv->max_slots);
+    sv->d_scal.alloc(sv->max_slots);
+    HIPCHECK(hipMemset(sv->d_scal.p, 0, sv->d_scal.bytes()));
+    HIPCHECK(hipHostMalloc((void **)&sv->h_scal, sizeof(BScal) * sv->max_slots,
+                           hipHostMallocDefault));
+  } catch (...) {
+    delete sv;
+    throw;
+  }
+  return sv;
+}
+
+void server_free(Server *sv) { delete sv; }
+
+int server_add_topology(Server *sv, System *s, double jitter) {
+  Topo T;
+  T.sys = s;
+  T.jitter = jitter;
+  T.nblk = (s->npad + BATCH_NT - 1) / BATCH_NT;
+  T.bytes = Server::layout(*s, nullptr, nullptr, std::max(1, s->n / 100));
+  for (int u = 0; u < s->n; u++)
+    if (s->row_bd[s->mesh->perm[u]] < 0) T.interior.push_back(u);
+  REQUIRE(!T.interior.empty(), ADMFEM_ERR_ARG, "topology has no interior node");
+  sv->topos.push_back(std::move(T));
+  return (int)sv->topos.size() - 1;
+}
+
+int64_t server_submit(Server *sv, const admfem_request &r) {
+  REQUIRE(r.topo >= 0 && r.topo < (int)sv->topos.size(), ADMFEM_ERR_ARG, "bad topology id");
+  REQUIRE(r.kind == 0 || r.kind == 1, ADMFEM_ERR_ARG, "bad request kind");
+  REQUIRE(r.source >= 0 && r.source <= 2, ADMFEM_ERR_ARG, "bad request source");
+  REQUIRE(r.kind == 0 || r.dt > 0, ADMFEM_ERR_ARG, "heat step needs dt > 0");
+  REQUIRE(r.nobs >= 0, ADMFEM_ERR_ARG, "bad nobs");
+  Req q;
+  q.r = r;
+  q.res.t_arrival = r.arrival;
+  sv->reqs.push_back(q);
+  return (int64_t)sv->reqs.size() - 1;
+}
+
+void server_run(Server *sv, double timeout) {
+  if (!sv->started) {
+    sv->started = true;
+    sv->t0 = std::chrono::steady_clock::now();
+  }
+  if (sv->o.mode == ADMFEM_MODE_BATCHED) sv->run_batched(timeout);
+  else sv->run_seq(timeout);
+}
+
+void server_result(Server *sv, int64_t id, admfem_result &out) {
+  REQUIRE(id >= 0 && id < (int64_t)sv->reqs.size(), ADMFEM_ERR_ARG, "bad request id");
+  out = sv->reqs[id].res;
+}
+
+void server_stats(Server *sv, double *s) {
+  int done = 0, failed = 0;
+  for (auto &q : sv->reqs)
+    if (q.res.done) {
+      done++;
+      if (q.res.status != ADMFEM_OK) failed++;
+    }
+  s[0] = done;
+  s[1] = sv->wall;
+  s[2] = (double)sv->arena.peak;
+  s[3] = sv->peak_active;
+  s[4] = (double)sv->steps;
+  s[5] = (double)sv->iters;
+  s[6] = failed;
+  s[7] = sv->steps ? (double)sv->batch_sum / sv->steps : 0.0;
+}
+
+}  // namespace admfem

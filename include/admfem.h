@@ -269,9 +269,90 @@ int admfem_system_spmv_repeat(admfem_system s, int precision, int repeat,
 
 /* ======================================================================
  * E. Serving: continuous batching of forward+adjoint requests under an HBM
- *    budget (no reference counterpart: the reference runs one solve at a
- *    time on the host).  See admfem_server_* in admfem_server.h.
+ *    budget.  No reference counterpart (the reference runs one solve at a
+ *    time on the host); each request is the reference's forward + gradient
+ *    pipeline (compute_fem_* -> impose_Dirichlet -> K\F -> gradients) for
+ *    L = sum_k (u[obs_k] - obs_val_k)^2.
+ *
+ *    Requests are admitted when they have arrived and their device memory
+ *    fits the budget; admitted systems advance together, one batched launch
+ *    per CG step for all of them (mixed precision: FP32 inner iterations,
+ *    FP64 residual replacement); a system that converges moves on (forward
+ *    -> adjoint -> gradient -> done) and frees its memory for the next
+ *    arrivals.  Results do not depend on what else is in the batch.
  * ==================================================================== */
+
+typedef struct admfem_server_s *admfem_server;
+
+#define ADMFEM_MODE_BATCHED 0     /* continuous batching, mixed precision */
+#define ADMFEM_MODE_SEQ_FP64 1    /* one request at a time, FP64 solver */
+
+typedef struct {
+  double hbm_budget_bytes; /* device memory for admitted requests */
+  int max_active;          /* cap on concurrently admitted requests */
+  int mode;                /* ADMFEM_MODE_* */
+  double rtol;             /* true relative residual of every solve */
+  int max_iter;
+  double replace_delta;    /* mixed: residual replacement factor (0: 0.1) */
+  int chunk;               /* CG iterations between host checks (0: 16) */
+} admfem_server_opts;
+
+#define ADMFEM_SRC_HOST 0    /* inputs are host pointers */
+#define ADMFEM_SRC_DEVICE 1  /* inputs are device pointers */
+#define ADMFEM_SRC_SYNTH 2   /* inputs generated on the device from `seed` */
+
+typedef struct {
+  int topo;                /* id from admfem_server_add_topology */
+  int kind;                /* 0: Poisson (K u = F); 1: heat step
+                              ((M + dt K) u = M u_prev + dt F) */
+  double dt;               /* heat step */
+  double arrival;          /* seconds after admfem_server_run starts */
+  int source;              /* ADMFEM_SRC_* */
+  /* HOST / DEVICE inputs (user node order; must stay valid until the
+   * request is admitted): coords nnode x 2 (NULL: the topology's own),
+   * kappa/f [ngauss], u_prev [nnode] (heat), obs_idx (0-based) / obs_val */
+  const double *coords, *kappa, *f, *u_prev;
+  const int *obs_idx;
+  const double *obs_val;
+  int nobs;
+  /* SYNTH: interior nodes jittered by U[-a, a]^2 (a from the topology),
+   * kappa = exp(GRF) (32 random Fourier modes, sigma 1, length 0.1) and
+   * f ~ U[-1, 1] at the Gauss points, u_prev a GRF (heat), 1% of the
+   * interior nodes observed with obs_val = 0; all from `seed`. */
+  unsigned long long seed;
+  /* host outputs (may be NULL), valid once the request is done */
+  double *u_out;           /* [nnode] */
+  double *grad_out;        /* [ngauss] dL/dkappa */
+} admfem_request;
+
+typedef struct {
+  int status;              /* ADMFEM_OK or the failing solve's code;
+                              ADMFEM_ERR_BUDGET if it can never fit */
+  int done;
+  double t_arrival, t_admit, t_done; /* seconds after run start */
+  double loss;
+  int iters_fwd, iters_adj;
+  double relres_fwd, relres_adj;     /* true FP64 relative residuals */
+  double device_bytes;
+} admfem_result;
+
+int admfem_server_create(const admfem_server_opts *o, admfem_server *out);
+int admfem_server_destroy(admfem_server sv);
+/* Registers a mesh + boundary set (a system made by admfem_system_create; it
+ * must outlive the server).  jitter: absolute amplitude of the SYNTH node
+ * jitter.  Returns the topology id. */
+int admfem_server_add_topology(admfem_server sv, admfem_system s, double jitter,
+                               int *topo_id);
+/* Queues a request; *req_id receives its id (0, 1, ...). */
+int admfem_server_submit(admfem_server sv, const admfem_request *r, int64_t *req_id);
+/* Serves every queued request (arrival times are measured from the first
+ * call), returns when all are done or after timeout_s seconds. */
+int admfem_server_run(admfem_server sv, double timeout_s);
+int admfem_server_result(admfem_server sv, int64_t req_id, admfem_result *out);
+/* stats[0..7] = requests done, wall seconds, peak device bytes, peak
+ * concurrent requests, batched CG steps, total CG iterations, failed,
+ * mean batch size */
+int admfem_server_stats(admfem_server sv, double stats[8]);
 
 #ifdef __cplusplus
 }

@@ -41,8 +41,8 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_python_binding_covers_header():
-    from admfem import _lib
-    bound = set(_lib.exported_symbols())
+    from admfem import _lib, server
+    bound = set(_lib.exported_symbols()) | set(server.exported_symbols())
     missing = [n for n in declared_functions() if n not in bound]
     assert not missing, missing
 
