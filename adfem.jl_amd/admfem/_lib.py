@@ -49,6 +49,11 @@ _SIGS = {
     "admfem_last_error": (ctypes.c_char_p, []),
     "admfem_version": (c_int, []),
     "admfem_system_spmv_repeat": (c_int, [c_vp, c_int, c_int, c_vp, c_vp]),
+    "admfem_sparse_solve": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp,
+                                    ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats), c_vp]),
+    "admfem_sparse_solve_grad": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp,
+                                         ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats),
+                                         c_vp]),
     # A. reference-named
     "init_nnfem_mesh": (c_ll_p, [c_vp, c_int, c_vp, c_int, c_int, c_int, c_int, c_vp]),
     "mfem_get_ngauss": (c_int, []),

@@ -437,6 +437,32 @@ int admfem_system_forward_adjoint(admfem_system h, const admfem_problem *p,
   });
 }
 
+int admfem_sparse_solve(const long long *indices, const double *vv, int64_t nnz, int n,
+                        const double *rhs, double *u, const admfem_solve_opts *o,
+                        admfem_solve_stats *st, void *stream) {
+  return guarded([&] {
+    REQUIRE(rhs && u && o && (nnz == 0 || (indices && vv)), ADMFEM_ERR_ARG, "null pointer");
+    admfem_solve_stats tmp{};
+    sparse_solve(indices, vv, nnz, n, rhs, u, *o, tmp, S(stream));
+    if (st) *st = tmp;
+    if (tmp.status != ADMFEM_OK) throw Error{tmp.status, "sparse solve did not converge"};
+  });
+}
+
+int admfem_sparse_solve_grad(const long long *indices, const double *vv, int64_t nnz, int n,
+                             const double *u, const double *grad_u, double *grad_vv,
+                             double *grad_rhs, const admfem_solve_opts *o,
+                             admfem_solve_stats *st, void *stream) {
+  return guarded([&] {
+    REQUIRE(u && grad_u && grad_rhs && o && (nnz == 0 || (indices && vv && grad_vv)),
+            ADMFEM_ERR_ARG, "null pointer");
+    admfem_solve_stats tmp{};
+    sparse_solve_grad(indices, vv, nnz, n, u, grad_u, grad_vv, grad_rhs, *o, tmp, S(stream));
+    if (st) *st = tmp;
+    if (tmp.status != ADMFEM_OK) throw Error{tmp.status, "adjoint solve did not converge"};
+  });
+}
+
 int admfem_system_spmv_repeat(admfem_system h, int precision, int repeat, double *bytes,
                               void *stream) {
   return guarded([&] {

@@ -194,6 +194,14 @@ void cg_solve(System &s, const double *b_int, double *x_int,
               hipStream_t st);
 void cg_work_free(CGWork *w);
 void spmv_repeat(System &s, int precision, int repeat, hipStream_t st);
+// sparse_op.hip
+void sparse_solve(const long long *ind, const double *vv, int64_t nnz, int n, const double *rhs,
+                  double *u, const admfem_solve_opts &o, admfem_solve_stats &stats,
+                  hipStream_t st);
+void sparse_solve_grad(const long long *ind, const double *vv, int64_t nnz, int n,
+                       const double *u, const double *grad_u, double *grad_vv,
+                       double *grad_rhs, const admfem_solve_opts &o, admfem_solve_stats &stats,
+                       hipStream_t st);
 // server.cpp
 struct Server;
 Server *server_create(const admfem_server_opts &o);

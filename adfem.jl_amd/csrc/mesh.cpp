@@ -322,6 +322,58 @@ size_t System::device_bytes() const {
          d_dinv.bytes() + d_diag32.bytes() + d_dinv32.bytes();
 }
 
+// SELL-64 layout of the off-diagonal entries of the system's CSR pattern
+// (slices of 64 rows, column-major inside a slice, width = the slice's
+// widest row; padding entries point at the row itself with value 0) and the
+// device buffers of the assembled values and of the solver.
+static void build_solver_layout(System *s) {
+  const int n = s->n;
+  s->nslices = s->npad / 64;
+  s->slice_ptr.assign(s->nslices + 1, 0);
+  auto offdiag = [&](int r) {
+    int c = 0;
+    for (int k = s->rp[r]; k < s->rp[r + 1]; k++) c += s->col[k] != r;
+    return c;
+  };
+  for (int sl = 0; sl < s->nslices; sl++) {
+    int wmax = 0;
+    for (int l = 0; l < 64; l++) {
+      int r = sl * 64 + l;
+      if (r >= n) break;
+      wmax = std::max(wmax, offdiag(r));
+    }
+    s->slice_ptr[sl + 1] = s->slice_ptr[sl] + 64 * wmax;
+  }
+  std::vector<int> sell_col(std::max(1, s->slice_ptr[s->nslices]));
+  for (int sl = 0; sl < s->nslices; sl++) {
+    int off = s->slice_ptr[sl];
+    int wmax = (s->slice_ptr[sl + 1] - off) / 64;
+    for (int l = 0; l < 64; l++) {
+      int r = sl * 64 + l;
+      int kk = 0;
+      if (r < n)
+        for (int k = s->rp[r]; k < s->rp[r + 1]; k++)
+          if (s->col[k] != r) sell_col[off + kk++ * 64 + l] = s->col[k];
+      for (; kk < wmax; kk++) sell_col[off + kk * 64 + l] = r < n ? r : 0;
+    }
+  }
+  s->d_rp.upload(s->rp);
+  s->d_col.upload(s->col);
+  s->val.alloc(std::max<size_t>(1, s->col.size()));
+  s->rhs.alloc(s->npad);
+  HIPCHECK(hipMemset(s->rhs.p, 0, s->rhs.bytes()));
+  s->d_slice_ptr.upload(s->slice_ptr);
+  s->d_sell_col.upload(sell_col);
+  s->d_sell_val.alloc(sell_col.size());
+  s->d_sell_val32.alloc(sell_col.size());
+  HIPCHECK(hipMemset(s->d_sell_val.p, 0, s->d_sell_val.bytes()));
+  HIPCHECK(hipMemset(s->d_sell_val32.p, 0, s->d_sell_val32.bytes()));
+  s->d_diag.alloc(s->npad);
+  s->d_dinv.alloc(s->npad);
+  s->d_diag32.alloc(s->npad);
+  s->d_dinv32.alloc(s->npad);
+}
+
 System *system_build(Mesh *m, const long long *bdnode, int nbd) {
   REQUIRE(m, ADMFEM_ERR_ARG, "null mesh");
   REQUIRE(nbd >= 0 && (nbd == 0 || bdnode), ADMFEM_ERR_ARG, "bad bdnode");
@@ -375,50 +427,32 @@ System *system_build(Mesh *m, const long long *bdnode, int nbd) {
         slots[t] = w;
       }
     }
-    // SELL-64 layout of the off-diagonal entries
-    s->nslices = s->npad / 64;
-    s->slice_ptr.assign(s->nslices + 1, 0);
-    for (int sl = 0; sl < s->nslices; sl++) {
-      int wmax = 0;
-      for (int l = 0; l < 64; l++) {
-        int r = sl * 64 + l;
-        if (r >= n) break;
-        wmax = std::max(wmax, s->rp[r + 1] - s->rp[r] - 1);
-      }
-      s->slice_ptr[sl + 1] = s->slice_ptr[sl] + 64 * wmax;
-    }
-    std::vector<int> sell_col(std::max(1, s->slice_ptr[s->nslices]));
-    for (int sl = 0; sl < s->nslices; sl++) {
-      int off = s->slice_ptr[sl];
-      int wmax = (s->slice_ptr[sl + 1] - off) / 64;
-      for (int l = 0; l < 64; l++) {
-        int r = sl * 64 + l;
-        int kk = 0;
-        if (r < n)
-          for (int k = s->rp[r]; k < s->rp[r + 1]; k++)
-            if (s->col[k] != r) sell_col[off + kk++ * 64 + l] = s->col[k];
-        for (; kk < wmax; kk++) sell_col[off + kk * 64 + l] = r < n ? r : 0;
-      }
-    }
-    s->d_rp.upload(s->rp);
-    s->d_col.upload(s->col);
+    build_solver_layout(s);
     s->d_row_bd.upload(s->row_bd);
     s->d_slots.upload(slots);
-    s->val.alloc(s->col.size());
-    s->rhs.alloc(s->npad);
-    HIPCHECK(hipMemset(s->rhs.p, 0, s->rhs.bytes()));
-    s->d_slice_ptr.upload(s->slice_ptr);
-    s->d_sell_col.upload(sell_col);
-    s->d_sell_val.alloc(sell_col.size());
-    s->d_sell_val32.alloc(sell_col.size());
-    HIPCHECK(hipMemset(s->d_sell_val.p, 0, s->d_sell_val.bytes()));
-    HIPCHECK(hipMemset(s->d_sell_val32.p, 0, s->d_sell_val32.bytes()));
-    s->d_diag.alloc(s->npad);
-    s->d_dinv.alloc(s->npad);
-    s->d_diag32.alloc(s->npad);
-    s->d_dinv32.alloc(s->npad);
     for (int r0 = 0; r0 < n; r0 += 256)
       s->max_block_nnz = std::max(s->max_block_nnz, s->rp[std::min(r0 + 256, n)] - s->rp[r0]);
+  } catch (...) {
+    system_free(s);
+    throw;
+  }
+  return s;
+}
+
+// A system with no mesh: the CSR pattern of a sparse matrix given by the
+// caller (the general K\F op), identity node order, no boundary rows.
+System *system_from_csr(int n, std::vector<int> rp, std::vector<int> col) {
+  REQUIRE(n > 0 && (int)rp.size() == n + 1, ADMFEM_ERR_ARG, "bad CSR pattern");
+  System *s = new System();
+  try {
+    s->mesh = nullptr;
+    s->n = n;
+    s->npad = (n + 63) / 64 * 64;
+    s->row_bd.assign(n, -1);
+    s->rp = std::move(rp);
+    s->col = std::move(col);
+    build_solver_layout(s);
+    s->d_row_bd.upload(s->row_bd);
   } catch (...) {
     system_free(s);
     throw;

@@ -110,6 +110,68 @@ def spmv_roofline(S, precision, repeat=200):
                 avg_us=round(avg_s * 1e6, 2), algorithmic_bytes=int(nbytes.value))
 
 
+def serving(nreq, fp64_reqs, seed=7):
+    """Serving workload: forward+adjoint requests, each Poisson or an implicit
+    heat step (M + dt K, dt ~ logU[1e-4, 1e-1]) with p = 1/2, on a jittered
+    unit-square mesh whose side is log-uniform in [32, 512] quads (rounded to
+    a multiple of 16), kappa = exp(GRF), f ~ U[-1, 1], FP32 inner solves with
+    FP64 residual replacement to a 1e-8 true relative residual, 64 GB HBM
+    admission budget.  Three runs: capacity (all requests queued at t = 0),
+    load (Poisson arrivals at 80% of the measured capacity: p50 / p99
+    latency), and the plain FP64 path serving the first requests one at a
+    time (its solves/s)."""
+    import admfem
+    from admfem import meshgen
+    from admfem.server import Server
+    rng = np.random.default_rng(seed)
+    sides = (np.round(np.exp(rng.uniform(np.log(32), np.log(512), nreq)) / 16) * 16)
+    sides = np.clip(sides, 32, 512).astype(int)
+    heat = rng.random(nreq) < 0.5
+    dts = np.exp(rng.uniform(np.log(1e-4), np.log(1e-1), nreq))
+    t0 = time.perf_counter()
+    systems = {}
+    for s in sorted(set(sides.tolist())):
+        c, e = meshgen.jittered_square(s, seed=0, jitter=0.0, permute=False)
+        mesh = admfem.Mesh(c, e, zero_based=True)
+        systems[s] = admfem.DirichletSystem(mesh, admfem.bcnode(mesh))
+    setup_s = time.perf_counter() - t0
+
+    def run(mode, n, arrivals):
+        sv = Server(64e9, mode=mode, rtol=1e-8)
+        tid = {s: sv.add_topology(S, 0.2 / s) for s, S in systems.items()}
+        ids = [sv.submit(tid[int(sides[i])], kind="heat" if heat[i] else "poisson",
+                         dt=float(dts[i]), seed=1000 + i, arrival=float(arrivals[i]),
+                         want_u=False, want_grad=False) for i in range(n)]
+        t = time.perf_counter()
+        sv.run()
+        wall = time.perf_counter() - t
+        rs = [sv.result(i) for i in ids]
+        bad = sum(r["status"] != 0 for r in rs)
+        lat = np.array([r["latency"] for r in rs])
+        return dict(requests=n, failed=bad, wall_s=round(wall, 3),
+                    solves_per_s=round(n / max(wall, 1e-9), 2),
+                    p50_ms=round(1e3 * float(np.percentile(lat, 50)), 2),
+                    p99_ms=round(1e3 * float(np.percentile(lat, 99)), 2),
+                    peak_active=int(sv.stats()["peak_active"]),
+                    peak_gb=round(sv.stats()["peak_bytes"] / 1e9, 3),
+                    mean_batch=round(sv.stats()["mean_batch"], 1))
+
+    cap = run("batched", nreq, np.zeros(nreq))
+    rate = 0.8 * cap["solves_per_s"]
+    arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, nreq))
+    load = run("batched", nreq, arr)
+    load["arrival_rate_per_s"] = round(rate, 2)
+    nf = min(nreq, fp64_reqs)
+    fp64 = run("seq_fp64", nf, np.zeros(nf))
+    cap_sub = run("batched", nf, np.zeros(nf))
+    return dict(workload=f"{nreq} forward+adjoint requests, Poisson or heat (p=1/2), side "
+                         f"logU[32,512] (x16), jittered square, kappa=exp(GRF), rtol 1e-8, "
+                         f"64 GB budget", topologies=len(systems), setup_s=round(setup_s, 1),
+                capacity=cap, load=load, fp64_plain_subset=fp64, batched_subset=cap_sub,
+                speedup_vs_fp64_plain=round(cap_sub["solves_per_s"] /
+                                            max(fp64["solves_per_s"], 1e-9), 2))
+
+
 def cpu_baseline(host, m):
     """One forward+adjoint of the same workload with the CPU oracle (serial C
     restatement of the reference loops + SciPy SuperLU), pinned to one core."""
@@ -147,6 +209,10 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=512,
+                    help="serving workload size (0: skip; 4096 for the full stream)")
+    ap.add_argument("--serving-fp64-requests", type=int, default=48,
+                    help="requests served by the plain FP64 path for the comparison")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}): this build is "
@@ -213,6 +279,9 @@ def main():
         "speedup_vs_fp64": round(value / value64, 3),
         "roofline": roof,
     }
+    if a.serving_requests > 0:
+        print(f"[bench] serving {a.serving_requests} requests", file=sys.stderr, flush=True)
+        line["serving"] = serving(a.serving_requests, a.serving_fp64_requests)
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb, cres = cpu_baseline(host, m)

@@ -259,6 +259,24 @@ int admfem_system_forward_adjoint(admfem_system s, const admfem_problem *p,
                                   double *loss, admfem_solve_stats st[2],
                                   void *stream);
 
+/* ADCME's SparseSolver (the `\` of a SparseTensor, call sites README.md:54,75;
+ * ADCME is not vendored in the reference): u = A \ rhs for A given as COO
+ * triplets indices [nnz x 2] (0-based int64) / vv [nnz], duplicates summed
+ * in triplet order; rhs, u [n].  A must be symmetric positive definite
+ * (Jacobi-PCG; ADMFEM_ERR_BREAKDOWN otherwise).  Device pointers;
+ * synchronises the stream. */
+int admfem_sparse_solve(const long long *indices, const double *vv, int64_t nnz,
+                        int n, const double *rhs, double *u,
+                        const admfem_solve_opts *o, admfem_solve_stats *st,
+                        void *stream);
+/* SparseSolverGrad: lambda = A^-T grad_u (= A^-1 grad_u, A symmetric);
+ * grad_vv[k] = -lambda[i_k] * u[j_k]; grad_rhs = lambda. */
+int admfem_sparse_solve_grad(const long long *indices, const double *vv,
+                             int64_t nnz, int n, const double *u,
+                             const double *grad_u, double *grad_vv,
+                             double *grad_rhs, const admfem_solve_opts *o,
+                             admfem_solve_stats *st, void *stream);
+
 /* Instrumentation (no reference counterpart): launches the solver's SpMV
  * kernel (FP64 or FP32 matrix per `precision`) `repeat` times back to back on
  * `stream`, on the system's work vectors, for timing the dominant kernel.

@@ -1,0 +1,118 @@
+"""The reference's operator chain through torch autograd on the GPU:
+compute_fem_laplace_matrix1 -> compute_fem_source_term1 ->
+impose_Dirichlet_boundary_conditions -> A \\ F -> loss.backward(), against the
+oracle's composition of the reference's forward and Grad ops."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+import torch
+
+import admfem
+from admfem import meshgen
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(m, seed):
+    c, e = meshgen.jittered_square(m, seed=seed)
+    gm = admfem.Mesh(c, e, zero_based=True)
+    om = O.OracleMesh(c, e)
+    rng = np.random.default_rng(seed)
+    kap = np.exp(meshgen.grf(np.stack([om.gx, om.gy], 1), seed + 1))
+    f = rng.uniform(-1, 1, om.ngauss)
+    bd = om.bcnode()
+    interior = np.setdiff1d(np.arange(om.nnode), bd - 1)
+    obs = rng.choice(interior, size=max(3, interior.size // 10), replace=False)
+    uobs = rng.standard_normal(obs.size) * 1e-2
+    return gm, om, kap, f, bd, obs, uobs
+
+
+def dloss(obs, uobs):
+    def g(u):
+        out = np.zeros_like(u)
+        out[obs] = 2 * (u[obs] - uobs)
+        return out
+    return g
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def test_poisson_chain_autograd_matches_oracle():
+    gm, om, kap, f, bd, obs, uobs = setup(14, 3)
+    k = torch.tensor(kap, device="cuda", requires_grad=True)
+    fk = torch.tensor(f, device="cuda")
+    K = admfem.compute_fem_laplace_matrix1(k, gm)
+    F = admfem.compute_fem_source_term1(fk, gm)
+    Kb, Fb = admfem.impose_Dirichlet_boundary_conditions(K, F, bd, np.zeros(bd.size))
+    u = admfem.solve(Kb, Fb, rtol=1e-13)
+    o = torch.as_tensor(obs, device="cuda")
+    loss = ((u[o] - torch.as_tensor(uobs, device="cuda")) ** 2).sum()
+    loss.backward()
+    ref = O.forward_adjoint(om, kap, f, bd, np.zeros(bd.size), dloss(obs, uobs))
+    assert rel(u.detach().cpu().numpy(), ref["u"]) < 1e-10
+    assert rel(k.grad.cpu().numpy(), ref["grad_kappa"]) < 1e-9
+
+
+def test_heat_chain_autograd_matches_oracle():
+    gm, om, kap, f, bd, obs, uobs = setup(11, 5)
+    dt = 0.02
+    up = np.sin(np.pi * om.coords[:, 0]) * np.sin(np.pi * om.coords[:, 1])
+    k = torch.tensor(kap, device="cuda", requires_grad=True)
+    M = admfem.compute_fem_mass_matrix1(gm)
+    K = admfem.compute_fem_laplace_matrix1(k, gm)
+    A = M + K * dt
+    rhs = M @ torch.as_tensor(up, device="cuda") + dt * admfem.compute_fem_source_term1(
+        torch.as_tensor(f, device="cuda"), gm)
+    Ab, rb = admfem.impose_Dirichlet_boundary_conditions(A, rhs, bd, np.zeros(bd.size))
+    u = admfem.solve(Ab, rb, rtol=1e-13)
+    o = torch.as_tensor(obs, device="cuda")
+    ((u[o] - torch.as_tensor(uobs, device="cuda")) ** 2).sum().backward()
+    ref = O.forward_adjoint(om, kap, f, bd, np.zeros(bd.size), dloss(obs, uobs), dt=dt,
+                            u_prev=up)
+    assert rel(u.detach().cpu().numpy(), ref["u"]) < 1e-10
+    assert rel(k.grad.cpu().numpy(), ref["grad_kappa"]) < 1e-9
+
+
+def test_array_paths_and_unit_coefficient_helpers():
+    gm, om, kap, f, bd, obs, uobs = setup(6, 1)
+    K = admfem.compute_fem_laplace_matrix1(kap, gm)  # numpy -> scipy (MCore.jl:112-120)
+    K0 = O.csr_matrix(*om.laplace_coo(kap), om.nnode)
+    assert abs(K - K0).max() < 1e-13 * abs(K0).max()
+    K1 = admfem.compute_fem_laplace_matrix1(gm)      # kappa = 1 (MCore.jl:137)
+    assert abs(K1.to_scipy() - O.csr_matrix(*om.laplace_coo(np.ones(om.ngauss)),
+                                            om.nnode)).max() < 1e-13
+    F = admfem.compute_fem_source_term1(f, gm)
+    np.testing.assert_allclose(F, om.source(f), rtol=1e-14, atol=1e-17)
+    B = admfem.impose_Dirichlet_boundary_conditions(K1, bd)  # homogeneous helper
+    oi, ov, _ = O.dirichlet(*om.laplace_coo(np.ones(om.ngauss)), bd, np.zeros(om.nnode),
+                            np.zeros(bd.size))
+    np.testing.assert_array_equal(B.indices.cpu().numpy(), oi)
+    np.testing.assert_array_equal(B.values.detach().cpu().numpy(), ov)
+
+
+def test_sparse_solve_op_and_gradient_general_spd():
+    """SparseSolver / SparseSolverGrad on a random SPD matrix with duplicate
+    triplets, against SciPy and the closed-form gradient."""
+    rng = np.random.default_rng(0)
+    n = 60
+    R = sp.random(n, n, density=0.08, random_state=1)
+    A = (R @ R.T + sp.eye(n) * 2.0).tocoo()
+    I = np.concatenate([A.row, A.row[:20]])
+    J = np.concatenate([A.col, A.col[:20]])
+    V = np.concatenate([A.data, np.zeros(20)])  # explicit duplicate zeros
+    b = rng.standard_normal(n)
+    ind = torch.as_tensor(np.stack([I, J], 1), device="cuda")
+    vv = torch.tensor(V, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    u = admfem.solve(admfem.SparseTensor(ind, vv, n), bt, rtol=1e-14)
+    w = torch.as_tensor(rng.standard_normal(n), device="cuda")
+    (u * w).sum().backward()
+    u0 = spl.spsolve(A.tocsc(), b)
+    lam = spl.spsolve(A.T.tocsc(), w.cpu().numpy())
+    assert rel(u.detach().cpu().numpy(), u0) < 1e-10
+    assert rel(bt.grad.cpu().numpy(), lam) < 1e-10
+    assert rel(vv.grad.cpu().numpy(), -lam[I] * u0[J]) < 1e-10
