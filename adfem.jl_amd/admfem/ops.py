@@ -33,8 +33,10 @@ class SparseTensor:
     differentiable ``values`` [nnz] float64, both on the GPU."""
 
     def __init__(self, indices, values, n):
-        self.indices = indices
-        self.values = values
+        # the reference converts indices to Int64 and values to Float64
+        # (convert_to_tensor in MUtils.jl:210); .to() keeps the autograd graph
+        self.indices = torch.as_tensor(indices).to(device=_DEV, dtype=torch.int64).contiguous()
+        self.values = torch.as_tensor(values).to(device=_DEV, dtype=torch.float64)
         self.n = int(n)
 
     @property
@@ -190,7 +192,19 @@ class _Dirichlet(torch.autograd.Function):
 def impose_Dirichlet_boundary_conditions(A, rhs, bdnode=None, bdval=None):
     """ImposeDirichlet (deps/MFEM/ImposeDirichlet/ImposeDirichlet.h:27-93).
     ``bdnode`` 1-based as returned by bcnode.  ``(A, bdnode)`` is the
-    homogeneous helper (MUtils.jl:220-225) returning the matrix only."""
+    homogeneous helper (MUtils.jl:220-225) returning the matrix only.
+    A SciPy sparse ``A`` with host arrays takes the Array path
+    (MUtils.jl:184-198): the same kernels, returning ``(B, r)`` as SciPy CSC
+    and NumPy."""
+    if sp.issparse(A):
+        if bdnode is None or bdval is None:
+            raise ValueError("the SparseMatrixCSC form takes (A, rhs, bdnode, bdval)")
+        C = A.tocoo()
+        ind = torch.as_tensor(np.stack([C.row, C.col], 1).astype(np.int64)).to(_DEV)
+        T = SparseTensor(ind, torch.as_tensor(C.data.astype(np.float64)).to(_DEV), A.shape[0])
+        B, r = impose_Dirichlet_boundary_conditions(T, np.asarray(rhs, dtype=np.float64),
+                                                    bdnode, bdval)
+        return B.to_scipy().tocsc(), r.detach().cpu().numpy()
     homogeneous = bdval is None and bdnode is None
     if homogeneous:  # impose_Dirichlet_boundary_conditions(A, bdnode)
         bdnode, rhs = rhs, torch.zeros(A.n, dtype=torch.float64, device=_DEV)

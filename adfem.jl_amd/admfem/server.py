@@ -24,7 +24,7 @@ SRC_HOST, SRC_DEVICE, SRC_SYNTH = 0, 1, 2
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", c_double), ("max_active", c_int), ("mode", c_int),
                 ("rtol", c_double), ("max_iter", c_int), ("replace_delta", c_double),
-                ("chunk", c_int)]
+                ("chunk", c_int), ("use_graph", c_int)]
 
 
 class Request(ctypes.Structure):
@@ -83,10 +83,10 @@ class Server:
     """
 
     def __init__(self, budget_bytes, max_active=1024, mode="batched", rtol=1e-8,
-                 max_iter=200000, replace_delta=0.0, chunk=0):
+                 max_iter=200000, replace_delta=0.0, chunk=0, use_graph=True):
         m = {"batched": MODE_BATCHED, "seq_fp64": MODE_SEQ_FP64}[mode]
         o = ServerOpts(float(budget_bytes), int(max_active), m, float(rtol), int(max_iter),
-                       float(replace_delta), int(chunk))
+                       float(replace_delta), int(chunk), int(bool(use_graph)))
         h = c_vp()
         _lib.check(_L().admfem_server_create(ctypes.byref(o), ctypes.byref(h)))
         self._h = h

@@ -7,12 +7,15 @@
 // (FemLaplaceScalar.h:3-26, ComputeFemMassMatrixMfem.h:4-29,
 // FemSourceScalar.h:4-15, ImposeDirichlet.h:27-60), summed into CSR.
 //
-// One thread per matrix row, 256 rows per block.  Each row walks its incident
-// elements (precomputed node->element list, ascending element id) and adds
-// the element-matrix row into the row's slots, which live in LDS for the
-// whole block; the block then writes its contiguous CSR range coalesced.
-// Every slot has exactly one writer, so there are no atomics and the result
-// is deterministic.
+// A block owns AS_ROWS consecutive rows and works in two passes over LDS:
+//  1. one thread per (row, incident element) pair computes that element's
+//     matrix row for the row node (stiffness and mass parts), its load-vector
+//     term and its Dirichlet lifting -- independent work items, so the
+//     element gathers of many pairs are in flight at once;
+//  2. one thread per row adds its pairs, in ascending element order, into the
+//     row's CSR slots and its right-hand side.
+// The block's contiguous CSR range is then written out coalesced.  Every slot
+// has one writer and a fixed summation order: no atomics, deterministic.
 #include "geom.h"
 
 namespace admfem {
@@ -30,73 +33,103 @@ __global__ void __launch_bounds__(AS_NT)
                     const uint32_t *__restrict__ slots,
                     const int *__restrict__ rp, const int *__restrict__ row_bd,
                     admfem_problem P, double *__restrict__ val,
-                    double *__restrict__ rhs) {
+                    double *__restrict__ rhs, int max_inc, int max_nnz) {
   extern __shared__ double lds[];
-  const int r0 = blockIdx.x * AS_NT;
-  const int r = r0 + threadIdx.x;
-  const int rend = min(r0 + AS_NT, n);
+  double *ca = lds;                        // [max_inc][3] matrix-row terms
+  double *rc = ca + 3 * (size_t)max_inc;   // [max_inc] right-hand-side terms
+  double *vl = rc + max_inc;               // [max_nnz] the block's CSR values
+  uint32_t *ws = (uint32_t *)(vl + max_nnz);  // [max_inc] slot words
+  int *ip = (int *)(ws + max_inc);         // [AS_ROWS + 1] incidence offsets
+  const int r0 = blockIdx.x * AS_ROWS;
+  const int rend = min(r0 + AS_ROWS, n);
+  const int nr = rend - r0;
+  const int i0 = n2e_ptr[r0];
+  const int ninc = n2e_ptr[rend] - i0;
   const int base = rp[r0];
   const int bnnz = rp[rend] - base;
-  for (int k = threadIdx.x; k < bnnz; k += AS_NT) lds[k] = 0.0;
+  for (int k = threadIdx.x; k <= nr; k += AS_NT) ip[k] = n2e_ptr[r0 + k] - i0;
+  for (int k = threadIdx.x; k < bnnz; k += AS_NT) vl[k] = 0.0;
   __syncthreads();
-  if (r < n) {
-    const int bi = row_bd[r];
-    const int off = rp[r] - base;
-    if (bi >= 0) {
-      lds[off] = 1.0;  // the appended (b,b,1.0) of ImposeDirichlet.h:46-50
-      rhs[r] = P.bdval ? P.bdval[bi] : 0.0;
-    } else {
-      double src = 0.0, mu = 0.0, lift = 0.0;
-      for (int t = n2e_ptr[r]; t < n2e_ptr[r + 1]; t++) {
-        const int e = n2e_elem[t];
-        const uint32_t wd = slots[t];
-        const int p = wd & 3u;
-        int v[3];
-        double ex[3], ey[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          v[k] = conn[(size_t)k * nelem + e];
-          ex[k] = x[v[k]];
-          ey[k] = y[v[k]];
-        }
-        ElemGeom G = elem_geom(ex[0], ey[0], ex[1], ey[1], ex[2], ey[2]);
-        // K_e = (sum_k w_k kappa_k) * grad phi_p . grad phi_q  (P1: constant
-        // gradients, so the Gauss sum factors out)
-        double ck = 0.0;
-        double mrow[3] = {0.0, 0.0, 0.0};
-        for (int k = 0; k < rule.nq; k++) {
-          const size_t g = (size_t)rule.nq * e + k;
-          const double w = gauss_weight(rule.w[k], G.area);
-          ck += w * (P.kappa ? P.kappa[g] : 1.0);
-          const double hp = shape(p, rule.xi[k], rule.eta[k]);
-          if (P.mass_coef != 0.0 || P.u_prev) {
-            const double wr = w * (P.rho ? P.rho[g] : 1.0) * hp;
-#pragma unroll
-            for (int q = 0; q < 3; q++) mrow[q] += wr * shape(q, rule.xi[k], rule.eta[k]);
-          }
-          if (P.f) src += P.f[g] * hp * w;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-          double a = P.lap_coef * ck * (G.gx[p] * G.gx[q] + G.gy[p] * G.gy[q]);
-          if (P.mass_coef != 0.0) a += P.mass_coef * mrow[q];
-          if (P.u_prev) mu += mrow[q] * P.u_prev[v[q]];
-          const uint32_t s = slot_of(wd, q);
-          if (s == SLOT_LIFT) {
-            if (P.bdval) lift += a * P.bdval[row_bd[v[q]]];
-          } else {
-            lds[off + s] += a;
-          }
-        }
-      }
-      rhs[r] = P.src_coef * src + mu - lift;
+  // pass 1: (row, element) pairs
+  for (int t = threadIdx.x; t < ninc; t += AS_NT) {
+    int lo = 0, hi = nr - 1;  // row of pair t: last k with ip[k] <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (ip[mid] <= t) lo = mid;
+      else hi = mid - 1;
     }
+    const int r = r0 + lo;
+    const uint32_t wd = slots[i0 + t];
+    ws[t] = wd;
+    if (row_bd[r] >= 0) continue;  // boundary row: only its unit diagonal
+    const int e = n2e_elem[i0 + t];
+    const int p = wd & 3u;
+    int v[3];
+    double ex[3], ey[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      v[k] = conn[(size_t)k * nelem + e];
+      ex[k] = x[v[k]];
+      ey[k] = y[v[k]];
+    }
+    const ElemGeom G = elem_geom(ex[0], ey[0], ex[1], ey[1], ex[2], ey[2]);
+    // K_e = (sum_k w_k kappa_k) grad phi_p . grad phi_q (P1 gradients are
+    // constant, so the Gauss sum factors out)
+    double ck = 0.0, src = 0.0;
+    double mrow[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < rule.nq; k++) {
+      const size_t g = (size_t)rule.nq * e + k;
+      const double w = gauss_weight(rule.w[k], G.area);
+      ck += w * (P.kappa ? P.kappa[g] : 1.0);
+      const double hp = shape(p, rule.xi[k], rule.eta[k]);
+      if (P.mass_coef != 0.0 || P.u_prev) {
+        const double wr = w * (P.rho ? P.rho[g] : 1.0) * hp;
+#pragma unroll
+        for (int q = 0; q < 3; q++) mrow[q] += wr * shape(q, rule.xi[k], rule.eta[k]);
+      }
+      if (P.f) src += P.f[g] * hp * w;
+    }
+    double rt = P.src_coef * src;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      double a = P.lap_coef * ck * (G.gx[p] * G.gx[q] + G.gy[p] * G.gy[q]);
+      if (P.mass_coef != 0.0) a += P.mass_coef * mrow[q];
+      if (P.u_prev) rt += mrow[q] * P.u_prev[v[q]];
+      if (slot_of(wd, q) == SLOT_LIFT && P.bdval) rt -= a * P.bdval[row_bd[v[q]]];
+      ca[3 * t + q] = a;
+    }
+    rc[t] = rt;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < bnnz; k += AS_NT) val[base + k] = lds[k];
+  // pass 2: rows sum their pairs in ascending element order
+  for (int k = threadIdx.x; k < nr; k += AS_NT) {
+    const int r = r0 + k;
+    const int off = rp[r] - base;
+    const int bi = row_bd[r];
+    if (bi >= 0) {
+      vl[off] = 1.0;  // the appended (b,b,1.0) of ImposeDirichlet.h:46-50
+      rhs[r] = P.bdval ? P.bdval[bi] : 0.0;
+      continue;
+    }
+    double acc = 0.0;
+    for (int t = ip[k]; t < ip[k + 1]; t++) {
+      const uint32_t wd = ws[t];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const uint32_t sl = slot_of(wd, q);
+        if (sl != SLOT_LIFT) vl[off + sl] += ca[3 * t + q];
+      }
+      acc += rc[t];
+    }
+    rhs[r] = acc;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < bnnz; k += AS_NT) val[base + k] = vl[k];
 }
 
 // CSR (post-BC) -> SELL-64 off-diagonals + diagonal, FP64 and FP32 copies.
+// Every entry of the slice is written, the padding ones with 0, so the
+// destination needs no clearing beforehand (serving reuses device memory).
 __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
                                const int *__restrict__ col,
                                const double *__restrict__ val,
@@ -109,25 +142,26 @@ __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
                                float *__restrict__ dinv32, int npad) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npad) return;
-  if (r >= n) {  // padding rows: identity
-    diag[r] = 1.0;
-    dinv[r] = 1.0;
-    diag32[r] = 1.0f;
-    dinv32[r] = 1.0f;
-    return;
-  }
-  const int off = slice_ptr[r >> 6] + (r & 63);
+  const int s0 = slice_ptr[r >> 6], width = (slice_ptr[(r >> 6) + 1] - s0) >> 6;
+  const int off = s0 + (r & 63);
   int kk = 0;
-  double d = 0.0;
-  for (int k = rp[r]; k < rp[r + 1]; k++) {
-    double v = val[k];
-    if (col[k] == r) {
-      d = v;
-    } else {
-      sv[off + 64 * kk] = v;
-      sv32[off + 64 * kk] = (float)v;
-      kk++;
+  double d = 1.0;  // padding rows: identity
+  if (r < n) {
+    d = 0.0;
+    for (int k = rp[r]; k < rp[r + 1]; k++) {
+      double v = val[k];
+      if (col[k] == r) {
+        d = v;
+      } else {
+        sv[off + 64 * kk] = v;
+        sv32[off + 64 * kk] = (float)v;
+        kk++;
+      }
     }
+  }
+  for (; kk < width; kk++) {
+    sv[off + 64 * kk] = 0.0;
+    sv32[off + 64 * kk] = 0.0f;
   }
   diag[r] = d;
   dinv[r] = d != 0.0 ? 1.0 / d : 0.0;
@@ -215,12 +249,14 @@ void launch_assemble_to(const System &s, const double *x, const double *y,
                         const admfem_problem &p, double *val, double *rhs,
                         hipStream_t st) {
   const Mesh &m = *s.mesh;
-  // the largest per-block CSR range decides the dynamic LDS size
-  size_t lds = (size_t)std::max(s.max_block_nnz, 1) * sizeof(double);
+  // the block with the most (row, element) pairs / CSR entries sizes the LDS
+  const int mi = std::max(s.asm_max_inc, 1), mn = std::max(s.asm_max_nnz, 1);
+  const size_t lds = (size_t)mi * (4 * sizeof(double) + sizeof(uint32_t)) +
+                     (size_t)mn * sizeof(double) + (AS_ROWS + 1) * sizeof(int);
   REQUIRE(lds <= 160 * 1024, ADMFEM_ERR_UNSUPPORTED, "row block too dense for LDS");
-  assemble_kernel<<<nblk(s.n, AS_NT), AS_NT, lds, st>>>(
+  assemble_kernel<<<nblk(s.n, AS_ROWS), AS_NT, lds, st>>>(
       s.n, m.nelem, m.rule, m.conn.p, x, y, m.d_n2e_ptr.p, m.d_n2e_elem.p,
-      s.d_slots.p, s.d_rp.p, s.d_row_bd.p, p, val, rhs);
+      s.d_slots.p, s.d_rp.p, s.d_row_bd.p, p, val, rhs, mi, mn);
   HIPCHECK(hipGetLastError());
 }
 

@@ -18,7 +18,7 @@ struct BSys {
   double *x[2];             // FP64 solution, two buffers (swapped on replacement)
   float *xs[2];             // FP32 correction since the last replacement
   float *r, *z, *p[2], *q;
-  double *part;             // 3 * nblk block partials
+  double *part;             // 4 * nblk block partials
   int n, npad, nblk;
 };
 

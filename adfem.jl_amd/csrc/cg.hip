@@ -404,6 +404,7 @@ void spmv_repeat(System &s, int precision, int repeat, hipStream_t st) {
   const int nb = w->nblocks;
   CGScalars init{};
   HIPCHECK(hipMemcpyAsync(w->sc.p, &init, sizeof(CGScalars), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipStreamSynchronize(st));  // init lives on this stack frame
   const unsigned fb = (unsigned)((s.npad + 255) / 256);
   if (precision == ADMFEM_PREC_MIXED) {
     k_fill<float><<<fb, 256, 0, st>>>(w->z32.p, s.npad, 1.0f);

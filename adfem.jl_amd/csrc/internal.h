@@ -68,6 +68,7 @@ __host__ __device__ inline uint32_t slot_of(uint32_t w, int q) {
   return (w >> (2 + SLOT_BITS * q)) & 63u;
 }
 
+constexpr int AS_ROWS = 128;  // rows per assembly block
 constexpr int MAXQ = 3;  // quadrature points per element carried here
 struct QuadRule {
   int nq;
@@ -132,7 +133,9 @@ struct System {
   DevBuf<float> d_sell_val32;  // FP32 copy for the mixed-precision solver
   DevBuf<double> d_diag, d_dinv;
   DevBuf<float> d_diag32, d_dinv32;
-  int max_block_nnz = 0;  // largest CSR range of a 256-row assembly block
+  int max_block_nnz = 0;  // largest CSR range of a 256-row block
+  int asm_max_inc = 0;    // assembly: most (row, element) pairs in a block
+  int asm_max_nnz = 0;    // assembly: largest CSR range of a block
   CGWork *work = nullptr;
   size_t device_bytes() const;
 };

@@ -432,6 +432,11 @@ System *system_build(Mesh *m, const long long *bdnode, int nbd) {
     s->d_slots.upload(slots);
     for (int r0 = 0; r0 < n; r0 += 256)
       s->max_block_nnz = std::max(s->max_block_nnz, s->rp[std::min(r0 + 256, n)] - s->rp[r0]);
+    for (int r0 = 0; r0 < n; r0 += AS_ROWS) {
+      const int r1 = std::min(r0 + AS_ROWS, n);
+      s->asm_max_inc = std::max(s->asm_max_inc, m->n2e_ptr[r1] - m->n2e_ptr[r0]);
+      s->asm_max_nnz = std::max(s->asm_max_nnz, s->rp[r1] - s->rp[r0]);
+    }
   } catch (...) {
     system_free(s);
     throw;

@@ -117,12 +117,14 @@ struct Server {
   DevBuf<BSys> d_desc;
   DevBuf<BScal> d_scal;
   BScal *h_scal = nullptr;       // pinned mirror
+  BSys *h_desc = nullptr;        // pinned staging of the descriptors
   int2 *h_map = nullptr;         // pinned staging for the block map
   int *h_list = nullptr;
   size_t map_cap = 0, list_cap = 0;
   DevBuf<int2> d_map;
   DevBuf<int> d_list;
   bool started = false;
+  bool use_graph = false;
   std::chrono::steady_clock::time_point t0;
   // stats
   int64_t steps = 0, iters = 0, batch_sum = 0;
@@ -130,7 +132,9 @@ struct Server {
   double wall = 0;
 
   ~Server() {
+    drop_graphs();
     if (h_scal) (void)hipHostFree(h_scal);
+    if (h_desc) (void)hipHostFree(h_desc);
     if (h_map) (void)hipHostFree(h_map);
     if (h_list) (void)hipHostFree(h_list);
     if (st) (void)hipStreamDestroy(st);
@@ -153,7 +157,7 @@ struct Server {
     double *x = D(np), *y = D(np), *kappa = D(ng), *f = D(ng), *uprev = D(np), *val = D(nz),
            *b = D(np), *sv64 = D(ns), *diag64 = D(np), *dinv64 = D(np), *xa = D(np),
            *xb = D(np), *u64 = D(np), *lam64 = D(np), *grad = D(ng), *obsv = D(nobs + 1),
-           *loss2 = D(2), *part = D(3 * nblk), *stage = D(2 * np);
+           *loss2 = D(2), *part = D(4 * nblk), *stage = D(2 * np);
     float *sv32 = F(ns), *diag32 = F(np), *dinv32 = F(np), *xsa = F(np), *xsb = F(np),
           *r32 = F(np), *z32 = F(np), *pa = F(np), *pb = F(np), *q32 = F(np);
     int *obsi = (int *)take((nobs + 1) * sizeof(int));
@@ -177,7 +181,10 @@ struct Server {
   // ------------------------------------------------------------ phases
   void write_desc(Req &q, const double *b) {
     const System &s = *topos[q.r.topo].sys;
-    BSys d{};
+    // staged in pinned memory: the copy is asynchronous, and a slot's entry
+    // is rewritten only after the stream has drained its previous request
+    BSys &d = h_desc[q.slot];
+    d = BSys{};
     d.sp = s.d_slice_ptr.p;
     d.sc = s.d_sell_col.p;
     d.row_bd = s.d_row_bd.p;
@@ -389,13 +396,43 @@ struct Server {
       map_cap = std::max(nmap, 2 * map_cap);
       HIPCHECK(hipHostMalloc((void **)&h_map, sizeof(int2) * map_cap, hipHostMallocDefault));
       d_map.alloc(map_cap);
+      drop_graphs();  // captured graphs point at the old buffers
     }
     if (nlist > list_cap) {
       if (h_list) HIPCHECK(hipHostFree(h_list));
       list_cap = std::max(nlist, 2 * list_cap);
       HIPCHECK(hipHostMalloc((void **)&h_list, sizeof(int) * list_cap, hipHostMallocDefault));
       d_list.alloc(list_cap);
+      drop_graphs();
     }
+  }
+
+  // One CG chunk for a (map size, list size) bucket, captured once as a HIP
+  // graph of kernel nodes only and replayed for every batch of that size.
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  void drop_graphs() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
+  static int bucket(size_t v) {
+    size_t b = 64;
+    while (b < v) b <<= 1;
+    return (int)b;
+  }
+  void launch_chunk(int nmap, int nlist, int chunk) {
+    auto key = std::make_pair(nmap, nlist);
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      hipGraph_t g;
+      HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      batch_iterations(d_map.p, nmap, d_list.p, nlist, d_desc.p, d_scal.p, chunk, st);
+      HIPCHECK(hipStreamEndCapture(st, &g));
+      hipGraphExec_t ex;
+      HIPCHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      HIPCHECK(hipGraphDestroy(g));
+      it = graphs.emplace(key, ex).first;
+    }
+    HIPCHECK(hipGraphLaunch(it->second, st));
   }
 
   void run_batched(double timeout) {
@@ -428,21 +465,29 @@ struct Server {
         if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
         continue;
       }
-      // block map over the active systems
+      // block map over the active systems; with graphs, both grids are padded
+      // to power-of-two buckets with entries that do no work (map: system -1,
+      // list: the never-running dummy slot)
       size_t nmap = 0;
       for (int i : active) nmap += topos[reqs[i].r.topo].nblk;
-      ensure_map(nmap, active.size());
+      const size_t gmap = use_graph ? (size_t)bucket(nmap) : nmap;
+      const size_t glist = use_graph ? (size_t)bucket(active.size()) : active.size();
+      ensure_map(gmap, glist);
       size_t k = 0;
       for (size_t a = 0; a < active.size(); a++) {
         const Req &q = reqs[active[a]];
         for (int b = 0; b < topos[q.r.topo].nblk; b++) h_map[k++] = make_int2(q.slot, b);
         h_list[a] = q.slot;
       }
-      HIPCHECK(hipMemcpyAsync(d_map.p, h_map, sizeof(int2) * nmap, hipMemcpyHostToDevice, st));
-      HIPCHECK(hipMemcpyAsync(d_list.p, h_list, sizeof(int) * active.size(),
-                              hipMemcpyHostToDevice, st));
-      batch_iterations(d_map.p, (int)nmap, d_list.p, (int)active.size(), d_desc.p, d_scal.p,
-                       chunk, st);
+      for (; k < gmap; k++) h_map[k] = make_int2(-1, 0);
+      for (size_t a = active.size(); a < glist; a++) h_list[a] = max_slots;  // dummy slot
+      HIPCHECK(hipMemcpyAsync(d_map.p, h_map, sizeof(int2) * gmap, hipMemcpyHostToDevice, st));
+      HIPCHECK(hipMemcpyAsync(d_list.p, h_list, sizeof(int) * glist, hipMemcpyHostToDevice, st));
+      if (use_graph)
+        launch_chunk((int)gmap, (int)glist, chunk);
+      else
+        batch_iterations(d_map.p, (int)nmap, d_list.p, (int)active.size(), d_desc.p, d_scal.p,
+                         chunk, st);
       steps += 1;
       iters += chunk;
       batch_sum += (int64_t)active.size();
@@ -575,10 +620,20 @@ Server *server_create(const admfem_server_opts &o) {
     sv->arena.init((size_t)o.hbm_budget_bytes);
     sv->max_slots = o.max_active > 0 ? o.max_active : 1024;
     for (int i = sv->max_slots - 1; i >= 0; i--) sv->free_slots.push_back(i);
-    sv->d_desc.alloc(sv->max_slots);
-    sv->d_scal.alloc(sv->max_slots);
+    sv->use_graph = o.use_graph != 0;
+    // one extra slot: the dummy the padded graph launches point at; its
+    // status is never "running", so every kernel skips it
+    sv->d_desc.alloc(sv->max_slots + 1);
+    sv->d_scal.alloc(sv->max_slots + 1);
+    HIPCHECK(hipMemset(sv->d_desc.p, 0, sv->d_desc.bytes()));
     HIPCHECK(hipMemset(sv->d_scal.p, 0, sv->d_scal.bytes()));
+    BScal dummy{};
+    dummy.status = 99;
+    HIPCHECK(hipMemcpy(sv->d_scal.p + sv->max_slots, &dummy, sizeof(BScal),
+                       hipMemcpyHostToDevice));
     HIPCHECK(hipHostMalloc((void **)&sv->h_scal, sizeof(BScal) * sv->max_slots,
+                           hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void **)&sv->h_desc, sizeof(BSys) * sv->max_slots,
                            hipHostMallocDefault));
   } catch (...) {
     delete sv;

@@ -313,6 +313,7 @@ typedef struct {
   int max_iter;
   double replace_delta;    /* mixed: residual replacement factor (0: 0.1) */
   int chunk;               /* CG iterations between host checks (0: 16) */
+  int use_graph;           /* replay each chunk as a captured HIP graph */
 } admfem_server_opts;
 
 #define ADMFEM_SRC_HOST 0    /* inputs are host pointers */

@@ -82,14 +82,30 @@ def test_array_paths_and_unit_coefficient_helpers():
     K = admfem.compute_fem_laplace_matrix1(kap, gm)  # numpy -> scipy (MCore.jl:112-120)
     K0 = O.csr_matrix(*om.laplace_coo(kap), om.nnode)
     assert abs(K - K0).max() < 1e-13 * abs(K0).max()
-    K1 = admfem.compute_fem_laplace_matrix1(gm)      # kappa = 1 (MCore.jl:137)
-    assert abs(K1.to_scipy() - O.csr_matrix(*om.laplace_coo(np.ones(om.ngauss)),
-                                            om.nnode)).max() < 1e-13
+    K1 = admfem.compute_fem_laplace_matrix1(gm)      # kappa = 1, Array path (MCore.jl:137)
+    assert abs(K1 - O.csr_matrix(*om.laplace_coo(np.ones(om.ngauss)), om.nnode)).max() < 1e-13
     F = admfem.compute_fem_source_term1(f, gm)
     np.testing.assert_allclose(F, om.source(f), rtol=1e-14, atol=1e-17)
-    B = admfem.impose_Dirichlet_boundary_conditions(K1, bd)  # homogeneous helper
-    oi, ov, _ = O.dirichlet(*om.laplace_coo(np.ones(om.ngauss)), bd, np.zeros(om.nnode),
-                            np.zeros(bd.size))
+    # SparseMatrixCSC form (MUtils.jl:184-198): [A_II 0; 0 I], [r_I - A_IB u_B; u_B]
+    bv = np.linspace(0.5, 1.5, bd.size)
+    Bs, rs = admfem.impose_Dirichlet_boundary_conditions(K, F, bd, bv)
+    keep = np.ones(om.nnode, bool)
+    keep[bd - 1] = False
+    Kd = K.toarray()
+    Bref = np.zeros_like(Kd)
+    Bref[np.ix_(keep, keep)] = Kd[np.ix_(keep, keep)]
+    Bref[bd - 1, bd - 1] = 1.0
+    rref = F.copy()
+    rref[keep] -= Kd[np.ix_(keep, bd - 1)] @ bv
+    rref[bd - 1] = bv
+    assert np.abs(Bs.toarray() - Bref).max() < 1e-14 * np.abs(Kd).max()
+    np.testing.assert_allclose(rs, rref, rtol=1e-13, atol=1e-13 * np.abs(rref).max())
+    K1t = admfem.compute_fem_laplace_matrix1(torch.ones(om.ngauss, dtype=torch.float64), gm)
+    ki, kv = K1t.indices.cpu().numpy(), K1t.values.detach().cpu().numpy()
+    _, kv0 = om.laplace_coo(np.ones(om.ngauss))
+    np.testing.assert_allclose(kv, kv0, rtol=1e-14, atol=1e-14 * np.abs(kv0).max())
+    B = admfem.impose_Dirichlet_boundary_conditions(K1t, bd)  # homogeneous helper
+    oi, ov, _ = O.dirichlet(ki, kv, bd, np.zeros(om.nnode), np.zeros(bd.size))
     np.testing.assert_array_equal(B.indices.cpu().numpy(), oi)
     np.testing.assert_array_equal(B.values.detach().cpu().numpy(), ov)
 
