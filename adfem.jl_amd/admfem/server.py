@@ -49,6 +49,7 @@ _SIGS = {
     "admfem_server_run": (c_int, [c_vp, c_double]),
     "admfem_server_result": (c_int, [c_vp, ctypes.c_int64, ctypes.POINTER(Result)]),
     "admfem_server_stats": (c_int, [c_vp, c_vp]),
+    "admfem_server_spmv_repeat": (c_int, [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
 }
 _bound = False
 
@@ -151,6 +152,18 @@ class Server:
         d["u"], d["grad"] = self._out[rid]
         d["latency"] = d["t_done"] - d["t_arrival"]
         return d
+
+    def spmv_repeat(self, topos, repeat):
+        """Instrumentation: the batched SpMV kernel over one synthetic Poisson
+        system per entry of ``topos``, ``repeat`` launches on torch's current
+        stream.  Returns (algorithmic bytes per launch, average seconds per
+        launch from HIP events on that stream)."""
+        t = np.ascontiguousarray(topos, dtype=np.int32)
+        b, sec = ctypes.c_double(), ctypes.c_double()
+        _lib.check(_L().admfem_server_spmv_repeat(self._h, _lib.ptr(t), int(t.size), int(repeat),
+                                                  ctypes.byref(b), ctypes.byref(sec),
+                                                  _lib.stream_ptr()))
+        return b.value, sec.value
 
     def stats(self):
         s = np.zeros(8)

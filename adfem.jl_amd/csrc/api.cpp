@@ -532,6 +532,17 @@ int admfem_server_stats(admfem_server h, double stats[8]) {
   });
 }
 
+int admfem_server_spmv_repeat(admfem_server h, const int *topos, int nsys, int repeat,
+                              double *bytes, double *seconds, void *stream) {
+  return guarded([&] {
+    REQUIRE(topos && nsys > 0 && repeat >= 0, ADMFEM_ERR_ARG, "bad arguments");
+    double sec = 0.0;
+    const double b = server_spmv_repeat(&SV(h), topos, nsys, repeat, S(stream), &sec);
+    if (bytes) *bytes = b;
+    if (seconds) *seconds = sec;
+  });
+}
+
 // ------------------------------------------------- A. reference-named symbols
 admfem_mesh admfem_default_mesh(void) {
   std::lock_guard<std::mutex> lk(g_default_mu);

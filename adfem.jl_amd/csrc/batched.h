@@ -40,6 +40,8 @@ void batch_init(const BSys *D, BScal *S, int sys, int nblk, double rtol2, double
                 int maxit, hipStream_t st);
 void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, const BSys *D,
                       BScal *S, int iters, hipStream_t st);
+void batch_spmv_repeat(const int2 *map, int nmap, const BSys *D, const BScal *S, int repeat,
+                       hipStream_t st);
 void batch_final(const BSys *D, const BScal *S, int sys, int npad, double *out,
                  hipStream_t st);
 void synth_inputs(const System &s, const double *base_x, const double *base_y, uint64_t seed,

@@ -410,6 +410,12 @@ void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, con
   HIPCHECK(hipGetLastError());
 }
 
+void batch_spmv_repeat(const int2 *map, int nmap, const BSys *D, const BScal *S, int repeat,
+                       hipStream_t st) {
+  for (int k = 0; k < repeat; k++) bk_spmv<<<nmap, NT, 0, st>>>(map, D, S);
+  HIPCHECK(hipGetLastError());
+}
+
 void batch_final(const BSys *D, const BScal *S, int sys, int npad, double *out,
                  hipStream_t st) {
   bk_final<<<nb(npad, 256), 256, 0, st>>>(D, S, sys, out);

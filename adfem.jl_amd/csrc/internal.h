@@ -214,6 +214,8 @@ int64_t server_submit(Server *sv, const admfem_request &r);
 void server_run(Server *sv, double timeout);
 void server_result(Server *sv, int64_t id, admfem_result &out);
 void server_stats(Server *sv, double *stats);
+double server_spmv_repeat(Server *sv, const int *topos, int nsys, int repeat,
+                          hipStream_t st, double *seconds);
 // loss gradient: g_int = 0; g[perm[idx]] = 2 (u - obs); returns loss on host
 double loss_and_grad(const System &s, const double *u_int, const int *obs_idx,
                      const double *obs_val, int nobs, double *g_int,

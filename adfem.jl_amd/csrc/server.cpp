@@ -607,6 +607,67 @@ struct Server {
   }
 };
 
+// Instrumentation: the batched SpMV kernel over a batch of synthetic Poisson
+// systems.  bk_spmv reads z and p_old and writes p_new and q only, and the
+// scalars it reads (beta, it, status) stay as batch_init left them, so every
+// launch does the same work.
+double server_spmv_repeat(Server *sv, const int *topos, int nsys, int repeat,
+                          hipStream_t st, double *seconds) {
+  REQUIRE(nsys <= (int)sv->free_slots.size(), ADMFEM_ERR_ARG, "more systems than free slots");
+  for (int i = 0; i < nsys; i++)
+    REQUIRE(topos[i] >= 0 && topos[i] < (int)sv->topos.size(), ADMFEM_ERR_ARG,
+            "bad topology id");
+  std::vector<Req> qs(nsys);
+  int admitted = 0;
+  double bytes = 0.0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  auto cleanup = [&] {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(sv->st);
+    for (int i = 0; i < admitted; i++) sv->release(qs[i]);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  try {
+    size_t nmap = 0;
+    for (int i = 0; i < nsys; i++) {
+      Req &q = qs[i];
+      q.r = admfem_request{};
+      q.r.topo = topos[i];
+      q.r.source = ADMFEM_SRC_SYNTH;
+      q.r.seed = (unsigned long long)i;
+      const bool ok = sv->admit(q, 0.0);
+      REQUIRE(ok && q.state == Q_FWD, ADMFEM_ERR_BUDGET, "batch does not fit the budget");
+      admitted++;
+      nmap += sv->topos[q.r.topo].nblk;
+      const System &s = *sv->topos[q.r.topo].sys;
+      bytes += (double)s.col.size() * 8.0 + 4.0 * (s.n + 1) + 2.0 * s.n * 4.0;
+    }
+    sv->ensure_map(nmap, (size_t)nsys);
+    size_t k = 0;
+    for (int i = 0; i < nsys; i++)
+      for (int b = 0; b < sv->topos[qs[i].r.topo].nblk; b++)
+        sv->h_map[k++] = make_int2(qs[i].slot, b);
+    HIPCHECK(hipMemcpyAsync(sv->d_map.p, sv->h_map, sizeof(int2) * nmap, hipMemcpyHostToDevice,
+                            sv->st));
+    HIPCHECK(hipStreamSynchronize(sv->st));
+    HIPCHECK(hipEventCreate(&ev[0]));
+    HIPCHECK(hipEventCreate(&ev[1]));
+    HIPCHECK(hipEventRecord(ev[0], st));
+    batch_spmv_repeat(sv->d_map.p, (int)nmap, sv->d_desc.p, sv->d_scal.p, repeat, st);
+    HIPCHECK(hipEventRecord(ev[1], st));
+    HIPCHECK(hipEventSynchronize(ev[1]));
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    *seconds = repeat > 0 ? 1e-3 * ms / repeat : 0.0;
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+  return bytes;
+}
+
 // ---------------------------------------------------------------- C-ABI glue
 Server *server_create(const admfem_server_opts &o) {
   REQUIRE(o.hbm_budget_bytes > 0, ADMFEM_ERR_ARG, "budget must be positive");

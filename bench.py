@@ -1,19 +1,37 @@
-"""Benchmark: forward+adjoint solves/s of the P1 Poisson path on one MI355X.
+"""Benchmark: forward+adjoint solves/s at 1e-8 relative error on one MI355X,
+with p50/p99 latency, against the plain FP64 GPU path.
 
-Workload (one "step"): on a 1024x1024-quad jittered, node-permuted unit-square
-mesh (2,097,152 triangles, 1,050,625 nodes), kappa = exp(GRF, 32 Fourier
-modes, sigma 1, length 0.1) and f ~ U[-1,1] at the 3 Gauss points of every
-triangle, homogeneous Dirichlet on the boundary:
-  assemble K, F -> impose Dirichlet -> u = K^-1 F -> L = sum (u - u_obs)^2
-  over a seeded 1% of interior nodes -> lam = K^-1 dL/du -> dL/dkappa.
-The product path is the mixed-precision one (FP32 Jacobi-PCG with FP64
-residual replacement and FP64 solution accumulation); the plain FP64 path is
-timed beside it, and both report their relative error against an FP64
-reference (FP64 CG to a 1e-13 true residual; at the start of the run also
-checked against the CPU direct solve of the oracle when the cpu_baseline leg
-runs).  Inputs are resident in HBM before timing starts.
+Workload (the serving form): a stream of 4096 forward+adjoint requests.  Each
+request is P1 Poisson (K u = F) or an implicit heat step ((M + dt K) u =
+M u_prev + dt F, dt ~ logU[1e-4, 1e-1]) with probability 1/2, on a unit-square
+mesh of side s = round(exp(U[ln 32, ln 512])) quads (2 s^2 triangles), interior
+nodes jittered by U[-0.2h, 0.2h]^2; kappa = exp(GRF: 32 random Fourier modes,
+sigma 1, length 0.1) and f ~ U[-1, 1] at the Gauss points, homogeneous
+Dirichlet on the boundary, L = sum (u - u_obs)^2 over a seeded 1% of the
+interior nodes; each request returns u, L and dL/dkappa.  Requests are served
+by the continuous-batching engine (FP32-storage Jacobi-PCG with FP64 residual
+replacement, every system's true FP64 relative residual driven below rtol)
+under a 64 GB HBM admission budget.  Inputs are generated on the device from
+each request's seed; mesh topologies are resident before timing starts.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--m 1024]
+One step = one pass of the whole stream through a fresh engine with every
+request queued at t = 0 (throughput at capacity).  After the timed steps:
+  * load: the same stream with Poisson arrivals at 80% of the measured
+    capacity -> p50 / p99 latency (arrival to result);
+  * fp64_plain: the plain FP64 GPU path (one request at a time, FP64
+    Jacobi-PCG) on the first requests of the stream -> its solves/s and
+    latency; the batched engine on the same subset beside it;
+  * accuracy: requests of the stream with host-side inputs, served by both
+    paths and compared with the CPU oracle's direct solve (u and dL/dkappa);
+  * roofline: the batched SpMV kernel over the stream's first batch, timed
+    with HIP events on the stream it runs on;
+  * precision_form: one 2048x2048-quad system (8.4M triangles), mixed vs plain
+    FP64 forward+adjoint, errors against an FP64 CG solve to 1e-13, the
+    single-system SpMV kernels and the assembly timed with HIP events;
+  * cpu_baseline: the CPU oracle (C restatement of the reference loops + SciPy
+    SuperLU) on the first requests of the stream, one core.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 Prints ONE JSON line.
 """
 import argparse
@@ -29,54 +47,135 @@ sys.path.insert(0, os.path.join(ROOT, "adfem.jl_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SEEDS = dict(mesh=20240601, kappa=11, kappa_obs=12, f=13, obs=14)
+SEEDS = dict(stream=7, arrivals=8, mesh=20240601, kappa=11, kappa_obs=12, f=13, obs=14,
+             host=5000)
+BUDGET = 64e9
 
 
-def build_problem(m):
-    import torch
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# --------------------------------------------------------------- the stream
+def make_stream(n):
+    rng = np.random.default_rng(SEEDS["stream"])
+    sides = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), n))).astype(int)
+    heat = rng.random(n) < 0.5
+    dts = np.exp(rng.uniform(np.log(1e-4), np.log(1e-1), n))
+    return sides, heat, dts
+
+
+def make_topologies(sides):
     import admfem
     from admfem import meshgen
-    coords, elems = meshgen.jittered_square(m, seed=SEEDS["mesh"])
-    mesh = admfem.Mesh(coords, elems, zero_based=True)
-    g = admfem.gauss_nodes(mesh)
-    kappa = np.exp(meshgen.grf(g, SEEDS["kappa"]))
-    kappa_obs = np.exp(meshgen.grf(g, SEEDS["kappa_obs"]))
-    f = np.random.default_rng(SEEDS["f"]).uniform(-1.0, 1.0, mesh.ngauss)
-    bd = admfem.bcnode(mesh)
-    interior = np.setdiff1d(np.arange(mesh.nnode), bd - 1)
-    obs = np.sort(np.random.default_rng(SEEDS["obs"]).choice(
-        interior, size=max(1, mesh.nnode // 100), replace=False))
-    S = admfem.DirichletSystem(mesh, bd)
-    dev = dict(kappa=torch.as_tensor(kappa, device="cuda"),
-               kappa_obs=torch.as_tensor(kappa_obs, device="cuda"),
-               f=torch.as_tensor(f, device="cuda"),
-               obs=torch.as_tensor(obs.astype(np.int32), device="cuda"))
-    # u_obs: the state of a second seeded kappa (FP64 solve)
-    S.assemble(kappa=dev["kappa_obs"], f=dev["f"])
-    u2, _ = S.solve(precision="fp64", rtol=1e-12)
-    dev["uobs"] = u2[dev["obs"].long()].contiguous()
-    host = dict(coords=coords, elems=elems, kappa=kappa, f=f, bd=bd, obs=obs,
-                uobs=dev["uobs"].cpu().numpy())
-    return mesh, S, dev, host
+    systems = {}
+    for s in sorted(set(sides.tolist())):
+        c, e = meshgen.jittered_square(s, seed=0, jitter=0.0, permute=False)
+        mesh = admfem.Mesh(c, e, zero_based=True)
+        systems[s] = admfem.DirichletSystem(mesh, admfem.bcnode(mesh))
+    return systems
 
 
-def run_steps(S, dev, precision, rtol, steps, warmup):
+def serve(systems, sides, heat, dts, idx, arrivals, mode, rtol, want=False):
+    """Serves requests ``idx`` of the stream through a fresh engine; returns
+    (wall seconds of run(), results, stats)."""
     import torch
-    out = None
-    for _ in range(warmup):
-        out = S.forward_adjoint(dev["obs"], dev["uobs"], precision=precision, rtol=rtol,
-                                kappa=dev["kappa"], f=dev["f"], want_lam=False)
+    from admfem.server import Server
+    sv = Server(BUDGET, mode=mode, rtol=rtol)
+    tid = {s: sv.add_topology(S, 0.2 / s) for s, S in systems.items()}
+    ids = [sv.submit(tid[int(sides[i])], kind="heat" if heat[i] else "poisson",
+                     dt=float(dts[i]), seed=1000 + int(i), arrival=float(a), want_u=want,
+                     want_grad=want) for i, a in zip(idx, arrivals)]
     torch.cuda.synchronize()
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        a = time.perf_counter()
-        out = S.forward_adjoint(dev["obs"], dev["uobs"], precision=precision, rtol=rtol,
-                                kappa=dev["kappa"], f=dev["f"], want_lam=False)
-        lat.append(time.perf_counter() - a)
+    t = time.perf_counter()
+    sv.run()
     torch.cuda.synchronize()
-    total = time.perf_counter() - t0
-    return out, total, np.array(lat)
+    wall = time.perf_counter() - t
+    rs = [sv.result(r) for r in ids]
+    st = sv.stats()
+    sv.close()
+    return wall, rs, st
+
+
+def summary(n, wall, rs, st):
+    lat = np.array([r["latency"] for r in rs])
+    svc = np.array([r["t_done"] - r["t_admit"] for r in rs])
+    return dict(requests=n, failed=int(sum(r["status"] != 0 for r in rs)),
+                wall_s=round(wall, 3), solves_per_s=round(n / max(wall, 1e-9), 2),
+                p50_ms=round(1e3 * float(np.percentile(lat, 50)), 2),
+                p99_ms=round(1e3 * float(np.percentile(lat, 99)), 2),
+                service_p50_ms=round(1e3 * float(np.percentile(svc, 50)), 2),
+                service_p99_ms=round(1e3 * float(np.percentile(svc, 99)), 2),
+                max_relres=float(max(max(r["relres_fwd"], r["relres_adj"]) for r in rs)),
+                mean_iters=round(float(np.mean([r["iters_fwd"] + r["iters_adj"] for r in rs])), 1),
+                peak_active=int(st["peak_active"]), peak_gb=round(st["peak_bytes"] / 1e9, 3),
+                mean_batch=round(st["mean_batch"], 1))
+
+
+# ------------------------------------------------- host-side request inputs
+def host_request(side, heat, dt, seed):
+    """Inputs of one request of the stream generated on the host (same
+    distributions as the engine's device generator, own random numbers)."""
+    from admfem import meshgen
+    rng = np.random.default_rng(seed)
+    c, e = meshgen.jittered_square(side, seed=seed, jitter=0.2, permute=False)
+    g = meshgen.gauss_points(c, e)
+    kappa = np.exp(meshgen.grf(g, seed + 1))
+    f = rng.uniform(-1.0, 1.0, g.shape[0])
+    up = meshgen.grf(c, seed + 2) if heat else None
+    n = c.shape[0]
+    j, i = np.divmod(np.arange(n), side + 1)
+    inner = np.flatnonzero((i > 0) & (i < side) & (j > 0) & (j < side))
+    obs = np.sort(rng.choice(inner, size=max(1, inner.size // 100), replace=False))
+    return dict(side=side, coords=c, elems=e, kappa=kappa, f=f, u_prev=up,
+                dt=dt if heat else None, obs=obs.astype(np.int32), obs_val=np.zeros(obs.size))
+
+
+def oracle_solve(P):
+    from oracle import oracle as O
+    om = O.OracleMesh(P["coords"], P["elems"])
+    bd = om.bcnode()
+    obs = P["obs"]
+
+    def dl(u):
+        g = np.zeros_like(u)
+        g[obs] = 2.0 * (u[obs] - P["obs_val"])
+        return g
+    kw = {} if P["dt"] is None else dict(dt=P["dt"], u_prev=P["u_prev"])
+    return O.forward_adjoint(om, P["kappa"], P["f"], bd, np.zeros(bd.size), dl, **kw)
+
+
+def accuracy(systems, sides, heat, dts, rtol, nsample, max_side):
+    """Host-input requests through both GPU paths against the CPU oracle."""
+    from admfem.server import Server
+    pick = [i for i in range(len(sides)) if sides[i] <= max_side][:nsample]
+    Ps = [host_request(int(sides[i]), bool(heat[i]), float(dts[i]), SEEDS["host"] + int(i))
+          for i in pick]
+    out = {}
+    for mode in ("batched", "seq_fp64"):
+        sv = Server(BUDGET, mode=mode, rtol=rtol)
+        tid = {s: sv.add_topology(S, 0.0) for s, S in systems.items()}
+        ids = [sv.submit(tid[P["side"]], kind="heat" if P["dt"] else "poisson",
+                         dt=P["dt"] or 0.0, coords=P["coords"], kappa=P["kappa"], f=P["f"],
+                         u_prev=P["u_prev"], obs_idx=P["obs"], obs_val=P["obs_val"])
+               for P in Ps]
+        sv.run()
+        out[mode] = [sv.result(r) for r in ids]
+        sv.close()
+    eu = {m: 0.0 for m in out}
+    eg = {m: 0.0 for m in out}
+    for k, P in enumerate(Ps):
+        ref = oracle_solve(P)
+        for m in out:
+            r = out[m][k]
+            assert r["status"] == 0, (m, k, r["status"])
+            eu[m] = max(eu[m], relerr(r["u"], ref["u"]))
+            eg[m] = max(eg[m], relerr(r["grad"], ref["grad_kappa"]))
+    return dict(requests=len(Ps), sides=[P["side"] for P in Ps],
+                reference="CPU oracle: C restatement of the reference ops + SciPy SuperLU",
+                max_rel_err_u=eu["batched"], max_rel_err_grad=eg["batched"],
+                fp64_plain_max_rel_err_u=eu["seq_fp64"],
+                fp64_plain_max_rel_err_grad=eg["seq_fp64"])
 
 
 def relerr(a, b):
@@ -85,134 +184,152 @@ def relerr(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def spmv_roofline(S, precision, repeat=200):
-    """Average duration of the solver's SpMV kernel, timed with HIP events on
-    the stream it is launched on (torch's current stream, which the library
-    is handed), and its algorithmic-byte rate against HBM peak."""
-    import ctypes
+# ------------------------------------------------------------------ roofline
+def hip_time(fn, repeat):
+    """Average seconds per call of fn() over ``repeat`` calls, from HIP events
+    recorded on torch's current stream (the stream the library is handed)."""
     import torch
-    from admfem import _lib
-    L = _lib.lib()
-    prec = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
-    nbytes = ctypes.c_double()
-    _lib.check(L.admfem_system_spmv_repeat(S._h, prec, 20, ctypes.byref(nbytes), _lib.stream_ptr()))
-    torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
     e0.record()
-    _lib.check(L.admfem_system_spmv_repeat(S._h, prec, repeat, None, _lib.stream_ptr()))
+    fn()
     e1.record()
     torch.cuda.synchronize()
-    avg_s = e0.elapsed_time(e1) / 1e3 / repeat
-    gbs = nbytes.value / avg_s / 1e9
-    return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
-                kernel=f"k_spmv<{'float' if prec == _lib.PREC_MIXED else 'double'}>",
-                avg_us=round(avg_s * 1e6, 2), algorithmic_bytes=int(nbytes.value))
+    return e0.elapsed_time(e1) / 1e3 / repeat
 
 
-def serving(nreq, fp64_reqs, seed=7):
-    """Serving workload: forward+adjoint requests, each Poisson or an implicit
-    heat step (M + dt K, dt ~ logU[1e-4, 1e-1]) with p = 1/2, on a jittered
-    unit-square mesh whose side is log-uniform in [32, 512] quads (rounded to
-    a multiple of 16), kappa = exp(GRF), f ~ U[-1, 1], FP32 inner solves with
-    FP64 residual replacement to a 1e-8 true relative residual, 64 GB HBM
-    admission budget.  Three runs: capacity (all requests queued at t = 0),
-    load (Poisson arrivals at 80% of the measured capacity: p50 / p99
-    latency), and the plain FP64 path serving the first requests one at a
-    time (its solves/s)."""
-    import admfem
-    from admfem import meshgen
+def roof(nbytes, avg_s, kernel, **extra):
+    gbs = nbytes / avg_s / 1e9
+    d = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+             frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, kernel=kernel,
+             avg_us=round(avg_s * 1e6, 2), algorithmic_bytes=int(nbytes))
+    d.update(extra)
+    return d
+
+
+def batched_spmv_roofline(systems, sides, nsys, repeat=200):
+    """bk_spmv over the stream's first nsys requests.  The library records
+    the HIP events on torch's current stream around the kernel launches only
+    (the batch is admitted and assembled beforehand in the same call)."""
     from admfem.server import Server
-    rng = np.random.default_rng(seed)
-    sides = (np.round(np.exp(rng.uniform(np.log(32), np.log(512), nreq)) / 16) * 16)
-    sides = np.clip(sides, 32, 512).astype(int)
-    heat = rng.random(nreq) < 0.5
-    dts = np.exp(rng.uniform(np.log(1e-4), np.log(1e-1), nreq))
+    sv = Server(BUDGET, mode="batched")
+    tid = {s: sv.add_topology(S, 0.2 / s) for s, S in systems.items()}
+    topos = [tid[int(s)] for s in sides[:nsys]]
+    sv.spmv_repeat(topos, 5)
+    nbytes, avg = sv.spmv_repeat(topos, repeat)
+    sv.close()
+    return roof(nbytes, avg, "bk_spmv", batch=f"the stream's first {nsys} requests, one "
+                "Poisson system each, one launch over all of them",
+                format="SELL-64, FP32 values, FP64 arithmetic, fused p = z + beta p_old")
+
+
+# ------------------------------------------------------------ precision form
+def precision_form(m, rtol):
+    import ctypes
+    import torch
+    import admfem
+    from admfem import _lib, meshgen
     t0 = time.perf_counter()
-    systems = {}
-    for s in sorted(set(sides.tolist())):
-        c, e = meshgen.jittered_square(s, seed=0, jitter=0.0, permute=False)
-        mesh = admfem.Mesh(c, e, zero_based=True)
-        systems[s] = admfem.DirichletSystem(mesh, admfem.bcnode(mesh))
-    setup_s = time.perf_counter() - t0
-
-    def run(mode, n, arrivals):
-        sv = Server(64e9, mode=mode, rtol=1e-8)
-        tid = {s: sv.add_topology(S, 0.2 / s) for s, S in systems.items()}
-        ids = [sv.submit(tid[int(sides[i])], kind="heat" if heat[i] else "poisson",
-                         dt=float(dts[i]), seed=1000 + i, arrival=float(arrivals[i]),
-                         want_u=False, want_grad=False) for i in range(n)]
+    coords, elems = meshgen.jittered_square(m, seed=SEEDS["mesh"])
+    mesh = admfem.Mesh(coords, elems, zero_based=True)
+    g = admfem.gauss_nodes(mesh)
+    kappa = torch.as_tensor(np.exp(meshgen.grf(g, SEEDS["kappa"])), device="cuda")
+    kappa2 = torch.as_tensor(np.exp(meshgen.grf(g, SEEDS["kappa_obs"])), device="cuda")
+    f = torch.as_tensor(np.random.default_rng(SEEDS["f"]).uniform(-1, 1, mesh.ngauss),
+                        device="cuda")
+    bd = admfem.bcnode(mesh)
+    interior = np.setdiff1d(np.arange(mesh.nnode), bd - 1)
+    obs = torch.as_tensor(np.sort(np.random.default_rng(SEEDS["obs"]).choice(
+        interior, size=mesh.nnode // 100, replace=False)).astype(np.int32), device="cuda")
+    S = admfem.DirichletSystem(mesh, bd)
+    S.assemble(kappa=kappa2, f=f)
+    u2, _ = S.solve(precision="fp64", rtol=1e-12)
+    uobs = u2[obs.long()].contiguous()
+    setup = time.perf_counter() - t0
+    log(f"precision form m={m}: setup {setup:.1f}s")
+    ref = S.forward_adjoint(obs, uobs, precision="fp64", rtol=1e-13, kappa=kappa, f=f)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        S.forward_adjoint(obs, uobs, precision=prec, rtol=rtol, kappa=kappa, f=f, want_lam=False)
+        torch.cuda.synchronize()
         t = time.perf_counter()
-        sv.run()
-        wall = time.perf_counter() - t
-        rs = [sv.result(i) for i in ids]
-        bad = sum(r["status"] != 0 for r in rs)
-        lat = np.array([r["latency"] for r in rs])
-        return dict(requests=n, failed=bad, wall_s=round(wall, 3),
-                    solves_per_s=round(n / max(wall, 1e-9), 2),
-                    p50_ms=round(1e3 * float(np.percentile(lat, 50)), 2),
-                    p99_ms=round(1e3 * float(np.percentile(lat, 99)), 2),
-                    peak_active=int(sv.stats()["peak_active"]),
-                    peak_gb=round(sv.stats()["peak_bytes"] / 1e9, 3),
-                    mean_batch=round(sv.stats()["mean_batch"], 1))
+        out = S.forward_adjoint(obs, uobs, precision=prec, rtol=rtol, kappa=kappa, f=f,
+                                want_lam=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        res[prec] = dict(seconds=round(dt, 4), solves_per_s=round(1.0 / dt, 4),
+                         rel_err_u=relerr(out["u"], ref["u"]),
+                         rel_err_grad=relerr(out["grad_kappa"], ref["grad_kappa"]),
+                         iterations=[out["stats"][0]["iterations"],
+                                     out["stats"][1]["iterations"]])
+    L = _lib.lib()
+    rl = {}
+    for prec, code in (("mixed", _lib.PREC_MIXED), ("fp64", _lib.PREC_FP64)):
+        nb = ctypes.c_double()
+        _lib.check(L.admfem_system_spmv_repeat(S._h, code, 10, ctypes.byref(nb),
+                                               _lib.stream_ptr()))
+        avg = hip_time(lambda: _lib.check(L.admfem_system_spmv_repeat(
+            S._h, code, 200, None, _lib.stream_ptr())), 200)
+        rl[prec] = roof(nb.value, avg, "k_spmv<%s>" % ("float" if prec == "mixed" else "double"))
+    # assembly: fused K, F and Dirichlet into CSR values + rhs, then the SELL copy
+    nnz, N, ne, ng = S.nnz, mesh.nnode, mesh.nelem, mesh.ngauss
+    b_asm = ng * 8 + 12 * ne + 16 * N + nnz * 8
+    b_fused = b_asm + ng * 8 + N * 8
+    avg = hip_time(lambda: [S.assemble(kappa=kappa, f=f) for _ in range(20)], 20)
+    rl["assembly"] = roof(b_fused, avg, "assemble_kernel + to_sell_kernel",
+                          bytes_note="kappa, f, connectivity, coordinates read; CSR values "
+                                     "and rhs written (the kernel computes K and F)",
+                          survey_bytes=int(b_asm))
+    return dict(workload=f"{m}x{m}-quad jittered node-permuted mesh ({ne} triangles, {N} "
+                         f"nodes, nnz {nnz}), Poisson, kappa=exp(GRF), f~U[-1,1], 1% observed "
+                         f"nodes, rtol {rtol:g}; errors vs FP64 CG to 1e-13",
+                setup_s=round(setup, 1), mixed=res["mixed"], fp64_plain=res["fp64"],
+                speedup_mixed_vs_fp64=round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3),
+                roofline=rl)
 
-    cap = run("batched", nreq, np.zeros(nreq))
-    rate = 0.8 * cap["solves_per_s"]
-    arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, nreq))
-    load = run("batched", nreq, arr)
-    load["arrival_rate_per_s"] = round(rate, 2)
-    nf = min(nreq, fp64_reqs)
-    fp64 = run("seq_fp64", nf, np.zeros(nf))
-    cap_sub = run("batched", nf, np.zeros(nf))
-    return dict(workload=f"{nreq} forward+adjoint requests, Poisson or heat (p=1/2), side "
-                         f"logU[32,512] (x16), jittered square, kappa=exp(GRF), rtol 1e-8, "
-                         f"64 GB budget", topologies=len(systems), setup_s=round(setup_s, 1),
-                capacity=cap, load=load, fp64_plain_subset=fp64, batched_subset=cap_sub,
-                speedup_vs_fp64_plain=round(cap_sub["solves_per_s"] /
-                                            max(fp64["solves_per_s"], 1e-9), 2))
 
-
-def cpu_baseline(host, m):
-    """One forward+adjoint of the same workload with the CPU oracle (serial C
-    restatement of the reference loops + SciPy SuperLU), pinned to one core."""
-    from oracle import oracle as O
-    cores = 1
+# -------------------------------------------------------------- CPU baseline
+def cpu_baseline(sides, heat, dts, budget_s=20.0):
+    """The CPU oracle on the first requests of the stream, pinned to one
+    core, until ``budget_s`` of CPU work has been done."""
     old = os.sched_getaffinity(0)
+    done, spent, nodes = 0, 0.0, 0
     try:
         os.sched_setaffinity(0, {min(old)})
-        t0 = time.perf_counter()
-        om = O.OracleMesh(host["coords"], host["elems"])
-        obs, uobs = host["obs"], host["uobs"]
-
-        def dloss(u):
-            g = np.zeros_like(u)
-            g[obs] = 2.0 * (u[obs] - uobs)
-            return g
-
-        res = O.forward_adjoint(om, host["kappa"], host["f"], host["bd"],
-                                np.zeros(host["bd"].size), dloss)
-        dt = time.perf_counter() - t0
+        for i in range(len(sides)):
+            P = host_request(int(sides[i]), bool(heat[i]), float(dts[i]), SEEDS["host"] + i)
+            t = time.perf_counter()
+            oracle_solve(P)
+            spent += time.perf_counter() - t
+            done += 1
+            nodes += (int(sides[i]) + 1) ** 2
+            if spent >= budget_s:
+                break
     finally:
         os.sched_setaffinity(0, old)
-    return dict(value=round(1.0 / dt, 5), unit="solves/s", cores=cores, kind="port",
-                sample=f"one forward+adjoint at m={m} (mesh precompute, COO emission, "
-                       f"Dirichlet, CSR, SuperLU factor+solve, adjoint solve, gradient chain), "
-                       f"{dt:.1f} s on 1 core"), res
+    return dict(value=round(done / spent, 4), unit="solves/s", cores=1, kind="port",
+                sample=f"the stream's first {done} requests (sides {sides[:done].min()}-"
+                       f"{sides[:done].max()}, {nodes} nodes in all), host-generated inputs, "
+                       f"each: mesh precompute, COO emission, Dirichlet, CSR, SuperLU "
+                       f"factor+solve, adjoint solve, gradient chain; {spent:.1f} s on 1 core")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--m", type=int, default=1024)
-    ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1, help="timed passes of the stream")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes before them")
+    ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--rtol", type=float, default=1e-9,
+                    help="true relative residual of every solve")
+    ap.add_argument("--fp64-requests", type=int, default=512)
+    ap.add_argument("--accuracy-requests", type=int, default=16)
+    ap.add_argument("--roofline-systems", type=int, default=1024)
+    ap.add_argument("--precision-m", type=int, default=2048, help="0: skip")
+    ap.add_argument("--precision-rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serving-requests", type=int, default=512,
-                    help="serving workload size (0: skip; 4096 for the full stream)")
-    ap.add_argument("--serving-fp64-requests", type=int, default=48,
-                    help="requests served by the plain FP64 path for the comparison")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="set up the stream and time only the batched SpMV kernel")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}): this build is "
@@ -220,30 +337,49 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "no HIP device"
-    m = a.m
-    print(f"[bench] building m={m} problem", file=sys.stderr, flush=True)
+    n = a.requests
+    sides, heat, dts = make_stream(n)
     t0 = time.perf_counter()
-    mesh, S, dev, host = build_problem(m)
-    print(f"[bench] setup {time.perf_counter() - t0:.1f}s; reference solve", file=sys.stderr,
-          flush=True)
-    ref = S.forward_adjoint(dev["obs"], dev["uobs"], precision="fp64", rtol=1e-13,
-                            kappa=dev["kappa"], f=dev["f"])
-    torch.cuda.synchronize()
-    print(f"[bench] mixed steps={a.steps}", file=sys.stderr, flush=True)
-    out, total, lat = run_steps(S, dev, "mixed", a.rtol, a.steps, a.warmup)
-    err_u = relerr(out["u"], ref["u"])
-    err_g = relerr(out["grad_kappa"], ref["grad_kappa"])
-    print(f"[bench] fp64 steps={a.fp64_steps}", file=sys.stderr, flush=True)
-    out64, total64, lat64 = run_steps(S, dev, "fp64", a.rtol, a.fp64_steps, 1)
-    err_u64 = relerr(out64["u"], ref["u"])
-    err_g64 = relerr(out64["grad_kappa"], ref["grad_kappa"])
-    roof = spmv_roofline(S, "mixed")
-    roof64 = spmv_roofline(S, "fp64")
-    value = a.steps / total
-    value64 = a.fp64_steps / total64
+    systems = make_topologies(sides)
+    setup_s = time.perf_counter() - t0
+    log(f"{len(systems)} topologies in {setup_s:.1f}s")
+    nroof = min(a.roofline_systems, n)
+    if a.roofline_only:
+        print(json.dumps(batched_spmv_roofline(systems, sides, nroof)), flush=True)
+        return
+    idx = np.arange(n)
+    zeros = np.zeros(n)
+    for w in range(a.warmup):
+        wall, rs, st = serve(systems, sides, heat, dts, idx, zeros, "batched", a.rtol)
+        log(f"warmup pass {w}: {n / wall:.1f} solves/s")
+    walls, last = [], None
+    for k in range(a.steps):
+        wall, rs, st = serve(systems, sides, heat, dts, idx, zeros, "batched", a.rtol)
+        walls.append(wall)
+        last = summary(n, wall, rs, st)
+        log(f"step {k}: {n / wall:.1f} solves/s, failed {last['failed']}")
+    total = float(np.sum(walls))
+    value = n * a.steps / total
+    rate = 0.8 * value
+    arr = np.cumsum(np.random.default_rng(SEEDS["arrivals"]).exponential(1.0 / rate, n))
+    wall, rs, st = serve(systems, sides, heat, dts, idx, arr, "batched", a.rtol)
+    load = summary(n, wall, rs, st)
+    load["arrival_rate_per_s"] = round(rate, 2)
+    log(f"load: p50 {load['p50_ms']} ms p99 {load['p99_ms']} ms")
+    nf = min(n, a.fp64_requests)
+    sub = np.arange(nf)
+    wall, rs, st = serve(systems, sides, heat, dts, sub, np.zeros(nf), "seq_fp64", a.rtol)
+    fp64 = summary(nf, wall, rs, st)
+    wall, rs, st = serve(systems, sides, heat, dts, sub, np.zeros(nf), "batched", a.rtol)
+    bsub = summary(nf, wall, rs, st)
+    log(f"fp64 plain: {fp64['solves_per_s']} solves/s; batched same subset "
+        f"{bsub['solves_per_s']}")
+    acc = accuracy(systems, sides, heat, dts, a.rtol, a.accuracy_requests, 256)
+    log(f"accuracy: u {acc['max_rel_err_u']:.2e} grad {acc['max_rel_err_grad']:.2e}")
+    rl = batched_spmv_roofline(systems, sides, nroof)
     line = {
-        "metric": "forward+adjoint solves/s at 1e-8 rel err",
-        "value": round(value, 4),
+        "metric": "forward+adjoint solves/s at 1e-8 rel err; p50/p99 latency vs plain FP64 GPU",
+        "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -255,40 +391,32 @@ def main():
         "dtype": "fp32+fp64",
         "data": "synthetic",
         "config": {
-            "workload": f"P1 Poisson forward+adjoint, {m}x{m}-quad jittered node-permuted "
-                        f"mesh ({mesh.nelem} triangles, {mesh.nnode} nodes), kappa=exp(GRF), "
-                        f"f~U[-1,1], 1% observed nodes, Jacobi-PCG rtol {a.rtol:g}",
-            "precision": "FP32 inner CG + FP64 residual replacement",
-            "nnz": S.nnz,
+            "workload": f"serving: {n} forward+adjoint requests per step, Poisson or implicit "
+                        f"heat step (p=1/2, dt~logU[1e-4,1e-1]), side round(exp(U[ln32,ln512])) "
+                        f"quads, jittered unit square, kappa=exp(GRF), f~U[-1,1], 1% observed "
+                        f"interior nodes, all queued at t=0, {BUDGET / 1e9:g} GB HBM budget",
+            "topologies": len(systems),
+            "rtol": a.rtol,
+            "precision": "FP32 storage, FP64 arithmetic inner CG; FP64 residual replacement "
+                         "and solution",
         },
-        "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
-        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
-        "rel_err_u": err_u,
-        "rel_err_grad": err_g,
-        "iterations": [out["stats"][0]["iterations"], out["stats"][1]["iterations"]],
-        "fp64_plain": {
-            "value": round(value64, 4),
-            "ms_per_step": round(1e3 * total64 / a.fp64_steps, 3),
-            "p50_ms": round(1e3 * float(np.percentile(lat64, 50)), 3),
-            "p99_ms": round(1e3 * float(np.percentile(lat64, 99)), 3),
-            "rel_err_u": err_u64,
-            "rel_err_grad": err_g64,
-            "iterations": [out64["stats"][0]["iterations"], out64["stats"][1]["iterations"]],
-            "roofline": roof64,
-        },
-        "speedup_vs_fp64": round(value / value64, 3),
-        "roofline": roof,
+        "p50_ms": load["p50_ms"],
+        "p99_ms": load["p99_ms"],
+        "capacity": last,
+        "load": load,
+        "fp64_plain": fp64,
+        "batched_same_subset": bsub,
+        "speedup_vs_fp64": round(value / max(fp64["solves_per_s"], 1e-9), 3),
+        "speedup_vs_fp64_same_subset": round(bsub["solves_per_s"] /
+                                             max(fp64["solves_per_s"], 1e-9), 3),
+        "accuracy": acc,
+        "roofline": rl,
     }
-    if a.serving_requests > 0:
-        print(f"[bench] serving {a.serving_requests} requests", file=sys.stderr, flush=True)
-        line["serving"] = serving(a.serving_requests, a.serving_fp64_requests)
+    if a.precision_m > 0:
+        line["precision_form"] = precision_form(a.precision_m, a.precision_rtol)
     if not a.no_cpu:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        cb, cres = cpu_baseline(host, m)
-        line["cpu_baseline"] = cb
-        line["rel_err_u_vs_cpu_direct"] = relerr(out["u"], cres["u"])
-        line["rel_err_grad_vs_cpu_direct"] = relerr(out["grad_kappa"], cres["grad_kappa"])
-        line["fp64_plain"]["rel_err_u_vs_cpu_direct"] = relerr(out64["u"], cres["u"])
+        log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(sides, heat, dts)
     print(json.dumps(line), flush=True)
 
 

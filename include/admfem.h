@@ -373,6 +373,18 @@ int admfem_server_result(admfem_server sv, int64_t req_id, admfem_result *out);
  * mean batch size */
 int admfem_server_stats(admfem_server sv, double stats[8]);
 
+/* Instrumentation (no reference counterpart): admits one synthetic Poisson
+ * request (seed i) on topology topos[i] for i < nsys, as a batch, launches
+ * the batched SpMV kernel over all of them `repeat` times back to back on
+ * `stream`, then releases them.  *bytes (may be NULL) receives the
+ * algorithmic bytes per launch: the sum over the systems of the canonical
+ * CSR figure nnz*(4+4) + 4*(n+1) + 2*n*4 (FP32 values).  *seconds (may be
+ * NULL) receives the average time per launch, from HIP events recorded on
+ * `stream` right before the first and after the last launch.  Not to be
+ * called while admfem_server_run is running on the same server. */
+int admfem_server_spmv_repeat(admfem_server sv, const int *topos, int nsys, int repeat,
+                              double *bytes, double *seconds, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
