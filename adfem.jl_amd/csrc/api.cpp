@@ -330,9 +330,11 @@ int admfem_system_csr(admfem_system h, int *row_ptr, int *col, double *val, doub
   return guarded([&] {
     System &s = SY(h);
     const Mesh &m = *s.mesh;
-    std::vector<double> v(s.col.size()), b(s.n);
+    // the assembled values live in the solver format: diagonal + SELL slices
+    std::vector<double> sv(std::max(1, s.slice_ptr.back())), dg(s.npad), b(s.n);
     HIPCHECK(hipDeviceSynchronize());
-    d2h(v.data(), s.val.p, v.size());
+    d2h(sv.data(), s.d_sell_val.p, sv.size());
+    d2h(dg.data(), s.d_diag.p, dg.size());
     d2h(b.data(), s.rhs.p, b.size());
     int pos = 0;
     row_ptr[0] = 0;
@@ -340,7 +342,10 @@ int admfem_system_csr(admfem_system h, int *row_ptr, int *col, double *val, doub
     for (int u = 0; u < s.n; u++) {
       int r = m.perm[u];
       row.clear();
-      for (int k = s.rp[r]; k < s.rp[r + 1]; k++) row.push_back({m.iperm[s.col[k]], v[k]});
+      const int off = s.slice_ptr[r >> 6] + (r & 63);
+      int kk = 0;
+      for (int k = s.rp[r]; k < s.rp[r + 1]; k++)
+        row.push_back({m.iperm[s.col[k]], s.col[k] == r ? dg[r] : sv[off + 64 * kk++]});
       std::sort(row.begin(), row.end(),
                 [](const std::pair<int, double> &a, const std::pair<int, double> &c) {
                   return a.first < c.first;
@@ -365,7 +370,6 @@ static void assemble_impl(admfem_system h, const admfem_problem *p, hipStream_t 
     q.u_prev = h->tmp_int.p;
   }
   launch_assemble(s, q, st);
-  launch_to_sell(s, st);
 }
 
 int admfem_system_assemble(admfem_system h, const admfem_problem *p, void *stream) {

@@ -1,130 +1,211 @@
-// Fused, atomic-free CSR assembly of the Dirichlet-eliminated system
+// Fused, atomic-free assembly of the Dirichlet-eliminated system
 //   A = mass_coef * M(rho) + lap_coef * K(kappa),
 //   rhs = src_coef * F(f) + M(rho) u_prev - A[:, bd] bdval  (interior rows),
 //   A[b, :] = e_b, rhs[b] = bdval  (boundary rows),
 // which is what compute_fem_{laplace,mass}_matrix1 + compute_fem_source_term1
 // + impose_Dirichlet_boundary_conditions produce in the reference
 // (FemLaplaceScalar.h:3-26, ComputeFemMassMatrixMfem.h:4-29,
-// FemSourceScalar.h:4-15, ImposeDirichlet.h:27-60), summed into CSR.
+// FemSourceScalar.h:4-15, ImposeDirichlet.h:27-60), written straight into
+// the solver's format: SELL-64 off-diagonal values in FP64 and FP32, the
+// diagonal and its inverse in both precisions, and the right-hand side.
 //
-// A block owns AS_ROWS consecutive rows and works in two passes over LDS:
-//  1. one thread per (row, incident element) pair computes that element's
-//     matrix row for the row node (stiffness and mass parts), its load-vector
-//     term and its Dirichlet lifting -- independent work items, so the
-//     element gathers of many pairs are in flight at once;
-//  2. one thread per row adds its pairs, in ascending element order, into the
-//     row's CSR slots and its right-hand side.
-// The block's contiguous CSR range is then written out coalesced.  Every slot
-// has one writer and a fixed summation order: no atomics, deterministic.
+// A block owns one 64-row SELL slice (blocks mapped XCD-contiguously) and
+// works in two passes over LDS:
+//  1. one thread per element incident to the block's interior rows computes
+//     the element's symmetric 3x3 matrix and its 3 lifted load terms -- each
+//     element once per block, its gathers independent of each other;
+//  2. one thread per row adds, in ascending element order, its pairs' matrix
+//     row into its SELL slots (LDS) and diagonal, and their load terms.
+// Each row's thread then writes its slice lanes: consecutive rows are
+// consecutive addresses.  Every value has one writer and a fixed summation
+// order: no atomics, deterministic.
 #include "geom.h"
 
 namespace admfem {
 
 namespace {
 
-constexpr int AS_NT = 256;
+constexpr int AS_NT = 128;
 
-__global__ void __launch_bounds__(AS_NT)
-    assemble_kernel(int n, int nelem, QuadRule rule,
-                    const int *__restrict__ conn, const double *__restrict__ x,
-                    const double *__restrict__ y,
-                    const int *__restrict__ n2e_ptr,
-                    const int *__restrict__ n2e_elem,
-                    const uint32_t *__restrict__ slots,
-                    const int *__restrict__ rp, const int *__restrict__ row_bd,
-                    admfem_problem P, double *__restrict__ val,
-                    double *__restrict__ rhs, int max_inc, int max_nnz) {
-  extern __shared__ double lds[];
-  double *ca = lds;                        // [max_inc][3] matrix-row terms
-  double *rc = ca + 3 * (size_t)max_inc;   // [max_inc] right-hand-side terms
-  double *vl = rc + max_inc;               // [max_nnz] the block's CSR values
-  uint32_t *ws = (uint32_t *)(vl + max_nnz);  // [max_inc] slot words
-  int *ip = (int *)(ws + max_inc);         // [AS_ROWS + 1] incidence offsets
-  const int r0 = blockIdx.x * AS_ROWS;
-  const int rend = min(r0 + AS_ROWS, n);
-  const int nr = rend - r0;
-  const int i0 = n2e_ptr[r0];
-  const int ninc = n2e_ptr[rend] - i0;
-  const int base = rp[r0];
-  const int bnnz = rp[rend] - base;
-  for (int k = threadIdx.x; k <= nr; k += AS_NT) ip[k] = n2e_ptr[r0 + k] - i0;
-  for (int k = threadIdx.x; k < bnnz; k += AS_NT) vl[k] = 0.0;
-  __syncthreads();
-  // pass 1: (row, element) pairs
-  for (int t = threadIdx.x; t < ninc; t += AS_NT) {
-    int lo = 0, hi = nr - 1;  // row of pair t: last k with ip[k] <= t
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (ip[mid] <= t) lo = mid;
-      else hi = mid - 1;
+// index of (p, q) in the packed upper triangle [00 01 02 11 12 22]
+__device__ __forceinline__ int sym3(int p, int q) {
+  const int lo = min(p, q), hi = max(p, q);
+  return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+}
+
+constexpr int PF = 8;  // pair words of a row fetched before the element pass
+
+// One element's gathered inputs: internal vertex ids, vertex coordinates,
+// and its Gauss-point coefficients.
+template <int NQ>
+struct ElemIn {
+  int v[3];
+  double ex[3], ey[3], kq[NQ], fq[NQ], rq[NQ];
+  template <bool MASS>
+  __device__ __forceinline__ void load(const int4 rec, const double *__restrict__ x,
+                                       const double *__restrict__ y, const admfem_problem &P) {
+    v[0] = rec.y;
+    v[1] = rec.z;
+    v[2] = rec.w;
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      const size_t g = (size_t)NQ * rec.x + k;
+      kq[k] = P.kappa ? P.kappa[g] : 1.0;
+      fq[k] = P.f ? P.f[g] : 0.0;
+      rq[k] = MASS && P.rho ? P.rho[g] : 1.0;
     }
-    const int r = r0 + lo;
-    const uint32_t wd = slots[i0 + t];
-    ws[t] = wd;
-    if (row_bd[r] >= 0) continue;  // boundary row: only its unit diagonal
-    const int e = n2e_elem[i0 + t];
-    const int p = wd & 3u;
-    int v[3];
-    double ex[3], ey[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      v[k] = conn[(size_t)k * nelem + e];
       ex[k] = x[v[k]];
       ey[k] = y[v[k]];
     }
-    const ElemGeom G = elem_geom(ex[0], ey[0], ex[1], ey[1], ex[2], ey[2]);
-    // K_e = (sum_k w_k kappa_k) grad phi_p . grad phi_q (P1 gradients are
-    // constant, so the Gauss sum factors out)
-    double ck = 0.0, src = 0.0;
-    double mrow[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < rule.nq; k++) {
-      const size_t g = (size_t)rule.nq * e + k;
-      const double w = gauss_weight(rule.w[k], G.area);
-      ck += w * (P.kappa ? P.kappa[g] : 1.0);
-      const double hp = shape(p, rule.xi[k], rule.eta[k]);
-      if (P.mass_coef != 0.0 || P.u_prev) {
-        const double wr = w * (P.rho ? P.rho[g] : 1.0) * hp;
+  }
+};
+
+// The element's symmetric matrix (packed upper triangle, 6 values) and its
+// three lifted load terms.
+template <int NQ, bool MASS>
+__device__ __forceinline__ void element(const ElemIn<NQ> &in, const QuadRule &rule,
+                                        const admfem_problem &P, const int *__restrict__ row_bd,
+                                        double *ae, double *rl) {
+  const ElemGeom G = elem_geom(in.ex[0], in.ey[0], in.ex[1], in.ey[1], in.ex[2], in.ey[2]);
+  double ck = 0.0, mm[6] = {0, 0, 0, 0, 0, 0}, src[3] = {0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < 3; q++) mrow[q] += wr * shape(q, rule.xi[k], rule.eta[k]);
+  for (int k = 0; k < NQ; k++) {
+    const double w = gauss_weight(rule.w[k], G.area);
+    ck += w * in.kq[k];
+    const double h[3] = {shape(0, rule.xi[k], rule.eta[k]), shape(1, rule.xi[k], rule.eta[k]),
+                         shape(2, rule.xi[k], rule.eta[k])};
+    if (MASS) {
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        const double wr = w * in.rq[k] * h[p];
+#pragma unroll
+        for (int q = p; q < 3; q++) mm[sym3(p, q)] += wr * h[q];
       }
-      if (P.f) src += P.f[g] * hp * w;
     }
-    double rt = P.src_coef * src;
+    if (P.f) {
+#pragma unroll
+      for (int p = 0; p < 3; p++) src[p] += in.fq[k] * h[p] * w;
+    }
+  }
+  double a[6];
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+#pragma unroll
+    for (int q = p; q < 3; q++) {
+      double val = P.lap_coef * ck * (G.gx[p] * G.gx[q] + G.gy[p] * G.gy[q]);
+      if (MASS && P.mass_coef != 0.0) val += P.mass_coef * mm[sym3(p, q)];
+      a[sym3(p, q)] = val;
+    }
+  double r[3];
+#pragma unroll
+  for (int p = 0; p < 3; p++) r[p] = P.src_coef * src[p];
+  if (MASS && P.u_prev) {
+    double up[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) up[q] = P.u_prev[in.v[q]];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int q = 0; q < 3; q++) r[p] += mm[sym3(p, q)] * up[q];
+  }
+  if (P.bdval) {  // Dirichlet lifting of the boundary columns
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-      double a = P.lap_coef * ck * (G.gx[p] * G.gx[q] + G.gy[p] * G.gy[q]);
-      if (P.mass_coef != 0.0) a += P.mass_coef * mrow[q];
-      if (P.u_prev) rt += mrow[q] * P.u_prev[v[q]];
-      if (slot_of(wd, q) == SLOT_LIFT && P.bdval) rt -= a * P.bdval[row_bd[v[q]]];
-      ca[3 * t + q] = a;
-    }
-    rc[t] = rt;
-  }
-  __syncthreads();
-  // pass 2: rows sum their pairs in ascending element order
-  for (int k = threadIdx.x; k < nr; k += AS_NT) {
-    const int r = r0 + k;
-    const int off = rp[r] - base;
-    const int bi = row_bd[r];
-    if (bi >= 0) {
-      vl[off] = 1.0;  // the appended (b,b,1.0) of ImposeDirichlet.h:46-50
-      rhs[r] = P.bdval ? P.bdval[bi] : 0.0;
-      continue;
-    }
-    double acc = 0.0;
-    for (int t = ip[k]; t < ip[k + 1]; t++) {
-      const uint32_t wd = ws[t];
+      const int bq = row_bd[in.v[q]];
+      if (bq >= 0) {
+        const double bv = P.bdval[bq];
 #pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const uint32_t sl = slot_of(wd, q);
-        if (sl != SLOT_LIFT) vl[off + sl] += ca[3 * t + q];
+        for (int p = 0; p < 3; p++) r[p] -= a[sym3(p, q)] * bv;
       }
-      acc += rc[t];
     }
-    rhs[r] = acc;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) ae[k] = a[k];
+#pragma unroll
+  for (int p = 0; p < 3; p++) rl[p] = r[p];
+}
+
+template <int NQ, bool MASS>
+__global__ void __launch_bounds__(AS_NT)
+    assemble_kernel(int n, int npad, QuadRule rule, const double *__restrict__ x,
+                    const double *__restrict__ y, const int *__restrict__ blk_eptr,
+                    const int4 *__restrict__ blk_rec, const int *__restrict__ pair_ptr,
+                    const uint32_t *__restrict__ pairs, const int *__restrict__ row_bd,
+                    const int *__restrict__ slice_ptr, admfem_problem P, SellOut o, int max_e,
+                    int wmax) {
+  extern __shared__ double lds[];
+  double *ae = lds;                     // [max_e][6] element matrices
+  double *rl = ae + 6 * (size_t)max_e;  // [max_e][3] element load terms
+  double *vl = rl + 3 * (size_t)max_e;  // [AS_ROWS][wmax] row accumulators
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int row = blk * AS_ROWS + (int)threadIdx.x;  // phase 2: threads < AS_ROWS
+  const int e0 = blk_eptr[blk];
+  const int ne = blk_eptr[blk + 1] - e0;
+  // the row's pair words, fetched now so that their latency overlaps pass 1
+  int bi = -1, t0 = 0, t1 = 0;
+  uint32_t wb[PF];
+  if (threadIdx.x < AS_ROWS && row < n) {
+    bi = row_bd[row];
+    if (bi < 0) {
+      t0 = pair_ptr[row];
+      t1 = pair_ptr[row + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < PF; k++) wb[k] = t0 + k < t1 ? pairs[t0 + k] : 0u;
+  }
+  for (int k = threadIdx.x; k < AS_ROWS * wmax; k += AS_NT) vl[k] = 0.0;
+  // pass 1: elements (MASS: a mass term or u_prev is present), two per
+  // thread and step with every load of both issued before either computes
+  for (int t = threadIdx.x; t < ne; t += 2 * AS_NT) {
+    ElemIn<NQ> in[2];
+    const bool two = t + AS_NT < ne;
+    in[0].template load<MASS>(blk_rec[e0 + t], x, y, P);
+    if (two) in[1].template load<MASS>(blk_rec[e0 + t + AS_NT], x, y, P);
+    element<NQ, MASS>(in[0], rule, P, row_bd, ae + 6 * t, rl + 3 * t);
+    if (two) element<NQ, MASS>(in[1], rule, P, row_bd, ae + 6 * (t + AS_NT), rl + 3 * (t + AS_NT));
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < bnnz; k += AS_NT) val[base + k] = vl[k];
+  // pass 2: rows
+  if (threadIdx.x >= AS_ROWS || row >= npad) return;
+  double *my = vl + threadIdx.x * wmax;
+  double d = 1.0, acc = 0.0;  // padding rows: identity, rhs 0
+  if (row < n) {
+    if (bi >= 0) {
+      acc = P.bdval ? P.bdval[bi] : 0.0;  // (b,b,1.0) of ImposeDirichlet.h:46-50
+    } else {
+      d = 0.0;
+      auto add = [&](uint32_t w) {
+        const int p = (int)(w & 3u), le = (int)(w >> 20);
+        const double *a = ae + 6 * le;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          const uint32_t code = (w >> (2 + SLOT_BITS * q)) & 63u;
+          const double aq = a[sym3(p, q)];
+          if (code == PAIR_DIAG) d += aq;
+          else if (code != SLOT_LIFT) my[code] += aq;
+        }
+        acc += rl[3 * le + p];
+      };
+#pragma unroll
+      for (int k = 0; k < PF; k++)
+        if (t0 + k < t1) add(wb[k]);
+      for (int t = t0 + PF; t < t1; t++) add(pairs[t]);
+    }
+  }
+  const int s0 = slice_ptr[blk], width = (slice_ptr[blk + 1] - s0) >> 6;  // block = slice
+  const int off = s0 + (row & 63);
+  for (int k = 0; k < width; k++) {
+    const double v = my[k];
+    o.sv[off + 64 * k] = v;
+    o.sv32[off + 64 * k] = (float)v;
+  }
+  o.diag[row] = d;
+  o.dinv[row] = d != 0.0 ? 1.0 / d : 0.0;
+  o.diag32[row] = (float)d;
+  o.dinv32[row] = d != 0.0 ? (float)(1.0 / d) : 0.0f;
+  o.rhs[row] = acc;
 }
 
 // CSR (post-BC) -> SELL-64 off-diagonals + diagonal, FP64 and FP32 copies.
@@ -246,25 +327,36 @@ inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 }  // namespace
 
 void launch_assemble_to(const System &s, const double *x, const double *y,
-                        const admfem_problem &p, double *val, double *rhs,
-                        hipStream_t st) {
+                        const admfem_problem &p, const SellOut &o, hipStream_t st) {
   const Mesh &m = *s.mesh;
-  // the block with the most (row, element) pairs / CSR entries sizes the LDS
-  const int mi = std::max(s.asm_max_inc, 1), mn = std::max(s.asm_max_nnz, 1);
-  const size_t lds = (size_t)mi * (4 * sizeof(double) + sizeof(uint32_t)) +
-                     (size_t)mn * sizeof(double) + (AS_ROWS + 1) * sizeof(int);
-  REQUIRE(lds <= 160 * 1024, ADMFEM_ERR_UNSUPPORTED, "row block too dense for LDS");
-  assemble_kernel<<<nblk(s.n, AS_ROWS), AS_NT, lds, st>>>(
-      s.n, m.nelem, m.rule, m.conn.p, x, y, m.d_n2e_ptr.p, m.d_n2e_elem.p,
-      s.d_slots.p, s.d_rp.p, s.d_row_bd.p, p, val, rhs, mi, mn);
+  const int me = std::max(s.asm_max_e, 1), w = std::max(s.sell_wmax, 1);
+  const size_t lds = (size_t)me * 9 * sizeof(double) + (size_t)AS_ROWS * w * sizeof(double);
+  REQUIRE(lds <= 160 * 1024, ADMFEM_ERR_UNSUPPORTED, "assembly block too dense for LDS");
+  const unsigned grid = nblk(s.npad, AS_ROWS);
+  static_assert(AS_ROWS == 64, "an assembly block is one SELL slice");
+  const bool mass = p.mass_coef != 0.0 || p.u_prev;
+  auto go = [&](auto kern) {
+    kern<<<grid, AS_NT, lds, st>>>(s.n, s.npad, m.rule, x, y, s.d_blk_eptr.p, s.d_blk_rec.p,
+                                   m.d_n2e_ptr.p, s.d_pairs.p, s.d_row_bd.p, s.d_slice_ptr.p, p,
+                                   o, me, w);
+  };
+  if (m.rule.nq == 3) {
+    if (mass) go(assemble_kernel<3, true>);
+    else go(assemble_kernel<3, false>);
+  } else {
+    if (mass) go(assemble_kernel<1, true>);
+    else go(assemble_kernel<1, false>);
+  }
   HIPCHECK(hipGetLastError());
 }
 
-void launch_assemble(const System &s, const admfem_problem &p, hipStream_t st) {
-  launch_assemble_to(s, s.mesh->x.p, s.mesh->y.p, p, s.val.p, s.rhs.p, st);
+void launch_assemble(System &s, const admfem_problem &p, hipStream_t st) {
+  SellOut o{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p,
+            s.d_sell_val32.p, s.d_diag32.p, s.d_dinv32.p};
+  launch_assemble_to(s, s.mesh->x.p, s.mesh->y.p, p, o, st);
 }
 
-void launch_to_sell_to(const System &s, const double *val, double *sv, float *sv32,
+static void launch_to_sell_to(const System &s, const double *val, double *sv, float *sv32,
                        double *diag, double *dinv, float *diag32, float *dinv32,
                        hipStream_t st) {
   to_sell_kernel<<<nblk(s.npad, 256), 256, 0, st>>>(s.n, s.d_rp.p, s.d_col.p, val,

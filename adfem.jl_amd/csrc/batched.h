@@ -5,6 +5,7 @@
 namespace admfem {
 
 constexpr int BATCH_NT = 256;  // rows per block of the batched kernels
+constexpr int REPLACE_EVERY = 8;  // iterations between replacement points
 
 // One system of the batch: its matrix (topology arrays shared with every
 // system of the same mesh) and its vectors.  Node vectors are in the

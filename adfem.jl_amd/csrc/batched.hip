@@ -13,9 +13,13 @@
 //   bk_spmv   p = z + beta p_old, q = A p (FP32 storage, FP64 arithmetic)
 //   bs_alpha  one block per system: alpha
 //   bk_update xs += alpha p, r -= alpha q, z = D^-1 r
-//   bs_beta   beta, convergence / replacement decision
-//   bk_replace (systems flagged): x' = x + xs, r = b - A64 x' in FP64, xs' = 0
+//   bs_beta   beta; a converged-looking residual freezes the system
+// and every 8th iteration of a chunk (a replacement point):
+//   bk_replace (frozen systems, or residual fell by delta since the last
+//              replacement): x' = x + xs, r = b - A64 x' in FP64, xs' = 0
 //   bs_replace true-residual convergence test, swap the x / xs buffers
+// Systems start at chunk boundaries, so a system's replacement points fall
+// on the same iterations of its own count whatever else is in the batch.
 // No atomics, no fences (see cg.hip).
 #include <cmath>
 
@@ -162,7 +166,7 @@ __global__ void __launch_bounds__(NT)
   const int2 m = map[blockIdx.x];
   if (m.x < 0) return;
   const BScal &s = S[m.x];
-  if (s.status != ST_RUN) return;
+  if (s.status != ST_RUN || s.replace) return;
   const BSys &d = D[m.x];
   const int i = m.y * NT + threadIdx.x;
   const double beta = s.beta;
@@ -188,7 +192,7 @@ __global__ void __launch_bounds__(NT)
     bs_alpha(const int *__restrict__ list, const BSys *__restrict__ D, BScal *S) {
   const int sys = list[blockIdx.x];
   BScal &s = S[sys];
-  if (s.status != ST_RUN) return;
+  if (s.status != ST_RUN || s.replace) return;
   double pq, unused;
   sum2(D[sys].part, D[sys].nblk, pq, unused);
   if (threadIdx.x != 0) return;
@@ -204,7 +208,7 @@ __global__ void __launch_bounds__(NT)
   const int2 m = map[blockIdx.x];
   if (m.x < 0) return;
   const BScal &s = S[m.x];
-  if (s.status != ST_RUN) return;
+  if (s.status != ST_RUN || s.replace) return;
   const BSys &d = D[m.x];
   const int i = m.y * NT + threadIdx.x;
   const double alpha = s.alpha;
@@ -227,7 +231,7 @@ __global__ void __launch_bounds__(NT)
     bs_beta(const int *__restrict__ list, const BSys *__restrict__ D, BScal *S) {
   const int sys = list[blockIdx.x];
   BScal &s = S[sys];
-  if (s.status != ST_RUN) return;
+  if (s.status != ST_RUN || s.replace) return;
   double rz, rr;
   sum2(D[sys].part, D[sys].nblk, rz, rr);
   if (threadIdx.x != 0) return;
@@ -240,19 +244,26 @@ __global__ void __launch_bounds__(NT)
     s.status = ST_NAN;
     return;
   }
-  // replacement: when the residual looks converged, or (checked every 8th
-  // iteration of this system) when it fell by delta since the last one
-  if (rr <= s.tol2 || ((s.it & 7) == 0 && rr <= s.delta2 * s.rr_ref)) s.replace = 1;
+  // a residual that looks converged freezes the system until the next
+  // replacement point, where the FP64 residual decides
+  if (rr <= s.tol2) s.replace = 1;
   else if (s.it >= s.maxit) s.status = ST_MAXIT;
 }
 
-// flagged systems: x' = x + xs into the other buffer, r = b - A64 x', xs' = 0
+// A system is replaced at a replacement point when it is frozen (converged
+// looking) or its residual fell by delta since the last replacement.  Every
+// kernel of the point reads the same, unchanged scalars: one decision.
+__device__ __forceinline__ bool replace_due(const BScal &s) {
+  return s.status == ST_RUN && (s.replace || s.rr <= s.delta2 * s.rr_ref);
+}
+
+// due systems: x' = x + xs into the other buffer, r = b - A64 x', xs' = 0
 __global__ void __launch_bounds__(NT)
     bk_replace(const int2 *__restrict__ map, const BSys *__restrict__ D, const BScal *S) {
   const int2 m = map[blockIdx.x];
   if (m.x < 0) return;
   const BScal &s = S[m.x];
-  if (s.status != ST_RUN || !s.replace) return;
+  if (!replace_due(s)) return;
   const BSys &d = D[m.x];
   const int i = m.y * NT + threadIdx.x;
   const double *__restrict__ x = d.x[s.sel];
@@ -280,9 +291,10 @@ __global__ void __launch_bounds__(NT)
     bs_replace(const int *__restrict__ list, const BSys *__restrict__ D, BScal *S) {
   const int sys = list[blockIdx.x];
   BScal &s = S[sys];
-  if (s.status != ST_RUN || !s.replace) return;
+  if (!replace_due(s)) return;
   double rz, rr;
   sum2(D[sys].part, D[sys].nblk, rz, rr);
+  __syncthreads();  // every thread has read the decision's inputs
   if (threadIdx.x != 0) return;
   s.rz = rz;
   s.beta = rz / s.rz_prev;
@@ -399,13 +411,16 @@ void batch_init(const BSys *D, BScal *S, int sys, int nblk, double rtol2, double
 
 void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, const BSys *D,
                       BScal *S, int iters, hipStream_t st) {
+  REQUIRE(iters % REPLACE_EVERY == 0, ADMFEM_ERR_ARG, "chunk must be a multiple of 8");
   for (int k = 0; k < iters; k++) {
     bk_spmv<<<nmap, NT, 0, st>>>(map, D, S);
     bs_alpha<<<nlist, NT, 0, st>>>(list, D, S);
     bk_update<<<nmap, NT, 0, st>>>(map, D, S);
     bs_beta<<<nlist, NT, 0, st>>>(list, D, S);
-    bk_replace<<<nmap, NT, 0, st>>>(map, D, S);
-    bs_replace<<<nlist, NT, 0, st>>>(list, D, S);
+    if (k % REPLACE_EVERY == REPLACE_EVERY - 1) {  // replacement point
+      bk_replace<<<nmap, NT, 0, st>>>(map, D, S);
+      bs_replace<<<nlist, NT, 0, st>>>(list, D, S);
+    }
   }
   HIPCHECK(hipGetLastError());
 }

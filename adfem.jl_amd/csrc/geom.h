@@ -54,6 +54,14 @@ __device__ __forceinline__ double gauss_weight(double ipw, double area) {
   return ipw * area / 0.5;
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (block b to XCD b % 8).
+// This bijection gives each XCD one contiguous range of logical blocks, so
+// data shared by neighbouring blocks is reused in that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int per = nb >> 3, rem = nb & 7, x = b & 7;
+  return x * per + min(x, rem) + (b >> 3);
+}
+
 // Wavefront / block reductions (deterministic: fixed butterfly order).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

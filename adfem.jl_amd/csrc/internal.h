@@ -68,7 +68,15 @@ __host__ __device__ inline uint32_t slot_of(uint32_t w, int q) {
   return (w >> (2 + SLOT_BITS * q)) & 63u;
 }
 
-constexpr int AS_ROWS = 128;  // rows per assembly block
+// Assembly pair word of a (row, incident element) pair of a post-BC system:
+// bits 0-1 the row node's local vertex p, then three 6-bit codes for the
+// element's vertices q = 0,1,2 -- the index of column conn[e][q] among the
+// row's off-diagonal entries (its SELL position), PAIR_DIAG for the row
+// itself or SLOT_LIFT for a boundary column -- and bits 20-31 the element's
+// index in its assembly block's element list.
+constexpr uint32_t PAIR_DIAG = 62;
+constexpr int PAIR_MAX_ELEMS = 4096;
+constexpr int AS_ROWS = 64;  // rows per assembly block (one SELL slice)
 constexpr int MAXQ = 3;  // quadrature points per element carried here
 struct QuadRule {
   int nq;
@@ -121,8 +129,15 @@ struct System {
   std::vector<int> rp, col;  // post-BC pattern, internal numbering
   // device: pattern + assembly maps
   DevBuf<int> d_rp, d_col, d_row_bd;
-  DevBuf<uint32_t> d_slots;  // per incidence, slots in the post-BC row
-  // assembled FP64 values / rhs (internal order)
+  // assembly: per block of AS_ROWS rows the distinct elements incident to
+  // its interior rows (ascending) as {element, 3 internal vertex ids}, and a
+  // pair word per incidence (row-major, parallel to the mesh's n2e lists)
+  DevBuf<int> d_blk_eptr;
+  DevBuf<int4> d_blk_rec;
+  DevBuf<uint32_t> d_pairs;
+  int asm_max_e = 0;   // most elements of one block
+  int sell_wmax = 0;   // widest SELL slice
+  // CSR values (general sparse systems only) and the right-hand side
   DevBuf<double> val, rhs;
   // solver format: SELL-64 off-diagonals + separate diagonal
   int nslices = 0;
@@ -133,9 +148,6 @@ struct System {
   DevBuf<float> d_sell_val32;  // FP32 copy for the mixed-precision solver
   DevBuf<double> d_diag, d_dinv;
   DevBuf<float> d_diag32, d_dinv32;
-  int max_block_nnz = 0;  // largest CSR range of a 256-row block
-  int asm_max_inc = 0;    // assembly: most (row, element) pairs in a block
-  int asm_max_nnz = 0;    // assembly: largest CSR range of a block
   CGWork *work = nullptr;
   size_t device_bytes() const;
 };
@@ -159,14 +171,17 @@ void launch_source(const Mesh &m, const double *f, double *rhs_user,
 void launch_source_grad(const Mesh &m, const double *grhs_user, double *gf,
                         hipStream_t st);
 // assemble.hip
-void launch_assemble(const System &s, const admfem_problem &p, hipStream_t st);
+// Fused assembly of a mesh system straight into the solver format: SELL
+// values (FP64 and FP32), diagonal and its inverse (both precisions), rhs.
+struct SellOut {
+  double *sv, *diag, *dinv, *rhs;
+  float *sv32, *diag32, *dinv32;
+};
+void launch_assemble(System &s, const admfem_problem &p, hipStream_t st);
 void launch_assemble_to(const System &s, const double *x, const double *y,
-                        const admfem_problem &p, double *val, double *rhs,
-                        hipStream_t st);
+                        const admfem_problem &p, const SellOut &o, hipStream_t st);
+// CSR values (s.val, general sparse systems) -> solver format
 void launch_to_sell(System &s, hipStream_t st);
-void launch_to_sell_to(const System &s, const double *val, double *sv, float *sv32,
-                       double *diag, double *dinv, float *diag32, float *dinv32,
-                       hipStream_t st);
 void launch_kappa_gradient(const System &s, const double *u_int,
                            const double *lam_int, double scale, double *gk,
                            hipStream_t st);

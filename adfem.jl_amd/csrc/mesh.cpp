@@ -126,6 +126,8 @@ static std::vector<int> rcm_order(int n, const std::vector<int> &rp,
   return order;
 }
 
+
+
 Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
                  int order) {
   REQUIRE(nnode > 0 && nelem > 0, ADMFEM_ERR_ARG, "empty mesh");
@@ -186,7 +188,10 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
       ucol.insert(ucol.end(), cand.begin(), cand.end());
       urp[i + 1] = (int)ucol.size();
     }
-    // internal numbering: reverse Cuthill-McKee over the node graph
+    // internal numbering: reverse Cuthill-McKee over the node graph.  Rows
+    // then run along BFS levels: the k-th neighbours of consecutive rows are
+    // consecutive nodes of the adjacent levels, so the solver's column
+    // gathers coalesce within a wavefront.
     {
       std::vector<int> arp(nnode + 1, 0), acol;
       acol.reserve(ucol.size());
@@ -316,8 +321,9 @@ void mesh_edges(Mesh &m) {
 }
 
 size_t System::device_bytes() const {
-  return d_rp.bytes() + d_col.bytes() + d_row_bd.bytes() + d_slots.bytes() +
-         val.bytes() + rhs.bytes() + d_slice_ptr.bytes() + d_sell_col.bytes() +
+  return d_rp.bytes() + d_col.bytes() + d_row_bd.bytes() + d_blk_eptr.bytes() +
+         d_blk_rec.bytes() + d_pairs.bytes() + val.bytes() + rhs.bytes() +
+         d_slice_ptr.bytes() + d_sell_col.bytes() +
          d_sell_val.bytes() + d_sell_val32.bytes() + d_diag.bytes() +
          d_dinv.bytes() + d_diag32.bytes() + d_dinv32.bytes();
 }
@@ -343,6 +349,7 @@ static void build_solver_layout(System *s) {
       wmax = std::max(wmax, offdiag(r));
     }
     s->slice_ptr[sl + 1] = s->slice_ptr[sl] + 64 * wmax;
+    s->sell_wmax = std::max(s->sell_wmax, wmax);
   }
   std::vector<int> sell_col(std::max(1, s->slice_ptr[s->nslices]));
   for (int sl = 0; sl < s->nslices; sl++) {
@@ -359,7 +366,7 @@ static void build_solver_layout(System *s) {
   }
   s->d_rp.upload(s->rp);
   s->d_col.upload(s->col);
-  s->val.alloc(std::max<size_t>(1, s->col.size()));
+  if (!s->mesh) s->val.alloc(std::max<size_t>(1, s->col.size()));
   s->rhs.alloc(s->npad);
   HIPCHECK(hipMemset(s->rhs.p, 0, s->rhs.bytes()));
   s->d_slice_ptr.upload(s->slice_ptr);
@@ -405,38 +412,57 @@ System *system_build(Mesh *m, const long long *bdnode, int nbd) {
       }
       s->rp[r + 1] = (int)s->col.size();
     }
-    // slot map of the post-BC rows
-    std::vector<uint32_t> slots(m->n2e_elem.size(), 0);
-    for (int r = 0; r < n; r++) {
-      if (s->row_bd[r] >= 0) continue;
-      const int *c = &s->col[s->rp[r]];
-      int len = s->rp[r + 1] - s->rp[r];
-      for (int t = m->n2e_ptr[r]; t < m->n2e_ptr[r + 1]; t++) {
-        int e = m->n2e_elem[t];
-        uint32_t w = m->n2e_p[t];
-        for (int q = 0; q < 3; q++) {
-          int cq = m->perm[m->elems[3 * (size_t)e + q]];
-          uint32_t sl;
-          if (s->row_bd[cq] >= 0) {
-            sl = SLOT_LIFT;
-          } else {
-            sl = (uint32_t)(std::lower_bound(c, c + len, cq) - c);
+    build_solver_layout(s);
+    // assembly blocks: elements incident to the block's interior rows, and
+    // one pair word per (row, element) incidence
+    const int nblk = (s->npad + AS_ROWS - 1) / AS_ROWS;
+    std::vector<int> beptr(nblk + 1, 0), belem;
+    std::vector<uint32_t> pairs(m->n2e_elem.size(), 0);
+    std::vector<int> offd;
+    for (int b = 0; b < nblk; b++) {
+      const int r0 = b * AS_ROWS, r1 = std::min(r0 + AS_ROWS, n);
+      const size_t first = belem.size();
+      for (int r = r0; r < r1; r++)
+        if (s->row_bd[r] < 0)
+          for (int t = m->n2e_ptr[r]; t < m->n2e_ptr[r + 1]; t++) belem.push_back(m->n2e_elem[t]);
+      std::sort(belem.begin() + first, belem.end());
+      belem.erase(std::unique(belem.begin() + first, belem.end()), belem.end());
+      const int ne = (int)(belem.size() - first);
+      REQUIRE(ne <= PAIR_MAX_ELEMS, ADMFEM_ERR_UNSUPPORTED, "assembly block has too many elements");
+      s->asm_max_e = std::max(s->asm_max_e, ne);
+      beptr[b + 1] = (int)belem.size();
+      for (int r = r0; r < r1; r++) {
+        if (s->row_bd[r] >= 0) continue;
+        offd.clear();
+        for (int k = s->rp[r]; k < s->rp[r + 1]; k++)
+          if (s->col[k] != r) offd.push_back(s->col[k]);
+        for (int t = m->n2e_ptr[r]; t < m->n2e_ptr[r + 1]; t++) {
+          const int e = m->n2e_elem[t];
+          const uint32_t lei = (uint32_t)(std::lower_bound(belem.begin() + first, belem.end(), e) -
+                                          (belem.begin() + first));
+          uint32_t w = m->n2e_p[t] | (lei << 20);
+          for (int q = 0; q < 3; q++) {
+            const int cq = m->perm[m->elems[3 * (size_t)e + q]];
+            uint32_t code;
+            if (s->row_bd[cq] >= 0) code = SLOT_LIFT;
+            else if (cq == r) code = PAIR_DIAG;
+            else code = (uint32_t)(std::lower_bound(offd.begin(), offd.end(), cq) - offd.begin());
+            w |= code << (2 + SLOT_BITS * q);
           }
-          w |= sl << (2 + SLOT_BITS * q);
+          pairs[t] = w;
         }
-        slots[t] = w;
       }
     }
-    build_solver_layout(s);
-    s->d_row_bd.upload(s->row_bd);
-    s->d_slots.upload(slots);
-    for (int r0 = 0; r0 < n; r0 += 256)
-      s->max_block_nnz = std::max(s->max_block_nnz, s->rp[std::min(r0 + 256, n)] - s->rp[r0]);
-    for (int r0 = 0; r0 < n; r0 += AS_ROWS) {
-      const int r1 = std::min(r0 + AS_ROWS, n);
-      s->asm_max_inc = std::max(s->asm_max_inc, m->n2e_ptr[r1] - m->n2e_ptr[r0]);
-      s->asm_max_nnz = std::max(s->asm_max_nnz, s->rp[r1] - s->rp[r0]);
+    REQUIRE(s->sell_wmax < (int)PAIR_DIAG, ADMFEM_ERR_UNSUPPORTED, "row wider than 61");
+    std::vector<int4> brec(std::max<size_t>(1, belem.size()), make_int4(0, 0, 0, 0));
+    for (size_t k = 0; k < belem.size(); k++) {
+      const int *v = &m->elems[3 * (size_t)belem[k]];
+      brec[k] = make_int4(belem[k], m->perm[v[0]], m->perm[v[1]], m->perm[v[2]]);
     }
+    s->d_row_bd.upload(s->row_bd);
+    s->d_blk_eptr.upload(beptr);
+    s->d_blk_rec.upload(brec);
+    s->d_pairs.upload(pairs.empty() ? std::vector<uint32_t>(1, 0) : pairs);
   } catch (...) {
     system_free(s);
     throw;

@@ -89,7 +89,7 @@ struct Req {
   int slot = -1;
   size_t off = 0;
   // device buffers (inside the arena)
-  double *x, *y, *kappa, *f, *uprev, *val, *b, *sv64, *diag64, *dinv64, *xa, *xb, *u64, *lam64,
+  double *x, *y, *kappa, *f, *uprev, *b, *sv64, *diag64, *dinv64, *xa, *xb, *u64, *lam64,
       *grad, *obsv, *loss2, *part, *stage;
   float *sv32, *diag32, *dinv32, *xsa, *xsb, *r32, *z32, *pa, *pb, *q32;
   int *obsi;
@@ -143,8 +143,7 @@ struct Server {
   // ------------------------------------------------------------ layout
   static size_t layout(const System &s, Req *q, char *base, int nobs) {
     const Mesh &m = *s.mesh;
-    const size_t np = s.npad, ng = m.ngauss, nz = s.col.size(),
-                 ns = std::max<size_t>(1, s.slice_ptr.back());
+    const size_t np = s.npad, ng = m.ngauss, ns = std::max<size_t>(1, s.slice_ptr.back());
     const size_t nblk = (np + BATCH_NT - 1) / BATCH_NT;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -154,16 +153,14 @@ struct Server {
     };
     auto D = [&](size_t n) { return (double *)take(n * sizeof(double)); };
     auto F = [&](size_t n) { return (float *)take(n * sizeof(float)); };
-    double *x = D(np), *y = D(np), *kappa = D(ng), *f = D(ng), *uprev = D(np), *val = D(nz),
-           *b = D(np), *sv64 = D(ns), *diag64 = D(np), *dinv64 = D(np), *xa = D(np),
+    double *x = D(np), *y = D(np), *kappa = D(ng), *f = D(ng), *uprev = D(np), *b = D(np), *sv64 = D(ns), *diag64 = D(np), *dinv64 = D(np), *xa = D(np),
            *xb = D(np), *u64 = D(np), *lam64 = D(np), *grad = D(ng), *obsv = D(nobs + 1),
            *loss2 = D(2), *part = D(4 * nblk), *stage = D(2 * np);
     float *sv32 = F(ns), *diag32 = F(np), *dinv32 = F(np), *xsa = F(np), *xsb = F(np),
           *r32 = F(np), *z32 = F(np), *pa = F(np), *pb = F(np), *q32 = F(np);
     int *obsi = (int *)take((nobs + 1) * sizeof(int));
     if (q) {
-      q->x = x; q->y = y; q->kappa = kappa; q->f = f; q->uprev = uprev; q->val = val;
-      q->b = b; q->sv64 = sv64; q->diag64 = diag64; q->dinv64 = dinv64; q->xa = xa;
+      q->x = x; q->y = y; q->kappa = kappa; q->f = f; q->uprev = uprev; q->b = b; q->sv64 = sv64; q->diag64 = diag64; q->dinv64 = dinv64; q->xa = xa;
       q->xb = xb; q->u64 = u64; q->lam64 = lam64; q->grad = grad; q->obsv = obsv;
       q->loss2 = loss2; q->part = part; q->stage = stage; q->sv32 = sv32; q->diag32 = diag32;
       q->dinv32 = dinv32; q->xsa = xsa; q->xsb = xsb; q->r32 = r32; q->z32 = z32;
@@ -291,8 +288,8 @@ struct Server {
     const System &s = *topos[q.r.topo].sys;
     inputs(q);
     admfem_problem p = problem(q);
-    launch_assemble_to(s, q.x, q.y, p, q.val, q.b, st);
-    launch_to_sell_to(s, q.val, q.sv64, q.sv32, q.diag64, q.dinv64, q.diag32, q.dinv32, st);
+    launch_assemble_to(s, q.x, q.y, p,
+                       SellOut{q.sv64, q.diag64, q.dinv64, q.b, q.sv32, q.diag32, q.dinv32}, st);
   }
 
   void start_cg(Req &q) {
@@ -436,7 +433,8 @@ struct Server {
   }
 
   void run_batched(double timeout) {
-    const int chunk = o.chunk > 0 ? o.chunk : 16;
+    const int chunk = o.chunk > 0 ? (o.chunk + REPLACE_EVERY - 1) / REPLACE_EVERY * REPLACE_EVERY
+                                  : 2 * REPLACE_EVERY;
     std::vector<int> order(reqs.size());
     for (size_t i = 0; i < reqs.size(); i++) order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -557,8 +555,10 @@ struct Server {
       // FP64 single-system path on the topology's own solver buffers
       inputs(q);
       admfem_problem p = problem(q);
-      launch_assemble_to(s, q.x, q.y, p, s.val.p, s.rhs.p, st);
-      launch_to_sell(s, st);
+      launch_assemble_to(s, q.x, q.y, p,
+                         SellOut{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p,
+                                 s.d_sell_val32.p, s.d_diag32.p, s.d_dinv32.p},
+                         st);
       admfem_solve_opts so{ADMFEM_PREC_FP64, o.max_iter, o.rtol, 0, 0.0};
       admfem_solve_stats s0{}, s1{};
       cg_solve(s, s.rhs.p, q.u64, so, s0, st);

@@ -312,7 +312,8 @@ typedef struct {
   double rtol;             /* true relative residual of every solve */
   int max_iter;
   double replace_delta;    /* mixed: residual replacement factor (0: 0.1) */
-  int chunk;               /* CG iterations between host checks (0: 16) */
+  int chunk;               /* CG iterations between host checks (0: 16;
+                              rounded up to a multiple of 8) */
   int use_graph;           /* replay each chunk as a captured HIP graph */
 } admfem_server_opts;
 
