@@ -207,6 +207,33 @@ def roof(nbytes, avg_s, kernel, **extra):
     return d
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "bk_spmv_stream_first1024.md")
+
+
+def measured_traffic(nsys):
+    """HBM bytes per bk_spmv launch over the stream's first 1024 requests,
+    from the committed PMC summary (FETCH_SIZE + WRITE_SIZE, each scaled by
+    a copy of known size); None for another batch or without the file."""
+    if nsys != 1024 or not os.path.exists(PMC_SUMMARY):
+        return None
+    for ln in open(PMC_SUMMARY):
+        if ln.startswith("traffic_bytes_per_launch"):
+            return int(ln.split("=")[1])
+    return None
+
+
+def copy_rate(mib=1024):
+    """GB/s of a streaming elementwise kernel (read + write of ``mib`` MiB,
+    torch's vectorised elementwise kernel), HIP events: the bandwidth a
+    streaming kernel reaches on this GPU."""
+    import torch
+    src = torch.ones(mib << 17, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    avg = hip_time(lambda: [torch.mul(src, 1.0, out=dst) for _ in range(5)], 5)
+    return round(2 * src.numel() * 8 / avg / 1e9, 1)
+
+
 def batched_spmv_roofline(systems, sides, nsys, repeat=200):
     """bk_spmv over the stream's first nsys requests.  The library records
     the HIP events on torch's current stream around the kernel launches only
@@ -218,9 +245,15 @@ def batched_spmv_roofline(systems, sides, nsys, repeat=200):
     sv.spmv_repeat(topos, 5)
     nbytes, avg = sv.spmv_repeat(topos, repeat)
     sv.close()
-    return roof(nbytes, avg, "bk_spmv", batch=f"the stream's first {nsys} requests, one "
-                "Poisson system each, one launch over all of them",
-                format="SELL-64, FP32 values, FP64 arithmetic, fused p = z + beta p_old")
+    r = roof(nbytes, avg, "bk_spmv", batch=f"the stream's first {nsys} requests, one "
+             "Poisson system each, one launch over all of them",
+             format="SELL-64, FP32 values, FP64 arithmetic, fused p = z + beta p_old")
+    t = measured_traffic(nsys)
+    if t is not None:
+        r["traffic"] = t
+        r["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy " \
+                              "(profiles/bk_spmv_stream_first1024.md)"
+    return r
 
 
 # ------------------------------------------------------------ precision form
@@ -276,10 +309,11 @@ def precision_form(m, rtol):
     b_asm = ng * 8 + 12 * ne + 16 * N + nnz * 8
     b_fused = b_asm + ng * 8 + N * 8
     avg = hip_time(lambda: [S.assemble(kappa=kappa, f=f) for _ in range(20)], 20)
-    rl["assembly"] = roof(b_fused, avg, "assemble_kernel + to_sell_kernel",
-                          bytes_note="kappa, f, connectivity, coordinates read; CSR values "
-                                     "and rhs written (the kernel computes K and F)",
-                          survey_bytes=int(b_asm))
+    rl["assembly"] = roof(b_fused, avg, "assemble_kernel (fused K, F, Dirichlet -> SELL)",
+                          bytes_note="kappa, f, connectivity, coordinates read; values (once, "
+                                     "FP64) and rhs written: the kernel computes K and F",
+                          survey_bytes=int(b_asm),
+                          survey_frac=round(b_asm / avg / 1e9 / HBM_PEAK_GBS, 4))
     return dict(workload=f"{m}x{m}-quad jittered node-permuted mesh ({ne} triangles, {N} "
                          f"nodes, nnz {nnz}), Poisson, kappa=exp(GRF), f~U[-1,1], 1% observed "
                          f"nodes, rtol {rtol:g}; errors vs FP64 CG to 1e-13",
@@ -330,6 +364,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
                     help="set up the stream and time only the batched SpMV kernel")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="with --roofline-only: first copy a MIB-MiB buffer once (a plain "
+                         "copy kernel of known size for scaling memory counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}): this build is "
@@ -345,6 +382,14 @@ def main():
     log(f"{len(systems)} topologies in {setup_s:.1f}s")
     nroof = min(a.roofline_systems, n)
     if a.roofline_only:
+        if a.calibrate_copy > 0:
+            src = torch.ones(a.calibrate_copy << 17, dtype=torch.float64, device="cuda")
+            dst = torch.empty_like(src)
+            torch.cuda.synchronize()
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            log(f"calibration copy: {a.calibrate_copy} MiB read, {a.calibrate_copy} MiB written")
+            del src, dst
         print(json.dumps(batched_spmv_roofline(systems, sides, nroof)), flush=True)
         return
     idx = np.arange(n)
@@ -377,6 +422,7 @@ def main():
     acc = accuracy(systems, sides, heat, dts, a.rtol, a.accuracy_requests, 256)
     log(f"accuracy: u {acc['max_rel_err_u']:.2e} grad {acc['max_rel_err_grad']:.2e}")
     rl = batched_spmv_roofline(systems, sides, nroof)
+    rl["copy_GBs"] = copy_rate()
     line = {
         "metric": "forward+adjoint solves/s at 1e-8 rel err; p50/p99 latency vs plain FP64 GPU",
         "value": round(value, 3),
