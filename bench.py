@@ -257,7 +257,7 @@ def batched_spmv_roofline(systems, sides, nsys, repeat=200):
 
 
 # ------------------------------------------------------------ precision form
-def precision_form(m, rtol):
+def precision_form(m, rtol, kernels_only=False):
     import ctypes
     import torch
     import admfem
@@ -276,13 +276,14 @@ def precision_form(m, rtol):
         interior, size=mesh.nnode // 100, replace=False)).astype(np.int32), device="cuda")
     S = admfem.DirichletSystem(mesh, bd)
     S.assemble(kappa=kappa2, f=f)
-    u2, _ = S.solve(precision="fp64", rtol=1e-12)
-    uobs = u2[obs.long()].contiguous()
+    res = {}
+    if not kernels_only:
+        u2, _ = S.solve(precision="fp64", rtol=1e-12)
+        uobs = u2[obs.long()].contiguous()
+        ref = S.forward_adjoint(obs, uobs, precision="fp64", rtol=1e-13, kappa=kappa, f=f)
     setup = time.perf_counter() - t0
     log(f"precision form m={m}: setup {setup:.1f}s")
-    ref = S.forward_adjoint(obs, uobs, precision="fp64", rtol=1e-13, kappa=kappa, f=f)
-    res = {}
-    for prec in ("mixed", "fp64"):
+    for prec in () if kernels_only else ("mixed", "fp64"):
         S.forward_adjoint(obs, uobs, precision=prec, rtol=rtol, kappa=kappa, f=f, want_lam=False)
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -314,12 +315,15 @@ def precision_form(m, rtol):
                                      "FP64) and rhs written: the kernel computes K and F",
                           survey_bytes=int(b_asm),
                           survey_frac=round(b_asm / avg / 1e9 / HBM_PEAK_GBS, 4))
-    return dict(workload=f"{m}x{m}-quad jittered node-permuted mesh ({ne} triangles, {N} "
-                         f"nodes, nnz {nnz}), Poisson, kappa=exp(GRF), f~U[-1,1], 1% observed "
-                         f"nodes, rtol {rtol:g}; errors vs FP64 CG to 1e-13",
-                setup_s=round(setup, 1), mixed=res["mixed"], fp64_plain=res["fp64"],
-                speedup_mixed_vs_fp64=round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3),
-                roofline=rl)
+    out = dict(workload=f"{m}x{m}-quad jittered node-permuted mesh ({ne} triangles, {N} "
+                        f"nodes, nnz {nnz}), Poisson, kappa=exp(GRF), f~U[-1,1], 1% observed "
+                        f"nodes, rtol {rtol:g}; errors vs FP64 CG to 1e-13",
+               setup_s=round(setup, 1), roofline=rl)
+    if res:
+        out.update(mixed=res["mixed"], fp64_plain=res["fp64"],
+                   speedup_mixed_vs_fp64=round(res["fp64"]["seconds"] /
+                                               res["mixed"]["seconds"], 3))
+    return out
 
 
 # -------------------------------------------------------------- CPU baseline
@@ -364,9 +368,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
                     help="set up the stream and time only the batched SpMV kernel")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="run only the single-system precision form (for profiling)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="with --precision-only: time the SpMV and assembly kernels, no solves")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="with --roofline-only: first copy a MIB-MiB buffer once (a plain "
-                         "copy kernel of known size for scaling memory counters)")
+                    help="with --roofline-only or --precision-only: first copy a MIB-MiB "
+                         "buffer once (a plain copy kernel of known size for scaling memory "
+                         "counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}): this build is "
@@ -374,6 +383,18 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "no HIP device"
+    if a.calibrate_copy > 0 and (a.roofline_only or a.precision_only):
+        src = torch.ones(a.calibrate_copy << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {a.calibrate_copy} MiB read, {a.calibrate_copy} MiB written")
+        del src, dst
+    if a.precision_only:
+        print(json.dumps(precision_form(a.precision_m, a.precision_rtol, a.kernels_only)),
+              flush=True)
+        return
     n = a.requests
     sides, heat, dts = make_stream(n)
     t0 = time.perf_counter()
@@ -382,14 +403,6 @@ def main():
     log(f"{len(systems)} topologies in {setup_s:.1f}s")
     nroof = min(a.roofline_systems, n)
     if a.roofline_only:
-        if a.calibrate_copy > 0:
-            src = torch.ones(a.calibrate_copy << 17, dtype=torch.float64, device="cuda")
-            dst = torch.empty_like(src)
-            torch.cuda.synchronize()
-            dst.copy_(src)
-            torch.cuda.synchronize()
-            log(f"calibration copy: {a.calibrate_copy} MiB read, {a.calibrate_copy} MiB written")
-            del src, dst
         print(json.dumps(batched_spmv_roofline(systems, sides, nroof)), flush=True)
         return
     idx = np.arange(n)
