@@ -132,3 +132,22 @@ def test_sparse_solve_op_and_gradient_general_spd():
     assert rel(u.detach().cpu().numpy(), u0) < 1e-10
     assert rel(bt.grad.cpu().numpy(), lam) < 1e-10
     assert rel(vv.grad.cpu().numpy(), -lam[I] * u0[J]) < 1e-10
+
+
+def test_sparse_solve_far_coupling():
+    """A chain with one coupling 69,999 rows away (a wide SELL gather, in the
+    caller's numbering since a general sparse system keeps it) against
+    SciPy."""
+    n = 70000
+    main = np.full(n, 4.0)
+    off = np.full(n - 1, -1.0)
+    A = sp.diags([off, main, off], [-1, 0, 1], format="coo")
+    I = np.concatenate([A.row, [0, n - 1]])
+    J = np.concatenate([A.col, [n - 1, 0]])
+    V = np.concatenate([A.data, [-0.5, -0.5]])
+    b = np.random.default_rng(3).standard_normal(n)
+    ind = torch.as_tensor(np.stack([I, J], 1), device="cuda")
+    u = admfem.solve(admfem.SparseTensor(ind, torch.tensor(V, device="cuda"), n),
+                     torch.tensor(b, device="cuda"), rtol=1e-13)
+    ref = spl.spsolve(sp.csr_matrix((V, (I, J)), shape=(n, n)), b)
+    assert np.abs(u.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-10
