@@ -11,4 +11,6 @@ from .mesh import (Mesh, bcedge, bcnode, fem_nodes, gauss_nodes, gauss_weights, 
 from .system import DirichletSystem, solve_opts  # noqa: F401
 from . import meshgen  # noqa: F401
 from .ops import (SparseTensor, compute_fem_laplace_matrix1, compute_fem_mass_matrix1,  # noqa: F401,E402
-                  compute_fem_source_term1, impose_Dirichlet_boundary_conditions, solve)
+                  compute_fem_source_term1, impose_Dirichlet_boundary_conditions, solve,
+                  fem_to_gauss_points, dof_to_gauss_points, eval_grad_on_gauss_pts1,
+                  compute_fem_laplace_term1)

@@ -66,6 +66,9 @@ _SIGS = {
     "mfem_get_element_to_vertices": (None, [c_vp]),
     "FemLaplaceScalar_forward_Julia": (None, [c_vp, c_vp, c_vp]),
     "FemSourceScalar_forward_Julia": (None, [c_vp, c_vp]),
+    "FemToGaussPointsMfem_Julia": (None, [c_vp, c_vp]),
+    "DofToGaussPointsMfem_forward_Julia": (None, [c_vp, c_vp]),
+    "ComputeLaplaceTermMfem_forward_Julia": (None, [c_vp, c_vp, c_vp]),
     "admfem_default_mesh": (c_vp, []),
     # B. meshes
     "admfem_mesh_create": (c_int, [c_vp, c_int, c_vp, c_int, c_int, c_int, c_vp]),
@@ -85,6 +88,14 @@ _SIGS = {
     "admfem_fem_mass_matrix_grad": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "admfem_fem_source_scalar": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "admfem_fem_source_scalar_grad": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "admfem_fem_to_gauss_points": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "admfem_fem_to_gauss_points_grad": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "admfem_dof_to_gauss_points": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "admfem_dof_to_gauss_points_grad": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "admfem_fem_grad": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "admfem_fem_grad_grad": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "admfem_fem_laplace_term": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "admfem_fem_laplace_term_grad": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "admfem_impose_dirichlet": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp, c_int,
                                         c_vp, c_vp, c_vp, c_vp, c_vp]),
     "admfem_impose_dirichlet_grad": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_int,

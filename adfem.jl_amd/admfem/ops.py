@@ -153,6 +153,111 @@ def compute_fem_source_term1(f, mesh):
     return _Source.apply(f, mesh)
 
 
+class _ToGauss(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u, mesh, fwd, bwd):
+        uu = _dev(u)
+        out = torch.empty(mesh.ngauss, dtype=torch.float64, device=_DEV)
+        _lib.check(fwd(mesh.handle, _lib.ptr(uu), _lib.ptr(out), _lib.stream_ptr()))
+        ctx.mesh, ctx.bwd = mesh, bwd
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        g = _dev(g)
+        gu = torch.empty(ctx.mesh.ndof, dtype=torch.float64, device=_DEV)
+        _lib.check(ctx.bwd(ctx.mesh.handle, _lib.ptr(g), _lib.ptr(gu), _lib.stream_ptr()))
+        return gu, None, None, None
+
+
+def _nodal(u, mesh):
+    if len(u) != mesh.ndof:
+        raise ValueError("length(u) must be mesh.ndof")
+    return isinstance(u, np.ndarray)
+
+
+def fem_to_gauss_points(u, mesh):
+    """FemToGaussPointsMfem (deps/MFEM/FemToGaussPoints/FemToGaussPointsMfem.h:
+    6-33; src/MFEM/MUtils.jl:231-250): nodal values -> Gauss points."""
+    L = _lib.lib()
+    host = _nodal(u, mesh)
+    out = _ToGauss.apply(torch.as_tensor(u) if host else u, mesh,
+                         L.admfem_fem_to_gauss_points, L.admfem_fem_to_gauss_points_grad)
+    return out.cpu().numpy() if host else out
+
+
+def dof_to_gauss_points(u, mesh):
+    """DofToGaussPointsMfem (deps/MFEM/DofToGaussPoints/DofToGaussPointsMfem.h:
+    6-32; src/MFEM/MUtils.jl:253-270): for P1 the dofs are the nodes."""
+    L = _lib.lib()
+    host = _nodal(u, mesh)
+    out = _ToGauss.apply(torch.as_tensor(u) if host else u, mesh,
+                         L.admfem_dof_to_gauss_points, L.admfem_dof_to_gauss_points_grad)
+    return out.cpu().numpy() if host else out
+
+
+class _FemGrad(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u, mesh):
+        uu = _dev(u)
+        out = torch.empty((mesh.ngauss, 2), dtype=torch.float64, device=_DEV)
+        _lib.check(_lib.lib().admfem_fem_grad(mesh.handle, _lib.ptr(uu), _lib.ptr(out),
+                                              _lib.stream_ptr()))
+        ctx.mesh = mesh
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        g = _dev(g)
+        gu = torch.empty(ctx.mesh.ndof, dtype=torch.float64, device=_DEV)
+        _lib.check(_lib.lib().admfem_fem_grad_grad(ctx.mesh.handle, _lib.ptr(g), _lib.ptr(gu),
+                                                   _lib.stream_ptr()))
+        return gu, None
+
+
+def eval_grad_on_gauss_pts1(u, mesh):
+    """FemGradMfem (deps/MFEM/FemGrad/FemGradMfem.h:8-41; src/MFEM/MCore.jl:
+    239-246): (du/dx, du/dy) at every Gauss point, shape (ngauss, 2)."""
+    host = _nodal(u, mesh)
+    out = _FemGrad.apply(torch.as_tensor(u) if host else u, mesh)
+    return out.cpu().numpy() if host else out
+
+
+class _LaplaceTerm(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u, nu, mesh):
+        uu, nn = _dev(u), _dev(nu)
+        out = torch.empty(mesh.ndof, dtype=torch.float64, device=_DEV)
+        _lib.check(_lib.lib().admfem_fem_laplace_term(mesh.handle, _lib.ptr(uu), _lib.ptr(nn),
+                                                      _lib.ptr(out), _lib.stream_ptr()))
+        ctx.save_for_backward(uu, nn)
+        ctx.mesh = mesh
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        uu, nn = ctx.saved_tensors
+        g = _dev(g)
+        gu = torch.empty_like(uu) if ctx.needs_input_grad[0] else None
+        gn = torch.empty_like(nn) if ctx.needs_input_grad[1] else None
+        _lib.check(_lib.lib().admfem_fem_laplace_term_grad(
+            ctx.mesh.handle, _lib.ptr(g), _lib.ptr(uu), _lib.ptr(nn), _lib.ptr(gu),
+            _lib.ptr(gn), _lib.stream_ptr()))
+        return gu, gn, None
+
+
+def compute_fem_laplace_term1(u, nu, mesh):
+    """ComputeLaplaceTermMfem (deps/MFEM/ComputeLaplaceTermMfem/
+    ComputeLaplaceTermMfem.h:4-39; src/MFEM/MCore.jl:740-763): K(nu) u,
+    matrix-free."""
+    if len(u) != mesh.ndof or len(nu) != mesh.ngauss:
+        raise ValueError("u must have mesh.ndof entries and nu get_ngauss(mesh)")
+    host = isinstance(u, np.ndarray) and isinstance(nu, np.ndarray)
+    t = (lambda a: torch.as_tensor(a) if isinstance(a, np.ndarray) else a)
+    out = _LaplaceTerm.apply(t(u), t(nu), mesh)
+    return out.cpu().numpy() if host else out
+
+
 class _Dirichlet(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vv, rhs, bdval, ind, bd, N):

@@ -24,7 +24,7 @@ COMMON = [
 ]
 
 SOURCES = ["mesh.cpp", "api.cpp", "geometry.hip", "assemble.hip", "cg.hip",
-           "dirichlet.hip", "batched.hip", "server.cpp", "sparse_op.hip"]
+           "dirichlet.hip", "batched.hip", "server.cpp", "sparse_op.hip", "fields.hip"]
 
 
 def _deps_mtime():

@@ -241,6 +241,62 @@ int admfem_fem_source_scalar_grad(admfem_mesh h, const double *grad_rhs, double 
   });
 }
 
+int admfem_fem_to_gauss_points(admfem_mesh h, const double *u, double *out, void *stream) {
+  return guarded([&] {
+    REQUIRE(u && out, ADMFEM_ERR_ARG, "null pointer");
+    launch_to_gauss(M(h), u, out, S(stream));
+  });
+}
+
+int admfem_fem_to_gauss_points_grad(admfem_mesh h, const double *grad_out, double *grad_u,
+                                    void *stream) {
+  return guarded([&] {
+    REQUIRE(grad_out && grad_u, ADMFEM_ERR_ARG, "null pointer");
+    launch_to_gauss_grad(M(h), grad_out, grad_u, S(stream));
+  });
+}
+
+int admfem_dof_to_gauss_points(admfem_mesh h, const double *u, double *out, void *stream) {
+  return admfem_fem_to_gauss_points(h, u, out, stream);
+}
+
+int admfem_dof_to_gauss_points_grad(admfem_mesh h, const double *grad_out, double *grad_u,
+                                    void *stream) {
+  return admfem_fem_to_gauss_points_grad(h, grad_out, grad_u, stream);
+}
+
+int admfem_fem_grad(admfem_mesh h, const double *u, double *out, void *stream) {
+  return guarded([&] {
+    REQUIRE(u && out, ADMFEM_ERR_ARG, "null pointer");
+    launch_fem_grad(M(h), u, out, S(stream));
+  });
+}
+
+int admfem_fem_grad_grad(admfem_mesh h, const double *grad_out, double *grad_u, void *stream) {
+  return guarded([&] {
+    REQUIRE(grad_out && grad_u, ADMFEM_ERR_ARG, "null pointer");
+    launch_fem_grad_grad(M(h), grad_out, grad_u, S(stream));
+  });
+}
+
+int admfem_fem_laplace_term(admfem_mesh h, const double *u, const double *nu, double *out,
+                            void *stream) {
+  return guarded([&] {
+    REQUIRE(u && nu && out, ADMFEM_ERR_ARG, "null pointer");
+    launch_laplace_term(M(h), u, nu, out, S(stream));
+  });
+}
+
+int admfem_fem_laplace_term_grad(admfem_mesh h, const double *grad_out, const double *u,
+                                 const double *nu, double *grad_u, double *grad_nu,
+                                 void *stream) {
+  return guarded([&] {
+    REQUIRE(grad_out && u && nu, ADMFEM_ERR_ARG, "null pointer");
+    if (grad_u) launch_laplace_term(M(h), grad_out, nu, grad_u, S(stream));
+    if (grad_nu) launch_laplace_term_nu_grad(M(h), grad_out, u, grad_nu, S(stream));
+  });
+}
+
 int admfem_impose_dirichlet(const long long *indices, const double *vv, int64_t sN,
                             const long long *bd, const double *bdval, int bdN,
                             const double *rhs, int N, long long *oindices, double *ov,
@@ -670,6 +726,37 @@ void FemSourceScalar_forward_Julia(double *rhs, const double *f) {
     d2h(rhs, dr.p, (size_t)m->nnode);
   });
   if (rc) fprintf(stderr, "FemSourceScalar_forward_Julia: %s\n", admfem_last_error());
+}
+
+void FemToGaussPointsMfem_Julia(double *out, const double *u) {
+  Mesh *m = default_or_warn("FemToGaussPointsMfem_Julia");
+  if (!m) return;
+  int rc = guarded([&] {
+    DevBuf<double> du, dout(m->ngauss);
+    du.upload(u, m->nnode);
+    launch_to_gauss(*m, du.p, dout.p, nullptr);
+    d2h(out, dout.p, (size_t)m->ngauss);
+  });
+  if (rc) fprintf(stderr, "FemToGaussPointsMfem_Julia: %s\n", admfem_last_error());
+}
+
+void DofToGaussPointsMfem_forward_Julia(double *out, const double *u) {
+  FemToGaussPointsMfem_Julia(out, u);
+}
+
+void ComputeLaplaceTermMfem_forward_Julia(double *out, const double *nu, const double *u) {
+  Mesh *m = default_or_warn("ComputeLaplaceTermMfem_forward_Julia");
+  if (!m) return;
+  int rc = guarded([&] {
+    DevBuf<double> du, dnu, dk(m->nnode);
+    du.upload(u, m->nnode);
+    dnu.upload(nu, m->ngauss);
+    launch_laplace_term(*m, du.p, dnu.p, dk.p, nullptr);
+    std::vector<double> k(m->nnode);
+    d2h(k.data(), dk.p, k.size());
+    for (int i = 0; i < m->nnode; i++) out[i] += k[i];  // accumulates, as the reference
+  });
+  if (rc) fprintf(stderr, "ComputeLaplaceTermMfem_forward_Julia: %s\n", admfem_last_error());
 }
 
 }  // extern "C"

@@ -1,0 +1,189 @@
+// Nodal fields <-> Gauss points, and the matrix-free Laplace term, for P1:
+//   FemToGaussPointsMfem / DofToGaussPointsMfem (deps/MFEM/FemToGaussPoints/
+//     FemToGaussPointsMfem.h:6-33, DofToGaussPoints/DofToGaussPointsMfem.h:
+//     6-32): out[g] = sum_r u[node_r] h_r(g)
+//   FemGradMfem (deps/MFEM/FemGrad/FemGradMfem.h:8-41):
+//     out[g] = (sum_r u[node_r] dh_r/dx, sum_r u[node_r] dh_r/dy)
+//   ComputeLaplaceTermMfem (deps/MFEM/ComputeLaplaceTermMfem/
+//     ComputeLaplaceTermMfem.h:4-39): out = K(nu) u without forming K.
+// Fields are in the user's node order, Gauss arrays in the user's element
+// order (g = nq e + k).  Forward maps to Gauss points run one thread per
+// Gauss point; every map to the nodes (the backward ops and the Laplace
+// term) is a per-node gather over the node's incident elements: one writer
+// per output, no atomics.
+#include "geom.h"
+
+namespace admfem {
+
+namespace {
+
+struct Tri {
+  double x[3], y[3];
+};
+
+__device__ __forceinline__ Tri load_tri(const int *__restrict__ conn, int nelem, int e,
+                                        const double *__restrict__ x,
+                                        const double *__restrict__ y) {
+  Tri t;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int v = conn[(size_t)k * nelem + e];
+    t.x[k] = x[v];
+    t.y[k] = y[v];
+  }
+  return t;
+}
+
+__global__ void to_gauss_kernel(int nelem, QuadRule rule, const int *__restrict__ conn_user,
+                                const double *__restrict__ u, double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rule.nq * nelem) return;
+  const int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
+  const double xi = rule.xi[k], eta = rule.eta[k];
+  out[g] = u[conn_user[e]] * shape(0, xi, eta) +
+           u[conn_user[(size_t)nelem + e]] * shape(1, xi, eta) +
+           u[conn_user[2 * (size_t)nelem + e]] * shape(2, xi, eta);
+}
+
+__global__ void fem_grad_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
+                                const int *__restrict__ conn_user, const double *__restrict__ x,
+                                const double *__restrict__ y, const double *__restrict__ u,
+                                double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rule.nq * nelem) return;
+  const int e = (int)(g / rule.nq);
+  const Tri t = load_tri(conn, nelem, e, x, y);
+  const ElemGeom G = elem_geom(t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2]);
+  double gx = 0.0, gy = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) gx += u[conn_user[(size_t)r * nelem + e]] * G.gx[r];
+#pragma unroll
+  for (int r = 0; r < 3; r++) gy += u[conn_user[(size_t)r * nelem + e]] * G.gy[r];
+  out[2 * g] = gx;
+  out[2 * g + 1] = gy;
+}
+
+// Per node: the transposes of the two maps above, and K(nu) u.
+enum { NODE_TO_GAUSS_T = 0, NODE_GRAD_T = 1, NODE_LAPLACE = 2 };
+
+template <int OP>
+__global__ void node_gather_kernel(int n, int nelem, QuadRule rule, const int *__restrict__ conn,
+                                   const int *__restrict__ conn_user,
+                                   const double *__restrict__ x, const double *__restrict__ y,
+                                   const int *__restrict__ n2e_ptr,
+                                   const int *__restrict__ n2e_elem,
+                                   const uint32_t *__restrict__ slots,
+                                   const int *__restrict__ iperm,
+                                   const double *__restrict__ in,   // Gauss array or field
+                                   const double *__restrict__ nu,   // NODE_LAPLACE
+                                   double *__restrict__ out_user) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double acc = 0.0;
+  for (int t = n2e_ptr[r]; t < n2e_ptr[r + 1]; t++) {
+    const int e = n2e_elem[t];
+    const int p = (int)(slots[t] & 3u);
+    if (OP == NODE_TO_GAUSS_T) {
+      for (int k = 0; k < rule.nq; k++)
+        acc += in[(size_t)rule.nq * e + k] * shape(p, rule.xi[k], rule.eta[k]);
+    } else {
+      const Tri tr = load_tri(conn, nelem, e, x, y);
+      const ElemGeom G = elem_geom(tr.x[0], tr.y[0], tr.x[1], tr.y[1], tr.x[2], tr.y[2]);
+      if (OP == NODE_GRAD_T) {
+        for (int k = 0; k < rule.nq; k++) {
+          const size_t g = (size_t)rule.nq * e + k;
+          acc += in[2 * g] * G.gx[p] + in[2 * g + 1] * G.gy[p];
+        }
+      } else {  // K(nu) u: sum_k nu_k w_k grad phi_p . grad u_e
+        double ux = 0.0, uy = 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          const double uq = in[conn_user[(size_t)q * nelem + e]];
+          ux += G.gx[q] * uq;
+          uy += G.gy[q] * uq;
+        }
+        double cw = 0.0;
+        for (int k = 0; k < rule.nq; k++)
+          cw += nu[(size_t)rule.nq * e + k] * gauss_weight(rule.w[k], G.area);
+        acc += cw * (G.gx[p] * ux + G.gy[p] * uy);
+      }
+    }
+  }
+  out_user[iperm[r]] = acc;
+}
+
+// ComputeLaplaceTermMfemGrad, nu part: w_g grad(grad_out) . grad(u) per Gauss
+// point (the term is linear in nu).
+__global__ void laplace_term_nu_grad_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
+                                            const int *__restrict__ conn_user,
+                                            const double *__restrict__ x,
+                                            const double *__restrict__ y,
+                                            const double *__restrict__ go,
+                                            const double *__restrict__ u,
+                                            double *__restrict__ gnu) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rule.nq * nelem) return;
+  const int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
+  const Tri t = load_tri(conn, nelem, e, x, y);
+  const ElemGeom G = elem_geom(t.x[0], t.y[0], t.x[1], t.y[1], t.x[2], t.y[2]);
+  double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int v = conn_user[(size_t)q * nelem + e];
+    ax += G.gx[q] * go[v];
+    ay += G.gy[q] * go[v];
+    bx += G.gx[q] * u[v];
+    by += G.gy[q] * u[v];
+  }
+  gnu[g] = gauss_weight(rule.w[k], G.area) * (ax * bx + ay * by);
+}
+
+inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int OP>
+void node_gather(const Mesh &m, const double *in, const double *nu, double *out_user,
+                 hipStream_t st) {
+  node_gather_kernel<OP><<<nb(m.nnode, 256), 256, 0, st>>>(
+      m.nnode, m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, m.d_n2e_ptr.p,
+      m.d_n2e_elem.p, m.d_slots_full.p, m.d_iperm.p, in, nu, out_user);
+  HIPCHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_to_gauss(const Mesh &m, const double *u, double *out, hipStream_t st) {
+  to_gauss_kernel<<<nb(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn_user.p, u, out);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_to_gauss_grad(const Mesh &m, const double *grad_out, double *grad_u,
+                          hipStream_t st) {
+  node_gather<NODE_TO_GAUSS_T>(m, grad_out, nullptr, grad_u, st);
+}
+
+void launch_fem_grad(const Mesh &m, const double *u, double *out, hipStream_t st) {
+  fem_grad_kernel<<<nb(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p, m.conn_user.p,
+                                                     m.x.p, m.y.p, u, out);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_fem_grad_grad(const Mesh &m, const double *grad_out, double *grad_u,
+                          hipStream_t st) {
+  node_gather<NODE_GRAD_T>(m, grad_out, nullptr, grad_u, st);
+}
+
+void launch_laplace_term(const Mesh &m, const double *u, const double *nu, double *out,
+                         hipStream_t st) {
+  node_gather<NODE_LAPLACE>(m, u, nu, out, st);
+}
+
+void launch_laplace_term_nu_grad(const Mesh &m, const double *grad_out, const double *u,
+                                 double *grad_nu, hipStream_t st) {
+  laplace_term_nu_grad_kernel<<<nb(m.ngauss, 256), 256, 0, st>>>(
+      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, grad_out, u, grad_nu);
+  HIPCHECK(hipGetLastError());
+}
+
+}  // namespace admfem

@@ -170,6 +170,17 @@ void launch_source(const Mesh &m, const double *f, double *rhs_user,
                    bool accumulate, hipStream_t st);
 void launch_source_grad(const Mesh &m, const double *grhs_user, double *gf,
                         hipStream_t st);
+// fields.hip (user node order / user Gauss order)
+void launch_to_gauss(const Mesh &m, const double *u, double *out, hipStream_t st);
+void launch_to_gauss_grad(const Mesh &m, const double *grad_out, double *grad_u,
+                          hipStream_t st);
+void launch_fem_grad(const Mesh &m, const double *u, double *out, hipStream_t st);
+void launch_fem_grad_grad(const Mesh &m, const double *grad_out, double *grad_u,
+                          hipStream_t st);
+void launch_laplace_term(const Mesh &m, const double *u, const double *nu, double *out,
+                         hipStream_t st);
+void launch_laplace_term_nu_grad(const Mesh &m, const double *grad_out, const double *u,
+                                 double *grad_nu, hipStream_t st);
 // assemble.hip
 // Fused assembly of a mesh system straight into the solver format: SELL
 // values (FP64 and FP32), diagonal and its inverse (both precisions), rhs.

@@ -84,6 +84,17 @@ void FemLaplaceScalar_forward_Julia(long long *indices, double *vv,
 /* deps/MFEM/FemSource1/FemSourceScalar.h:35-37 (forward of h:4-15):
  * rhs [ndof] is ACCUMULATED into (the caller zeroes it). */
 void FemSourceScalar_forward_Julia(double *rhs, const double *f);
+/* deps/MFEM/FemToGaussPoints/FemToGaussPointsMfem.h:36-48 (called from
+ * src/MFEM/MUtils.jl:248): out [ngauss] = u [nnode] interpolated to the
+ * Gauss points (overwritten). */
+void FemToGaussPointsMfem_Julia(double *out, const double *u);
+/* deps/MFEM/DofToGaussPoints/DofToGaussPointsMfem.h:35-37 (MUtils.jl:268):
+ * the same for the element's dofs (P1: the nodes). */
+void DofToGaussPointsMfem_forward_Julia(double *out, const double *u);
+/* deps/MFEM/ComputeLaplaceTermMfem/ComputeLaplaceTermMfem.h:42-44
+ * (src/MFEM/MCore.jl:744-745): out [ndof] += K(nu) u, ACCUMULATED (the
+ * caller zeroes it). */
+void ComputeLaplaceTermMfem_forward_Julia(double *out, const double *nu, const double *u);
 /* The default mesh as a handle, for sections B-E (NULL if none). */
 admfem_mesh admfem_default_mesh(void);
 
@@ -144,6 +155,38 @@ int admfem_fem_source_scalar(admfem_mesh m, const double *f, double *rhs,
 /* FemSourceScalarGrad (FemSourceScalar.h:17-32): grad_rhs -> grad_f. */
 int admfem_fem_source_scalar_grad(admfem_mesh m, const double *grad_rhs,
                                   double *grad_f, void *stream);
+
+/* FemToGaussPointsMfem (deps/MFEM/FemToGaussPoints/FemToGaussPointsMfem.h:
+ * 6-18): u [nnode] (user node order) -> out [ngauss] (overwritten). */
+int admfem_fem_to_gauss_points(admfem_mesh m, const double *u, double *out, void *stream);
+/* FemToGaussPointsMfemGrad (h:20-33): grad_out [ngauss] -> grad_u [nnode]
+ * (overwritten).  Atomic-free node gather. */
+int admfem_fem_to_gauss_points_grad(admfem_mesh m, const double *grad_out, double *grad_u,
+                                    void *stream);
+/* DofToGaussPointsMfem / ...Grad (deps/MFEM/DofToGaussPoints/
+ * DofToGaussPointsMfem.h:6-32): for P1 the dofs are the nodes, so these are
+ * the two functions above under the op's own names. */
+int admfem_dof_to_gauss_points(admfem_mesh m, const double *u, double *out, void *stream);
+int admfem_dof_to_gauss_points_grad(admfem_mesh m, const double *grad_out, double *grad_u,
+                                    void *stream);
+/* FemGradMfem (deps/MFEM/FemGrad/FemGradMfem.h:8-22), the op behind
+ * eval_grad_on_gauss_pts1 (src/MFEM/MCore.jl:239-246): u [nnode] ->
+ * out [ngauss x 2] row-major (du/dx, du/dy) per Gauss point. */
+int admfem_fem_grad(admfem_mesh m, const double *u, double *out, void *stream);
+/* FemGradMfemGrad (h:24-41): grad_out [ngauss x 2] -> grad_u [nnode]. */
+int admfem_fem_grad_grad(admfem_mesh m, const double *grad_out, double *grad_u, void *stream);
+/* ComputeLaplaceTermMfem (deps/MFEM/ComputeLaplaceTermMfem/
+ * ComputeLaplaceTermMfem.h:4-17), the op behind compute_fem_laplace_term1
+ * (src/MFEM/MCore.jl:740-763): out [nnode] = K(nu) u without forming K
+ * (overwritten); u [nnode], nu [ngauss]. */
+int admfem_fem_laplace_term(admfem_mesh m, const double *u, const double *nu, double *out,
+                            void *stream);
+/* ComputeLaplaceTermMfemGrad (h:19-39): grad_u = K(nu) grad_out (K is
+ * symmetric), grad_nu[g] = w_g grad(grad_out) . grad(u) on g's element.
+ * Either output may be NULL. */
+int admfem_fem_laplace_term_grad(admfem_mesh m, const double *grad_out, const double *u,
+                                 const double *nu, double *grad_u, double *grad_nu,
+                                 void *stream);
 /* ImposeDirichlet (deps/MFEM/ImposeDirichlet/ImposeDirichlet.cpp:14-40,
  * forward ImposeDirichlet.h:27-60).  indices [sN x 2] int64 0-based, vv [sN],
  * bd [bdN] int64 1-based, rhs [N], bdval [bdN].  Outputs: oindices

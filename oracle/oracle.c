@@ -390,3 +390,88 @@ int orc_bcnode(const int *elems, int nelem, int nnode, int64_t *out) {
   free(isbd); free(keys);
   return n;
 }
+
+/* FemToGaussPointsMfem.h:6-18 (forward) and 20-33 (backward); for P1 the
+ * DofToGaussPointsMfem.h:6-32 loops are the same (h = hs). */
+void orc_fem_to_gauss_forward(const int *elems, int nelem, int nq, const double *h,
+                              const double *u, double *out) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++) {
+      out[k] = u[elems[3 * i + 0]] * h[3 * k + 0] + u[elems[3 * i + 1]] * h[3 * k + 1] +
+               u[elems[3 * i + 2]] * h[3 * k + 2];
+      k++;
+    }
+}
+
+void orc_fem_to_gauss_backward(const int *elems, int nelem, int nq, const double *h,
+                               const double *grad_out, double *grad_u) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++) {
+      for (int r = 0; r < 3; r++) grad_u[elems[3 * i + r]] += grad_out[k] * h[3 * k + r];
+      k++;
+    }
+}
+
+/* FemGradMfem.h:8-22 (forward) and 24-41 (backward). */
+void orc_fem_grad_forward(const int *elems, int nelem, int nq, const double *hx,
+                          const double *hy, const double *u, double *out) {
+  int64_t k = 0, g = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++, g++) {
+      out[k] = 0.0;
+      for (int r = 0; r < 3; r++) out[k] += u[elems[3 * i + r]] * hx[3 * g + r];
+      k++;
+      out[k] = 0.0;
+      for (int r = 0; r < 3; r++) out[k] += u[elems[3 * i + r]] * hy[3 * g + r];
+      k++;
+    }
+}
+
+void orc_fem_grad_backward(const int *elems, int nelem, int nq, const double *hx,
+                           const double *hy, const double *grad_out, double *grad_u) {
+  int64_t k = 0, g = 0;
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++, g++) {
+      for (int r = 0; r < 3; r++) grad_u[elems[3 * i + r]] += grad_out[k] * hx[3 * g + r];
+      k++;
+      for (int r = 0; r < 3; r++) grad_u[elems[3 * i + r]] += grad_out[k] * hy[3 * g + r];
+      k++;
+    }
+}
+
+/* ComputeLaplaceTermMfem.h:4-17 (forward) and 19-39 (backward). */
+void orc_laplace_term_forward(const int *elems, int nelem, int nq, const double *w,
+                              const double *hx, const double *hy, const double *nu,
+                              const double *u, double *out) {
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++) {
+      const int64_t g = (int64_t)nq * i + j;
+      const double nv = nu[g];
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++)
+          out[elems[3 * i + p]] +=
+              nv * (hx[3 * g + p] * hx[3 * g + q] * u[elems[3 * i + q]] * w[g] +
+                    hy[3 * g + p] * hy[3 * g + q] * u[elems[3 * i + q]] * w[g]);
+    }
+}
+
+void orc_laplace_term_backward(const int *elems, int nelem, int nq, const double *w,
+                               const double *hx, const double *hy, const double *nu,
+                               const double *u, const double *grad_out, double *grad_nu,
+                               double *grad_u) {
+  for (int64_t i = 0; i < nelem; i++)
+    for (int j = 0; j < nq; j++) {
+      const int64_t g = (int64_t)nq * i + j;
+      const double nv = nu[g];
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) {
+          const double go = grad_out[elems[3 * i + p]];
+          grad_nu[g] += go * (hx[3 * g + p] * hx[3 * g + q] * u[elems[3 * i + q]] * w[g] +
+                              hy[3 * g + p] * hy[3 * g + q] * u[elems[3 * i + q]] * w[g]);
+          grad_u[elems[3 * i + q]] += go * nv * hx[3 * g + p] * hx[3 * g + q] * w[g];
+          grad_u[elems[3 * i + q]] += go * nv * hy[3 * g + p] * hy[3 * g + q] * w[g];
+        }
+    }
+}

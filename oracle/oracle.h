@@ -96,6 +96,36 @@ void orc_dirichlet_backward(double *grad_vv_ipt, double *grad_rhs_ipt,
 int64_t orc_coo_to_csr(const int64_t *indices, const double *vv, int64_t sN,
                        int N, int64_t *row_ptr, int32_t *col, double *val);
 
+/* FemToGaussPointsMfem_forward/_backward
+ * (deps/MFEM/FemToGaussPoints/FemToGaussPointsMfem.h:6-33): out[g] =
+ * sum_r u[node_r] hs(r, j).  DofToGaussPointsMfem (DofToGaussPointsMfem.h:
+ * 6-32) is the same loop over the element's dofs with h, identical for P1.
+ * The backward accumulates into grad_u (caller zeroes). */
+void orc_fem_to_gauss_forward(const int *elems, int nelem, int nq, const double *h,
+                              const double *u, double *out);
+void orc_fem_to_gauss_backward(const int *elems, int nelem, int nq, const double *h,
+                               const double *grad_out, double *grad_u);
+
+/* FemGradMfem_forward/_backward (deps/MFEM/FemGrad/FemGradMfem.h:8-41):
+ * out[2g] = sum_r u[dof_r] hx(r, j), out[2g+1] = sum_r u[dof_r] hy(r, j).
+ * The backward accumulates into grad_u (caller zeroes). */
+void orc_fem_grad_forward(const int *elems, int nelem, int nq, const double *hx,
+                          const double *hy, const double *u, double *out);
+void orc_fem_grad_backward(const int *elems, int nelem, int nq, const double *hx,
+                           const double *hy, const double *grad_out, double *grad_u);
+
+/* ComputeLaplaceTermMfem_forward/_backward
+ * (deps/MFEM/ComputeLaplaceTermMfem/ComputeLaplaceTermMfem.h:4-39):
+ * out[dof_p] += nu_j sum_q (hx_p hx_q + hy_p hy_q) u[dof_q] w_j.  Both
+ * accumulate (caller zeroes out, grad_nu and grad_u). */
+void orc_laplace_term_forward(const int *elems, int nelem, int nq, const double *w,
+                              const double *hx, const double *hy, const double *nu,
+                              const double *u, double *out);
+void orc_laplace_term_backward(const int *elems, int nelem, int nq, const double *w,
+                               const double *hx, const double *hy, const double *nu,
+                               const double *u, const double *grad_out, double *grad_nu,
+                               double *grad_u);
+
 /* Boundary nodes of a triangle mesh: vertices of edges used by exactly one
  * element (src/MFEM/MCore.jl:385-406, 439-450), returned ascending, 1-based.
  * out capacity nnode.  Returns the count. */

@@ -54,6 +54,13 @@ def lib():
         "orc_dirichlet_backward": (None, [_dp, _dp, _dp, _dp, _dp, _lp, _dp, _L, _lp, _dp, _I, _I]),
         "orc_coo_to_csr": (_L, [_lp, _dp, _L, _I, _lp, _ip, _dp]),
         "orc_bcnode": (_I, [_ip, _I, _I, _lp]),
+        "orc_fem_to_gauss_forward": (None, [_ip, _I, _I, _dp, _dp, _dp]),
+        "orc_fem_to_gauss_backward": (None, [_ip, _I, _I, _dp, _dp, _dp]),
+        "orc_fem_grad_forward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp]),
+        "orc_fem_grad_backward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp]),
+        "orc_laplace_term_forward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_laplace_term_backward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                             _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -145,6 +152,42 @@ class OracleMesh:
         lib().orc_source_backward(self.elems, self.nelem, self.nq, self.w, self.h,
                                   _f64(grad_rhs), g)
         return g
+
+    # FemToGaussPoints / DofToGaussPoints (P1), FemGrad, ComputeLaplaceTerm
+    def fem_to_gauss(self, u):
+        out = np.zeros(self.ngauss)
+        lib().orc_fem_to_gauss_forward(self.elems, self.nelem, self.nq, self.h, _f64(u), out)
+        return out
+
+    def fem_to_gauss_backward(self, grad_out):
+        g = np.zeros(self.nnode)
+        lib().orc_fem_to_gauss_backward(self.elems, self.nelem, self.nq, self.h,
+                                        _f64(grad_out), g)
+        return g
+
+    def fem_grad(self, u):
+        out = np.zeros(2 * self.ngauss)
+        lib().orc_fem_grad_forward(self.elems, self.nelem, self.nq, self.hx, self.hy, _f64(u),
+                                   out)
+        return out.reshape(-1, 2)
+
+    def fem_grad_backward(self, grad_out):
+        g = np.zeros(self.nnode)
+        lib().orc_fem_grad_backward(self.elems, self.nelem, self.nq, self.hx, self.hy,
+                                    _f64(grad_out).reshape(-1), g)
+        return g
+
+    def laplace_term(self, u, nu):
+        out = np.zeros(self.nnode)
+        lib().orc_laplace_term_forward(self.elems, self.nelem, self.nq, self.w, self.hx,
+                                       self.hy, _f64(nu), _f64(u), out)
+        return out
+
+    def laplace_term_backward(self, u, nu, grad_out):
+        gn, gu = np.zeros(self.ngauss), np.zeros(self.nnode)
+        lib().orc_laplace_term_backward(self.elems, self.nelem, self.nq, self.w, self.hx,
+                                        self.hy, _f64(nu), _f64(u), _f64(grad_out), gn, gu)
+        return gn, gu
 
 
 def dirichlet(indices, vv, bd, rhs, bdval):

@@ -338,3 +338,56 @@ def test_heat_step_gradient_matches_differences():
     eps = 1e-6
     fd = (L(kap + eps * d) - L(kap - eps * d)) / (2 * eps)
     assert out["grad_kappa"] @ d == pytest.approx(fd, rel=1e-6)
+
+
+# ------------------------------------------------- fields at Gauss points
+def test_fem_to_gauss_exact_for_linear_and_second_order_for_quadratic():
+    """deps/MFEM/FemToGaussPoints/TestFemToGaussPoints.jl: P1 interpolation of
+    nodal values to the Gauss points; exact for linear fields, O(h^2) for the
+    test's 2x^2 + y."""
+    for m in (8, 16):
+        mesh = om(m)
+        x, y = mesh.coords[:, 0], mesh.coords[:, 1]
+        np.testing.assert_allclose(mesh.fem_to_gauss(1.5 + 2 * x - 3 * y),
+                                   1.5 + 2 * mesh.gx - 3 * mesh.gy, rtol=1e-13, atol=1e-13)
+        err = np.abs(mesh.fem_to_gauss(2 * x ** 2 + y) - (2 * mesh.gx ** 2 + mesh.gy)).max()
+        assert err < 2 * (1.0 / m) ** 2
+    assert err > 0.1 * 2 * (1.0 / 16) ** 2 / 4   # not exact: really interpolates
+
+
+def test_fem_grad_exact_for_linear_fields():
+    """deps/MFEM/FemGrad/TestFemGrad.jl: f = 2x + x + 14y has gradient (3, 14)
+    at every Gauss point; layout [g][x, y]."""
+    c, e = meshgen.jittered_square(7, seed=5)
+    mesh = O.OracleMesh(c, e)
+    g = mesh.fem_grad(3 * c[:, 0] + 14 * c[:, 1])
+    assert g.shape == (mesh.ngauss, 2)
+    np.testing.assert_allclose(g[:, 0], 3.0, rtol=1e-12)
+    np.testing.assert_allclose(g[:, 1], 14.0, rtol=1e-12)
+
+
+def test_laplace_term_equals_stiffness_times_u():
+    """deps/MFEM/ComputeLaplaceTermMfem/ftest.jl: compute_fem_laplace_term1(u,
+    nu) = compute_fem_laplace_matrix1(nu) * u."""
+    c, e = meshgen.jittered_square(9, seed=2)
+    mesh = O.OracleMesh(c, e)
+    rng = np.random.default_rng(0)
+    nu, u = rng.random(mesh.ngauss) + 0.5, rng.standard_normal(mesh.nnode)
+    K = O.csr_matrix(*mesh.laplace_coo(nu), mesh.nnode)
+    np.testing.assert_allclose(mesh.laplace_term(u, nu), K @ u, rtol=1e-12,
+                               atol=1e-12 * np.abs(K @ u).max())
+
+
+def test_field_op_backwards_are_adjoints():
+    c, e = meshgen.jittered_square(6, seed=8)
+    mesh = O.OracleMesh(c, e)
+    rng = np.random.default_rng(1)
+    u, gq = rng.standard_normal(mesh.nnode), rng.standard_normal(mesh.ngauss)
+    assert np.isclose(gq @ mesh.fem_to_gauss(u), mesh.fem_to_gauss_backward(gq) @ u, rtol=1e-12)
+    g2 = rng.standard_normal((mesh.ngauss, 2))
+    assert np.isclose((g2 * mesh.fem_grad(u)).sum(), mesh.fem_grad_backward(g2) @ u, rtol=1e-12)
+    nu, go = rng.random(mesh.ngauss) + 0.5, rng.standard_normal(mesh.nnode)
+    gn, gu = mesh.laplace_term_backward(u, nu, go)
+    out = mesh.laplace_term(u, nu)
+    assert np.isclose(go @ out, gu @ u, rtol=1e-12)      # linear in u
+    assert np.isclose(go @ out, gn @ nu, rtol=1e-12)     # linear in nu
