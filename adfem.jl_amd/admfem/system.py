@@ -111,7 +111,7 @@ class DirichletSystem:
         return g
 
     def forward_adjoint(self, obs_idx, obs_val, precision="fp64", rtol=1e-10,
-                        max_iter=100000, want_lam=True, **problem):
+                        max_iter=100000, want_lam=True, replace_delta=0.0, **problem):
         """Assemble, u = A^-1 rhs, L = sum (u[obs_idx]-obs_val)^2 (obs_idx
         0-based user node ids), lam = A^-1 dL/du, dL/dkappa.  Returns dict."""
         P = self.problem(**problem)
@@ -123,7 +123,7 @@ class DirichletSystem:
         gk = torch.empty(self.mesh.ngauss, dtype=torch.float64, device=_DEV)
         loss = ctypes.c_double()
         st = (_lib.SolveStats * 2)()
-        o = solve_opts(precision, rtol, max_iter)
+        o = solve_opts(precision, rtol, max_iter, replace_delta=replace_delta)
         _lib.check(_lib.lib().admfem_system_forward_adjoint(
             self._h, ctypes.byref(P), _lib.ptr(idx), _lib.ptr(ov), idx.numel(), ctypes.byref(o),
             _lib.ptr(u), _lib.ptr(lam), _lib.ptr(gk), ctypes.byref(loss), st,

@@ -25,8 +25,8 @@ struct BSys {
 
 // CG state of one system.
 struct BScal {
-  double rtol2, delta2, bb, tol2, rz, rz_prev, alpha, beta, rr, rr_ref;
-  int status, it, maxit, replace, nrepl, sel;
+  double rtol2, delta2, bb, tol2, rz, rz_prev, alpha, beta, rr, rr_ref, rr_min;
+  int status, it, maxit, replace, nrepl, sel, stall;
 };
 
 // A Gaussian random field of random Fourier features:

@@ -152,6 +152,8 @@ __global__ void __launch_bounds__(NT) bs_init(const BSys *__restrict__ D, BScal 
   s.rz_prev = rz;
   s.rr = rr;
   s.rr_ref = rr;
+  s.rr_min = rr;
+  s.stall = 0;
   s.alpha = 0.0;
   s.beta = 0.0;
   s.it = 0;
@@ -244,6 +246,12 @@ __global__ void __launch_bounds__(NT)
     s.status = ST_NAN;
     return;
   }
+  if (rr < 0.98 * s.rr_min) {
+    s.rr_min = rr;
+    s.stall = 0;
+  } else {
+    s.stall += 1;
+  }
   // a residual that looks converged freezes the system until the next
   // replacement point, where the FP64 residual decides
   if (rr <= s.tol2) s.replace = 1;
@@ -251,10 +259,12 @@ __global__ void __launch_bounds__(NT)
 }
 
 // A system is replaced at a replacement point when it is frozen (converged
-// looking) or its residual fell by delta since the last replacement.  Every
+// looking), its residual fell by delta since the last replacement, or its
+// FP32 residual has stalled.  Every
 // kernel of the point reads the same, unchanged scalars: one decision.
 __device__ __forceinline__ bool replace_due(const BScal &s) {
-  return s.status == ST_RUN && (s.replace || s.rr <= s.delta2 * s.rr_ref);
+  return s.status == ST_RUN &&
+         (s.replace || s.rr <= s.delta2 * s.rr_ref || s.stall >= STALL_ITERS);
 }
 
 // due systems: x' = x + xs into the other buffer, r = b - A64 x', xs' = 0
@@ -300,6 +310,8 @@ __global__ void __launch_bounds__(NT)
   s.beta = rz / s.rz_prev;
   s.rr = rr;
   s.rr_ref = rr;
+  s.rr_min = rr;
+  s.stall = 0;
   s.replace = 0;
   s.nrepl += 1;
   s.sel ^= 1;

@@ -44,12 +44,15 @@ struct CGScalars {
   double alpha, beta;
   double rr;             // ||r||^2 of the current residual (recursive)
   double rr_ref;         // mixed: ||r||^2 at the last replacement
+  double rr_min;         // mixed: smallest recursive ||r||^2 since then
   int mixed;
   int status;            // ST_*
   int it;                // completed iterations
   int replace;           // mixed: replacement pending
   int nrepl;
+  int stall;             // mixed: iterations without a new rr_min
 };
+
 
 constexpr int NT = 256;   // threads per wide block = 4 slices of 64 rows
 constexpr int RT = 1024;  // threads of the single-block scalar kernels
@@ -192,6 +195,8 @@ __global__ void __launch_bounds__(RT) s_init(int G, const double *part, CGScalar
   S->rz = t[0];
   S->rr = t[1];
   S->rr_ref = t[1];
+  S->rr_min = t[1];
+  S->stall = 0;
   S->bb = t[2];
   S->tol2 = S->rtol2 * t[2];
   S->beta = 0.0;
@@ -278,7 +283,14 @@ __global__ void __launch_bounds__(RT) s_beta(int G, const double *part, CGScalar
   if (!(t[1] == t[1]) || !isfinite(t[1])) {
     S->status = ST_NAN;
   } else if (S->mixed) {
-    if (t[1] <= S->tol2 || t[1] <= S->delta2 * S->rr_ref) S->replace = 1;
+    if (t[1] < 0.98 * S->rr_min) {
+      S->rr_min = t[1];
+      S->stall = 0;
+    } else {
+      S->stall += 1;
+    }
+    if (t[1] <= S->tol2 || t[1] <= S->delta2 * S->rr_ref || S->stall >= STALL_ITERS)
+      S->replace = 1;
     if (S->it >= maxit && !S->replace) S->status = ST_MAXIT;
   } else {
     if (t[1] <= S->tol2) S->status = ST_CONV;
@@ -332,6 +344,8 @@ __global__ void __launch_bounds__(RT) s_replace(int G, const double *part, CGSca
   S->beta = t[0] / S->rz_prev;
   S->rr = t[1];
   S->rr_ref = t[1];
+  S->rr_min = t[1];
+  S->stall = 0;
   S->replace = 0;
   S->nrepl += 1;
   if (!(t[1] == t[1]) || !isfinite(t[1])) S->status = ST_NAN;
@@ -434,7 +448,7 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
   REQUIRE(o.rtol > 0 && o.max_iter > 0, ADMFEM_ERR_ARG, "bad rtol/max_iter");
   CGScalars init{};
   init.rtol2 = o.rtol * o.rtol;
-  const double delta = o.replace_delta > 0 ? o.replace_delta : 0.1;
+  const double delta = o.replace_delta > 0 ? o.replace_delta : REPLACE_DELTA;
   init.delta2 = delta * delta;
   init.mixed = mixed ? 1 : 0;
   HIPCHECK(hipMemcpyAsync(w->sc.p, &init, sizeof(CGScalars), hipMemcpyHostToDevice, st));

@@ -76,6 +76,12 @@ __host__ __device__ inline uint32_t slot_of(uint32_t w, int q) {
 // index in its assembly block's element list.
 constexpr uint32_t PAIR_DIAG = 62;
 constexpr int PAIR_MAX_ELEMS = 4096;
+// Mixed-precision CG: default residual replacement factor (a replacement
+// when the FP32 residual fell by this factor since the last one) and the
+// stall guard (a replacement when the FP32 residual has not fallen by 1%
+// for this many iterations).
+constexpr double REPLACE_DELTA = 0.3;
+constexpr int STALL_ITERS = 64;
 constexpr int AS_ROWS = 64;  // rows per assembly block (one SELL slice)
 constexpr int MAXQ = 3;  // quadrature points per element carried here
 struct QuadRule {

@@ -293,7 +293,7 @@ struct Server {
   }
 
   void start_cg(Req &q) {
-    const double d = o.replace_delta > 0 ? o.replace_delta : 0.1;
+    const double d = o.replace_delta > 0 ? o.replace_delta : REPLACE_DELTA;
     batch_init(d_desc.p, d_scal.p, q.slot, topos[q.r.topo].nblk, o.rtol * o.rtol, d * d,
                o.max_iter, st);
   }

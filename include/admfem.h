@@ -262,8 +262,10 @@ typedef struct {
   int max_iter;       /* total inner iterations */
   double rtol;        /* ||b - A x||_2 <= rtol * ||b||_2 (true residual) */
   int check_every;    /* iterations between host convergence checks (0: auto) */
-  double replace_delta; /* mixed: residual replacement when the inner residual
-                           fell by this factor (0: auto 1e-2) */
+  double replace_delta; /* mixed: FP64 residual replacement when the inner
+                           residual fell by this factor since the last one
+                           (0: 0.3); also forced after 64 iterations without
+                           a 1% drop */
 } admfem_solve_opts;
 
 typedef struct {
@@ -354,7 +356,7 @@ typedef struct {
   int mode;                /* ADMFEM_MODE_* */
   double rtol;             /* true relative residual of every solve */
   int max_iter;
-  double replace_delta;    /* mixed: residual replacement factor (0: 0.1) */
+  double replace_delta;    /* mixed: residual replacement factor (0: 0.3) */
   int chunk;               /* CG iterations between host checks (0: 16;
                               rounded up to a multiple of 8) */
   int use_graph;           /* replay each chunk as a captured HIP graph */
