@@ -196,15 +196,13 @@ __global__ void __launch_bounds__(AS_NT)
   }
   const int s0 = slice_ptr[blk], width = (slice_ptr[blk + 1] - s0) >> 6;  // block = slice
   const int off = s0 + (row & 63);
-  for (int k = 0; k < width; k++) {
-    const double v = my[k];
-    o.sv[off + 64 * k] = v;
-    o.sv32[off + 64 * k] = (float)v;
-  }
+  for (int k = 0; k < width; k++) o.sv[off + 64 * k] = my[k];
+  if (o.sv32)
+    for (int k = 0; k < width; k++) o.sv32[off + 64 * k] = (float)my[k];
   o.diag[row] = d;
-  o.dinv[row] = d != 0.0 ? 1.0 / d : 0.0;
-  o.diag32[row] = (float)d;
-  o.dinv32[row] = d != 0.0 ? (float)(1.0 / d) : 0.0f;
+  if (o.dinv) o.dinv[row] = d != 0.0 ? 1.0 / d : 0.0;
+  if (o.diag32) o.diag32[row] = (float)d;
+  if (o.dinv32) o.dinv32[row] = d != 0.0 ? (float)(1.0 / d) : 0.0f;
   o.rhs[row] = acc;
 }
 

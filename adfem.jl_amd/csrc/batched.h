@@ -9,23 +9,26 @@ constexpr int REPLACE_EVERY = 8;  // iterations between replacement points
 
 // One system of the batch: its matrix (topology arrays shared with every
 // system of the same mesh) and its vectors.  Node vectors are in the
-// topology's internal order, padded to npad.
+// topology's internal order, padded to npad.  The FP32 inner iteration works
+// on the symmetrically scaled system (see batched.hip).
 struct BSys {
   const int *sp, *sc;       // SELL slice offsets / columns (topology)
   const int *row_bd;        // boundary index per row (topology)
-  const double *sv64, *diag64;
-  const float *sv32, *diag32, *dinv32;
+  const double *sv64, *diag64;  // FP64 matrix (replacement residuals)
+  const float *sv32;        // FP32 scaled off-diagonals a_ij dsq_i dsq_j
+  const double *dsq;        // 1 / sqrt(a_ii)
   const double *b;          // right-hand side
   double *x[2];             // FP64 solution, two buffers (swapped on replacement)
-  float *xs[2];             // FP32 correction since the last replacement
-  float *r, *z, *p[2], *q;
+  float *ys[2];             // FP32 scaled correction since the last replacement
+  float *r, *p[2], *q;      // scaled residual, directions, A p
   double *part;             // 4 * nblk block partials
   int n, npad, nblk;
 };
 
-// CG state of one system.
+// CG state of one system: rz (= scaled r.r) drives the iteration, rr is the
+// true FP64 ||b - A x||^2 of the last replacement.
 struct BScal {
-  double rtol2, delta2, bb, tol2, rz, rz_prev, alpha, beta, rr, rr_ref, rr_min;
+  double rtol2, delta2, bb, tol2, rz, rz_prev, alpha, beta, rr, rr_ref, rz_ref, rz_min;
   int status, it, maxit, replace, nrepl, sel, stall;
 };
 
@@ -43,6 +46,8 @@ void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, con
                       BScal *S, int iters, hipStream_t st);
 void batch_spmv_repeat(const int2 *map, int nmap, const BSys *D, const BScal *S, int repeat,
                        hipStream_t st);
+void batch_scale(const System &s, const double *sv64, const double *diag64, double *dsq,
+                 float *sv32, hipStream_t st);
 void batch_final(const BSys *D, const BScal *S, int sys, int npad, double *out,
                  hipStream_t st);
 void synth_inputs(const System &s, const double *base_x, const double *base_y, uint64_t seed,

@@ -4,20 +4,26 @@
 //
 // One launch serves every active system: a block map assigns each block of
 // the grid to (system, block-within-system); per-system descriptors hold the
-// system's SELL matrix and vectors, per-system scalars its CG state.  Each
+// system's matrix and vectors, per-system scalars its CG state.  Each
 // system's reductions run over its own partials in a fixed order, and its
 // replacement schedule depends only on its own iteration count, so a system
 // produces bit-identical results alone or in any mix.
 //
-// Per iteration (all active systems):
-//   bk_spmv   p = z + beta p_old, q = A p (FP32 storage, FP64 arithmetic)
-//   bs_alpha  one block per system: alpha
-//   bk_update xs += alpha p, r -= alpha q, z = D^-1 r
+// Jacobi-PCG on A x = b is run in its symmetrically scaled form: plain CG on
+// Â y = S r with Â = S A S, S = diag(1/sqrt(a_ii)) (unit diagonal), x = S y
+// -- the same iterates in exact arithmetic, but the FP32 inner iteration
+// reads no diagonal, no inverse diagonal and keeps no separate z = D^-1 r.
+//
+// Per iteration (all active systems), FP32 storage, FP64 arithmetic:
+//   bk_spmv   p = r + beta p_old, q = Â p
+//   bs_alpha  one block per system: alpha = r.r / p.q
+//   bk_update y += alpha p, r -= alpha q
 //   bs_beta   beta; a converged-looking residual freezes the system
 // and every 8th iteration of a chunk (a replacement point):
-//   bk_replace (frozen systems, or residual fell by delta since the last
-//              replacement): x' = x + xs, r = b - A64 x' in FP64, xs' = 0
-//   bs_replace true-residual convergence test, swap the x / xs buffers
+//   bk_replace (frozen systems, residual fell by delta since the last
+//              replacement, or stalled): x' = x + S y in FP64, the true
+//              residual b - A x' with the FP64 matrix, r = S (b - A x'), y' = 0
+//   bs_replace true-residual convergence test, swap the x / y buffers
 // Systems start at chunk boundaries, so a system's replacement points fall
 // on the same iterations of its own count whatever else is in the batch.
 // No atomics, no fences (see cg.hip).
@@ -108,8 +114,10 @@ __device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
 }
 
 // --------------------------------------------------------------- CG kernels
-// init of ONE system (grid = its nblk): x0 = b on identity rows; r0, z0;
-// p_old = 0; partials of r.z, r.r; and b.b into part[2G..3G)
+// init of ONE system (grid = its nblk): x0 = b on identity rows, 0 elsewhere;
+// r0 = S (b - A x0) (b on the other rows, 0 on identity rows: their columns
+// are eliminated); p_old = 0, y = 0; partials of r.r (scaled), ||b - A x0||^2
+// and, into part[2G..4G), b.b
 __global__ void __launch_bounds__(NT)
     bk_init(const BSys *__restrict__ D, int sys) {
   const BSys &d = D[sys];
@@ -120,13 +128,11 @@ __global__ void __launch_bounds__(NT)
     const bool ident = i >= d.n || d.row_bd[i] >= 0;
     const double ri = ident ? 0.0 : bi;
     d.x[0][i] = (ident && i < d.n) ? bi : 0.0;
-    d.xs[0][i] = 0.0f;
-    const float rv = (float)ri;
-    const float zv = (float)((double)rv * (double)d.dinv32[i]);
+    d.ys[0][i] = 0.0f;
+    const float rv = (float)(ri * d.dsq[i]);
     d.r[i] = rv;
-    d.z[i] = zv;
     d.p[0][i] = 0.0f;
-    rz = (double)rv * (double)zv;
+    rz = (double)rv * (double)rv;
     rr = ri * ri;
     bb = bi * bi;
   }
@@ -150,9 +156,10 @@ __global__ void __launch_bounds__(NT) bs_init(const BSys *__restrict__ D, BScal 
   s.tol2 = rtol2 * bb;
   s.rz = rz;
   s.rz_prev = rz;
+  s.rz_ref = rz;
+  s.rz_min = rz;
   s.rr = rr;
   s.rr_ref = rr;
-  s.rr_min = rr;
   s.stall = 0;
   s.alpha = 0.0;
   s.beta = 0.0;
@@ -174,13 +181,12 @@ __global__ void __launch_bounds__(NT)
   const double beta = s.beta;
   const float *__restrict__ po = d.p[s.it & 1];
   float *__restrict__ pn = d.p[(s.it + 1) & 1];
-  const float *__restrict__ z = d.z;
+  const float *__restrict__ r = d.r;
   double pq = 0;
   if (i < d.npad) {
-    const double p_i = (double)z[i] + beta * (double)po[i];
-    const double a = (double)d.diag32[i] * p_i +
-                     sell_dot<float>(i, d.sp, d.sc, d.sv32, [&](int c) {
-                       return (double)z[c] + beta * (double)po[c];
+    const double p_i = (double)r[i] + beta * (double)po[i];
+    const double a = p_i + sell_dot<float>(i, d.sp, d.sc, d.sv32, [&](int c) {
+                       return (double)r[c] + beta * (double)po[c];
                      });
     const float pv = (float)p_i, qv = (float)a;
     pn[i] = pv;
@@ -215,18 +221,15 @@ __global__ void __launch_bounds__(NT)
   const int i = m.y * NT + threadIdx.x;
   const double alpha = s.alpha;
   const float *__restrict__ p = d.p[(s.it + 1) & 1];
-  float *__restrict__ xs = d.xs[s.sel];
-  double rz = 0, rr = 0;
+  float *__restrict__ ys = d.ys[s.sel];
+  double rz = 0;
   if (i < d.npad) {
-    xs[i] = (float)((double)xs[i] + alpha * (double)p[i]);
+    ys[i] = (float)((double)ys[i] + alpha * (double)p[i]);
     const float rv = (float)((double)d.r[i] - alpha * (double)d.q[i]);
-    const float zv = (float)((double)rv * (double)d.dinv32[i]);
     d.r[i] = rv;
-    d.z[i] = zv;
-    rz = (double)rv * (double)zv;
-    rr = (double)rv * (double)rv;
+    rz = (double)rv * (double)rv;
   }
-  put_partials2(rz, rr, d.part, d.nblk, m.y);
+  put_partials2(rz, 0.0, d.part, d.nblk, m.y);
 }
 
 __global__ void __launch_bounds__(NT)
@@ -234,40 +237,41 @@ __global__ void __launch_bounds__(NT)
   const int sys = list[blockIdx.x];
   BScal &s = S[sys];
   if (s.status != ST_RUN || s.replace) return;
-  double rz, rr;
-  sum2(D[sys].part, D[sys].nblk, rz, rr);
+  double rz, unused;
+  sum2(D[sys].part, D[sys].nblk, rz, unused);
   if (threadIdx.x != 0) return;
   s.rz_prev = s.rz;
   s.rz = rz;
   s.beta = rz / s.rz_prev;
-  s.rr = rr;
   s.it += 1;
-  if (!(rr == rr) || !isfinite(rr)) {
+  if (!(rz == rz) || !isfinite(rz)) {
     s.status = ST_NAN;
     return;
   }
-  if (rr < 0.98 * s.rr_min) {
-    s.rr_min = rr;
+  if (rz < 0.98 * s.rz_min) {
+    s.rz_min = rz;
     s.stall = 0;
   } else {
     s.stall += 1;
   }
-  // a residual that looks converged freezes the system until the next
-  // replacement point, where the FP64 residual decides
-  if (rr <= s.tol2) s.replace = 1;
+  // a scaled residual that looks converged -- below the tolerance times the
+  // scaled-to-true ratio of the last replacement -- freezes the system until
+  // the next replacement point, where the FP64 residual decides
+  if (rz * s.rr_ref <= s.tol2 * s.rz_ref) s.replace = 1;
   else if (s.it >= s.maxit) s.status = ST_MAXIT;
 }
 
 // A system is replaced at a replacement point when it is frozen (converged
 // looking), its residual fell by delta since the last replacement, or its
-// FP32 residual has stalled.  Every
-// kernel of the point reads the same, unchanged scalars: one decision.
+// FP32 residual has stalled.  Every kernel of the point reads the same,
+// unchanged scalars: one decision.
 __device__ __forceinline__ bool replace_due(const BScal &s) {
   return s.status == ST_RUN &&
-         (s.replace || s.rr <= s.delta2 * s.rr_ref || s.stall >= STALL_ITERS);
+         (s.replace || s.rz <= s.delta2 * s.rz_ref || s.stall >= STALL_ITERS);
 }
 
-// due systems: x' = x + xs into the other buffer, r = b - A64 x', xs' = 0
+// due systems: x' = x + S y into the other buffer, the true residual
+// b - A64 x', r = S (b - A64 x'), y' = 0
 __global__ void __launch_bounds__(NT)
     bk_replace(const int2 *__restrict__ map, const BSys *__restrict__ D, const BScal *S) {
   const int2 m = map[blockIdx.x];
@@ -277,21 +281,20 @@ __global__ void __launch_bounds__(NT)
   const BSys &d = D[m.x];
   const int i = m.y * NT + threadIdx.x;
   const double *__restrict__ x = d.x[s.sel];
-  const float *__restrict__ xs = d.xs[s.sel];
+  const float *__restrict__ ys = d.ys[s.sel];
+  const double *__restrict__ dsq = d.dsq;
   double rz = 0, rr = 0;
   if (i < d.npad) {
-    const double xi = x[i] + (double)xs[i];
+    const double xi = x[i] + dsq[i] * (double)ys[i];
     const double ax = d.diag64[i] * xi + sell_dot<double>(i, d.sp, d.sc, d.sv64, [&](int c) {
-                        return x[c] + (double)xs[c];
+                        return x[c] + dsq[c] * (double)ys[c];
                       });
     d.x[s.sel ^ 1][i] = xi;
-    d.xs[s.sel ^ 1][i] = 0.0f;
+    d.ys[s.sel ^ 1][i] = 0.0f;
     const double ri = (i < d.n ? d.b[i] : 0.0) - ax;
-    const float rf = (float)ri;
-    const float zf = (float)((double)rf * (double)d.dinv32[i]);
+    const float rf = (float)(ri * dsq[i]);
     d.r[i] = rf;
-    d.z[i] = zf;
-    rz = (double)rf * (double)zf;
+    rz = (double)rf * (double)rf;
     rr = ri * ri;
   }
   put_partials2(rz, rr, d.part, d.nblk, m.y);
@@ -310,7 +313,8 @@ __global__ void __launch_bounds__(NT)
   s.beta = rz / s.rz_prev;
   s.rr = rr;
   s.rr_ref = rr;
-  s.rr_min = rr;
+  s.rz_ref = rz;
+  s.rz_min = rz;
   s.stall = 0;
   s.replace = 0;
   s.nrepl += 1;
@@ -320,14 +324,30 @@ __global__ void __launch_bounds__(NT)
   else if (s.it >= s.maxit) s.status = ST_MAXIT;
 }
 
-// solution of a finished system: out = x + xs (xs is zero after a final
+// solution of a finished system: out = x + S y (y is zero after a final
 // replacement)
 __global__ void bk_final(const BSys *__restrict__ D, const BScal *S, int sys,
                          double *__restrict__ out) {
   const BSys &d = D[sys];
   const int sel = S[sys].sel;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < d.npad) out[i] = d.x[sel][i] + (double)d.xs[sel][i];
+  if (i < d.npad) out[i] = d.x[sel][i] + d.dsq[i] * (double)d.ys[sel][i];
+}
+
+// Scaling of one assembled system: dsq = 1/sqrt(a_ii) (FP64) and the FP32
+// scaled off-diagonals a_ij dsq_i dsq_j in the SELL layout.
+__global__ void bk_scale(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
+                         const double *__restrict__ sv64, const double *__restrict__ diag64,
+                         double *__restrict__ dsq, float *__restrict__ sv32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  const double si = 1.0 / sqrt(diag64[i]);
+  dsq[i] = si;
+  const int s0 = sp[i >> 6], w = (sp[(i >> 6) + 1] - s0) >> 6, off = s0 + (i & 63);
+  for (int k = 0; k < w; k++) {
+    const int e = off + 64 * k;
+    sv32[e] = (float)(sv64[e] * si * (1.0 / sqrt(diag64[sc[e]])));
+  }
 }
 
 // --------------------------------------------------------------- synthesis
@@ -440,6 +460,13 @@ void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, con
 void batch_spmv_repeat(const int2 *map, int nmap, const BSys *D, const BScal *S, int repeat,
                        hipStream_t st) {
   for (int k = 0; k < repeat; k++) bk_spmv<<<nmap, NT, 0, st>>>(map, D, S);
+  HIPCHECK(hipGetLastError());
+}
+
+void batch_scale(const System &s, const double *sv64, const double *diag64, double *dsq,
+                 float *sv32, hipStream_t st) {
+  bk_scale<<<nb(s.npad, 256), 256, 0, st>>>(s.npad, s.d_slice_ptr.p, s.d_sell_col.p, sv64,
+                                            diag64, dsq, sv32);
   HIPCHECK(hipGetLastError());
 }
 

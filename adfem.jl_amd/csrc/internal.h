@@ -190,6 +190,8 @@ void launch_laplace_term_nu_grad(const Mesh &m, const double *grad_out, const do
 // assemble.hip
 // Fused assembly of a mesh system straight into the solver format: SELL
 // values (FP64 and FP32), diagonal and its inverse (both precisions), rhs.
+// sv, diag and rhs are always written; a null dinv / sv32 / diag32 / dinv32
+// is skipped.
 struct SellOut {
   double *sv, *diag, *dinv, *rhs;
   float *sv32, *diag32, *dinv32;

@@ -89,9 +89,9 @@ struct Req {
   int slot = -1;
   size_t off = 0;
   // device buffers (inside the arena)
-  double *x, *y, *kappa, *f, *uprev, *b, *sv64, *diag64, *dinv64, *xa, *xb, *u64, *lam64,
+  double *x, *y, *kappa, *f, *uprev, *b, *sv64, *diag64, *dsq, *xa, *xb, *u64, *lam64,
       *grad, *obsv, *loss2, *part, *stage;
-  float *sv32, *diag32, *dinv32, *xsa, *xsb, *r32, *z32, *pa, *pb, *q32;
+  float *sv32, *ysa, *ysb, *r32, *pa, *pb, *q32;
   int *obsi;
   int nobs = 0;
 };
@@ -153,18 +153,19 @@ struct Server {
     };
     auto D = [&](size_t n) { return (double *)take(n * sizeof(double)); };
     auto F = [&](size_t n) { return (float *)take(n * sizeof(float)); };
-    double *x = D(np), *y = D(np), *kappa = D(ng), *f = D(ng), *uprev = D(np), *b = D(np), *sv64 = D(ns), *diag64 = D(np), *dinv64 = D(np), *xa = D(np),
-           *xb = D(np), *u64 = D(np), *lam64 = D(np), *grad = D(ng), *obsv = D(nobs + 1),
-           *loss2 = D(2), *part = D(4 * nblk), *stage = D(2 * np);
-    float *sv32 = F(ns), *diag32 = F(np), *dinv32 = F(np), *xsa = F(np), *xsb = F(np),
-          *r32 = F(np), *z32 = F(np), *pa = F(np), *pb = F(np), *q32 = F(np);
+    double *x = D(np), *y = D(np), *kappa = D(ng), *f = D(ng), *uprev = D(np), *b = D(np),
+           *sv64 = D(ns), *diag64 = D(np), *dsq = D(np), *xa = D(np), *xb = D(np),
+           *u64 = D(np), *lam64 = D(np), *grad = D(ng), *obsv = D(nobs + 1), *loss2 = D(2),
+           *part = D(4 * nblk), *stage = D(2 * np);
+    float *sv32 = F(ns), *ysa = F(np), *ysb = F(np), *r32 = F(np), *pa = F(np), *pb = F(np),
+          *q32 = F(np);
     int *obsi = (int *)take((nobs + 1) * sizeof(int));
     if (q) {
-      q->x = x; q->y = y; q->kappa = kappa; q->f = f; q->uprev = uprev; q->b = b; q->sv64 = sv64; q->diag64 = diag64; q->dinv64 = dinv64; q->xa = xa;
-      q->xb = xb; q->u64 = u64; q->lam64 = lam64; q->grad = grad; q->obsv = obsv;
-      q->loss2 = loss2; q->part = part; q->stage = stage; q->sv32 = sv32; q->diag32 = diag32;
-      q->dinv32 = dinv32; q->xsa = xsa; q->xsb = xsb; q->r32 = r32; q->z32 = z32;
-      q->pa = pa; q->pb = pb; q->q32 = q32; q->obsi = obsi;
+      q->x = x; q->y = y; q->kappa = kappa; q->f = f; q->uprev = uprev; q->b = b;
+      q->sv64 = sv64; q->diag64 = diag64; q->dsq = dsq; q->xa = xa; q->xb = xb;
+      q->u64 = u64; q->lam64 = lam64; q->grad = grad; q->obsv = obsv; q->loss2 = loss2;
+      q->part = part; q->stage = stage; q->sv32 = sv32; q->ysa = ysa; q->ysb = ysb;
+      q->r32 = r32; q->pa = pa; q->pb = pb; q->q32 = q32; q->obsi = obsi;
     }
     return off;
   }
@@ -188,15 +189,13 @@ struct Server {
     d.sv64 = q.sv64;
     d.diag64 = q.diag64;
     d.sv32 = q.sv32;
-    d.diag32 = q.diag32;
-    d.dinv32 = q.dinv32;
+    d.dsq = q.dsq;
     d.b = b;
     d.x[0] = q.xa;
     d.x[1] = q.xb;
-    d.xs[0] = q.xsa;
-    d.xs[1] = q.xsb;
+    d.ys[0] = q.ysa;
+    d.ys[1] = q.ysb;
     d.r = q.r32;
-    d.z = q.z32;
     d.p[0] = q.pa;
     d.p[1] = q.pb;
     d.q = q.q32;
@@ -288,8 +287,10 @@ struct Server {
     const System &s = *topos[q.r.topo].sys;
     inputs(q);
     admfem_problem p = problem(q);
+    // FP64 matrix and rhs; the FP32 scaled matrix of the inner iteration
     launch_assemble_to(s, q.x, q.y, p,
-                       SellOut{q.sv64, q.diag64, q.dinv64, q.b, q.sv32, q.diag32, q.dinv32}, st);
+                       SellOut{q.sv64, q.diag64, nullptr, q.b, nullptr, nullptr, nullptr}, st);
+    batch_scale(s, q.sv64, q.diag64, q.dsq, q.sv32, st);
   }
 
   void start_cg(Req &q) {

@@ -249,7 +249,7 @@ def batched_spmv_roofline(systems, sides, nsys, repeat=200):
     fused = nbytes + 2 * 4 * nrows   # + p_old read and p write of the fused p update
     r = roof(nbytes, avg, "bk_spmv", batch=f"the stream's first {nsys} requests, one "
              "Poisson system each, one launch over all of them",
-             format="SELL-64, FP32 values, FP64 arithmetic, fused p = z + beta p_old",
+             format="SELL-64, FP32 scaled off-diagonals, FP64 arithmetic, fused p = r + beta p_old",
              fused_op_bytes=int(fused),
              fused_op_frac=round(fused / avg / 1e9 / HBM_PEAK_GBS, 4))
     t = measured_traffic(nsys)
