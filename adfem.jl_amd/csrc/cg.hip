@@ -31,6 +31,7 @@
 #include <chrono>
 #include <cmath>
 
+#include "batched.h"
 #include "geom.h"
 
 namespace admfem {
@@ -43,14 +44,15 @@ struct CGScalars {
   double rz, rz_prev;    // r.z of the current / previous residual
   double alpha, beta;
   double rr;             // ||r||^2 of the current residual (recursive)
-  double rr_ref;         // mixed: ||r||^2 at the last replacement
-  double rr_min;         // mixed: smallest recursive ||r||^2 since then
+  double rr_ref;         // mixed: true ||b - A x||^2 at the last replacement
+  double rz_ref;         // mixed: scaled r.r at the last replacement
+  double rz_min;         // mixed: smallest scaled r.r since then
   int mixed;
   int status;            // ST_*
   int it;                // completed iterations
   int replace;           // mixed: replacement pending
   int nrepl;
-  int stall;             // mixed: iterations without a new rr_min
+  int stall;             // mixed: iterations without a new rz_min
 };
 
 
@@ -60,7 +62,9 @@ constexpr int RT = 1024;  // threads of the single-block scalar kernels
 struct CGWork {
   int npad = 0, nblocks = 0;
   DevBuf<double> x, r64, z64, pa64, pb64, q64;
-  DevBuf<float> xs32, r32, z32, pa32, pb32, q32;
+  DevBuf<float> xs32, r32, pa32, pb32, q32;
+  DevBuf<double> dsq;    // mixed: 1 / sqrt(a_ii)
+  DevBuf<float> sv32s;   // mixed: FP32 scaled off-diagonals a_ij dsq_i dsq_j
   DevBuf<double> part;  // 3 x nblocks block partials
   DevBuf<CGScalars> sc;
   CGScalars *h_sc = nullptr;  // pinned
@@ -195,7 +199,8 @@ __global__ void __launch_bounds__(RT) s_init(int G, const double *part, CGScalar
   S->rz = t[0];
   S->rr = t[1];
   S->rr_ref = t[1];
-  S->rr_min = t[1];
+  S->rz_ref = t[0];
+  S->rz_min = t[0];
   S->stall = 0;
   S->bb = t[2];
   S->tol2 = S->rtol2 * t[2];
@@ -278,57 +283,127 @@ __global__ void __launch_bounds__(RT) s_beta(int G, const double *part, CGScalar
   S->rz_prev = S->rz;
   S->rz = t[0];
   S->beta = t[0] / S->rz_prev;
-  S->rr = t[1];
   S->it += 1;
-  if (!(t[1] == t[1]) || !isfinite(t[1])) {
+  if (!(t[0] == t[0]) || !isfinite(t[0])) {
     S->status = ST_NAN;
   } else if (S->mixed) {
-    if (t[1] < 0.98 * S->rr_min) {
-      S->rr_min = t[1];
+    // scaled form: r.r of the scaled residual is the monitor; a replacement
+    // when it fell by delta, stalled, or looks converged (below the
+    // tolerance times the scaled-to-true ratio of the last replacement)
+    if (t[0] < 0.98 * S->rz_min) {
+      S->rz_min = t[0];
       S->stall = 0;
     } else {
       S->stall += 1;
     }
-    if (t[1] <= S->tol2 || t[1] <= S->delta2 * S->rr_ref || S->stall >= STALL_ITERS)
+    if (t[0] * S->rr_ref <= S->tol2 * S->rz_ref || t[0] <= S->delta2 * S->rz_ref ||
+        S->stall >= STALL_ITERS)
       S->replace = 1;
     if (S->it >= maxit && !S->replace) S->status = ST_MAXIT;
   } else {
+    S->rr = t[1];
     if (t[1] <= S->tol2) S->status = ST_CONV;
     else if (S->it >= maxit) S->status = ST_MAXIT;
   }
 }
 
-// mixed, when a replacement is pending (or forced at the end): x += xs
+// ---- mixed precision, scaled form: plain CG on S A S (S = diag^-1/2,
+// unit diagonal) with FP32 storage, the solution x = x64 + S y
+
+// x0 = b on identity rows; r = S r0 (FP32); y = 0; p_old = 0; partials of
+// the scaled r.r, the true ||r0||^2 and b.b
 __global__ void __launch_bounds__(NT)
-    k_fold(int npad, double *__restrict__ x, float *__restrict__ xs, const CGScalars *S,
-           int force) {
+    k_minit(int n, int npad, const int *__restrict__ row_bd, const double *__restrict__ b,
+            const double *__restrict__ dsq, double *__restrict__ x, float *__restrict__ ys,
+            float *__restrict__ r, float *__restrict__ po, double *part) {
+  const int i = my_row();
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < npad) {
+    const double bi = i < n ? b[i] : 0.0;
+    const bool ident = i >= n || row_bd[i] >= 0;
+    const double ri = ident ? 0.0 : bi;
+    x[i] = (ident && i < n) ? bi : 0.0;
+    ys[i] = 0.0f;
+    const float rv = (float)(ri * dsq[i]);
+    r[i] = rv;
+    po[i] = 0.0f;
+    v[0] = (double)rv * (double)rv;
+    v[1] = ri * ri;
+    v[2] = bi * bi;
+  }
+  put_partials<3>(v, part);
+}
+
+// p = r + beta p_old; q = (S A S) p (unit diagonal); partials of p.q
+__global__ void __launch_bounds__(NT)
+    k_mspmv(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
+            const float *__restrict__ sv, const float *__restrict__ r,
+            const float *__restrict__ po, float *__restrict__ pn, float *__restrict__ q,
+            double *part, const CGScalars *S) {
+  if (S->status != ST_RUN) return;
+  const double beta = S->beta;
+  const int i = my_row();
+  double v[1] = {0.0};
+  if (i < npad) {
+    const double p_i = (double)r[i] + beta * (double)po[i];
+    const double a = p_i + sell_dot<float>(i, sp, sc, sv, [&](int c) {
+                       return (double)r[c] + beta * (double)po[c];
+                     });
+    const float pv = (float)p_i, qv = (float)a;
+    pn[i] = pv;
+    q[i] = qv;
+    v[0] = (double)pv * (double)qv;
+  }
+  put_partials<1>(v, part);
+}
+
+// y += alpha p; r -= alpha q; partials of the scaled r.r
+__global__ void __launch_bounds__(NT)
+    k_mupdate(int npad, const float *__restrict__ p, const float *__restrict__ q,
+              float *__restrict__ ys, float *__restrict__ r, double *part, const CGScalars *S) {
+  if (S->status != ST_RUN) return;
+  const double alpha = S->alpha;
+  const int i = my_row();
+  double v[2] = {0.0, 0.0};
+  if (i < npad) {
+    ys[i] = (float)((double)ys[i] + alpha * (double)p[i]);
+    const float rv = (float)((double)r[i] - alpha * (double)q[i]);
+    r[i] = rv;
+    v[0] = (double)rv * (double)rv;
+  }
+  put_partials<2>(v, part);
+}
+
+// when a replacement is pending (or forced at the end): x += S y, y = 0
+__global__ void __launch_bounds__(NT)
+    k_mfold(int npad, double *__restrict__ x, float *__restrict__ ys,
+            const double *__restrict__ dsq, const CGScalars *S, int force) {
   if (!force && (S->status != ST_RUN || !S->replace)) return;
   const int i = my_row();
   if (i < npad) {
-    x[i] += (double)xs[i];
-    xs[i] = 0.0f;
+    x[i] += dsq[i] * (double)ys[i];
+    ys[i] = 0.0f;
   }
 }
 
-// mixed, when a replacement is pending: r = b - A x in FP64 (FP64 matrix),
-// handed to the FP32 iteration with z = D^-1 r; partials of r.z, r.r
+// when a replacement is pending: the true residual b - A x in FP64 (FP64
+// matrix), handed to the FP32 iteration as r = S (b - A x); partials of the
+// scaled r.r and the true ||b - A x||^2
 __global__ void __launch_bounds__(NT)
-    k_replace(int n, int npad, const int *__restrict__ sp, const int *__restrict__ sc,
-              const double *__restrict__ sv, const double *__restrict__ diag,
-              const double *__restrict__ b, const double *__restrict__ x,
-              float *__restrict__ r32, float *__restrict__ z32,
-              const float *__restrict__ dinv32, double *part, const CGScalars *S) {
+    k_mreplace(int n, int npad, const int *__restrict__ sp, const int *__restrict__ sc,
+               const double *__restrict__ sv, const double *__restrict__ diag,
+               const double *__restrict__ b, const double *__restrict__ x,
+               const double *__restrict__ dsq, float *__restrict__ r, double *part,
+               const CGScalars *S) {
   if (S->status != ST_RUN || !S->replace) return;
   const int i = my_row();
   double v[2] = {0.0, 0.0};
   if (i < npad) {
     const double ax = diag[i] * x[i] + sell_dot<double>(i, sp, sc, sv, [&](int c) { return x[c]; });
     const double ri = (i < n ? b[i] : 0.0) - ax;
-    const float rf = (float)ri;
-    const float zf = (float)((double)rf * (double)dinv32[i]);
-    r32[i] = rf;
-    z32[i] = zf;
-    v[0] = (double)rf * (double)zf;
+    const float rf = (float)(ri * dsq[i]);
+    r[i] = rf;
+    v[0] = (double)rf * (double)rf;
     v[1] = ri * ri;
   }
   put_partials<2>(v, part);
@@ -344,7 +419,8 @@ __global__ void __launch_bounds__(RT) s_replace(int G, const double *part, CGSca
   S->beta = t[0] / S->rz_prev;
   S->rr = t[1];
   S->rr_ref = t[1];
-  S->rr_min = t[1];
+  S->rz_ref = t[0];
+  S->rz_min = t[0];
   S->stall = 0;
   S->replace = 0;
   S->nrepl += 1;
@@ -393,7 +469,8 @@ static CGWork *ensure_work(System &s) {
     w->q64.alloc(s.npad);
     w->xs32.alloc(s.npad);
     w->r32.alloc(s.npad);
-    w->z32.alloc(s.npad);
+    w->dsq.alloc(s.npad);
+    w->sv32s.alloc(std::max(1, s.slice_ptr.back()));
     w->pa32.alloc(s.npad);
     w->pb32.alloc(s.npad);
     w->q32.alloc(s.npad);
@@ -420,13 +497,13 @@ void spmv_repeat(System &s, int precision, int repeat, hipStream_t st) {
   HIPCHECK(hipMemcpyAsync(w->sc.p, &init, sizeof(CGScalars), hipMemcpyHostToDevice, st));
   HIPCHECK(hipStreamSynchronize(st));  // init lives on this stack frame
   const unsigned fb = (unsigned)((s.npad + 255) / 256);
-  if (precision == ADMFEM_PREC_MIXED) {
-    k_fill<float><<<fb, 256, 0, st>>>(w->z32.p, s.npad, 1.0f);
+  if (precision == ADMFEM_PREC_MIXED) {  // the scaled-form kernel of the mixed solve
+    batch_scale(s, s.d_sell_val.p, s.d_diag.p, w->dsq.p, w->sv32s.p, st);
+    k_fill<float><<<fb, 256, 0, st>>>(w->r32.p, s.npad, 1.0f);
     k_fill<float><<<fb, 256, 0, st>>>(w->pa32.p, s.npad, 0.0f);
     for (int k = 0; k < repeat; k++)
-      k_spmv<float><<<nb, NT, 0, st>>>(s.npad, s.d_slice_ptr.p, s.d_sell_col.p,
-                                       s.d_sell_val32.p, s.d_diag32.p, w->z32.p, w->pa32.p,
-                                       w->pb32.p, w->q32.p, w->part.p, w->sc.p);
+      k_mspmv<<<nb, NT, 0, st>>>(s.npad, s.d_slice_ptr.p, s.d_sell_col.p, w->sv32s.p,
+                                 w->r32.p, w->pa32.p, w->pb32.p, w->q32.p, w->part.p, w->sc.p);
   } else {
     k_fill<double><<<fb, 256, 0, st>>>(w->z64.p, s.npad, 1.0);
     k_fill<double><<<fb, 256, 0, st>>>(w->pa64.p, s.npad, 0.0);
@@ -455,9 +532,10 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
   CGScalars *S = w->sc.p;
   double *part = w->part.p;
   const int *sp = s.d_slice_ptr.p, *scol = s.d_sell_col.p;
-  if (mixed) {
-    k_init<float><<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, s.d_dinv32.p, w->x.p,
-                                     w->xs32.p, w->r32.p, w->z32.p, w->pa32.p, part);
+  if (mixed) {  // scaled form: S A S from the FP64 matrix, then the FP32 iteration
+    batch_scale(s, s.d_sell_val.p, s.d_diag.p, w->dsq.p, w->sv32s.p, st);
+    k_minit<<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, w->dsq.p, w->x.p, w->xs32.p,
+                               w->r32.p, w->pa32.p, part);
   } else {
     k_init<double><<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, s.d_dinv.p, w->x.p, nullptr,
                                       w->r64.p, w->z64.p, w->pa64.p, part);
@@ -474,16 +552,15 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
       const bool odd = it & 1;
       if (mixed) {
         float *po = odd ? w->pb32.p : w->pa32.p, *pn = odd ? w->pa32.p : w->pb32.p;
-        k_spmv<float><<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val32.p, s.d_diag32.p,
-                                         w->z32.p, po, pn, w->q32.p, part, S);
+        k_mspmv<<<nb, NT, 0, st>>>(np, sp, scol, w->sv32s.p, w->r32.p, po, pn, w->q32.p, part,
+                                   S);
         s_alpha<<<1, RT, 0, st>>>(nb, part, S);
-        k_update<float, float><<<nb, NT, 0, st>>>(np, pn, w->q32.p, w->xs32.p, w->r32.p,
-                                                  w->z32.p, s.d_dinv32.p, part, S);
+        k_mupdate<<<nb, NT, 0, st>>>(np, pn, w->q32.p, w->xs32.p, w->r32.p, part, S);
         s_beta<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
         if ((it & 7) == 7 || it + 1 == o.max_iter) {  // replacement check every 8
-          k_fold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, S, 0);
-          k_replace<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b,
-                                       w->x.p, w->r32.p, w->z32.p, s.d_dinv32.p, part, S);
+          k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, w->dsq.p, S, 0);
+          k_mreplace<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b,
+                                        w->x.p, w->dsq.p, w->r32.p, part, S);
           s_replace<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
         }
       } else {
@@ -501,20 +578,24 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
     HIPCHECK(hipStreamSynchronize(st));
     const CGScalars &h = *w->h_sc;
     if (h.status != ST_RUN || it >= o.max_iter) break;
-    // next chunk: the iterations left, estimated from the observed rate
+    // next chunk: the iterations left, estimated from the observed rate of
+    // the monitored residual (mixed: the scaled one, against the tolerance
+    // times the scaled-to-true ratio of the last replacement)
+    const double mon = mixed ? h.rz : h.rr;
+    const double target = mixed ? h.tol2 * h.rz_ref / h.rr_ref : h.tol2;
     int next = 16;
-    if (rr_prev > 0 && h.rr > 0 && h.it > it_prev && h.rr < rr_prev) {
-      double rate = std::log(h.rr / rr_prev) / (h.it - it_prev);
-      double left = std::log(h.tol2 / h.rr) / rate;
+    if (rr_prev > 0 && mon > 0 && h.it > it_prev && mon < rr_prev) {
+      double rate = std::log(mon / rr_prev) / (h.it - it_prev);
+      double left = std::log(target / mon) / rate;
       if (std::isfinite(left)) next = (int)std::min(1024.0, std::max(8.0, 0.9 * left));
     }
-    rr_prev = h.rr;
+    rr_prev = mon;
     it_prev = h.it;
     chunk = o.check_every > 0 ? o.check_every : (next + 1) / 2 * 2;
   }
   const bool converged = w->h_sc->status == ST_CONV;
   if (mixed && !converged)  // stopped without a final replacement
-    k_fold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, S, 1);
+    k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, w->dsq.p, S, 1);
   k_true_residual<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b, w->x.p,
                                      part);
   HIPCHECK(hipGetLastError());

@@ -308,7 +308,7 @@ def precision_form(m, rtol, kernels_only=False):
                                                _lib.stream_ptr()))
         avg = hip_time(lambda: _lib.check(L.admfem_system_spmv_repeat(
             S._h, code, 200, None, _lib.stream_ptr())), 200)
-        rl[prec] = roof(nb.value, avg, "k_spmv<%s>" % ("float" if prec == "mixed" else "double"))
+        rl[prec] = roof(nb.value, avg, "k_mspmv (FP32, scaled)" if prec == "mixed" else "k_spmv<double>")
     # assembly: fused K, F and Dirichlet into CSR values + rhs, then the SELL copy
     nnz, N, ne, ng = S.nnz, mesh.nnode, mesh.nelem, mesh.ngauss
     b_asm = ng * 8 + 12 * ne + 16 * N + nnz * 8
