@@ -6,10 +6,12 @@ Node and element ids follow the reference's Julia API: 1-based in ``elems``,
 ccall getters do (deps/MFEM/API.cpp:17-65).
 """
 import ctypes
+import os
 
 import numpy as np
 
 from . import _lib
+from .meshfile import read_mesh
 from .meshgen import rectangle
 
 P1 = 1
@@ -22,14 +24,21 @@ class Mesh:
     elems nelem x 3 **1-based** (src/MFEM/MFEM.jl:60-119).
     ``Mesh(m, n, h; version=1)``: the structured generator
     (src/MFEM/MFEM.jl:134-171).
+    ``Mesh(filename, file_format=None)``: a mesh file (src/MFEM/MUtils.jl:
+    33-50; formats in admfem.meshfile).
     After construction ``elems`` holds the orientation-fixed connectivity, as
     the reference's mfem_get_element_to_vertices overwrite does
     (src/MFEM/MFEM.jl:106).
     """
 
     def __init__(self, *args, order=-1, degree=1, lorder=-1, version=1, seed=None,
-                 zero_based=False):
-        if len(args) == 3 and all(isinstance(a, (int, np.integer)) for a in args[:2]):
+                 zero_based=False, file_format=None):
+        if len(args) == 1 and isinstance(args[0], (str, os.PathLike)):
+            coords, elems = read_mesh(os.fspath(args[0]), file_format)
+            zero_based = True
+            if order == -1:
+                order = 2  # MUtils.jl:33 defaults order = 2
+        elif len(args) == 3 and all(isinstance(a, (int, np.integer)) for a in args[:2]):
             m, n, h = args
             coords, elems = rectangle(int(m), int(n), float(h), version=version, seed=seed)
             zero_based = True

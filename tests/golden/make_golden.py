@@ -12,6 +12,10 @@ c1_m64.npz  the same case on Mesh(64,64,1/64) (8192 triangles, 4225 nodes):
             from the generator and the coefficient formulas).
 heat_m6.npz a jittered 6x6 mesh, implicit heat step A = M + dt K,
             rhs = M u_prev + dt F: the Dirichlet output, u and dL/dkappa.
+twoholes_large.npz  the reference's twoholes_large.stl mesh (1303 nodes,
+            2432 triangles, two holes) with the Poisson example's
+            coefficients: mesh, bcnode, u and dL/dkappa.  Reads the reference
+            checkout, so it runs where /root/reference exists.
 """
 import os
 import sys
@@ -82,10 +86,26 @@ def heat(m, seed):
                 u=r["u"], grad_kappa=r["grad_kappa"], A_rhs=r["rhs_bc"])
 
 
+def twoholes():
+    """The mesh of docs/src/codes/Poisson/inverse.jl (deps/MFEM/MeshData/
+    twoholes_large.stl, read as Mesh(filename) does) with that example's
+    coefficients kappa = sin(x)(1+y^2)+1, f = 1e5(x+y), homogeneous
+    Dirichlet on bcnode; u and dL/dkappa of L = sum u^2."""
+    from admfem.meshfile import read_mesh
+    c, e = read_mesh("/root/reference/deps/MFEM/MeshData/twoholes_large.stl")
+    om = O.OracleMesh(c, e)
+    kappa = np.sin(om.gx) * (1 + om.gy ** 2) + 1.0
+    f = 1e5 * (om.gx + om.gy)
+    bd = om.bcnode()
+    r = O.forward_adjoint(om, kappa, f, bd, np.zeros(bd.size), lambda u: 2 * u)
+    return dict(coords=c, elems=e, bd=bd, u=r["u"], grad_kappa=r["grad_kappa"])
+
+
 def main():
     np.savez_compressed(os.path.join(HERE, "c1_m8.npz"), **c1(8, True))
     np.savez_compressed(os.path.join(HERE, "c1_m64.npz"), **c1(64, False))
     np.savez_compressed(os.path.join(HERE, "heat_m6.npz"), **heat(6, 3))
+    np.savez_compressed(os.path.join(HERE, "twoholes_large.npz"), **twoholes())
 
 
 if __name__ == "__main__":

@@ -176,3 +176,31 @@ def test_c4_gradient_m512_finite_differences_and_oracle():
     eg = rel(grad, ref["grad_kappa"])
     print(f"rel err vs oracle chain {eg:.2e}", flush=True)
     assert eg < 1e-8
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("mixed", 1e-8)])
+def test_twoholes_poisson_example_matches_golden(prec, tol, tmp_path):
+    """docs/src/codes/Poisson/inverse.jl's forward problem on the reference's
+    twoholes_large.stl (an unstructured mesh with two holes), read through
+    Mesh(filename) from an STL written from the fixture."""
+    d = load("twoholes_large.npz")
+    c, e = d["coords"], d["elems"]
+    p = tmp_path / "twoholes.stl"
+    with open(p, "w") as fh:
+        fh.write("solid twoholes\n")
+        for t in e:
+            fh.write("facet normal 0 0 1\n outer loop\n")
+            fh.write("".join(f"  vertex {float(c[v, 0])!r} {float(c[v, 1])!r} 0\n" for v in t))
+            fh.write(" endloop\nendfacet\n")
+        fh.write("endsolid twoholes\n")
+    gm = admfem.Mesh(str(p))
+    np.testing.assert_array_equal(gm.nodes, c)
+    gx, gy = admfem.gauss_nodes(gm).T
+    kappa, f = np.sin(gx) * (1 + gy ** 2) + 1.0, 1e5 * (gx + gy)
+    bd = admfem.bcnode(gm)
+    np.testing.assert_array_equal(np.sort(bd), np.sort(d["bd"]))
+    S = admfem.DirichletSystem(gm, bd)
+    out = S.forward_adjoint(np.arange(gm.nnode), np.zeros(gm.nnode), precision=prec,
+                            rtol=1e-13 if prec == "fp64" else 1e-11, kappa=kappa, f=f)
+    assert rel(host(out["u"]), d["u"]) < tol
+    assert rel(host(out["grad_kappa"]), d["grad_kappa"]) < tol
