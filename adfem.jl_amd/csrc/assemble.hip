@@ -6,8 +6,8 @@
 // + impose_Dirichlet_boundary_conditions produce in the reference
 // (FemLaplaceScalar.h:3-26, ComputeFemMassMatrixMfem.h:4-29,
 // FemSourceScalar.h:4-15, ImposeDirichlet.h:27-60), written straight into
-// the solver's format: SELL-64 off-diagonal values in FP64 and FP32, the
-// diagonal and its inverse in both precisions, and the right-hand side.
+// the solver's format: SELL-64 off-diagonal values (FP64), the diagonal and
+// its inverse, and the right-hand side.
 //
 // A block owns one 64-row SELL slice (blocks mapped XCD-contiguously) and
 // works in two passes over LDS:
@@ -63,44 +63,62 @@ struct ElemIn {
 };
 
 // The element's symmetric matrix (packed upper triangle, 6 values) and its
-// three lifted load terms.
+// three lifted load terms, in closed form for the affine P1 map.  With
+// d_p = (x_{p+2} - x_{p+1}, y_{p+1} - y_{p+2}) and det = 2 * signed area,
+// grad phi_p = (d_p.y, d_p.x) / det and w_k = ipw_k * area / 0.5 =
+// ipw_k * |det| (Common.cpp:116; |det| / 2 is Heron's area up to rounding),
+// so the per-Gauss-point sums of FemLaplaceScalar.h:7-25,
+// ComputeFemMassMatrixMfem.h:4-29 and FemSourceScalar.h:4-15 become
+//   K_pq = lap * (sum_k ipw_k kappa_k) * (d_p . d_q) / |det|,
+//   M_pq = |det| * sum_k ipw_k rho_k h_pk h_qk,
+//   F_p  = |det| * sum_k ipw_k f_k h_pk:
+// no square roots and one division per element.  The values agree with the
+// per-Gauss-point form to rounding (a few ulps); the COO ops keep that form.
 template <int NQ, bool MASS>
 __device__ __forceinline__ void element(const ElemIn<NQ> &in, const QuadRule &rule,
                                         const admfem_problem &P, const int *__restrict__ row_bd,
                                         double *ae, double *rl) {
-  const ElemGeom G = elem_geom(in.ex[0], in.ey[0], in.ex[1], in.ey[1], in.ex[2], in.ey[2]);
-  double ck = 0.0, mm[6] = {0, 0, 0, 0, 0, 0}, src[3] = {0, 0, 0};
+  const double dx[3] = {in.ex[2] - in.ex[1], in.ex[0] - in.ex[2], in.ex[1] - in.ex[0]};
+  const double dy[3] = {in.ey[1] - in.ey[2], in.ey[2] - in.ey[0], in.ey[0] - in.ey[1]};
+  const double ad = fabs(dx[2] * dy[1] - dx[1] * dy[2]);
+  double sk = 0.0, mm[6] = {0, 0, 0, 0, 0, 0}, src[3] = {0, 0, 0};
 #pragma unroll
   for (int k = 0; k < NQ; k++) {
-    const double w = gauss_weight(rule.w[k], G.area);
-    ck += w * in.kq[k];
+    sk += rule.w[k] * in.kq[k];
     const double h[3] = {shape(0, rule.xi[k], rule.eta[k]), shape(1, rule.xi[k], rule.eta[k]),
                          shape(2, rule.xi[k], rule.eta[k])};
     if (MASS) {
 #pragma unroll
       for (int p = 0; p < 3; p++) {
-        const double wr = w * in.rq[k] * h[p];
+        const double wr = rule.w[k] * in.rq[k] * h[p];
 #pragma unroll
         for (int q = p; q < 3; q++) mm[sym3(p, q)] += wr * h[q];
       }
     }
     if (P.f) {
+      const double wf = rule.w[k] * in.fq[k];
 #pragma unroll
-      for (int p = 0; p < 3; p++) src[p] += in.fq[k] * h[p] * w;
+      for (int p = 0; p < 3; p++) src[p] += wf * h[p];
     }
   }
+  if (MASS) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) mm[i] *= ad;
+  }
+  const double c = P.lap_coef * sk / ad;
   double a[6];
 #pragma unroll
   for (int p = 0; p < 3; p++)
 #pragma unroll
     for (int q = p; q < 3; q++) {
-      double val = P.lap_coef * ck * (G.gx[p] * G.gx[q] + G.gy[p] * G.gy[q]);
+      double val = c * (dx[p] * dx[q] + dy[p] * dy[q]);
       if (MASS && P.mass_coef != 0.0) val += P.mass_coef * mm[sym3(p, q)];
       a[sym3(p, q)] = val;
     }
   double r[3];
+  const double sc = P.src_coef * ad;
 #pragma unroll
-  for (int p = 0; p < 3; p++) r[p] = P.src_coef * src[p];
+  for (int p = 0; p < 3; p++) r[p] = sc * src[p];
   if (MASS && P.u_prev) {
     double up[3];
 #pragma unroll
@@ -138,7 +156,10 @@ __global__ void __launch_bounds__(AS_NT)
   extern __shared__ double lds[];
   double *ae = lds;                     // [max_e][6] element matrices
   double *rl = ae + 6 * (size_t)max_e;  // [max_e][3] element load terms
-  double *vl = rl + 3 * (size_t)max_e;  // [AS_ROWS][wmax] row accumulators
+  // [wmax + 2][AS_ROWS] row accumulators, slot-major: slots [0, wmax) are
+  // the row's SELL positions, slot wmax its diagonal, wmax + 1 a sink for
+  // lifted (boundary) columns, so pass 2 adds without branching
+  double *vl = rl + 3 * (size_t)max_e;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int row = blk * AS_ROWS + (int)threadIdx.x;  // phase 2: threads < AS_ROWS
   const int e0 = blk_eptr[blk];
@@ -155,36 +176,39 @@ __global__ void __launch_bounds__(AS_NT)
 #pragma unroll
     for (int k = 0; k < PF; k++) wb[k] = t0 + k < t1 ? pairs[t0 + k] : 0u;
   }
-  for (int k = threadIdx.x; k < AS_ROWS * wmax; k += AS_NT) vl[k] = 0.0;
-  // pass 1: elements (MASS: a mass term or u_prev is present), two per
-  // thread and step with every load of both issued before either computes
-  for (int t = threadIdx.x; t < ne; t += 2 * AS_NT) {
-    ElemIn<NQ> in[2];
-    const bool two = t + AS_NT < ne;
+  for (int k = threadIdx.x; k < AS_ROWS * (wmax + 2); k += AS_NT) vl[k] = 0.0;
+  // pass 1: elements (MASS: a mass term or u_prev is present), up to three
+  // per thread and step with every load issued before any computes (a
+  // 64-row block of a triangulation touches ~260 elements: one step)
+  for (int t = threadIdx.x; t < ne; t += 3 * AS_NT) {
+    ElemIn<NQ> in[3];
+    const bool b1 = t + AS_NT < ne, b2 = t + 2 * AS_NT < ne;
     in[0].template load<MASS>(blk_rec[e0 + t], x, y, P);
-    if (two) in[1].template load<MASS>(blk_rec[e0 + t + AS_NT], x, y, P);
+    if (b1) in[1].template load<MASS>(blk_rec[e0 + t + AS_NT], x, y, P);
+    if (b2) in[2].template load<MASS>(blk_rec[e0 + t + 2 * AS_NT], x, y, P);
     element<NQ, MASS>(in[0], rule, P, row_bd, ae + 6 * t, rl + 3 * t);
-    if (two) element<NQ, MASS>(in[1], rule, P, row_bd, ae + 6 * (t + AS_NT), rl + 3 * (t + AS_NT));
+    if (b1) element<NQ, MASS>(in[1], rule, P, row_bd, ae + 6 * (t + AS_NT), rl + 3 * (t + AS_NT));
+    if (b2)
+      element<NQ, MASS>(in[2], rule, P, row_bd, ae + 6 * (t + 2 * AS_NT),
+                        rl + 3 * (t + 2 * AS_NT));
   }
   __syncthreads();
   // pass 2: rows
   if (threadIdx.x >= AS_ROWS || row >= npad) return;
-  double *my = vl + threadIdx.x * wmax;
+  double *my = vl + threadIdx.x;  // slot k at my[AS_ROWS * k]: no bank conflicts
   double d = 1.0, acc = 0.0;  // padding rows: identity, rhs 0
   if (row < n) {
     if (bi >= 0) {
       acc = P.bdval ? P.bdval[bi] : 0.0;  // (b,b,1.0) of ImposeDirichlet.h:46-50
     } else {
-      d = 0.0;
       auto add = [&](uint32_t w) {
         const int p = (int)(w & 3u), le = (int)(w >> 20);
         const double *a = ae + 6 * le;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
           const uint32_t code = (w >> (2 + SLOT_BITS * q)) & 63u;
-          const double aq = a[sym3(p, q)];
-          if (code == PAIR_DIAG) d += aq;
-          else if (code != SLOT_LIFT) my[code] += aq;
+          const int sl = code < PAIR_DIAG ? (int)code : wmax + (int)(code - PAIR_DIAG);
+          my[AS_ROWS * sl] += a[sym3(p, q)];
         }
         acc += rl[3 * le + p];
       };
@@ -192,21 +216,18 @@ __global__ void __launch_bounds__(AS_NT)
       for (int k = 0; k < PF; k++)
         if (t0 + k < t1) add(wb[k]);
       for (int t = t0 + PF; t < t1; t++) add(pairs[t]);
+      d = my[AS_ROWS * wmax];
     }
   }
   const int s0 = slice_ptr[blk], width = (slice_ptr[blk + 1] - s0) >> 6;  // block = slice
   const int off = s0 + (row & 63);
-  for (int k = 0; k < width; k++) o.sv[off + 64 * k] = my[k];
-  if (o.sv32)
-    for (int k = 0; k < width; k++) o.sv32[off + 64 * k] = (float)my[k];
+  for (int k = 0; k < width; k++) o.sv[off + 64 * k] = my[AS_ROWS * k];
   o.diag[row] = d;
   if (o.dinv) o.dinv[row] = d != 0.0 ? 1.0 / d : 0.0;
-  if (o.diag32) o.diag32[row] = (float)d;
-  if (o.dinv32) o.dinv32[row] = d != 0.0 ? (float)(1.0 / d) : 0.0f;
   o.rhs[row] = acc;
 }
 
-// CSR (post-BC) -> SELL-64 off-diagonals + diagonal, FP64 and FP32 copies.
+// CSR (post-BC) -> SELL-64 off-diagonals + diagonal and its inverse (FP64).
 // Every entry of the slice is written, the padding ones with 0, so the
 // destination needs no clearing beforehand (serving reuses device memory).
 __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
@@ -214,11 +235,8 @@ __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
                                const double *__restrict__ val,
                                const int *__restrict__ slice_ptr,
                                double *__restrict__ sv,
-                               float *__restrict__ sv32,
                                double *__restrict__ diag,
-                               double *__restrict__ dinv,
-                               float *__restrict__ diag32,
-                               float *__restrict__ dinv32, int npad) {
+                               double *__restrict__ dinv, int npad) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npad) return;
   const int s0 = slice_ptr[r >> 6], width = (slice_ptr[(r >> 6) + 1] - s0) >> 6;
@@ -233,19 +251,13 @@ __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
         d = v;
       } else {
         sv[off + 64 * kk] = v;
-        sv32[off + 64 * kk] = (float)v;
         kk++;
       }
     }
   }
-  for (; kk < width; kk++) {
-    sv[off + 64 * kk] = 0.0;
-    sv32[off + 64 * kk] = 0.0f;
-  }
+  for (; kk < width; kk++) sv[off + 64 * kk] = 0.0;
   diag[r] = d;
   dinv[r] = d != 0.0 ? 1.0 / d : 0.0;
-  diag32[r] = (float)d;
-  dinv32[r] = d != 0.0 ? (float)(1.0 / d) : 0.0f;
 }
 
 // dL/dkappa_{e,k} = -scale * w_k * grad(lam~) . grad(u) on element e, with
@@ -328,7 +340,7 @@ void launch_assemble_to(const System &s, const double *x, const double *y,
                         const admfem_problem &p, const SellOut &o, hipStream_t st) {
   const Mesh &m = *s.mesh;
   const int me = std::max(s.asm_max_e, 1), w = std::max(s.sell_wmax, 1);
-  const size_t lds = (size_t)me * 9 * sizeof(double) + (size_t)AS_ROWS * w * sizeof(double);
+  const size_t lds = (size_t)me * 9 * sizeof(double) + (size_t)AS_ROWS * (w + 2) * sizeof(double);
   REQUIRE(lds <= 160 * 1024, ADMFEM_ERR_UNSUPPORTED, "assembly block too dense for LDS");
   const unsigned grid = nblk(s.npad, AS_ROWS);
   static_assert(AS_ROWS == 64, "an assembly block is one SELL slice");
@@ -349,23 +361,15 @@ void launch_assemble_to(const System &s, const double *x, const double *y,
 }
 
 void launch_assemble(System &s, const admfem_problem &p, hipStream_t st) {
-  SellOut o{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p,
-            s.d_sell_val32.p, s.d_diag32.p, s.d_dinv32.p};
+  SellOut o{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p};
   launch_assemble_to(s, s.mesh->x.p, s.mesh->y.p, p, o, st);
 }
 
-static void launch_to_sell_to(const System &s, const double *val, double *sv, float *sv32,
-                       double *diag, double *dinv, float *diag32, float *dinv32,
-                       hipStream_t st) {
-  to_sell_kernel<<<nblk(s.npad, 256), 256, 0, st>>>(s.n, s.d_rp.p, s.d_col.p, val,
-                                                    s.d_slice_ptr.p, sv, sv32, diag, dinv,
-                                                    diag32, dinv32, s.npad);
-  HIPCHECK(hipGetLastError());
-}
-
 void launch_to_sell(System &s, hipStream_t st) {
-  launch_to_sell_to(s, s.val.p, s.d_sell_val.p, s.d_sell_val32.p, s.d_diag.p, s.d_dinv.p,
-                    s.d_diag32.p, s.d_dinv32.p, st);
+  to_sell_kernel<<<nblk(s.npad, 256), 256, 0, st>>>(s.n, s.d_rp.p, s.d_col.p, s.val.p,
+                                                    s.d_slice_ptr.p, s.d_sell_val.p,
+                                                    s.d_diag.p, s.d_dinv.p, s.npad);
+  HIPCHECK(hipGetLastError());
 }
 
 void launch_kappa_gradient_to(const System &s, const double *x, const double *y,

@@ -151,9 +151,7 @@ struct System {
   DevBuf<int> d_slice_ptr;
   DevBuf<int> d_sell_col;      // absolute internal column ids
   DevBuf<double> d_sell_val;   // FP64 values
-  DevBuf<float> d_sell_val32;  // FP32 copy for the mixed-precision solver
   DevBuf<double> d_diag, d_dinv;
-  DevBuf<float> d_diag32, d_dinv32;
   CGWork *work = nullptr;
   size_t device_bytes() const;
 };
@@ -188,13 +186,11 @@ void launch_laplace_term(const Mesh &m, const double *u, const double *nu, doubl
 void launch_laplace_term_nu_grad(const Mesh &m, const double *grad_out, const double *u,
                                  double *grad_nu, hipStream_t st);
 // assemble.hip
-// Fused assembly of a mesh system straight into the solver format: SELL
-// values (FP64 and FP32), diagonal and its inverse (both precisions), rhs.
-// sv, diag and rhs are always written; a null dinv / sv32 / diag32 / dinv32
-// is skipped.
+// Fused assembly of a mesh system straight into the solver format: FP64
+// SELL values, diagonal and its inverse (skipped when null), rhs.  (The
+// mixed-precision solvers build their FP32 matrix from these, scaled.)
 struct SellOut {
   double *sv, *diag, *dinv, *rhs;
-  float *sv32, *diag32, *dinv32;
 };
 void launch_assemble(System &s, const admfem_problem &p, hipStream_t st);
 void launch_assemble_to(const System &s, const double *x, const double *y,
