@@ -144,36 +144,12 @@ int admfem_mesh_edges(admfem_mesh h, long long *edges) {
 int admfem_mesh_bcnode(admfem_mesh h, long long *out, int *count) {
   return guarded([&] {
     Mesh &m = M(h);
-    // boundary edges are used by exactly one element (src/MFEM/MCore.jl:385-406):
-    // count element-edge uses per pattern slot (lo, hi)
-    const int n = m.nnode;
-    std::vector<int> urp(n + 1, 0), ucol;
-    ucol.reserve(m.col.size());
-    for (int u = 0; u < n; u++) {
-      int r = m.perm[u];
-      for (int k = m.rp[r]; k < m.rp[r + 1]; k++) ucol.push_back(m.iperm[m.col[k]]);
-      std::sort(ucol.begin() + urp[u], ucol.end());
-      urp[u + 1] = (int)ucol.size();
-    }
-    std::vector<int> uses(ucol.size(), 0);
-    static const int ev[3][2] = {{0, 1}, {2, 1}, {0, 2}};
-    for (int e = 0; e < m.nelem; e++)
-      for (int j = 0; j < 3; j++) {
-        int a = m.elems[3 * (size_t)e + ev[j][0]], b = m.elems[3 * (size_t)e + ev[j][1]];
-        int lo = std::min(a, b), hi = std::max(a, b);
-        auto it = std::lower_bound(ucol.begin() + urp[lo], ucol.begin() + urp[lo + 1], hi);
-        uses[it - ucol.begin()]++;
-      }
-    std::vector<char> bd(n, 0);
-    for (int lo = 0; lo < n; lo++)
-      for (int k = urp[lo]; k < urp[lo + 1]; k++)
-        if (uses[k] % 2 == 1) {  // set toggling keeps odd counts
-          bd[lo] = 1;
-          bd[ucol[k]] = 1;
-        }
+    // nodes of the edges used by exactly one element (src/MFEM/MCore.jl:385-406,
+    // 439-450), flagged by the edge pass
+    mesh_edges(m);
     int c = 0;
-    for (int u = 0; u < n; u++)
-      if (bd[u]) out[c++] = u + 1;
+    for (int u = 0; u < m.nnode; u++)
+      if (m.bd_node[u]) out[c++] = u + 1;
     *count = c;
   });
 }
@@ -181,15 +157,8 @@ int admfem_mesh_bcnode(admfem_mesh h, long long *out, int *count) {
 int admfem_mesh_pattern(admfem_mesh h, int *row_ptr, int *col) {
   return guarded([&] {
     Mesh &m = M(h);
-    int pos = 0;
-    row_ptr[0] = 0;
-    for (int u = 0; u < m.nnode; u++) {
-      int r = m.perm[u];
-      int start = pos;
-      for (int k = m.rp[r]; k < m.rp[r + 1]; k++) col[pos++] = m.iperm[m.col[k]];
-      std::sort(col + start, col + pos);
-      row_ptr[u + 1] = pos;
-    }
+    std::copy(m.urp.begin(), m.urp.end(), row_ptr);
+    std::copy(m.ucol.begin(), m.ucol.end(), col);
   });
 }
 

@@ -2,9 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/admfem.h"
@@ -90,6 +93,36 @@ struct QuadRule {
 };
 int quad_rule(int order, QuadRule &r);
 
+// ------------------------------------------------------- host parallelism
+// Runs f(begin, end, chunk) over [0, n) in contiguous chunks, one per host
+// thread (at most 16, at least `grain` items each; chunk c covers
+// [c*n/nc, (c+1)*n/nc)).  Returns the number of chunks.  An exception thrown
+// by any chunk is rethrown here after every thread has joined.
+template <class F>
+int parallel_chunks(int64_t n, int64_t grain, F &&f) {
+  const int hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nc = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(hw, 16), n / std::max<int64_t>(grain, 1)));
+  if (nc == 1) {
+    f((int64_t)0, n, 0);
+    return 1;
+  }
+  std::vector<std::thread> th;
+  std::vector<std::exception_ptr> err(nc);
+  for (int c = 0; c < nc; c++)
+    th.emplace_back([&, c] {
+      try {
+        f(n * c / nc, n * (c + 1) / nc, c);
+      } catch (...) {
+        err[c] = std::current_exception();
+      }
+    });
+  for (auto &t : th) t.join();
+  for (auto &e : err)
+    if (e) std::rethrow_exception(e);
+  return nc;
+}
+constexpr int64_t PAR_GRAIN = 16384;  // items per host thread, at least
+
 // --------------------------------------------------------------------- mesh
 struct Mesh {
   int nnode = 0, nelem = 0, order = 2;
@@ -105,8 +138,10 @@ struct Mesh {
   std::vector<uint8_t> n2e_p;   // local vertex of the row node
   std::vector<int> rp;          // pre-BC pattern, internal numbering
   std::vector<int> col;
+  std::vector<int> urp, ucol;   // pre-BC pattern, user numbering (rows sorted)
   int64_t nedges = -1;
   std::vector<long long> edges;  // lazily built, 2*nedges, 1-based user ids
+  std::vector<char> bd_node;     // built with edges: node on a boundary edge
   // device
   DevBuf<double> x, y;          // internal order
   DevBuf<int> conn;             // [3][nelem] internal ids (SoA)
@@ -121,7 +156,7 @@ struct Mesh {
 
 Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
                  int order);
-void mesh_edges(Mesh &m);  // fills m.edges/m.nedges (MFEM numbering)
+void mesh_edges(Mesh &m);  // fills m.edges/m.nedges (MFEM numbering), m.bd_node
 
 // ------------------------------------------------------------------ system
 struct CGWork;

@@ -169,24 +169,38 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
       for (int e = 0; e < nelem; e++)
         for (int k = 0; k < 3; k++) uinc[pos[m->elems[3 * (size_t)e + k]]++] = e;
     }
-    // pre-BC pattern, user numbering: union of incident elements' vertices
-    std::vector<int> urp(nnode + 1, 0), ucol;
-    ucol.reserve((size_t)nnode * 8);
-    std::vector<int> cand;
-    for (int i = 0; i < nnode; i++) {
-      cand.clear();
-      for (int k = uptr[i]; k < uptr[i + 1]; k++) {
-        const int *v = &m->elems[3 * (size_t)uinc[k]];
-        cand.push_back(v[0]);
-        cand.push_back(v[1]);
-        cand.push_back(v[2]);
-      }
-      std::sort(cand.begin(), cand.end());
-      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-      REQUIRE((int)cand.size() <= MAX_ROW, ADMFEM_ERR_UNSUPPORTED,
-              "a node has more than 61 neighbours");
-      ucol.insert(ucol.end(), cand.begin(), cand.end());
-      urp[i + 1] = (int)ucol.size();
+    // pre-BC pattern, user numbering: union of incident elements' vertices,
+    // built per chunk of nodes on host threads, then concatenated
+    std::vector<int> &urp = m->urp, &ucol = m->ucol;
+    urp.assign(nnode + 1, 0);
+    {
+      std::vector<std::vector<int>> part(16);
+      auto rows = [&](int64_t b, int64_t e, int c) {
+        std::vector<int> &out = part[c];
+        out.reserve((size_t)(e - b) * 8);
+        std::vector<int> cand;
+        for (int i = (int)b; i < (int)e; i++) {
+          cand.clear();
+          for (int k = uptr[i]; k < uptr[i + 1]; k++) {
+            const int *v = &m->elems[3 * (size_t)uinc[k]];
+            cand.push_back(v[0]);
+            cand.push_back(v[1]);
+            cand.push_back(v[2]);
+          }
+          std::sort(cand.begin(), cand.end());
+          cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+          REQUIRE((int)cand.size() <= MAX_ROW, ADMFEM_ERR_UNSUPPORTED,
+                  "a node has more than 61 neighbours");
+          out.insert(out.end(), cand.begin(), cand.end());
+          urp[i + 1] = (int)cand.size();
+        }
+      };
+      parallel_chunks(nnode, PAR_GRAIN, rows);
+      for (int i = 0; i < nnode; i++) urp[i + 1] += urp[i];
+      ucol.resize(urp[nnode]);
+      parallel_chunks(nnode, PAR_GRAIN, [&](int64_t b, int64_t, int c) {
+        std::copy(part[c].begin(), part[c].end(), ucol.begin() + urp[b]);
+      });
     }
     // internal numbering: reverse Cuthill-McKee over the node graph.  Rows
     // then run along BFS levels: the k-th neighbours of consecutive rows are
@@ -220,13 +234,17 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
     }
     std::vector<uint32_t> slots(uinc.size());
     std::vector<int> conn_int(3 * (size_t)nelem), conn_usr(3 * (size_t)nelem);
-    for (int e = 0; e < nelem; e++)
-      for (int k = 0; k < 3; k++) {
-        int u = m->elems[3 * (size_t)e + k];
-        conn_usr[(size_t)k * nelem + e] = u;
-        conn_int[(size_t)k * nelem + e] = P[u];
-      }
-    for (int r = 0; r < nnode; r++) {
+    parallel_chunks(nelem, PAR_GRAIN, [&](int64_t b, int64_t e1, int) {
+      for (int e = (int)b; e < (int)e1; e++)
+        for (int k = 0; k < 3; k++) {
+          int u = m->elems[3 * (size_t)e + k];
+          conn_usr[(size_t)k * nelem + e] = u;
+          conn_int[(size_t)k * nelem + e] = P[u];
+        }
+    });
+    // rows are independent: host threads over row chunks
+    parallel_chunks(nnode, PAR_GRAIN, [&](int64_t rb, int64_t re, int) {
+    for (int r = (int)rb; r < (int)re; r++) {
       int u = IP[r];
       int *c = &m->col[m->rp[r]];
       int len = 0;
@@ -251,6 +269,7 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
         slots[t] = w | (uint32_t)p;
       }
     }
+    });
     // device copies
     std::vector<double> xs(nnode), ys(nnode);
     for (int r = 0; r < nnode; r++) {
@@ -284,40 +303,63 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
 // Edge table in MFEM's numbering: an edge gets the next id the first time it
 // is met walking the oriented elements and their edges (0,1),(1,2),(2,0)
 // (Mesh::GetVertexToVertexTable / DSTable::Push order); output as
-// Common.cpp:133-140: [min+1 ...; max+1 ...].
+// Common.cpp:133-140: [min+1 ...; max+1 ...].  The same pass counts each
+// edge's uses for the boundary nodes (edges used by one element,
+// src/MFEM/MCore.jl:385-406 and 439-450: set toggling keeps odd counts).
+//
+// Per node v (host threads over node chunks), walking v's incident elements
+// in ascending order: the first sighting of edge (v, w) with v < w is its
+// first appearance overall, since every element holding the edge holds v.
+// A serial scan of the (element, edge) keys in walk order then numbers them.
 void mesh_edges(Mesh &m) {
   if (m.nedges >= 0) return;
   const int n = m.nnode;
-  // pattern rows in user numbering give an O(1) slot per (lo, hi) pair
-  std::vector<int> urp(n + 1, 0), ucol;
-  ucol.reserve(m.col.size());
-  for (int u = 0; u < n; u++) {
-    int r = m.perm[u];
-    for (int k = m.rp[r]; k < m.rp[r + 1]; k++) ucol.push_back(m.iperm[m.col[k]]);
-    std::sort(ucol.begin() + urp[u], ucol.end());
-    urp[u + 1] = (int)ucol.size();
-  }
-  std::vector<int> id(ucol.size(), -1);
-  std::vector<int> lo_of, hi_of;
   static const int ev[3][2] = {{0, 1}, {1, 2}, {2, 0}};
-  for (int e = 0; e < m.nelem; e++)
-    for (int j = 0; j < 3; j++) {
-      int a = m.elems[3 * (size_t)e + ev[j][0]], b = m.elems[3 * (size_t)e + ev[j][1]];
-      int lo = std::min(a, b), hi = std::max(a, b);
-      int s = (int)(std::lower_bound(ucol.begin() + urp[lo], ucol.begin() + urp[lo + 1], hi) -
-                    ucol.begin());
-      if (id[s] < 0) {
-        id[s] = (int)lo_of.size();
-        lo_of.push_back(lo);
-        hi_of.push_back(hi);
+  std::vector<char> first(3 * (size_t)m.nelem, 0);
+  m.bd_node.assign(n, 0);
+  parallel_chunks(n, PAR_GRAIN, [&](int64_t b, int64_t e1, int) {
+    int cnt[MAX_ROW + 1];
+    char seen[MAX_ROW + 1];
+    for (int v = (int)b; v < (int)e1; v++) {
+      const int *c = &m.ucol[m.urp[v]];
+      const int len = m.urp[v + 1] - m.urp[v];
+      std::fill(cnt, cnt + len, 0);
+      std::fill(seen, seen + len, 0);
+      const int r = m.perm[v];
+      for (int t = m.n2e_ptr[r]; t < m.n2e_ptr[r + 1]; t++) {
+        const int el = m.n2e_elem[t];
+        const int *vv = &m.elems[3 * (size_t)el];
+        for (int j = 0; j < 3; j++) {
+          const int a = vv[ev[j][0]], bq = vv[ev[j][1]];
+          if (a == bq || (a != v && bq != v)) continue;
+          const int w = a == v ? bq : a;
+          const int sl = (int)(std::lower_bound(c, c + len, w) - c);
+          cnt[sl]++;
+          if (v < w && !seen[sl]) {
+            seen[sl] = 1;
+            first[3 * (size_t)el + j] = 1;
+          }
+        }
       }
+      char bd = 0;
+      for (int k = 0; k < len; k++) bd |= (char)(cnt[k] & 1);
+      m.bd_node[v] = bd;
+    }
+  });
+  std::vector<long long> lo_of, hi_of;
+  lo_of.reserve((size_t)n * 3);
+  hi_of.reserve((size_t)n * 3);
+  for (size_t k = 0; k < first.size(); k++)
+    if (first[k]) {
+      const int *vv = &m.elems[3 * (k / 3)];
+      const int a = vv[ev[k % 3][0]], bq = vv[ev[k % 3][1]];
+      lo_of.push_back(std::min(a, bq) + 1);
+      hi_of.push_back(std::max(a, bq) + 1);
     }
   m.nedges = (int64_t)lo_of.size();
   m.edges.resize(2 * lo_of.size());
-  for (size_t i = 0; i < lo_of.size(); i++) {
-    m.edges[i] = lo_of[i] + 1;
-    m.edges[lo_of.size() + i] = hi_of[i] + 1;
-  }
+  std::copy(lo_of.begin(), lo_of.end(), m.edges.begin());
+  std::copy(hi_of.begin(), hi_of.end(), m.edges.begin() + lo_of.size());
 }
 
 size_t System::device_bytes() const {
@@ -409,51 +451,67 @@ System *system_build(Mesh *m, const long long *bdnode, int nbd) {
     }
     build_solver_layout(s);
     // assembly blocks: elements incident to the block's interior rows, and
-    // one pair word per (row, element) incidence
+    // one pair word per (row, element) incidence; host threads over chunks
+    // of blocks, element lists concatenated in block order
     const int nblk = (s->npad + AS_ROWS - 1) / AS_ROWS;
     std::vector<int> beptr(nblk + 1, 0), belem;
     std::vector<uint32_t> pairs(m->n2e_elem.size(), 0);
-    std::vector<int> offd;
-    for (int b = 0; b < nblk; b++) {
-      const int r0 = b * AS_ROWS, r1 = std::min(r0 + AS_ROWS, n);
-      const size_t first = belem.size();
-      for (int r = r0; r < r1; r++)
-        if (s->row_bd[r] < 0)
-          for (int t = m->n2e_ptr[r]; t < m->n2e_ptr[r + 1]; t++) belem.push_back(m->n2e_elem[t]);
-      std::sort(belem.begin() + first, belem.end());
-      belem.erase(std::unique(belem.begin() + first, belem.end()), belem.end());
-      const int ne = (int)(belem.size() - first);
-      REQUIRE(ne <= PAIR_MAX_ELEMS, ADMFEM_ERR_UNSUPPORTED, "assembly block has too many elements");
-      s->asm_max_e = std::max(s->asm_max_e, ne);
-      beptr[b + 1] = (int)belem.size();
-      for (int r = r0; r < r1; r++) {
-        if (s->row_bd[r] >= 0) continue;
-        offd.clear();
-        for (int k = s->rp[r]; k < s->rp[r + 1]; k++)
-          if (s->col[k] != r) offd.push_back(s->col[k]);
-        for (int t = m->n2e_ptr[r]; t < m->n2e_ptr[r + 1]; t++) {
-          const int e = m->n2e_elem[t];
-          const uint32_t lei = (uint32_t)(std::lower_bound(belem.begin() + first, belem.end(), e) -
-                                          (belem.begin() + first));
-          uint32_t w = m->n2e_p[t] | (lei << 20);
-          for (int q = 0; q < 3; q++) {
-            const int cq = m->perm[m->elems[3 * (size_t)e + q]];
-            uint32_t code;
-            if (s->row_bd[cq] >= 0) code = SLOT_LIFT;
-            else if (cq == r) code = PAIR_DIAG;
-            else code = (uint32_t)(std::lower_bound(offd.begin(), offd.end(), cq) - offd.begin());
-            w |= code << (2 + SLOT_BITS * q);
+    std::vector<std::vector<int>> part(16);
+    std::vector<int> part_max(16, 0);
+    auto blocks = [&](int64_t bb, int64_t be, int c) {
+      std::vector<int> &bl = part[c];
+      std::vector<int> offd;
+      for (int b = (int)bb; b < (int)be; b++) {
+        const int r0 = b * AS_ROWS, r1 = std::min(r0 + AS_ROWS, n);
+        const size_t first = bl.size();
+        for (int r = r0; r < r1; r++)
+          if (s->row_bd[r] < 0)
+            for (int t = m->n2e_ptr[r]; t < m->n2e_ptr[r + 1]; t++) bl.push_back(m->n2e_elem[t]);
+        std::sort(bl.begin() + first, bl.end());
+        bl.erase(std::unique(bl.begin() + first, bl.end()), bl.end());
+        const int ne = (int)(bl.size() - first);
+        REQUIRE(ne <= PAIR_MAX_ELEMS, ADMFEM_ERR_UNSUPPORTED,
+                "assembly block has too many elements");
+        part_max[c] = std::max(part_max[c], ne);
+        beptr[b + 1] = ne;
+        for (int r = r0; r < r1; r++) {
+          if (s->row_bd[r] >= 0) continue;
+          offd.clear();
+          for (int k = s->rp[r]; k < s->rp[r + 1]; k++)
+            if (s->col[k] != r) offd.push_back(s->col[k]);
+          for (int t = m->n2e_ptr[r]; t < m->n2e_ptr[r + 1]; t++) {
+            const int e = m->n2e_elem[t];
+            const uint32_t lei = (uint32_t)(std::lower_bound(bl.begin() + first, bl.end(), e) -
+                                            (bl.begin() + first));
+            uint32_t w = m->n2e_p[t] | (lei << 20);
+            for (int q = 0; q < 3; q++) {
+              const int cq = m->perm[m->elems[3 * (size_t)e + q]];
+              uint32_t code;
+              if (s->row_bd[cq] >= 0) code = SLOT_LIFT;
+              else if (cq == r) code = PAIR_DIAG;
+              else code = (uint32_t)(std::lower_bound(offd.begin(), offd.end(), cq) - offd.begin());
+              w |= code << (2 + SLOT_BITS * q);
+            }
+            pairs[t] = w;
           }
-          pairs[t] = w;
         }
       }
-    }
+    };
+    parallel_chunks(nblk, 64, blocks);
+    for (int b = 0; b < nblk; b++) beptr[b + 1] += beptr[b];
+    for (int c = 0; c < 16; c++) s->asm_max_e = std::max(s->asm_max_e, part_max[c]);
+    belem.resize(beptr[nblk]);
+    parallel_chunks(nblk, 64, [&](int64_t bb, int64_t, int c) {
+      std::copy(part[c].begin(), part[c].end(), belem.begin() + beptr[bb]);
+    });
     REQUIRE(s->sell_wmax < (int)PAIR_DIAG, ADMFEM_ERR_UNSUPPORTED, "row wider than 61");
     std::vector<int4> brec(std::max<size_t>(1, belem.size()), make_int4(0, 0, 0, 0));
-    for (size_t k = 0; k < belem.size(); k++) {
-      const int *v = &m->elems[3 * (size_t)belem[k]];
-      brec[k] = make_int4(belem[k], m->perm[v[0]], m->perm[v[1]], m->perm[v[2]]);
-    }
+    parallel_chunks((int64_t)belem.size(), PAR_GRAIN, [&](int64_t kb, int64_t ke, int) {
+      for (int64_t k = kb; k < ke; k++) {
+        const int *v = &m->elems[3 * (size_t)belem[k]];
+        brec[k] = make_int4(belem[k], m->perm[v[0]], m->perm[v[1]], m->perm[v[2]]);
+      }
+    });
     s->d_row_bd.upload(s->row_bd);
     s->d_blk_eptr.upload(beptr);
     s->d_blk_rec.upload(brec);

@@ -41,7 +41,8 @@ def relerr(a, b):
 
 
 # ---------------------------------------------------------------- mesh
-@pytest.mark.parametrize("m,jit", [(2, False), (5, True), (17, True)])
+# m = 260 (67,861 nodes) is split over several host threads by the set-up
+@pytest.mark.parametrize("m,jit", [(2, False), (5, True), (17, True), (260, True)])
 def test_mesh_getters_match_oracle(m, jit):
     print(f"mesh getters m={m} jittered={jit}", flush=True)
     gm, om = pair(m, seed=m, jittered=jit)
@@ -62,8 +63,10 @@ def test_mesh_2x2_known_answers_gpu():
     np.testing.assert_allclose(admfem.get_area(gm), np.full(8, 0.125), atol=1e-15)
 
 
-def test_pattern_matches_oracle_full_and_post_bc():
-    gm, om = pair(9, seed=3)
+@pytest.mark.parametrize("m", [9, 260])
+def test_pattern_matches_oracle_full_and_post_bc(m):
+    print(f"pattern m={m}", flush=True)
+    gm, om = pair(m, seed=3)
     I, V = om.laplace_coo(np.ones(om.ngauss))
     rp, col, _ = O.coo_to_csr(I, V, om.nnode)
     grp, gcol = gm.pattern()
@@ -234,9 +237,10 @@ def _oracle_system(om, kap, f, bd, bdval, dt=None, u_prev=None, rho=None):
     return O.csr_matrix(oi, ov, om.nnode), orhs
 
 
-@pytest.mark.parametrize("heat", [False, True])
-def test_system_assembly_matches_oracle(heat):
-    gm, om = pair(12, seed=7)
+@pytest.mark.parametrize("m,heat", [(12, False), (12, True), (260, True)])
+def test_system_assembly_matches_oracle(m, heat):
+    print(f"system assembly m={m} heat={heat}", flush=True)
+    gm, om = pair(m, seed=7)
     rng = np.random.default_rng(5)
     kap = np.exp(meshgen.grf(np.stack([om.gx, om.gy], 1), 1))
     f = rng.uniform(-1, 1, om.ngauss)
