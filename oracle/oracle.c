@@ -32,6 +32,24 @@ int orc_tri_rule(int order, double *xi, double *eta, double *weight) {
     weight[0] = weight[1] = weight[2] = wt;
     return 3;
   }
+  if (order == 3 || order == 4) {
+    /* the 6-point degree-4 rule (Strang & Fix; Dunavant 1985, rule 4): two
+     * orbits (a,a),(a,b),(b,a), b = 1-2a, listed orbit by orbit in the same
+     * point order as the 3-point rule (P2's default order 4,
+     * src/MFEM/MFEM.jl:74-75) */
+    const double a[2] = {0.091576213509770743459571463402202508,
+                         0.44594849091596488631832925388305199};
+    const double wt[2] = {0.054975871827660933819163162450105264,
+                          0.11169079483900573284750350421656140};
+    for (int o = 0; o < 2; o++) {
+      const double b = 1.0 - 2.0 * a[o];
+      xi[3 * o + 0] = a[o]; eta[3 * o + 0] = a[o];
+      xi[3 * o + 1] = a[o]; eta[3 * o + 1] = b;
+      xi[3 * o + 2] = b;    eta[3 * o + 2] = a[o];
+      weight[3 * o] = weight[3 * o + 1] = weight[3 * o + 2] = wt[o];
+    }
+    return 6;
+  }
   return -1;
 }
 
@@ -97,6 +115,67 @@ int orc_mesh_init(const double *coords, int nnode, int *elems, int nelem,
   return nq;
 }
 
+/* P2 shape functions on the reference triangle in barycentric form
+ * (l0 = 1-xi-eta, l1 = xi, l2 = eta): vertices l_k(2 l_k - 1), then the edge
+ * midpoint functions 4 l0 l1, 4 l1 l2, 4 l2 l0 -- the nodal basis of MFEM's
+ * H1_TriangleElement(2) (vertices, then edges (0,1),(1,2),(2,0)), with their
+ * derivatives d/dxi, d/deta. */
+static void p2_ref(double x, double y, double h[6], double d[6][2]) {
+  const double l0 = 1.0 - x - y;
+  h[0] = l0 * (2.0 * l0 - 1.0);
+  h[1] = x * (2.0 * x - 1.0);
+  h[2] = y * (2.0 * y - 1.0);
+  h[3] = 4.0 * l0 * x;
+  h[4] = 4.0 * x * y;
+  h[5] = 4.0 * y * l0;
+  d[0][0] = -(4.0 * l0 - 1.0); d[0][1] = -(4.0 * l0 - 1.0);
+  d[1][0] = 4.0 * x - 1.0;     d[1][1] = 0.0;
+  d[2][0] = 0.0;               d[2][1] = 4.0 * y - 1.0;
+  d[3][0] = 4.0 * (l0 - x);    d[3][1] = -4.0 * x;
+  d[4][0] = 4.0 * y;           d[4][1] = 4.0 * x;
+  d[5][0] = -4.0 * y;          d[5][1] = 4.0 * (l0 - y);
+}
+
+/* deps/MFEM/Common.cpp:20-142, P2 branch (degree 2): elems must already be
+ * orientation-fixed (orc_mesh_init); elem_edge[3e+j] is the MFEM id of the
+ * element's edge j ((0,1),(1,2),(2,0), orc_mesh_edges).  dofs (nelem x 6):
+ * the three vertices, then nnode + edge id (Common.cpp:63-72).  h/hx/hy
+ * [ngauss*6] from CalcPhysShape / CalcPhysDShape (dshape_ref J^{-1}). */
+int orc_mesh_init_p2(const double *coords, int nnode, const int *elems, int nelem,
+                     int order, const int64_t *elem_edge, int *dofs, double *h,
+                     double *hx, double *hy) {
+  double qx[8], qy[8], qw[8];
+  int nq = orc_tri_rule(order, qx, qy, qw);
+  if (nq < 0) return -1;
+  for (int e = 0; e < nelem; e++) {
+    const int *v = elems + 3 * e;
+    for (int k = 0; k < 3; k++) {
+      dofs[6 * e + k] = v[k];
+      dofs[6 * e + 3 + k] = nnode + (int)elem_edge[3 * e + k];
+    }
+    double c[3][2];
+    for (int k = 0; k < 3; k++) {
+      c[k][0] = coords[2 * v[k]];
+      c[k][1] = coords[2 * v[k] + 1];
+    }
+    double J00 = c[1][0] - c[0][0], J01 = c[2][0] - c[0][0];
+    double J10 = c[1][1] - c[0][1], J11 = c[2][1] - c[0][1];
+    double det = J00 * J11 - J01 * J10;
+    double I00 = J11 / det, I01 = -J01 / det, I10 = -J10 / det, I11 = J00 / det;
+    for (int i = 0; i < nq; i++) {
+      int g = nq * e + i;
+      double hv[6], d[6][2];
+      p2_ref(qx[i], qy[i], hv, d);
+      for (int r = 0; r < 6; r++) {
+        h[6 * g + r] = hv[r];
+        hx[6 * g + r] = d[r][0] * I00 + d[r][1] * I10;
+        hy[6 * g + r] = d[r][0] * I01 + d[r][1] * I11;
+      }
+    }
+  }
+  return nq;
+}
+
 typedef struct { int64_t key; int64_t ord; } keyord;
 static int cmp_keyord(const void *a, const void *b) {
   const keyord *x = (const keyord *)a, *y = (const keyord *)b;
@@ -108,8 +187,10 @@ static int cmp_i64(const void *a, const void *b) {
   return x < y ? -1 : (x > y);
 }
 
-/* Common.cpp:133-140 on MFEM's edge numbering. */
-int64_t orc_mesh_edges(const int *elems, int nelem, int nnode, int64_t *out) {
+/* Common.cpp:133-140 on MFEM's edge numbering; elem_edge (optional,
+ * 3*nelem) receives each element edge's id. */
+int64_t orc_mesh_edges(const int *elems, int nelem, int nnode, int64_t *out,
+                       int64_t *elem_edge) {
   static const int ev[3][2] = {{0, 1}, {1, 2}, {2, 0}};
   int64_t M = 3 * (int64_t)nelem;
   keyord *ko = (keyord *)malloc(sizeof(keyord) * (M ? M : 1));
@@ -121,43 +202,54 @@ int64_t orc_mesh_edges(const int *elems, int nelem, int nnode, int64_t *out) {
       ko[3 * e + j].ord = 3 * e + j;
     }
   qsort(ko, M, sizeof(keyord), cmp_keyord);
-  /* unique keys with their first occurrence, then order by first occurrence */
+  /* unique keys (groups) with their first occurrence, then order the groups
+   * by first occurrence: a group's rank is its edge id */
   keyord *u = (keyord *)malloc(sizeof(keyord) * (M ? M : 1));
+  int64_t *grp = (int64_t *)malloc(sizeof(int64_t) * (M ? M : 1));
   int64_t ne = 0;
-  for (int64_t i = 0; i < M; i++)
+  for (int64_t i = 0; i < M; i++) {
     if (i == 0 || ko[i].key != ko[i - 1].key) {
       u[ne].key = ko[i].ord; /* sort key: first occurrence */
-      u[ne].ord = ko[i].key;
+      u[ne].ord = ne;        /* group number */
       ne++;
     }
+    grp[i] = ne - 1;
+  }
+  int64_t *gkey = (int64_t *)malloc(sizeof(int64_t) * (ne ? ne : 1));
+  for (int64_t i = 0; i < M; i++) gkey[grp[i]] = ko[i].key;
   qsort(u, ne, sizeof(keyord), cmp_keyord);
+  int64_t *gid = (int64_t *)malloc(sizeof(int64_t) * (ne ? ne : 1));
+  for (int64_t r = 0; r < ne; r++) gid[u[r].ord] = r;
   if (out)
-    for (int64_t i = 0; i < ne; i++) {
-      out[i] = u[i].ord / nnode + 1;
-      out[ne + i] = u[i].ord % nnode + 1;
+    for (int64_t r = 0; r < ne; r++) {
+      int64_t k = gkey[u[r].ord];
+      out[r] = k / nnode + 1;
+      out[ne + r] = k % nnode + 1;
     }
-  free(ko);
-  free(u);
+  if (elem_edge)
+    for (int64_t i = 0; i < M; i++) elem_edge[ko[i].ord] = gid[grp[i]];
+  free(ko); free(u); free(grp); free(gkey); free(gid);
   return ne;
 }
 
-/* FemLaplaceScalar.h:3-26 */
-void orc_laplace_forward(const int *elems, int nelem, int nq, const double *w,
+/* FemLaplaceScalar.h:3-26: nd*nd triplets per Gauss point, p-major then q.
+ * dofs nelem x nd (user dof ids); hx/hy [ngauss*nd]. */
+void orc_laplace_forward(const int *dofs, int nd, int nelem, int nq, const double *w,
                          const double *hx, const double *hy,
                          const double *kappa, int64_t *indices, double *vv) {
   int64_t s = 0, nz = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int k = 0; k < nq; k++) {
-      const double *dx = hx + 3 * s, *dy = hy + 3 * s;
+      const double *dx = hx + nd * s, *dy = hy + nd * s;
       double kw = kappa[s], ww = w[s];
       s++;
-      for (int p = 0; p < 3; p++)
-        for (int q = 0; q < 3; q++) {
-          /* N = D * D^T * kappa * w, evaluated as Eigen does: the 3x3
-           * product first, then the two scalar factors. */
+      for (int p = 0; p < nd; p++)
+        for (int q = 0; q < nd; q++) {
+          /* N = D * D^T * kappa * w, evaluated as Eigen does: the product
+           * first, then the two scalar factors. */
           double d = dx[p] * dx[q] + dy[p] * dy[q];
-          indices[2 * nz] = elems[3 * i + p];
-          indices[2 * nz + 1] = elems[3 * i + q];
+          indices[2 * nz] = dofs[nd * i + p];
+          indices[2 * nz + 1] = dofs[nd * i + q];
           vv[nz] = d * kw * ww;
           nz++;
         }
@@ -165,16 +257,16 @@ void orc_laplace_forward(const int *elems, int nelem, int nq, const double *w,
 }
 
 /* FemLaplaceScalar.h:28-54 */
-void orc_laplace_backward(int nelem, int nq, const double *w, const double *hx,
+void orc_laplace_backward(int nd, int nelem, int nq, const double *w, const double *hx,
                           const double *hy, const double *grad_vv,
                           double *grad_kappa) {
   int64_t s = 0, nz = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int k = 0; k < nq; k++) {
-      const double *dx = hx + 3 * s, *dy = hy + 3 * s;
+      const double *dx = hx + nd * s, *dy = hy + nd * s;
       double v = 0.0;
-      for (int p = 0; p < 3; p++)
-        for (int q = 0; q < 3; q++) {
+      for (int p = 0; p < nd; p++)
+        for (int q = 0; q < nd; q++) {
           double d = (dx[p] * dx[q] + dy[p] * dy[q]) * w[s];
           v += grad_vv[nz] * d;
           nz++;
@@ -185,19 +277,19 @@ void orc_laplace_backward(int nelem, int nq, const double *w, const double *hx,
 }
 
 /* ComputeFemMassMatrixMfem.h:4-29 */
-void orc_mass_forward(const int *elems, int nelem, int nq, const double *w,
+void orc_mass_forward(const int *dofs, int nd, int nelem, int nq, const double *w,
                       const double *h, const double *rho, int64_t *indices,
                       double *vv) {
   int64_t k = 0, k0 = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int j = 0; j < nq; j++) {
-      const double *N = h + 3 * k0;
+      const double *N = h + nd * k0;
       double r = rho[k0], ww = w[k0];
       k0++;
-      for (int l = 0; l < 3; l++)
-        for (int s = 0; s < 3; s++) {
-          indices[2 * k] = elems[3 * i + l];
-          indices[2 * k + 1] = elems[3 * i + s];
+      for (int l = 0; l < nd; l++)
+        for (int s = 0; s < nd; s++) {
+          indices[2 * k] = dofs[nd * i + l];
+          indices[2 * k + 1] = dofs[nd * i + s];
           vv[k] = N[l] * N[s] * r * ww;
           k++;
         }
@@ -205,14 +297,14 @@ void orc_mass_forward(const int *elems, int nelem, int nq, const double *w,
 }
 
 /* ComputeFemMassMatrixMfem.h:31-56 */
-void orc_mass_backward(int nelem, int nq, const double *w, const double *h,
+void orc_mass_backward(int nd, int nelem, int nq, const double *w, const double *h,
                        const double *grad_vv, double *grad_rho) {
   int64_t k = 0, k0 = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int j = 0; j < nq; j++) {
-      const double *N = h + 3 * k0;
-      for (int l = 0; l < 3; l++)
-        for (int s = 0; s < 3; s++) {
+      const double *N = h + nd * k0;
+      for (int l = 0; l < nd; l++)
+        for (int s = 0; s < nd; s++) {
           grad_rho[k0] += N[l] * N[s] * w[k0] * grad_vv[k];
           k++;
         }
@@ -221,27 +313,27 @@ void orc_mass_backward(int nelem, int nq, const double *w, const double *h,
 }
 
 /* FemSourceScalar.h:4-15 */
-void orc_source_forward(const int *elems, int nelem, int nq, const double *w,
+void orc_source_forward(const int *dofs, int nd, int nelem, int nq, const double *w,
                         const double *h, const double *f, double *rhs) {
   int64_t k = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int j = 0; j < nq; j++) {
-      for (int r = 0; r < 3; r++)
-        rhs[elems[3 * i + r]] += f[k] * h[3 * k + r] * w[k];
+      for (int r = 0; r < nd; r++)
+        rhs[dofs[nd * i + r]] += f[k] * h[nd * k + r] * w[k];
       k++;
     }
 }
 
 /* FemSourceScalar.h:17-32 */
-void orc_source_backward(const int *elems, int nelem, int nq, const double *w,
+void orc_source_backward(const int *dofs, int nd, int nelem, int nq, const double *w,
                          const double *h, const double *grad_rhs,
                          double *grad_f) {
   int64_t k = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int j = 0; j < nq; j++) {
       grad_f[k] = 0.0;
-      for (int r = 0; r < 3; r++)
-        grad_f[k] += h[3 * k + r] * w[k] * grad_rhs[elems[3 * i + r]];
+      for (int r = 0; r < nd; r++)
+        grad_f[k] += h[nd * k + r] * w[k] * grad_rhs[dofs[nd * i + r]];
       k++;
     }
 }
@@ -391,73 +483,75 @@ int orc_bcnode(const int *elems, int nelem, int nnode, int64_t *out) {
   return n;
 }
 
-/* FemToGaussPointsMfem.h:6-18 (forward) and 20-33 (backward); for P1 the
- * DofToGaussPointsMfem.h:6-32 loops are the same (h = hs). */
-void orc_fem_to_gauss_forward(const int *elems, int nelem, int nq, const double *h,
+/* FemToGaussPointsMfem.h:6-18 (forward) and 20-33 (backward) with the
+ * vertex functions hs (P1 h, nd = 3); DofToGaussPointsMfem.h:6-32 is the
+ * same loop over the element's dofs with h (nd = 3 or 6). */
+void orc_fem_to_gauss_forward(const int *dofs, int nd, int nelem, int nq, const double *h,
                               const double *u, double *out) {
   int64_t k = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int j = 0; j < nq; j++) {
-      out[k] = u[elems[3 * i + 0]] * h[3 * k + 0] + u[elems[3 * i + 1]] * h[3 * k + 1] +
-               u[elems[3 * i + 2]] * h[3 * k + 2];
+      double v = 0.0;
+      for (int r = 0; r < nd; r++) v += u[dofs[nd * i + r]] * h[nd * k + r];
+      out[k] = v;
       k++;
     }
 }
 
-void orc_fem_to_gauss_backward(const int *elems, int nelem, int nq, const double *h,
+void orc_fem_to_gauss_backward(const int *dofs, int nd, int nelem, int nq, const double *h,
                                const double *grad_out, double *grad_u) {
   int64_t k = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int j = 0; j < nq; j++) {
-      for (int r = 0; r < 3; r++) grad_u[elems[3 * i + r]] += grad_out[k] * h[3 * k + r];
+      for (int r = 0; r < nd; r++) grad_u[dofs[nd * i + r]] += grad_out[k] * h[nd * k + r];
       k++;
     }
 }
 
 /* FemGradMfem.h:8-22 (forward) and 24-41 (backward). */
-void orc_fem_grad_forward(const int *elems, int nelem, int nq, const double *hx,
+void orc_fem_grad_forward(const int *dofs, int nd, int nelem, int nq, const double *hx,
                           const double *hy, const double *u, double *out) {
   int64_t k = 0, g = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int j = 0; j < nq; j++, g++) {
       out[k] = 0.0;
-      for (int r = 0; r < 3; r++) out[k] += u[elems[3 * i + r]] * hx[3 * g + r];
+      for (int r = 0; r < nd; r++) out[k] += u[dofs[nd * i + r]] * hx[nd * g + r];
       k++;
       out[k] = 0.0;
-      for (int r = 0; r < 3; r++) out[k] += u[elems[3 * i + r]] * hy[3 * g + r];
+      for (int r = 0; r < nd; r++) out[k] += u[dofs[nd * i + r]] * hy[nd * g + r];
       k++;
     }
 }
 
-void orc_fem_grad_backward(const int *elems, int nelem, int nq, const double *hx,
+void orc_fem_grad_backward(const int *dofs, int nd, int nelem, int nq, const double *hx,
                            const double *hy, const double *grad_out, double *grad_u) {
   int64_t k = 0, g = 0;
   for (int64_t i = 0; i < nelem; i++)
     for (int j = 0; j < nq; j++, g++) {
-      for (int r = 0; r < 3; r++) grad_u[elems[3 * i + r]] += grad_out[k] * hx[3 * g + r];
+      for (int r = 0; r < nd; r++) grad_u[dofs[nd * i + r]] += grad_out[k] * hx[nd * g + r];
       k++;
-      for (int r = 0; r < 3; r++) grad_u[elems[3 * i + r]] += grad_out[k] * hy[3 * g + r];
+      for (int r = 0; r < nd; r++) grad_u[dofs[nd * i + r]] += grad_out[k] * hy[nd * g + r];
       k++;
     }
 }
 
 /* ComputeLaplaceTermMfem.h:4-17 (forward) and 19-39 (backward). */
-void orc_laplace_term_forward(const int *elems, int nelem, int nq, const double *w,
+void orc_laplace_term_forward(const int *dofs, int nd, int nelem, int nq, const double *w,
                               const double *hx, const double *hy, const double *nu,
                               const double *u, double *out) {
   for (int64_t i = 0; i < nelem; i++)
     for (int j = 0; j < nq; j++) {
       const int64_t g = (int64_t)nq * i + j;
       const double nv = nu[g];
-      for (int p = 0; p < 3; p++)
-        for (int q = 0; q < 3; q++)
-          out[elems[3 * i + p]] +=
-              nv * (hx[3 * g + p] * hx[3 * g + q] * u[elems[3 * i + q]] * w[g] +
-                    hy[3 * g + p] * hy[3 * g + q] * u[elems[3 * i + q]] * w[g]);
+      for (int p = 0; p < nd; p++)
+        for (int q = 0; q < nd; q++)
+          out[dofs[nd * i + p]] +=
+              nv * (hx[nd * g + p] * hx[nd * g + q] * u[dofs[nd * i + q]] * w[g] +
+                    hy[nd * g + p] * hy[nd * g + q] * u[dofs[nd * i + q]] * w[g]);
     }
 }
 
-void orc_laplace_term_backward(const int *elems, int nelem, int nq, const double *w,
+void orc_laplace_term_backward(const int *dofs, int nd, int nelem, int nq, const double *w,
                                const double *hx, const double *hy, const double *nu,
                                const double *u, const double *grad_out, double *grad_nu,
                                double *grad_u) {
@@ -465,13 +559,13 @@ void orc_laplace_term_backward(const int *elems, int nelem, int nq, const double
     for (int j = 0; j < nq; j++) {
       const int64_t g = (int64_t)nq * i + j;
       const double nv = nu[g];
-      for (int p = 0; p < 3; p++)
-        for (int q = 0; q < 3; q++) {
-          const double go = grad_out[elems[3 * i + p]];
-          grad_nu[g] += go * (hx[3 * g + p] * hx[3 * g + q] * u[elems[3 * i + q]] * w[g] +
-                              hy[3 * g + p] * hy[3 * g + q] * u[elems[3 * i + q]] * w[g]);
-          grad_u[elems[3 * i + q]] += go * nv * hx[3 * g + p] * hx[3 * g + q] * w[g];
-          grad_u[elems[3 * i + q]] += go * nv * hy[3 * g + p] * hy[3 * g + q] * w[g];
+      for (int p = 0; p < nd; p++)
+        for (int q = 0; q < nd; q++) {
+          const double go = grad_out[dofs[nd * i + p]];
+          grad_nu[g] += go * (hx[nd * g + p] * hx[nd * g + q] * u[dofs[nd * i + q]] * w[g] +
+                              hy[nd * g + p] * hy[nd * g + q] * u[dofs[nd * i + q]] * w[g]);
+          grad_u[dofs[nd * i + q]] += go * nv * hx[nd * g + p] * hx[nd * g + q] * w[g];
+          grad_u[dofs[nd * i + q]] += go * nv * hy[nd * g + p] * hy[nd * g + q] * w[g];
         }
     }
 }

@@ -1,5 +1,5 @@
 /*
- * CPU oracle for the AdFem P1 forward/adjoint path -- TEST INFRASTRUCTURE ONLY.
+ * CPU oracle for the AdFem P1/P2 forward/adjoint path -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C restatement of the reference's CPU loops, used by tests/, by
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
@@ -17,8 +17,9 @@ extern "C" {
 #endif
 
 /* Quadrature rule of the reference triangle (area 1/2) for an integration
- * order: fills xi/eta/weight (capacity 8) and returns the point count, or -1
- * for an order this restatement does not carry. */
+ * order (0-2: centroid / 3-point; 3-4: 6-point): fills xi/eta/weight
+ * (capacity 8) and returns the point count, or -1 for an order this
+ * restatement does not carry. */
 int orc_tri_rule(int order, double *xi, double *eta, double *weight);
 
 /*
@@ -41,36 +42,52 @@ int orc_mesh_init(const double *coords, int nnode, int *elems, int nelem,
                   int order, double *gx, double *gy, double *w, double *area,
                   double *h, double *hx, double *hy);
 
+/* P2 precompute (Common.cpp:20-142 with degree 2): elems orientation-fixed
+ * (orc_mesh_init first), elem_edge[3e+j] the MFEM id of element edge j
+ * ((0,1),(1,2),(2,0), from orc_mesh_edges).  Outputs dofs [nelem x 6] (the
+ * vertices, then nnode + edge id; Common.cpp:63-72) and h/hx/hy
+ * [ngauss*6] (CalcPhysShape / CalcPhysDShape of MFEM's order-2 H1 triangle:
+ * vertex functions l(2l-1), then 4 l0 l1, 4 l1 l2, 4 l2 l0).  Gauss points,
+ * weights and areas are those of orc_mesh_init.  Returns nq or -1. */
+int orc_mesh_init_p2(const double *coords, int nnode, const int *elems, int nelem,
+                     int order, const int64_t *elem_edge, int *dofs, double *h,
+                     double *hx, double *hy);
+
 /* Edge table of the finalized mesh (MFEM numbering: first appearance over the
  * oriented elements' edges (0,1),(1,2),(2,0)); returns nedges and, when out is
  * non-NULL, writes [min+1 ...; max+1 ...] (2*nedges, 1-based) as
- * deps/MFEM/Common.cpp:133-140 does.  nedges upper bound is 3*nelem. */
-int64_t orc_mesh_edges(const int *elems, int nelem, int nnode, int64_t *out);
+ * deps/MFEM/Common.cpp:133-140 does; elem_edge (optional, 3*nelem) receives
+ * each element edge's 0-based id.  nedges upper bound is 3*nelem. */
+int64_t orc_mesh_edges(const int *elems, int nelem, int nnode, int64_t *out,
+                       int64_t *elem_edge);
+
+/* In the element ops below, dofs is nelem x nd (the element's global dof ids,
+ * 0-based; for P1 nd = 3 and dofs = elems) and h/hx/hy are [ngauss*nd]. */
 
 /* FemLaplaceScalar_forward (deps/MFEM/FemLaplace1/FemLaplaceScalar.h:3-26):
- * 9 triplets per Gauss point, p-major then q. indices [2*9*ngauss]. */
-void orc_laplace_forward(const int *elems, int nelem, int nq, const double *w,
+ * nd*nd triplets per Gauss point, p-major then q. indices [2*nd*nd*ngauss]. */
+void orc_laplace_forward(const int *dofs, int nd, int nelem, int nq, const double *w,
                          const double *hx, const double *hy,
                          const double *kappa, int64_t *indices, double *vv);
 /* FemLaplaceScalar_backward (FemLaplaceScalar.h:28-54). */
-void orc_laplace_backward(int nelem, int nq, const double *w, const double *hx,
+void orc_laplace_backward(int nd, int nelem, int nq, const double *w, const double *hx,
                           const double *hy, const double *grad_vv,
                           double *grad_kappa);
 
 /* ComputeFemMassMatrix1_forward/_backward
  * (deps/MFEM/ComputeFemMassMatrix1/ComputeFemMassMatrixMfem.h:4-56).
  * The backward accumulates into grad_rho (caller zeroes, as the op does). */
-void orc_mass_forward(const int *elems, int nelem, int nq, const double *w,
+void orc_mass_forward(const int *dofs, int nd, int nelem, int nq, const double *w,
                       const double *h, const double *rho, int64_t *indices,
                       double *vv);
-void orc_mass_backward(int nelem, int nq, const double *w, const double *h,
+void orc_mass_backward(int nd, int nelem, int nq, const double *w, const double *h,
                        const double *grad_vv, double *grad_rho);
 
 /* FemSourceScalar_forward/_backward (deps/MFEM/FemSource1/FemSourceScalar.h:4-32).
  * Forward accumulates into rhs (caller zeroes). */
-void orc_source_forward(const int *elems, int nelem, int nq, const double *w,
+void orc_source_forward(const int *dofs, int nd, int nelem, int nq, const double *w,
                         const double *h, const double *f, double *rhs);
-void orc_source_backward(const int *elems, int nelem, int nq, const double *w,
+void orc_source_backward(const int *dofs, int nd, int nelem, int nq, const double *w,
                          const double *h, const double *grad_rhs,
                          double *grad_f);
 
@@ -101,27 +118,27 @@ int64_t orc_coo_to_csr(const int64_t *indices, const double *vv, int64_t sN,
  * sum_r u[node_r] hs(r, j).  DofToGaussPointsMfem (DofToGaussPointsMfem.h:
  * 6-32) is the same loop over the element's dofs with h, identical for P1.
  * The backward accumulates into grad_u (caller zeroes). */
-void orc_fem_to_gauss_forward(const int *elems, int nelem, int nq, const double *h,
+void orc_fem_to_gauss_forward(const int *dofs, int nd, int nelem, int nq, const double *h,
                               const double *u, double *out);
-void orc_fem_to_gauss_backward(const int *elems, int nelem, int nq, const double *h,
+void orc_fem_to_gauss_backward(const int *dofs, int nd, int nelem, int nq, const double *h,
                                const double *grad_out, double *grad_u);
 
 /* FemGradMfem_forward/_backward (deps/MFEM/FemGrad/FemGradMfem.h:8-41):
  * out[2g] = sum_r u[dof_r] hx(r, j), out[2g+1] = sum_r u[dof_r] hy(r, j).
  * The backward accumulates into grad_u (caller zeroes). */
-void orc_fem_grad_forward(const int *elems, int nelem, int nq, const double *hx,
+void orc_fem_grad_forward(const int *dofs, int nd, int nelem, int nq, const double *hx,
                           const double *hy, const double *u, double *out);
-void orc_fem_grad_backward(const int *elems, int nelem, int nq, const double *hx,
+void orc_fem_grad_backward(const int *dofs, int nd, int nelem, int nq, const double *hx,
                            const double *hy, const double *grad_out, double *grad_u);
 
 /* ComputeLaplaceTermMfem_forward/_backward
  * (deps/MFEM/ComputeLaplaceTermMfem/ComputeLaplaceTermMfem.h:4-39):
  * out[dof_p] += nu_j sum_q (hx_p hx_q + hy_p hy_q) u[dof_q] w_j.  Both
  * accumulate (caller zeroes out, grad_nu and grad_u). */
-void orc_laplace_term_forward(const int *elems, int nelem, int nq, const double *w,
+void orc_laplace_term_forward(const int *dofs, int nd, int nelem, int nq, const double *w,
                               const double *hx, const double *hy, const double *nu,
                               const double *u, double *out);
-void orc_laplace_term_backward(const int *elems, int nelem, int nq, const double *w,
+void orc_laplace_term_backward(const int *dofs, int nd, int nelem, int nq, const double *w,
                                const double *hx, const double *hy, const double *nu,
                                const double *u, const double *grad_out, double *grad_nu,
                                double *grad_u);

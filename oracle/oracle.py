@@ -1,4 +1,4 @@
-"""CPU oracle for the AdFem P1 forward/adjoint path -- TEST INFRASTRUCTURE ONLY.
+"""CPU oracle for the AdFem P1/P2 forward/adjoint path -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
 leg import this module, as the checker or the timed CPU baseline.  The product
@@ -43,24 +43,25 @@ def lib():
     sig = {
         "orc_tri_rule": (_I, [_I, _dp, _dp, _dp]),
         "orc_mesh_init": (_I, [_dp, _I, _ip, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
-        "orc_mesh_edges": (_L, [_ip, _I, _I, ctypes.c_void_p]),
-        "orc_laplace_forward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp, _lp, _dp]),
-        "orc_laplace_backward": (None, [_I, _I, _dp, _dp, _dp, _dp, _dp]),
-        "orc_mass_forward": (None, [_ip, _I, _I, _dp, _dp, _dp, _lp, _dp]),
-        "orc_mass_backward": (None, [_I, _I, _dp, _dp, _dp, _dp]),
-        "orc_source_forward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp]),
-        "orc_source_backward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp]),
+        "orc_mesh_init_p2": (_I, [_dp, _I, _ip, _I, _I, _lp, _ip, _dp, _dp, _dp]),
+        "orc_mesh_edges": (_L, [_ip, _I, _I, ctypes.c_void_p, ctypes.c_void_p]),
+        "orc_laplace_forward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp, _dp, _lp, _dp]),
+        "orc_laplace_backward": (None, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp]),
+        "orc_mass_forward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp, _lp, _dp]),
+        "orc_mass_backward": (None, [_I, _I, _I, _dp, _dp, _dp, _dp]),
+        "orc_source_forward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp, _dp]),
+        "orc_source_backward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp, _dp]),
         "orc_dirichlet_forward": (_L, [_lp, _dp, _L, _lp, _dp, _I, _dp, _I, _lp, _dp, _dp]),
         "orc_dirichlet_backward": (None, [_dp, _dp, _dp, _dp, _dp, _lp, _dp, _L, _lp, _dp, _I, _I]),
         "orc_coo_to_csr": (_L, [_lp, _dp, _L, _I, _lp, _ip, _dp]),
         "orc_bcnode": (_I, [_ip, _I, _I, _lp]),
-        "orc_fem_to_gauss_forward": (None, [_ip, _I, _I, _dp, _dp, _dp]),
-        "orc_fem_to_gauss_backward": (None, [_ip, _I, _I, _dp, _dp, _dp]),
-        "orc_fem_grad_forward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp]),
-        "orc_fem_grad_backward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp]),
-        "orc_laplace_term_forward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp]),
-        "orc_laplace_term_backward": (None, [_ip, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
-                                             _dp]),
+        "orc_fem_to_gauss_forward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp]),
+        "orc_fem_to_gauss_backward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp]),
+        "orc_fem_grad_forward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp, _dp]),
+        "orc_fem_grad_backward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp, _dp]),
+        "orc_laplace_term_forward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_laplace_term_backward": (None, [_ip, _I, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp,
+                                             _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -79,15 +80,25 @@ def _i64(a):
 
 
 class OracleMesh:
-    """P1 mesh precompute (deps/MFEM/Common.cpp:20-142) on the host.
+    """P1/P2 mesh precompute (deps/MFEM/Common.cpp:20-142) on the host.
 
     ``coords`` is nnode x 2, ``elems`` nelem x 3 with 0-based vertex ids.
     After construction ``elems`` holds the orientation-fixed connectivity
-    (deps/MFEM/API.cpp:58-65 hands that back to Julia).
+    (deps/MFEM/API.cpp:58-65 hands that back to Julia).  ``degree`` 1 or 2;
+    ``order`` -1 takes the reference's default (2 for P1, 4 for P2,
+    src/MFEM/MFEM.jl:71-77).  ``dofs`` (nelem x nd) are the element dofs:
+    the vertices, then for P2 nnode + the MFEM id of edges (0,1),(1,2),(2,0).
+    ``hs`` holds the linear vertex functions (FemToGaussPoints), ``h``,
+    ``hx``, ``hy`` the degree's basis (stride nd).
     """
 
-    def __init__(self, coords, elems, order=2):
+    def __init__(self, coords, elems, order=-1, degree=1):
         L = lib()
+        if degree not in (1, 2):
+            raise ValueError("degree must be 1 or 2")
+        if order == -1:
+            order = 2 if degree == 1 else 4
+        self.degree = degree
         self.coords = _f64(coords).reshape(-1, 2)
         self.elems = np.ascontiguousarray(elems, dtype=np.int32).reshape(-1, 3).copy()
         self.nnode = self.coords.shape[0]
@@ -101,92 +112,128 @@ class OracleMesh:
         self.ngauss = ng
         self.gx, self.gy, self.w = np.zeros(ng), np.zeros(ng), np.zeros(ng)
         self.area = np.zeros(self.nelem)
-        self.h, self.hx, self.hy = np.zeros(3 * ng), np.zeros(3 * ng), np.zeros(3 * ng)
+        self.hs, self.hx, self.hy = np.zeros(3 * ng), np.zeros(3 * ng), np.zeros(3 * ng)
         L.orc_mesh_init(self.coords, self.nnode, self.elems, self.nelem, order,
-                        self.gx, self.gy, self.w, self.area, self.h, self.hx, self.hy)
+                        self.gx, self.gy, self.w, self.area, self.hs, self.hx, self.hy)
+        if degree == 1:
+            self.nd, self.dofs, self.ndof, self.h = 3, self.elems, self.nnode, self.hs
+        else:
+            ee = np.zeros(3 * self.nelem, dtype=np.int64)
+            self.nedge = L.orc_mesh_edges(self.elems, self.nelem, self.nnode, None, ee.ctypes.data)
+            self.elem_edge = ee.reshape(-1, 3)
+            self.nd, self.ndof = 6, self.nnode + self.nedge
+            self.dofs = np.zeros((self.nelem, 6), dtype=np.int32)
+            self.h, self.hx, self.hy = np.zeros(6 * ng), np.zeros(6 * ng), np.zeros(6 * ng)
+            L.orc_mesh_init_p2(self.coords, self.nnode, self.elems, self.nelem, order, ee,
+                               self.dofs, self.h, self.hx, self.hy)
 
     # --- getters (deps/MFEM/API.cpp:17-65) ---
     def edges(self):
         L = lib()
-        ne = L.orc_mesh_edges(self.elems, self.nelem, self.nnode, None)
+        ne = L.orc_mesh_edges(self.elems, self.nelem, self.nnode, None, None)
         out = np.zeros(2 * ne, dtype=np.int64)
-        L.orc_mesh_edges(self.elems, self.nelem, self.nnode, out.ctypes.data)
+        L.orc_mesh_edges(self.elems, self.nelem, self.nnode, out.ctypes.data, None)
         return out.reshape(2, ne).T.copy()
 
     def bcnode(self):
+        """Boundary dofs, 1-based: vertices ascending; for P2 then
+        nnode + the ids of boundary edges, ascending (src/MFEM/MCore.jl:439-450
+        with get_edge_dof, src/MFEM/MUtils.jl:149-163)."""
         out = np.zeros(self.nnode, dtype=np.int64)
         n = lib().orc_bcnode(self.elems, self.nelem, self.nnode, out)
-        return out[:n].copy()
+        vert = out[:n].copy()
+        if self.degree == 1:
+            return vert
+        # bcedge (MCore.jl:385-406): edges used by an odd number of elements
+        ids, cnt = np.unique(self.elem_edge.ravel(), return_counts=True)
+        return np.concatenate([vert, self.nnode + ids[cnt % 2 == 1] + 1])
 
     # --- forward / backward ops ---
     def laplace_coo(self, kappa):
-        n = 9 * self.ngauss
+        n = self.nd * self.nd * self.ngauss
         ind, vv = np.zeros(2 * n, dtype=np.int64), np.zeros(n)
-        lib().orc_laplace_forward(self.elems, self.nelem, self.nq, self.w, self.hx, self.hy,
-                                  _f64(kappa), ind, vv)
+        lib().orc_laplace_forward(self.dofs, self.nd, self.nelem, self.nq, self.w, self.hx,
+                                  self.hy, _f64(kappa), ind, vv)
         return ind.reshape(n, 2), vv
 
     def laplace_backward(self, grad_vv):
         g = np.zeros(self.ngauss)
-        lib().orc_laplace_backward(self.nelem, self.nq, self.w, self.hx, self.hy, _f64(grad_vv), g)
+        lib().orc_laplace_backward(self.nd, self.nelem, self.nq, self.w, self.hx, self.hy,
+                                   _f64(grad_vv), g)
         return g
 
     def mass_coo(self, rho):
-        n = 9 * self.ngauss
+        n = self.nd * self.nd * self.ngauss
         ind, vv = np.zeros(2 * n, dtype=np.int64), np.zeros(n)
-        lib().orc_mass_forward(self.elems, self.nelem, self.nq, self.w, self.h, _f64(rho), ind, vv)
+        lib().orc_mass_forward(self.dofs, self.nd, self.nelem, self.nq, self.w, self.h,
+                               _f64(rho), ind, vv)
         return ind.reshape(n, 2), vv
 
     def mass_backward(self, grad_vv):
         g = np.zeros(self.ngauss)
-        lib().orc_mass_backward(self.nelem, self.nq, self.w, self.h, _f64(grad_vv), g)
+        lib().orc_mass_backward(self.nd, self.nelem, self.nq, self.w, self.h, _f64(grad_vv), g)
         return g
 
     def source(self, f):
-        rhs = np.zeros(self.nnode)
-        lib().orc_source_forward(self.elems, self.nelem, self.nq, self.w, self.h, _f64(f), rhs)
+        rhs = np.zeros(self.ndof)
+        lib().orc_source_forward(self.dofs, self.nd, self.nelem, self.nq, self.w, self.h,
+                                 _f64(f), rhs)
         return rhs
 
     def source_backward(self, grad_rhs):
         g = np.zeros(self.ngauss)
-        lib().orc_source_backward(self.elems, self.nelem, self.nq, self.w, self.h,
+        lib().orc_source_backward(self.dofs, self.nd, self.nelem, self.nq, self.w, self.h,
                                   _f64(grad_rhs), g)
         return g
 
-    # FemToGaussPoints / DofToGaussPoints (P1), FemGrad, ComputeLaplaceTerm
+    # FemToGaussPoints (vertex values, linear hs), DofToGaussPoints (dofs, h),
+    # FemGrad, ComputeLaplaceTerm
     def fem_to_gauss(self, u):
         out = np.zeros(self.ngauss)
-        lib().orc_fem_to_gauss_forward(self.elems, self.nelem, self.nq, self.h, _f64(u), out)
+        lib().orc_fem_to_gauss_forward(self.elems, 3, self.nelem, self.nq, self.hs, _f64(u), out)
         return out
 
     def fem_to_gauss_backward(self, grad_out):
         g = np.zeros(self.nnode)
-        lib().orc_fem_to_gauss_backward(self.elems, self.nelem, self.nq, self.h,
+        lib().orc_fem_to_gauss_backward(self.elems, 3, self.nelem, self.nq, self.hs,
+                                        _f64(grad_out), g)
+        return g
+
+    def dof_to_gauss(self, u):
+        out = np.zeros(self.ngauss)
+        lib().orc_fem_to_gauss_forward(self.dofs, self.nd, self.nelem, self.nq, self.h, _f64(u),
+                                       out)
+        return out
+
+    def dof_to_gauss_backward(self, grad_out):
+        g = np.zeros(self.ndof)
+        lib().orc_fem_to_gauss_backward(self.dofs, self.nd, self.nelem, self.nq, self.h,
                                         _f64(grad_out), g)
         return g
 
     def fem_grad(self, u):
         out = np.zeros(2 * self.ngauss)
-        lib().orc_fem_grad_forward(self.elems, self.nelem, self.nq, self.hx, self.hy, _f64(u),
-                                   out)
+        lib().orc_fem_grad_forward(self.dofs, self.nd, self.nelem, self.nq, self.hx, self.hy,
+                                   _f64(u), out)
         return out.reshape(-1, 2)
 
     def fem_grad_backward(self, grad_out):
-        g = np.zeros(self.nnode)
-        lib().orc_fem_grad_backward(self.elems, self.nelem, self.nq, self.hx, self.hy,
+        g = np.zeros(self.ndof)
+        lib().orc_fem_grad_backward(self.dofs, self.nd, self.nelem, self.nq, self.hx, self.hy,
                                     _f64(grad_out).reshape(-1), g)
         return g
 
     def laplace_term(self, u, nu):
-        out = np.zeros(self.nnode)
-        lib().orc_laplace_term_forward(self.elems, self.nelem, self.nq, self.w, self.hx,
+        out = np.zeros(self.ndof)
+        lib().orc_laplace_term_forward(self.dofs, self.nd, self.nelem, self.nq, self.w, self.hx,
                                        self.hy, _f64(nu), _f64(u), out)
         return out
 
     def laplace_term_backward(self, u, nu, grad_out):
-        gn, gu = np.zeros(self.ngauss), np.zeros(self.nnode)
-        lib().orc_laplace_term_backward(self.elems, self.nelem, self.nq, self.w, self.hx,
-                                        self.hy, _f64(nu), _f64(u), _f64(grad_out), gn, gu)
+        gn, gu = np.zeros(self.ngauss), np.zeros(self.ndof)
+        lib().orc_laplace_term_backward(self.dofs, self.nd, self.nelem, self.nq, self.w,
+                                        self.hx, self.hy, _f64(nu), _f64(u), _f64(grad_out),
+                                        gn, gu)
         return gn, gu
 
 
@@ -263,7 +310,7 @@ def forward_adjoint(mesh, kappa, f, bd, bdval, dloss_du, *, rho=None, dt=None,
       FemLaplaceScalarGrad (FemLaplaceScalar.h:28-54) → ∂L/∂κ.
     Returns dict(u, lam, grad_kappa, loss_grad, A_bc (csr), rhs_bc, K_coo).
     """
-    N = mesh.nnode
+    N = mesh.ndof
     Ki, Kv = mesh.laplace_coo(kappa)
     F = mesh.source(f)
     if dt is None:
