@@ -1,4 +1,4 @@
-"""admfem -- MI355X-native drop-in for AdFem.jl's P1 forward/adjoint path.
+"""admfem -- MI355X-native drop-in for AdFem.jl's P1/P2 forward/adjoint path.
 
 The Python mirror of the reference's Julia operator surface
 (src/MFEM/MFEM.jl, MCore.jl, MUtils.jl) over libadmfem.so's C-ABI.

@@ -78,6 +78,7 @@ _SIGS = {
     "admfem_mesh_weights": (c_int, [c_vp, c_vp]),
     "admfem_mesh_area": (c_int, [c_vp, c_vp]),
     "admfem_mesh_elems": (c_int, [c_vp, c_vp]),
+    "admfem_mesh_dofs": (c_int, [c_vp, c_vp]),
     "admfem_mesh_edges": (c_int, [c_vp, c_vp]),
     "admfem_mesh_bcnode": (c_int, [c_vp, c_vp, c_vp]),
     "admfem_mesh_pattern": (c_int, [c_vp, c_vp, c_vp]),

@@ -14,11 +14,11 @@ from . import _lib
 from .meshfile import read_mesh
 from .meshgen import rectangle
 
-P1 = 1
+P1, P2 = 1, 2
 
 
 class Mesh:
-    """P1 triangle mesh on the GPU.
+    """P1 or P2 triangle mesh on the GPU.
 
     ``Mesh(coords, elems, order=-1, degree=1, lorder=-1)``: coords N x 2,
     elems nelem x 3 **1-based** (src/MFEM/MFEM.jl:60-119).
@@ -52,8 +52,10 @@ class Mesh:
                 lorder = args[4]
         else:
             raise TypeError("Mesh(coords, elems[, order, degree, lorder]) or Mesh(m, n, h)")
-        if degree != 1:
-            raise NotImplementedError("only degree 1 (P1) elements are built")
+        if degree in ("P1", "P2"):
+            degree = int(degree[1])
+        if degree not in (1, 2):
+            raise NotImplementedError("only degree 1 (P1) and 2 (P2) elements are built")
         coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 2)
         e0 = np.asarray(elems, dtype=np.int64).reshape(-1, 3)
         if not zero_based:
@@ -70,14 +72,17 @@ class Mesh:
             int(v) for v in info[:6])
         self.nnz_full = int(info[6])
         self.nq = int(info[7])
-        self.order = 2 if order == -1 else int(order)
+        self.order = (2 if degree == 1 else 4) if order == -1 else int(order)
         self.lorder = 6 if lorder == -1 else int(lorder)
-        self.elem_type = P1
+        self.elem_type = P1 if degree == 1 else P2
+        self.degree = degree
         self.nodes = coords.copy()
         el = np.zeros((self.nelem, 3), dtype=np.int32)
         _lib.check(L.admfem_mesh_elems(h, _lib.ptr(el)))
         self.elems = el.astype(np.int64) + 1          # oriented, 1-based
-        self.conn = self.elems.copy()                 # P1: dofs are the vertices
+        dofs = np.zeros((self.nelem, self.elem_ndof), dtype=np.int32)
+        _lib.check(L.admfem_mesh_dofs(h, _lib.ptr(dofs)))
+        self.conn = dofs.astype(np.int64) + 1         # P1: the vertices; P2: + edge dofs
         ed = np.zeros(2 * self.nedge, dtype=np.int64)
         _lib.check(L.admfem_mesh_edges(h, _lib.ptr(ed)))
         self.edges = ed.reshape(2, self.nedge).T.copy()
@@ -140,9 +145,10 @@ def fem_nodes(mesh):
 
 
 def bcnode(mesh):
-    """Boundary node ids, 1-based, ascending (src/MFEM/MCore.jl:439-450; the
-    reference returns a Set's order)."""
-    out = np.zeros(mesh.nnode, dtype=np.int64)
+    """Boundary dofs, 1-based (src/MFEM/MCore.jl:439-450): the boundary
+    vertices ascending (the reference returns a Set's order), then for P2
+    nnode + the ids of the boundary edges, ascending."""
+    out = np.zeros(mesh.ndof, dtype=np.int64)
     cnt = ctypes.c_int()
     _lib.check(_lib.lib().admfem_mesh_bcnode(mesh.handle, _lib.ptr(out), ctypes.byref(cnt)))
     return out[:cnt.value].copy()

@@ -78,7 +78,7 @@ def _dev(t, dtype=torch.float64):
 class _LaplaceCOO(torch.autograd.Function):
     @staticmethod
     def forward(ctx, kappa, mesh, fwd, bwd):
-        nt = 9 * mesh.ngauss
+        nt = mesh.elem_ndof ** 2 * mesh.ngauss
         ind = torch.empty((nt, 2), dtype=torch.int64, device=_DEV)
         vv = torch.empty(nt, dtype=torch.float64, device=_DEV)
         k = _dev(kappa)
@@ -159,28 +159,30 @@ class _ToGauss(torch.autograd.Function):
         uu = _dev(u)
         out = torch.empty(mesh.ngauss, dtype=torch.float64, device=_DEV)
         _lib.check(fwd(mesh.handle, _lib.ptr(uu), _lib.ptr(out), _lib.stream_ptr()))
-        ctx.mesh, ctx.bwd = mesh, bwd
+        ctx.mesh, ctx.bwd, ctx.n = mesh, bwd, uu.numel()
         return out
 
     @staticmethod
     def backward(ctx, g):
         g = _dev(g)
-        gu = torch.empty(ctx.mesh.ndof, dtype=torch.float64, device=_DEV)
+        gu = torch.empty(ctx.n, dtype=torch.float64, device=_DEV)
         _lib.check(ctx.bwd(ctx.mesh.handle, _lib.ptr(g), _lib.ptr(gu), _lib.stream_ptr()))
         return gu, None, None, None
 
 
-def _nodal(u, mesh):
-    if len(u) != mesh.ndof:
-        raise ValueError("length(u) must be mesh.ndof")
+def _nodal(u, mesh, n=None):
+    n = mesh.ndof if n is None else n
+    if len(u) != n:
+        raise ValueError(f"length(u) must be {n}")
     return isinstance(u, np.ndarray)
 
 
 def fem_to_gauss_points(u, mesh):
     """FemToGaussPointsMfem (deps/MFEM/FemToGaussPoints/FemToGaussPointsMfem.h:
-    6-33; src/MFEM/MUtils.jl:231-250): nodal values -> Gauss points."""
+    6-33; src/MFEM/MUtils.jl:231-250): vertex values (nnode) -> Gauss points,
+    linear interpolation for either degree."""
     L = _lib.lib()
-    host = _nodal(u, mesh)
+    host = _nodal(u, mesh, mesh.nnode)
     out = _ToGauss.apply(torch.as_tensor(u) if host else u, mesh,
                          L.admfem_fem_to_gauss_points, L.admfem_fem_to_gauss_points_grad)
     return out.cpu().numpy() if host else out
@@ -188,7 +190,8 @@ def fem_to_gauss_points(u, mesh):
 
 def dof_to_gauss_points(u, mesh):
     """DofToGaussPointsMfem (deps/MFEM/DofToGaussPoints/DofToGaussPointsMfem.h:
-    6-32; src/MFEM/MUtils.jl:253-270): for P1 the dofs are the nodes."""
+    6-32; src/MFEM/MUtils.jl:253-270): dof values (ndof) -> Gauss points with
+    the degree's basis; for P1 the dofs are the nodes."""
     L = _lib.lib()
     host = _nodal(u, mesh)
     out = _ToGauss.apply(torch.as_tensor(u) if host else u, mesh,

@@ -24,7 +24,8 @@ COMMON = [
 ]
 
 SOURCES = ["mesh.cpp", "api.cpp", "geometry.hip", "assemble.hip", "cg.hip",
-           "dirichlet.hip", "batched.hip", "server.cpp", "sparse_op.hip", "fields.hip"]
+           "dirichlet.hip", "batched.hip", "server.cpp", "sparse_op.hip", "fields.hip",
+           "p2.hip"]
 
 
 def _deps_mtime():

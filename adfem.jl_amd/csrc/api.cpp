@@ -76,9 +76,9 @@ int admfem_mesh_create(const double *coords, int nnode, const int *elems, int ne
   return guarded([&] {
     REQUIRE(out, ADMFEM_ERR_ARG, "null out");
     *out = nullptr;
-    REQUIRE(degree == 1, ADMFEM_ERR_UNSUPPORTED,
-            "only degree 1 (P1) elements are built");
-    Mesh *m = mesh_build(coords, nnode, elems, nelem, order);
+    REQUIRE(degree == 1 || degree == 2, ADMFEM_ERR_UNSUPPORTED,
+            "only degree 1 (P1) and 2 (P2) elements are built");
+    Mesh *m = mesh_build(coords, nnode, elems, nelem, order, degree);
     *out = new admfem_mesh_s{m};
   });
 }
@@ -102,8 +102,8 @@ int admfem_mesh_info(admfem_mesh h, int64_t info[8]) {
     info[0] = m.nnode;
     info[1] = m.nelem;
     info[2] = m.ngauss;
-    info[3] = m.nnode;
-    info[4] = 3;
+    info[3] = m.ndof;
+    info[4] = m.nd;
     info[5] = m.nedges;
     info[6] = (int64_t)m.col.size();
     info[7] = m.rule.nq;
@@ -133,6 +133,15 @@ int admfem_mesh_elems(admfem_mesh h, int *elems) {
   });
 }
 
+int admfem_mesh_dofs(admfem_mesh h, int *dofs) {
+  return guarded([&] {
+    Mesh &m = M(h);
+    REQUIRE(dofs, ADMFEM_ERR_ARG, "null pointer");
+    if (m.degree == 2) std::copy(m.dofs.begin(), m.dofs.end(), dofs);
+    else std::copy(m.elems.begin(), m.elems.end(), dofs);
+  });
+}
+
 int admfem_mesh_edges(admfem_mesh h, long long *edges) {
   return guarded([&] {
     Mesh &m = M(h);
@@ -150,6 +159,9 @@ int admfem_mesh_bcnode(admfem_mesh h, long long *out, int *count) {
     int c = 0;
     for (int u = 0; u < m.nnode; u++)
       if (m.bd_node[u]) out[c++] = u + 1;
+    // P2: then the boundary edges' dofs, nnode + edge id (MCore.jl:441-443)
+    for (size_t i = 0; i < m.bd_edge.size(); i++)
+      if (m.bd_edge[i]) out[c++] = m.nnode + (long long)i + 1;
     *count = c;
   });
 }
@@ -226,12 +238,18 @@ int admfem_fem_to_gauss_points_grad(admfem_mesh h, const double *grad_out, doubl
 }
 
 int admfem_dof_to_gauss_points(admfem_mesh h, const double *u, double *out, void *stream) {
-  return admfem_fem_to_gauss_points(h, u, out, stream);
+  return guarded([&] {
+    REQUIRE(u && out, ADMFEM_ERR_ARG, "null pointer");
+    launch_dof_to_gauss(M(h), u, out, S(stream));
+  });
 }
 
 int admfem_dof_to_gauss_points_grad(admfem_mesh h, const double *grad_out, double *grad_u,
                                     void *stream) {
-  return admfem_fem_to_gauss_points_grad(h, grad_out, grad_u, stream);
+  return guarded([&] {
+    REQUIRE(grad_out && grad_u, ADMFEM_ERR_ARG, "null pointer");
+    launch_dof_to_gauss_grad(M(h), grad_out, grad_u, S(stream));
+  });
 }
 
 int admfem_fem_grad(admfem_mesh h, const double *u, double *out, void *stream) {
@@ -585,13 +603,14 @@ long long *init_nnfem_mesh(double *vertices, int num_vertices, int *element_indi
   long long *ret = nullptr;
   int rc = guarded([&] {
     REQUIRE(vertices && element_indices && nedges, ADMFEM_ERR_ARG, "null pointer");
-    REQUIRE(degree == 1, ADMFEM_ERR_UNSUPPORTED, "only degree 1 (P1) elements are built");
+    REQUIRE(degree == 1 || degree == 2, ADMFEM_ERR_UNSUPPORTED,
+            "only degree 1 (P1) and 2 (P2) elements are built");
     std::vector<double> c(2 * (size_t)num_vertices);
     for (int i = 0; i < num_vertices; i++) {
       c[2 * i] = vertices[3 * i];
       c[2 * i + 1] = vertices[3 * i + 1];
     }
-    Mesh *m = mesh_build(c.data(), num_vertices, element_indices, num_elements, order);
+    Mesh *m = mesh_build(c.data(), num_vertices, element_indices, num_elements, order, degree);
     admfem_mesh h = new admfem_mesh_s{m};
     {
       std::lock_guard<std::mutex> lk(g_default_mu);
@@ -647,17 +666,21 @@ void mfem_get_area(double *a) {
   if (m) guarded([&] { d2h(a, m->area.p, (size_t)m->nelem); });
 }
 
-int mfem_get_elem_ndof(void) { return default_or_warn("mfem_get_elem_ndof") ? 3 : 0; }
+int mfem_get_elem_ndof(void) {
+  Mesh *m = default_or_warn("mfem_get_elem_ndof");
+  return m ? m->nd : 0;
+}
 
 int mfem_get_ndof(void) {
   Mesh *m = default_or_warn("mfem_get_ndof");
-  return m ? m->nnode : 0;
+  return m ? (int)m->ndof : 0;
 }
 
 void mfem_get_connectivity(long long *conn) {
   Mesh *m = default_or_warn("mfem_get_connectivity");
   if (!m) return;
-  for (size_t k = 0; k < m->elems.size(); k++) conn[k] = m->elems[k] + 1;
+  const std::vector<int> &d = m->degree == 2 ? m->dofs : m->elems;
+  for (size_t k = 0; k < d.size(); k++) conn[k] = d[k] + 1;
 }
 
 void mfem_get_element_to_vertices(long long *elems) {
@@ -671,7 +694,7 @@ void FemLaplaceScalar_forward_Julia(long long *indices, double *vv, const double
   Mesh *m = default_or_warn("FemLaplaceScalar_forward_Julia");
   if (!m) return;
   int rc = guarded([&] {
-    size_t nt = 9 * (size_t)m->ngauss;
+    size_t nt = (size_t)m->nd * m->nd * (size_t)m->ngauss;
     DevBuf<double> dk, dv;
     DevBuf<long long> di;
     dk.upload(kappa, m->ngauss);
@@ -690,9 +713,9 @@ void FemSourceScalar_forward_Julia(double *rhs, const double *f) {
   int rc = guarded([&] {
     DevBuf<double> df, dr;
     df.upload(f, m->ngauss);
-    dr.upload(rhs, m->nnode);
+    dr.upload(rhs, m->ndof);
     launch_source(*m, df.p, dr.p, true, nullptr);
-    d2h(rhs, dr.p, (size_t)m->nnode);
+    d2h(rhs, dr.p, (size_t)m->ndof);
   });
   if (rc) fprintf(stderr, "FemSourceScalar_forward_Julia: %s\n", admfem_last_error());
 }
@@ -710,20 +733,28 @@ void FemToGaussPointsMfem_Julia(double *out, const double *u) {
 }
 
 void DofToGaussPointsMfem_forward_Julia(double *out, const double *u) {
-  FemToGaussPointsMfem_Julia(out, u);
+  Mesh *m = default_or_warn("DofToGaussPointsMfem_forward_Julia");
+  if (!m) return;
+  int rc = guarded([&] {
+    DevBuf<double> du, dout(m->ngauss);
+    du.upload(u, m->ndof);
+    launch_dof_to_gauss(*m, du.p, dout.p, nullptr);
+    d2h(out, dout.p, (size_t)m->ngauss);
+  });
+  if (rc) fprintf(stderr, "DofToGaussPointsMfem_forward_Julia: %s\n", admfem_last_error());
 }
 
 void ComputeLaplaceTermMfem_forward_Julia(double *out, const double *nu, const double *u) {
   Mesh *m = default_or_warn("ComputeLaplaceTermMfem_forward_Julia");
   if (!m) return;
   int rc = guarded([&] {
-    DevBuf<double> du, dnu, dk(m->nnode);
-    du.upload(u, m->nnode);
+    DevBuf<double> du, dnu, dk(m->ndof);
+    du.upload(u, m->ndof);
     dnu.upload(nu, m->ngauss);
     launch_laplace_term(*m, du.p, dnu.p, dk.p, nullptr);
-    std::vector<double> k(m->nnode);
+    std::vector<double> k(m->ndof);
     d2h(k.data(), dk.p, k.size());
-    for (int i = 0; i < m->nnode; i++) out[i] += k[i];  // accumulates, as the reference
+    for (size_t i = 0; i < k.size(); i++) out[i] += k[i];  // accumulates, as the reference
   });
   if (rc) fprintf(stderr, "ComputeLaplaceTermMfem_forward_Julia: %s\n", admfem_last_error());
 }

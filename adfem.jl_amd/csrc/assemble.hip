@@ -353,7 +353,11 @@ void launch_assemble_to(const System &s, const double *x, const double *y,
   if (m.rule.nq == 3) {
     if (mass) go(assemble_kernel<3, true>);
     else go(assemble_kernel<3, false>);
+  } else if (m.rule.nq == 6) {
+    if (mass) go(assemble_kernel<6, true>);
+    else go(assemble_kernel<6, false>);
   } else {
+    REQUIRE(m.rule.nq == 1, ADMFEM_ERR_UNSUPPORTED, "quadrature rule not built");
     if (mass) go(assemble_kernel<1, true>);
     else go(assemble_kernel<1, false>);
   }

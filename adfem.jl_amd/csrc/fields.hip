@@ -1,4 +1,6 @@
-// Nodal fields <-> Gauss points, and the matrix-free Laplace term, for P1:
+// Nodal fields <-> Gauss points, and the matrix-free Laplace term, for P1
+// (P2 meshes dispatch to p2.hip, except FemToGaussPoints, which maps vertex
+// values with the linear functions for either degree):
 //   FemToGaussPointsMfem / DofToGaussPointsMfem (deps/MFEM/FemToGaussPoints/
 //     FemToGaussPointsMfem.h:6-33, DofToGaussPoints/DofToGaussPointsMfem.h:
 //     6-32): out[g] = sum_r u[node_r] h_r(g)
@@ -163,7 +165,19 @@ void launch_to_gauss_grad(const Mesh &m, const double *grad_out, double *grad_u,
   node_gather<NODE_TO_GAUSS_T>(m, grad_out, nullptr, grad_u, st);
 }
 
+void launch_dof_to_gauss(const Mesh &m, const double *u, double *out, hipStream_t st) {
+  if (m.degree == 2) return launch_p2_dof_to_gauss(m, u, out, st);
+  launch_to_gauss(m, u, out, st);
+}
+
+void launch_dof_to_gauss_grad(const Mesh &m, const double *grad_out, double *grad_u,
+                              hipStream_t st) {
+  if (m.degree == 2) return launch_p2_dof_to_gauss_grad(m, grad_out, grad_u, st);
+  launch_to_gauss_grad(m, grad_out, grad_u, st);
+}
+
 void launch_fem_grad(const Mesh &m, const double *u, double *out, hipStream_t st) {
+  if (m.degree == 2) return launch_p2_fem_grad(m, u, out, st);
   fem_grad_kernel<<<nb(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p, m.conn_user.p,
                                                      m.x.p, m.y.p, u, out);
   HIPCHECK(hipGetLastError());
@@ -171,16 +185,19 @@ void launch_fem_grad(const Mesh &m, const double *u, double *out, hipStream_t st
 
 void launch_fem_grad_grad(const Mesh &m, const double *grad_out, double *grad_u,
                           hipStream_t st) {
+  if (m.degree == 2) return launch_p2_fem_grad_grad(m, grad_out, grad_u, st);
   node_gather<NODE_GRAD_T>(m, grad_out, nullptr, grad_u, st);
 }
 
 void launch_laplace_term(const Mesh &m, const double *u, const double *nu, double *out,
                          hipStream_t st) {
+  if (m.degree == 2) return launch_p2_laplace_term(m, u, nu, out, st);
   node_gather<NODE_LAPLACE>(m, u, nu, out, st);
 }
 
 void launch_laplace_term_nu_grad(const Mesh &m, const double *grad_out, const double *u,
                                  double *grad_nu, hipStream_t st) {
+  if (m.degree == 2) return launch_p2_laplace_term_nu_grad(m, grad_out, u, grad_nu, st);
   laplace_term_nu_grad_kernel<<<nb(m.ngauss, 256), 256, 0, st>>>(
       m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, grad_out, u, grad_nu);
   HIPCHECK(hipGetLastError());

@@ -219,6 +219,7 @@ void launch_geometry(const Mesh &m, hipStream_t st) {
 
 void launch_laplace_coo(const Mesh &m, const double *kappa, long long *ind,
                         double *vv, hipStream_t st) {
+  if (m.degree == 2) return launch_p2_coo(m, false, kappa, ind, vv, st);
   int64_t total = 9 * m.ngauss;
   laplace_coo_kernel<<<nblk(total, 256), 256, 0, st>>>(
       m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, kappa, ind, vv);
@@ -227,6 +228,7 @@ void launch_laplace_coo(const Mesh &m, const double *kappa, long long *ind,
 
 void launch_laplace_coo_grad(const Mesh &m, const double *gvv, double *gk,
                              hipStream_t st) {
+  if (m.degree == 2) return launch_p2_coo_grad(m, false, gvv, gk, st);
   laplace_coo_grad_kernel<<<nblk(m.ngauss, 256), 256, 0, st>>>(
       m.nelem, m.rule, m.conn.p, m.x.p, m.y.p, gvv, gk);
   HIPCHECK(hipGetLastError());
@@ -234,6 +236,7 @@ void launch_laplace_coo_grad(const Mesh &m, const double *gvv, double *gk,
 
 void launch_mass_coo(const Mesh &m, const double *rho, long long *ind,
                      double *vv, hipStream_t st) {
+  if (m.degree == 2) return launch_p2_coo(m, true, rho, ind, vv, st);
   int64_t total = 9 * m.ngauss;
   mass_coo_kernel<<<nblk(total, 256), 256, 0, st>>>(
       m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, rho, ind, vv);
@@ -242,6 +245,7 @@ void launch_mass_coo(const Mesh &m, const double *rho, long long *ind,
 
 void launch_mass_coo_grad(const Mesh &m, const double *gvv, double *gr,
                           hipStream_t st) {
+  if (m.degree == 2) return launch_p2_coo_grad(m, true, gvv, gr, st);
   mass_coo_grad_kernel<<<nblk(m.ngauss, 256), 256, 0, st>>>(
       m.nelem, m.rule, m.conn.p, m.x.p, m.y.p, gvv, gr);
   HIPCHECK(hipGetLastError());
@@ -249,6 +253,7 @@ void launch_mass_coo_grad(const Mesh &m, const double *gvv, double *gr,
 
 void launch_source(const Mesh &m, const double *f, double *rhs_user,
                    bool accumulate, hipStream_t st) {
+  if (m.degree == 2) return launch_p2_source(m, f, rhs_user, accumulate, st);
   source_kernel<<<nblk(m.nnode, 256), 256, 0, st>>>(
       m.nnode, m.rule, m.nelem, m.conn.p, m.x.p, m.y.p, m.d_n2e_ptr.p,
       m.d_n2e_elem.p, m.d_slots_full.p, m.d_iperm.p, f, rhs_user,
@@ -258,6 +263,7 @@ void launch_source(const Mesh &m, const double *f, double *rhs_user,
 
 void launch_source_grad(const Mesh &m, const double *grhs_user, double *gf,
                         hipStream_t st) {
+  if (m.degree == 2) return launch_p2_source_grad(m, grhs_user, gf, st);
   source_grad_kernel<<<nblk(m.ngauss, 256), 256, 0, st>>>(
       m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, grhs_user, gf);
   HIPCHECK(hipGetLastError());

@@ -86,7 +86,7 @@ constexpr int PAIR_MAX_ELEMS = 4096;
 constexpr double REPLACE_DELTA = 0.3;
 constexpr int STALL_ITERS = 64;
 constexpr int AS_ROWS = 64;  // rows per assembly block (one SELL slice)
-constexpr int MAXQ = 3;  // quadrature points per element carried here
+constexpr int MAXQ = 6;  // quadrature points per element carried here
 struct QuadRule {
   int nq;
   double xi[MAXQ], eta[MAXQ], w[MAXQ];  // reference triangle, weights sum 1/2
@@ -126,6 +126,8 @@ constexpr int64_t PAR_GRAIN = 16384;  // items per host thread, at least
 // --------------------------------------------------------------------- mesh
 struct Mesh {
   int nnode = 0, nelem = 0, order = 2;
+  int degree = 1, nd = 3;  // element degree and dofs per element (3 or 6)
+  int64_t ndof = 0;        // nnode (P1) or nnode + nedges (P2)
   QuadRule rule{};
   int64_t ngauss = 0;
   // host copies
@@ -142,6 +144,9 @@ struct Mesh {
   int64_t nedges = -1;
   std::vector<long long> edges;  // lazily built, 2*nedges, 1-based user ids
   std::vector<char> bd_node;     // built with edges: node on a boundary edge
+  std::vector<char> bd_edge;     // built with edges (P2): edge used by one element
+  std::vector<int> elem_edge;    // P2: MFEM id of element edge j at 3e+j
+  std::vector<int> dofs;         // P2: nelem x 6 user dof ids
   // device
   DevBuf<double> x, y;          // internal order
   DevBuf<int> conn;             // [3][nelem] internal ids (SoA)
@@ -151,11 +156,13 @@ struct Mesh {
   DevBuf<uint32_t> d_slots_full;  // per incidence, slots in the pre-BC row
   DevBuf<int> d_rp, d_col;
   DevBuf<double> gx, gy, gw, area;  // geometry (user element order)
+  DevBuf<int> conn_dof;             // P2: [6][nelem] user dof ids (SoA)
+  DevBuf<int> d_d2e_ptr, d_d2e;     // P2: dof -> (element << 3 | local dof), ascending
   size_t device_bytes() const;
 };
 
 Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
-                 int order);
+                 int order, int degree = 1);
 void mesh_edges(Mesh &m);  // fills m.edges/m.nedges (MFEM numbering), m.bd_node
 
 // ------------------------------------------------------------------ system
@@ -220,6 +227,27 @@ void launch_laplace_term(const Mesh &m, const double *u, const double *nu, doubl
                          hipStream_t st);
 void launch_laplace_term_nu_grad(const Mesh &m, const double *grad_out, const double *u,
                                  double *grad_nu, hipStream_t st);
+// DofToGaussPoints (DofToGaussPointsMfem.h:6-32): the degree's basis (for P1
+// the vertex functions, as launch_to_gauss).
+void launch_dof_to_gauss(const Mesh &m, const double *u, double *out, hipStream_t st);
+void launch_dof_to_gauss_grad(const Mesh &m, const double *grad_out, double *grad_u,
+                              hipStream_t st);
+// P2 kernels (p2.hip); the generic launchers above dispatch to them.
+void launch_p2_coo(const Mesh &m, bool mass, const double *coef, long long *ind, double *vv,
+                   hipStream_t st);
+void launch_p2_coo_grad(const Mesh &m, bool mass, const double *gvv, double *out,
+                        hipStream_t st);
+void launch_p2_source(const Mesh &m, const double *f, double *rhs, bool accumulate,
+                      hipStream_t st);
+void launch_p2_source_grad(const Mesh &m, const double *grhs, double *gf, hipStream_t st);
+void launch_p2_dof_to_gauss(const Mesh &m, const double *u, double *out, hipStream_t st);
+void launch_p2_dof_to_gauss_grad(const Mesh &m, const double *g, double *gu, hipStream_t st);
+void launch_p2_fem_grad(const Mesh &m, const double *u, double *out, hipStream_t st);
+void launch_p2_fem_grad_grad(const Mesh &m, const double *g, double *gu, hipStream_t st);
+void launch_p2_laplace_term(const Mesh &m, const double *u, const double *nu, double *out,
+                            hipStream_t st);
+void launch_p2_laplace_term_nu_grad(const Mesh &m, const double *go, const double *u,
+                                    double *gnu, hipStream_t st);
 // assemble.hip
 // Fused assembly of a mesh system straight into the solver format: FP64
 // SELL values, diagonal and its inverse (skipped when null), rhs.  (The

@@ -20,8 +20,8 @@ int quad_rule(int order, QuadRule &r) {
   // Reference-triangle rules (area 1/2).  Order 2 is the symmetric 3-point
   // rule (a,a),(a,b),(b,a) with a = 1/6, b = 2/3, weight 1/6 each, in MFEM's
   // listing order (the rule Common.cpp:42-44 takes for the P1 default
-  // order 2, src/MFEM/MFEM.jl:73); orders 0/1 are the centroid rule.
-  if (order == -1) order = 2;
+  // order 2, src/MFEM/MFEM.jl:73); orders 0/1 are the centroid rule.  The
+  // caller resolves the default order (-1) for its degree.
   if (order == 0 || order == 1) {
     r.nq = 1;
     r.xi[0] = r.eta[0] = 1.0 / 3.0;
@@ -37,6 +37,24 @@ int quad_rule(int order, QuadRule &r) {
     r.w[0] = r.w[1] = r.w[2] = 1.0 / 6.0;
     return 3;
   }
+  if (order == 3 || order == 4) {
+    // the 6-point degree-4 rule (Strang & Fix; Dunavant 1985): two orbits
+    // (a,a),(a,b),(b,a), b = 1-2a, listed orbit by orbit in the order of the
+    // 3-point rule (P2's default order 4, src/MFEM/MFEM.jl:74-75)
+    const double a[2] = {0.091576213509770743459571463402202508,
+                         0.44594849091596488631832925388305199};
+    const double w[2] = {0.054975871827660933819163162450105264,
+                         0.11169079483900573284750350421656140};
+    r.nq = 6;
+    for (int o = 0; o < 2; o++) {
+      const double b = 1.0 - 2.0 * a[o];
+      r.xi[3 * o + 0] = a[o]; r.eta[3 * o + 0] = a[o];
+      r.xi[3 * o + 1] = a[o]; r.eta[3 * o + 1] = b;
+      r.xi[3 * o + 2] = b;    r.eta[3 * o + 2] = a[o];
+      r.w[3 * o] = r.w[3 * o + 1] = r.w[3 * o + 2] = w[o];
+    }
+    return 6;
+  }
   return -1;
 }
 
@@ -44,7 +62,8 @@ size_t Mesh::device_bytes() const {
   return x.bytes() + y.bytes() + conn.bytes() + conn_user.bytes() +
          d_perm.bytes() + d_iperm.bytes() + d_n2e_ptr.bytes() +
          d_n2e_elem.bytes() + d_slots_full.bytes() + d_rp.bytes() +
-         d_col.bytes() + gx.bytes() + gy.bytes() + gw.bytes() + area.bytes();
+         d_col.bytes() + gx.bytes() + gy.bytes() + gw.bytes() + area.bytes() +
+         conn_dof.bytes() + d_d2e_ptr.bytes() + d_d2e.bytes();
 }
 
 // Reverse Cuthill-McKee order of the node graph (adjacency = pattern rows
@@ -129,18 +148,23 @@ static std::vector<int> rcm_order(int n, const std::vector<int> &rp,
 
 
 Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
-                 int order) {
+                 int order, int degree) {
   REQUIRE(nnode > 0 && nelem > 0, ADMFEM_ERR_ARG, "empty mesh");
   REQUIRE(coords && elems, ADMFEM_ERR_ARG, "null coords/elems");
+  REQUIRE(degree == 1 || degree == 2, ADMFEM_ERR_UNSUPPORTED,
+          "only degree 1 (P1) and 2 (P2) elements are built");
+  if (order == -1) order = degree == 1 ? 2 : 4;  // src/MFEM/MFEM.jl:71-77
   QuadRule rule;
   REQUIRE(quad_rule(order, rule) > 0, ADMFEM_ERR_UNSUPPORTED,
-          "integration order " + std::to_string(order) +
-              " not built (P1 supports orders 1 and 2)");
+          "integration order " + std::to_string(order) + " not built (orders 0-4 are)");
   Mesh *m = new Mesh();
   try {
     m->nnode = nnode;
     m->nelem = nelem;
-    m->order = order == -1 ? 2 : order;
+    m->order = order;
+    m->degree = degree;
+    m->nd = degree == 1 ? 3 : 6;
+    m->ndof = nnode;
     m->rule = rule;
     m->ngauss = (int64_t)rule.nq * nelem;
     m->coords.assign(coords, coords + 2 * (size_t)nnode);
@@ -292,6 +316,37 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
     m->gw.alloc(m->ngauss);
     m->area.alloc(nelem);
     launch_geometry(*m, nullptr);
+    if (degree == 2) {
+      // P2 dofs (Common.cpp:63-72): the vertices, then nnode + edge id of the
+      // element's edges (0,1),(1,2),(2,0); dof -> element incidences in
+      // ascending element order for the per-dof gathers
+      mesh_edges(*m);
+      m->ndof = nnode + m->nedges;
+      m->dofs.resize(6 * (size_t)nelem);
+      std::vector<int> cd(6 * (size_t)nelem);
+      parallel_chunks(nelem, PAR_GRAIN, [&](int64_t b, int64_t e1, int) {
+        for (int e = (int)b; e < (int)e1; e++)
+          for (int k = 0; k < 3; k++) {
+            const int v = m->elems[3 * (size_t)e + k];
+            const int d = nnode + m->elem_edge[3 * (size_t)e + k];
+            m->dofs[6 * (size_t)e + k] = v;
+            m->dofs[6 * (size_t)e + 3 + k] = d;
+            cd[(size_t)k * nelem + e] = v;
+            cd[(size_t)(3 + k) * nelem + e] = d;
+          }
+      });
+      REQUIRE(m->ndof < (int64_t)1 << 31, ADMFEM_ERR_UNSUPPORTED, "too many P2 dofs");
+      std::vector<int> dptr(m->ndof + 1, 0), d2e(6 * (size_t)nelem);
+      for (size_t k = 0; k < m->dofs.size(); k++) dptr[m->dofs[k] + 1]++;
+      for (int64_t i = 0; i < m->ndof; i++) dptr[i + 1] += dptr[i];
+      std::vector<int> pos(dptr.begin(), dptr.end() - 1);
+      for (int e = 0; e < nelem; e++)
+        for (int p = 0; p < 6; p++) d2e[pos[m->dofs[6 * (size_t)e + p]]++] = e * 8 + p;
+      REQUIRE(nelem < (1 << 28), ADMFEM_ERR_UNSUPPORTED, "too many P2 elements");
+      m->conn_dof.upload(cd);
+      m->d_d2e_ptr.upload(dptr);
+      m->d_d2e.upload(d2e);
+    }
     HIPCHECK(hipStreamSynchronize(nullptr));
   } catch (...) {
     delete m;
@@ -349,13 +404,47 @@ void mesh_edges(Mesh &m) {
   std::vector<long long> lo_of, hi_of;
   lo_of.reserve((size_t)n * 3);
   hi_of.reserve((size_t)n * 3);
+  const bool p2 = m.degree == 2;
+  std::vector<int> key_id(p2 ? first.size() : 0, -1);
   for (size_t k = 0; k < first.size(); k++)
     if (first[k]) {
       const int *vv = &m.elems[3 * (k / 3)];
       const int a = vv[ev[k % 3][0]], bq = vv[ev[k % 3][1]];
+      if (p2) key_id[k] = (int)lo_of.size();
       lo_of.push_back(std::min(a, bq) + 1);
       hi_of.push_back(std::max(a, bq) + 1);
     }
+  if (p2) {
+    // every element edge's id, and which edges are on the boundary: per
+    // lower endpoint v, the first sighting of (v, w) carries the id
+    m.elem_edge.assign(first.size(), -1);
+    m.bd_edge.assign(lo_of.size(), 0);
+    parallel_chunks(n, PAR_GRAIN, [&](int64_t b, int64_t e1, int) {
+      int sid[MAX_ROW + 1], cnt[MAX_ROW + 1];
+      for (int v = (int)b; v < (int)e1; v++) {
+        const int *c = &m.ucol[m.urp[v]];
+        const int len = m.urp[v + 1] - m.urp[v];
+        std::fill(sid, sid + len, -1);
+        std::fill(cnt, cnt + len, 0);
+        const int r = m.perm[v];
+        for (int t = m.n2e_ptr[r]; t < m.n2e_ptr[r + 1]; t++) {
+          const int el = m.n2e_elem[t];
+          const int *vv = &m.elems[3 * (size_t)el];
+          for (int j = 0; j < 3; j++) {
+            const int a = vv[ev[j][0]], bq = vv[ev[j][1]];
+            if (a == bq || std::min(a, bq) != v) continue;
+            const int w = std::max(a, bq);
+            const int sl = (int)(std::lower_bound(c, c + len, w) - c);
+            if (sid[sl] < 0) sid[sl] = key_id[3 * (size_t)el + j];
+            cnt[sl]++;
+            m.elem_edge[3 * (size_t)el + j] = sid[sl];
+          }
+        }
+        for (int k = 0; k < len; k++)
+          if (sid[k] >= 0) m.bd_edge[sid[k]] = (char)(cnt[k] & 1);
+      }
+    });
+  }
   m.nedges = (int64_t)lo_of.size();
   m.edges.resize(2 * lo_of.size());
   std::copy(lo_of.begin(), lo_of.end(), m.edges.begin());
@@ -420,6 +509,9 @@ static void build_solver_layout(System *s) {
 
 System *system_build(Mesh *m, const long long *bdnode, int nbd) {
   REQUIRE(m, ADMFEM_ERR_ARG, "null mesh");
+  REQUIRE(m->degree == 1, ADMFEM_ERR_UNSUPPORTED,
+          "the fused system path is built for P1 meshes; for P2 assemble with the COO "
+          "ops and solve with admfem_sparse_solve");
   REQUIRE(nbd >= 0 && (nbd == 0 || bdnode), ADMFEM_ERR_ARG, "bad bdnode");
   System *s = new System();
   try {

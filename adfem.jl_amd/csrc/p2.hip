@@ -1,0 +1,337 @@
+// P2 (degree 2) element kernels: the COO operators of FemLaplaceScalar and
+// ComputeFemMassMatrixMfem, the load vector of FemSourceScalar, the
+// DofToGaussPoints / FemGrad maps, the matrix-free Laplace term, and their
+// gradient ops, for the 6-dof quadratic triangle of NNFEM_Mesh::init with
+// degree 2 (deps/MFEM/Common.cpp:46-72): dofs are the three vertices, then
+// nnode + the MFEM id of edges (0,1), (1,2), (2,0).
+//
+// The basis is MFEM's order-2 H1 triangle, nodal at the vertices and edge
+// midpoints: with barycentric l0 = 1 - xi - eta, l1 = xi, l2 = eta,
+//   phi_k = l_k (2 l_k - 1) (k < 3), phi_3 = 4 l0 l1, phi_4 = 4 l1 l2,
+//   phi_5 = 4 l2 l0,
+// and its physical gradients combine the constant grad l_k of the affine map
+// (CalcPhysDShape, Common.cpp:119-126):
+//   grad phi_k = (4 l_k - 1) grad l_k, grad phi_3 = 4 (l1 grad l0 + l0 grad l1), ...
+//
+// Layouts as the P1 kernels: Gauss arrays in user element order
+// (g = nq e + k), dof arrays in user dof order, COO triplet
+// t = 36 g + 6 p + q.  Maps to Gauss points run one thread per Gauss point
+// (or triplet); maps to dofs gather per dof over its incident (element,
+// local dof) pairs in ascending element order -- the reference's addition
+// order -- with one writer per output and no atomics.
+#include "geom.h"
+
+namespace admfem {
+
+namespace {
+
+__device__ __forceinline__ ElemGeom p2_geom(const int *__restrict__ conn, int nelem, int e,
+                                            const double *__restrict__ x,
+                                            const double *__restrict__ y) {
+  double ex[3], ey[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int v = conn[(size_t)k * nelem + e];
+    ex[k] = x[v];
+    ey[k] = y[v];
+  }
+  return elem_geom(ex[0], ey[0], ex[1], ey[1], ex[2], ey[2]);
+}
+
+// P2 basis values at (xi, eta).
+__device__ __forceinline__ void p2_shape(double xi, double eta, double h[6]) {
+#pragma clang fp contract(off)
+  const double l0 = 1.0 - xi - eta;
+  h[0] = l0 * (2.0 * l0 - 1.0);
+  h[1] = xi * (2.0 * xi - 1.0);
+  h[2] = eta * (2.0 * eta - 1.0);
+  h[3] = 4.0 * l0 * xi;
+  h[4] = 4.0 * xi * eta;
+  h[5] = 4.0 * eta * l0;
+}
+
+// Physical gradients of the P2 basis at (xi, eta).
+__device__ __forceinline__ void p2_grad(double xi, double eta, const ElemGeom &G, double gx[6],
+                                        double gy[6]) {
+#pragma clang fp contract(off)
+  const double l[3] = {1.0 - xi - eta, xi, eta};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    gx[k] = (4.0 * l[k] - 1.0) * G.gx[k];
+    gy[k] = (4.0 * l[k] - 1.0) * G.gy[k];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++) {  // edge j joins vertices a = j, b = (j + 1) % 3
+    const int a = j, b = (j + 1) % 3;
+    gx[3 + j] = 4.0 * (l[b] * G.gx[a] + l[a] * G.gx[b]);
+    gy[3 + j] = 4.0 * (l[b] * G.gy[a] + l[a] * G.gy[b]);
+  }
+}
+
+// FemLaplaceScalar_forward / ComputeFemMassMatrix1_forward, P2: one thread
+// per triplet (FemLaplaceScalar.h:3-26, ComputeFemMassMatrixMfem.h:4-29).
+template <bool MASS>
+__global__ void p2_coo_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
+                              const int *__restrict__ conn_dof, const double *__restrict__ x,
+                              const double *__restrict__ y, const double *__restrict__ coef,
+                              long long *__restrict__ ind, double *__restrict__ vv) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)36 * rule.nq * nelem;
+  if (t >= total) return;
+  const int64_t g = t / 36;
+  const int pq = (int)(t - 36 * g);
+  const int p = pq / 6, q = pq - 6 * p;
+  const int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
+  const ElemGeom G = p2_geom(conn, nelem, e, x, y);
+  const double w = gauss_weight(rule.w[k], G.area);
+  double v;
+  if (MASS) {
+    double h[6];
+    p2_shape(rule.xi[k], rule.eta[k], h);
+    v = h[p] * h[q] * coef[g] * w;
+  } else {
+    double gx[6], gy[6];
+    p2_grad(rule.xi[k], rule.eta[k], G, gx, gy);
+    v = (gx[p] * gx[q] + gy[p] * gy[q]) * coef[g] * w;
+  }
+  ind[2 * t] = conn_dof[(size_t)p * nelem + e];
+  ind[2 * t + 1] = conn_dof[(size_t)q * nelem + e];
+  vv[t] = v;
+}
+
+// The two backward ops (FemLaplaceScalar.h:28-54, ComputeFemMassMatrixMfem.h:
+// 31-56): one thread per Gauss point, sum over its 36 triplets in order.
+template <bool MASS>
+__global__ void p2_coo_grad_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
+                                   const double *__restrict__ x, const double *__restrict__ y,
+                                   const double *__restrict__ gvv, double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rule.nq * nelem) return;
+  const int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
+  const ElemGeom G = p2_geom(conn, nelem, e, x, y);
+  const double w = gauss_weight(rule.w[k], G.area);
+  double a[6], b[6];
+  if (MASS) {
+    p2_shape(rule.xi[k], rule.eta[k], a);
+  } else {
+    p2_grad(rule.xi[k], rule.eta[k], G, a, b);
+  }
+  double v = 0.0;
+  for (int p = 0; p < 6; p++)
+    for (int q = 0; q < 6; q++) {
+      const double n = MASS ? a[p] * a[q] * w : (a[p] * a[q] + b[p] * b[q]) * w;
+      v += gvv[36 * g + 6 * p + q] * n;
+    }
+  out[g] = v;
+}
+
+enum { G_SOURCE_T = 0, G_DOF_TO_GAUSS = 1, G_FEM_GRAD = 2, G_LAP_NU = 3 };
+
+// Per Gauss point:
+//   G_SOURCE_T      FemSourceScalar_backward: out[g] = sum_r h_r w grad_rhs[dof_r]
+//   G_DOF_TO_GAUSS  DofToGaussPointsMfem_forward: out[g] = sum_r u[dof_r] h_r
+//   G_FEM_GRAD      FemGradMfem_forward: out[2g], out[2g+1] = sum_r u[dof_r] grad h_r
+//   G_LAP_NU        ComputeLaplaceTermMfem_backward, nu part:
+//                   out[g] = w grad(a) . grad(b) (the term is linear in nu)
+template <int OP>
+__global__ void p2_gauss_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
+                                const int *__restrict__ conn_dof, const double *__restrict__ x,
+                                const double *__restrict__ y, const double *__restrict__ a,
+                                const double *__restrict__ b, double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rule.nq * nelem) return;
+  const int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
+  int d[6];
+#pragma unroll
+  for (int r = 0; r < 6; r++) d[r] = conn_dof[(size_t)r * nelem + e];
+  if (OP == G_DOF_TO_GAUSS) {
+    double h[6];
+    p2_shape(rule.xi[k], rule.eta[k], h);
+    double v = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; r++) v += a[d[r]] * h[r];
+    out[g] = v;
+    return;
+  }
+  const ElemGeom G = p2_geom(conn, nelem, e, x, y);
+  const double w = gauss_weight(rule.w[k], G.area);
+  if (OP == G_SOURCE_T) {
+    double h[6];
+    p2_shape(rule.xi[k], rule.eta[k], h);
+    double v = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; r++) v += h[r] * w * a[d[r]];
+    out[g] = v;
+    return;
+  }
+  double gx[6], gy[6];
+  p2_grad(rule.xi[k], rule.eta[k], G, gx, gy);
+  if (OP == G_FEM_GRAD) {
+    double vx = 0.0, vy = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; r++) vx += a[d[r]] * gx[r];
+#pragma unroll
+    for (int r = 0; r < 6; r++) vy += a[d[r]] * gy[r];
+    out[2 * g] = vx;
+    out[2 * g + 1] = vy;
+  } else {  // G_LAP_NU
+    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      ax += gx[r] * a[d[r]];
+      ay += gy[r] * a[d[r]];
+      bx += gx[r] * b[d[r]];
+      by += gy[r] * b[d[r]];
+    }
+    out[g] = w * (ax * bx + ay * by);
+  }
+}
+
+enum { D_SOURCE = 0, D_DOF_TO_GAUSS_T = 1, D_FEM_GRAD_T = 2, D_LAPLACE = 3 };
+
+// Per dof, over its incident (element, local dof p) pairs, ascending element:
+//   D_SOURCE          FemSourceScalar_forward: sum_k f[g] h_p(k) w_k
+//   D_DOF_TO_GAUSS_T  DofToGaussPointsMfem_backward: sum_k in[g] h_p(k)
+//   D_FEM_GRAD_T      FemGradMfem_backward: sum_k in[2g] dh_p/dx + in[2g+1] dh_p/dy
+//   D_LAPLACE         ComputeLaplaceTermMfem_forward (= K(nu) u):
+//                     sum_k nu_k w_k grad h_p . grad u_e(k)
+template <int OP>
+__global__ void p2_dof_kernel(int ndof, int nelem, QuadRule rule, const int *__restrict__ conn,
+                              const int *__restrict__ conn_dof, const double *__restrict__ x,
+                              const double *__restrict__ y, const int *__restrict__ d2e_ptr,
+                              const int *__restrict__ d2e, const double *__restrict__ in,
+                              const double *__restrict__ nu, double *__restrict__ out,
+                              int accumulate) {
+#pragma clang fp contract(off)
+  const int dof = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dof >= ndof) return;
+  double acc = accumulate ? out[dof] : 0.0;
+  for (int t = d2e_ptr[dof]; t < d2e_ptr[dof + 1]; t++) {
+    const int e = d2e[t] >> 3, p = d2e[t] & 7;
+    if (OP == D_DOF_TO_GAUSS_T) {
+      for (int k = 0; k < rule.nq; k++) {
+        double h[6];
+        p2_shape(rule.xi[k], rule.eta[k], h);
+        acc += in[(size_t)rule.nq * e + k] * h[p];
+      }
+      continue;
+    }
+    const ElemGeom G = p2_geom(conn, nelem, e, x, y);
+    if (OP == D_SOURCE) {
+      for (int k = 0; k < rule.nq; k++) {
+        double h[6];
+        p2_shape(rule.xi[k], rule.eta[k], h);
+        const double w = gauss_weight(rule.w[k], G.area);
+        acc += in[(size_t)rule.nq * e + k] * h[p] * w;
+      }
+    } else if (OP == D_FEM_GRAD_T) {
+      for (int k = 0; k < rule.nq; k++) {
+        double gx[6], gy[6];
+        p2_grad(rule.xi[k], rule.eta[k], G, gx, gy);
+        const size_t g = (size_t)rule.nq * e + k;
+        acc += in[2 * g] * gx[p] + in[2 * g + 1] * gy[p];
+      }
+    } else {  // D_LAPLACE
+      double ue[6];
+#pragma unroll
+      for (int q = 0; q < 6; q++) ue[q] = in[conn_dof[(size_t)q * nelem + e]];
+      for (int k = 0; k < rule.nq; k++) {
+        double gx[6], gy[6];
+        p2_grad(rule.xi[k], rule.eta[k], G, gx, gy);
+        double ux = 0.0, uy = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          ux += gx[q] * ue[q];
+          uy += gy[q] * ue[q];
+        }
+        const double w = gauss_weight(rule.w[k], G.area);
+        acc += nu[(size_t)rule.nq * e + k] * w * (gx[p] * ux + gy[p] * uy);
+      }
+    }
+  }
+  out[dof] = acc;
+}
+
+inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int OP>
+void gauss_op(const Mesh &m, const double *a, const double *b, double *out, hipStream_t st) {
+  p2_gauss_kernel<OP><<<nb(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p,
+                                                          m.conn_dof.p, m.x.p, m.y.p, a, b, out);
+  HIPCHECK(hipGetLastError());
+}
+
+template <int OP>
+void dof_op(const Mesh &m, const double *in, const double *nu, double *out, bool accumulate,
+            hipStream_t st) {
+  p2_dof_kernel<OP><<<nb(m.ndof, 256), 256, 0, st>>>(
+      (int)m.ndof, m.nelem, m.rule, m.conn.p, m.conn_dof.p, m.x.p, m.y.p, m.d_d2e_ptr.p,
+      m.d_d2e.p, in, nu, out, accumulate ? 1 : 0);
+  HIPCHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_p2_coo(const Mesh &m, bool mass, const double *coef, long long *ind, double *vv,
+                   hipStream_t st) {
+  const int64_t total = 36 * m.ngauss;
+  if (mass)
+    p2_coo_kernel<true><<<nb(total, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p, m.conn_dof.p,
+                                                         m.x.p, m.y.p, coef, ind, vv);
+  else
+    p2_coo_kernel<false><<<nb(total, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p,
+                                                          m.conn_dof.p, m.x.p, m.y.p, coef, ind,
+                                                          vv);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_p2_coo_grad(const Mesh &m, bool mass, const double *gvv, double *out,
+                        hipStream_t st) {
+  if (mass)
+    p2_coo_grad_kernel<true><<<nb(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p,
+                                                                 m.x.p, m.y.p, gvv, out);
+  else
+    p2_coo_grad_kernel<false><<<nb(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p,
+                                                                  m.x.p, m.y.p, gvv, out);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_p2_source(const Mesh &m, const double *f, double *rhs, bool accumulate,
+                      hipStream_t st) {
+  dof_op<D_SOURCE>(m, f, nullptr, rhs, accumulate, st);
+}
+
+void launch_p2_source_grad(const Mesh &m, const double *grhs, double *gf, hipStream_t st) {
+  gauss_op<G_SOURCE_T>(m, grhs, nullptr, gf, st);
+}
+
+void launch_p2_dof_to_gauss(const Mesh &m, const double *u, double *out, hipStream_t st) {
+  gauss_op<G_DOF_TO_GAUSS>(m, u, nullptr, out, st);
+}
+
+void launch_p2_dof_to_gauss_grad(const Mesh &m, const double *g, double *gu, hipStream_t st) {
+  dof_op<D_DOF_TO_GAUSS_T>(m, g, nullptr, gu, false, st);
+}
+
+void launch_p2_fem_grad(const Mesh &m, const double *u, double *out, hipStream_t st) {
+  gauss_op<G_FEM_GRAD>(m, u, nullptr, out, st);
+}
+
+void launch_p2_fem_grad_grad(const Mesh &m, const double *g, double *gu, hipStream_t st) {
+  dof_op<D_FEM_GRAD_T>(m, g, nullptr, gu, false, st);
+}
+
+void launch_p2_laplace_term(const Mesh &m, const double *u, const double *nu, double *out,
+                            hipStream_t st) {
+  dof_op<D_LAPLACE>(m, u, nu, out, false, st);
+}
+
+void launch_p2_laplace_term_nu_grad(const Mesh &m, const double *go, const double *u,
+                                    double *gnu, hipStream_t st) {
+  gauss_op<G_LAP_NU>(m, go, u, gnu, st);
+}
+
+}  // namespace admfem

@@ -55,9 +55,10 @@ typedef struct admfem_system_s *admfem_system;
  * element_indices: 3*num_elements, 0-based.  Builds the default mesh
  * (replacing a previous one, with the reference's warning).  Returns a
  * malloc()ed array of 2*nedges 1-based vertex ids [min...; max...] that the
- * caller frees (Julia: unsafe_wrap(own=true)), and writes nedges.  Only
- * degree 1 (P1) and orders 1 and 2 (1- and 3-point rules) are built; other
- * values return NULL with admfem_last_error() set. */
+ * caller frees (Julia: unsafe_wrap(own=true)), and writes nedges.  Degrees
+ * 1 (P1) and 2 (P2) and orders 0-4 (1-, 3- and 6-point rules; -1: 2 for P1,
+ * 4 for P2) are built; other values return NULL with admfem_last_error()
+ * set. */
 long long *init_nnfem_mesh(double *vertices, int num_vertices,
                            int *element_indices, int num_elements, int order,
                            int lorder, int degree, long long *nedges);
@@ -78,7 +79,8 @@ void mfem_get_connectivity(long long *conn);
 /* deps/MFEM/API.cpp:58-65: column-major nelem x 3, 1-based, oriented. */
 void mfem_get_element_to_vertices(long long *elems);
 /* deps/MFEM/FemLaplace1/FemLaplaceScalar.h:59-61 (forward of h:3-26):
- * indices [2*9*ngauss] (i,j pairs, 0-based), vv [9*ngauss], kappa [ngauss]. */
+ * indices [2*nd*nd*ngauss] (i,j pairs, 0-based), vv [nd*nd*ngauss],
+ * kappa [ngauss]; nd = elem_ndof (3 for P1, 6 for P2). */
 void FemLaplaceScalar_forward_Julia(long long *indices, double *vv,
                                     const double *kappa);
 /* deps/MFEM/FemSource1/FemSourceScalar.h:35-37 (forward of h:4-15):
@@ -89,7 +91,7 @@ void FemSourceScalar_forward_Julia(double *rhs, const double *f);
  * Gauss points (overwritten). */
 void FemToGaussPointsMfem_Julia(double *out, const double *u);
 /* deps/MFEM/DofToGaussPoints/DofToGaussPointsMfem.h:35-37 (MUtils.jl:268):
- * the same for the element's dofs (P1: the nodes). */
+ * out [ngauss] from u [ndof] with the degree's basis (P1: the nodes). */
 void DofToGaussPointsMfem_forward_Julia(double *out, const double *u);
 /* deps/MFEM/ComputeLaplaceTermMfem/ComputeLaplaceTermMfem.h:42-44
  * (src/MFEM/MCore.jl:744-745): out [ndof] += K(nu) u, ACCUMULATED (the
@@ -105,8 +107,10 @@ admfem_mesh admfem_default_mesh(void);
 
 /* coords: host nnode x 2 row-major; elems: host nelem x 3 row-major 0-based
  * (orientation is fixed as NNFEM_Mesh::init does, Common.cpp:40).
- * order: integration order (-1 -> 2, the reference default for P1,
- * src/MFEM/MFEM.jl:73); degree must be 1. */
+ * order: integration order 0-4 (-1 -> the reference default: 2 for P1, 4
+ * for P2, src/MFEM/MFEM.jl:71-77); degree 1 (P1) or 2 (P2: dofs are the
+ * vertices, then nnode + the MFEM id of each element's edges (0,1),(1,2),
+ * (2,0), Common.cpp:63-72). */
 int admfem_mesh_create(const double *coords, int nnode, const int *elems,
                        int nelem, int order, int degree, admfem_mesh *out);
 int admfem_mesh_destroy(admfem_mesh m);
@@ -118,10 +122,15 @@ int admfem_mesh_gauss(admfem_mesh m, double *x, double *y);
 int admfem_mesh_weights(admfem_mesh m, double *w);
 int admfem_mesh_area(admfem_mesh m, double *a);
 int admfem_mesh_elems(admfem_mesh m, int *elems); /* nelem x 3 row-major 0-based, oriented */
+/* Element dofs, nelem x elem_ndof row-major, 0-based (what
+ * mfem_get_connectivity returns 1-based, deps/MFEM/API.cpp:49-56). */
+int admfem_mesh_dofs(admfem_mesh m, int *dofs);
 int admfem_mesh_edges(admfem_mesh m, long long *edges); /* 2*nedges, as init_nnfem_mesh */
 /* Boundary nodes (edges used by one element), ascending, 1-based
  * (src/MFEM/MCore.jl:385-406, 439-450 bcedge/bcnode; the reference returns a
- * Set's order, which nothing downstream depends on).  out capacity nnode. */
+ * Set's order, which nothing downstream depends on); for P2 then the
+ * boundary edges' dofs nnode + edge id + 1, ascending (MCore.jl:441-443).
+ * out capacity ndof. */
 int admfem_mesh_bcnode(admfem_mesh m, long long *out, int *count);
 /* Pre-BC CSR pattern (sorted columns, explicit zeros kept), host pointers. */
 int admfem_mesh_pattern(admfem_mesh m, int *row_ptr, int *col);
@@ -132,8 +141,9 @@ int admfem_mesh_pattern(admfem_mesh m, int *row_ptr, int *col);
  * ==================================================================== */
 
 /* FemLaplaceScalar (deps/MFEM/FemLaplace1/FemLaplaceScalar.cpp:14-33,
- * forward FemLaplaceScalar.h:3-26): kappa [ngauss] -> indices [9*ngauss x 2]
- * int64, vv [9*ngauss].  Same triplet order as the reference. */
+ * forward FemLaplaceScalar.h:3-26): kappa [ngauss] -> indices
+ * [nd*nd*ngauss x 2] int64, vv [nd*nd*ngauss] (nd = elem_ndof: 3 for P1, 6
+ * for P2).  Same triplet order as the reference. */
 int admfem_fem_laplace_scalar(admfem_mesh m, const double *kappa,
                               long long *indices, double *vv, void *stream);
 /* FemLaplaceScalarGrad (FemLaplaceScalar.h:28-54): grad_vv -> grad_kappa. */
@@ -157,28 +167,30 @@ int admfem_fem_source_scalar_grad(admfem_mesh m, const double *grad_rhs,
                                   double *grad_f, void *stream);
 
 /* FemToGaussPointsMfem (deps/MFEM/FemToGaussPoints/FemToGaussPointsMfem.h:
- * 6-18): u [nnode] (user node order) -> out [ngauss] (overwritten). */
+ * 6-18): u [nnode] (user node order) -> out [ngauss] (overwritten), with the
+ * linear vertex functions for either degree. */
 int admfem_fem_to_gauss_points(admfem_mesh m, const double *u, double *out, void *stream);
 /* FemToGaussPointsMfemGrad (h:20-33): grad_out [ngauss] -> grad_u [nnode]
  * (overwritten).  Atomic-free node gather. */
 int admfem_fem_to_gauss_points_grad(admfem_mesh m, const double *grad_out, double *grad_u,
                                     void *stream);
 /* DofToGaussPointsMfem / ...Grad (deps/MFEM/DofToGaussPoints/
- * DofToGaussPointsMfem.h:6-32): for P1 the dofs are the nodes, so these are
- * the two functions above under the op's own names. */
+ * DofToGaussPointsMfem.h:6-32): u [ndof] -> out [ngauss] with the degree's
+ * basis, and grad_out [ngauss] -> grad_u [ndof] (overwritten); for P1 the
+ * dofs are the nodes and these equal the two functions above. */
 int admfem_dof_to_gauss_points(admfem_mesh m, const double *u, double *out, void *stream);
 int admfem_dof_to_gauss_points_grad(admfem_mesh m, const double *grad_out, double *grad_u,
                                     void *stream);
 /* FemGradMfem (deps/MFEM/FemGrad/FemGradMfem.h:8-22), the op behind
- * eval_grad_on_gauss_pts1 (src/MFEM/MCore.jl:239-246): u [nnode] ->
+ * eval_grad_on_gauss_pts1 (src/MFEM/MCore.jl:239-246): u [ndof] ->
  * out [ngauss x 2] row-major (du/dx, du/dy) per Gauss point. */
 int admfem_fem_grad(admfem_mesh m, const double *u, double *out, void *stream);
-/* FemGradMfemGrad (h:24-41): grad_out [ngauss x 2] -> grad_u [nnode]. */
+/* FemGradMfemGrad (h:24-41): grad_out [ngauss x 2] -> grad_u [ndof]. */
 int admfem_fem_grad_grad(admfem_mesh m, const double *grad_out, double *grad_u, void *stream);
 /* ComputeLaplaceTermMfem (deps/MFEM/ComputeLaplaceTermMfem/
  * ComputeLaplaceTermMfem.h:4-17), the op behind compute_fem_laplace_term1
- * (src/MFEM/MCore.jl:740-763): out [nnode] = K(nu) u without forming K
- * (overwritten); u [nnode], nu [ngauss]. */
+ * (src/MFEM/MCore.jl:740-763): out [ndof] = K(nu) u without forming K
+ * (overwritten); u [ndof], nu [ngauss]. */
 int admfem_fem_laplace_term(admfem_mesh m, const double *u, const double *nu, double *out,
                             void *stream);
 /* ComputeLaplaceTermMfemGrad (h:19-39): grad_u = K(nu) grad_out (K is
@@ -219,7 +231,8 @@ int admfem_impose_dirichlet_grad(const double *grad_ov, const double *grad_orhs,
 /* bdnode: host, 1-based (as bcnode returns), duplicates allowed (the last
  * bdval entry of a node wins).  The post-BC CSR pattern is the reference's:
  * interior-interior element pairs (explicit zeros kept) plus the boundary
- * diagonals. */
+ * diagonals.  P1 meshes only (ADMFEM_ERR_UNSUPPORTED for P2: assemble P2
+ * with the section C ops, then admfem_sparse_solve). */
 int admfem_system_create(admfem_mesh m, const long long *bdnode, int nbd,
                          admfem_system *out);
 int admfem_system_destroy(admfem_system s);

@@ -62,7 +62,7 @@ def test_product_never_imports_oracle():
                 assert "oracle" not in re.sub(r"(#|//).*", "", txt).lower() or f == "build.py", f
 
 
-@pytest.mark.parametrize("bad", [{"degree": 2}])
+@pytest.mark.parametrize("bad", [{"degree": 3}, {"degree": "BDM1"}])
 def test_mesh_rejects_unsupported_degree_without_gpu(bad):
     from admfem import Mesh
     with pytest.raises(NotImplementedError):
