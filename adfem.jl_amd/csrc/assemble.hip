@@ -35,18 +35,16 @@ __device__ __forceinline__ int sym3(int p, int q) {
 
 constexpr int PF = 8;  // pair words of a row fetched before the element pass
 
-// One element's gathered inputs: internal vertex ids, vertex coordinates,
-// and its Gauss-point coefficients.
+// One element's inputs: its record {element id, 3 local vertex indices} and
+// its Gauss-point coefficients (the only per-element gathers: the vertex
+// data come from the block's staged vertices in LDS).
 template <int NQ>
 struct ElemIn {
-  int v[3];
-  double ex[3], ey[3], kq[NQ], fq[NQ], rq[NQ];
+  int loc;
+  double kq[NQ], fq[NQ], rq[NQ];
   template <bool MASS>
-  __device__ __forceinline__ void load(const int4 rec, const double *__restrict__ x,
-                                       const double *__restrict__ y, const admfem_problem &P) {
-    v[0] = rec.y;
-    v[1] = rec.z;
-    v[2] = rec.w;
+  __device__ __forceinline__ void load(const int2 rec, const admfem_problem &P) {
+    loc = rec.y;
 #pragma unroll
     for (int k = 0; k < NQ; k++) {
       const size_t g = (size_t)NQ * rec.x + k;
@@ -54,33 +52,61 @@ struct ElemIn {
       fq[k] = P.f ? P.f[g] : 0.0;
       rq[k] = MASS && P.rho ? P.rho[g] : 1.0;
     }
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      ex[k] = x[v[k]];
-      ey[k] = y[v[k]];
-    }
+  }
+  __device__ __forceinline__ int lv(int q) const {
+    return (loc >> (BLK_VERT_BITS * q)) & (BLK_MAX_VERTS - 1);
   }
 };
 
+// The block's vertex data in LDS: coordinates, and (when present) u_prev and
+// the Dirichlet value of boundary vertices (0 at interior vertices).
+struct VertLds {
+  double2 *xy;
+  double *up, *bv;
+};
+
+// Edge vectors of the element's vertices: d_p = (x_{p+2} - x_{p+1},
+// y_{p+1} - y_{p+2}), and |det| = 2 * area.
+struct ElemGeo {
+  double dx[3], dy[3], ad;
+};
+__device__ __forceinline__ ElemGeo elem_geo(const double (&ex)[3], const double (&ey)[3]) {
+  ElemGeo g;
+  g.dx[0] = ex[2] - ex[1];
+  g.dx[1] = ex[0] - ex[2];
+  g.dx[2] = ex[1] - ex[0];
+  g.dy[0] = ey[1] - ey[2];
+  g.dy[1] = ey[2] - ey[0];
+  g.dy[2] = ey[0] - ey[1];
+  g.ad = fabs(g.dx[2] * g.dy[1] - g.dx[1] * g.dy[2]);
+  return g;
+}
+
 // The element's symmetric matrix (packed upper triangle, 6 values) and its
 // three lifted load terms, in closed form for the affine P1 map.  With
-// d_p = (x_{p+2} - x_{p+1}, y_{p+1} - y_{p+2}) and det = 2 * signed area,
-// grad phi_p = (d_p.y, d_p.x) / det and w_k = ipw_k * area / 0.5 =
-// ipw_k * |det| (Common.cpp:116; |det| / 2 is Heron's area up to rounding),
-// so the per-Gauss-point sums of FemLaplaceScalar.h:7-25,
-// ComputeFemMassMatrixMfem.h:4-29 and FemSourceScalar.h:4-15 become
-//   K_pq = lap * (sum_k ipw_k kappa_k) * (d_p . d_q) / |det|,
+// d_p as above and det = 2 * signed area, grad phi_p = (d_p.y, d_p.x) / det
+// and w_k = ipw_k * area / 0.5 = ipw_k * |det| (Common.cpp:116; |det| / 2 is
+// Heron's area up to rounding), so the per-Gauss-point sums of
+// FemLaplaceScalar.h:7-25, ComputeFemMassMatrixMfem.h:4-29 and
+// FemSourceScalar.h:4-15 become
+//   K_pq = lap * (sum_k ipw_k kappa_k) * (d_p . d_q) / |det| = c (d_p . d_q),
 //   M_pq = |det| * sum_k ipw_k rho_k h_pk h_qk,
 //   F_p  = |det| * sum_k ipw_k f_k h_pk:
 // no square roots and one division per element.  The values agree with the
 // per-Gauss-point form to rounding (a few ulps); the COO ops keep that form.
+// Outputs: a (the matrix), c (the Laplace factor), r (the load terms).
 template <int NQ, bool MASS>
-__device__ __forceinline__ void element(const ElemIn<NQ> &in, const QuadRule &rule,
-                                        const admfem_problem &P, const int *__restrict__ row_bd,
-                                        double *ae, double *rl) {
-  const double dx[3] = {in.ex[2] - in.ex[1], in.ex[0] - in.ex[2], in.ex[1] - in.ex[0]};
-  const double dy[3] = {in.ey[1] - in.ey[2], in.ey[2] - in.ey[0], in.ey[0] - in.ey[1]};
-  const double ad = fabs(dx[2] * dy[1] - dx[1] * dy[2]);
+__device__ __forceinline__ void element(const ElemIn<NQ> &in, const VertLds &V,
+                                        const QuadRule &rule, const admfem_problem &P,
+                                        double (&a)[6], double &c, double (&r)[3]) {
+  double ex[3], ey[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const double2 xy = V.xy[in.lv(q)];
+    ex[q] = xy.x;
+    ey[q] = xy.y;
+  }
+  const ElemGeo G = elem_geo(ex, ey);
   double sk = 0.0, mm[6] = {0, 0, 0, 0, 0, 0}, src[3] = {0, 0, 0};
 #pragma unroll
   for (int k = 0; k < NQ; k++) {
@@ -103,67 +129,92 @@ __device__ __forceinline__ void element(const ElemIn<NQ> &in, const QuadRule &ru
   }
   if (MASS) {
 #pragma unroll
-    for (int i = 0; i < 6; i++) mm[i] *= ad;
+    for (int i = 0; i < 6; i++) mm[i] *= G.ad;
   }
-  const double c = P.lap_coef * sk / ad;
-  double a[6];
+  c = P.lap_coef * sk / G.ad;
 #pragma unroll
   for (int p = 0; p < 3; p++)
 #pragma unroll
     for (int q = p; q < 3; q++) {
-      double val = c * (dx[p] * dx[q] + dy[p] * dy[q]);
+      double val = c * (G.dx[p] * G.dx[q] + G.dy[p] * G.dy[q]);
       if (MASS && P.mass_coef != 0.0) val += P.mass_coef * mm[sym3(p, q)];
       a[sym3(p, q)] = val;
     }
-  double r[3];
-  const double sc = P.src_coef * ad;
+  const double sc = P.src_coef * G.ad;
 #pragma unroll
   for (int p = 0; p < 3; p++) r[p] = sc * src[p];
   if (MASS && P.u_prev) {
     double up[3];
 #pragma unroll
-    for (int q = 0; q < 3; q++) up[q] = P.u_prev[in.v[q]];
+    for (int q = 0; q < 3; q++) up[q] = V.up[in.lv(q)];
 #pragma unroll
     for (int p = 0; p < 3; p++)
 #pragma unroll
       for (int q = 0; q < 3; q++) r[p] += mm[sym3(p, q)] * up[q];
   }
-  if (P.bdval) {  // Dirichlet lifting of the boundary columns
+  if (P.bdval) {  // Dirichlet lifting of the boundary columns (value 0: nothing to lift)
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-      const int bq = row_bd[in.v[q]];
-      if (bq >= 0) {
-        const double bv = P.bdval[bq];
+      const double bv = V.bv[in.lv(q)];
+      if (bv != 0.0) {
 #pragma unroll
         for (int p = 0; p < 3; p++) r[p] -= a[sym3(p, q)] * bv;
       }
     }
   }
-#pragma unroll
-  for (int k = 0; k < 6; k++) ae[k] = a[k];
-#pragma unroll
-  for (int p = 0; p < 3; p++) rl[p] = r[p];
 }
 
-template <int NQ, bool MASS>
+__device__ __forceinline__ double pick3(const double (&v)[3], int i) {
+  return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
+
+// Per-block LDS, in doubles.  Two layouts:
+//  * full (a general mass density rho): [max_e][6] element matrices,
+//    [max_e][3] load terms;
+//  * compact (rho = 1 or no mass term: the matrix row is rebuilt in pass 2
+//    from the staged coordinates, halving the per-element footprint so more
+//    blocks fit a CU): [max_e][3] load terms, [max_e] Laplace factors c,
+//    [max_v] staged coordinates, [max_e] packed local vertex indices (int);
+// then one region used twice: first the staged vertex values read only by
+// pass 1 (full: coordinates, u_prev, boundary values; compact: u_prev,
+// boundary values), then, once every element is done, the row accumulators
+// [wmax + 2][AS_ROWS] (slot-major: slots [0, wmax) are the row's SELL
+// positions, slot wmax its diagonal, wmax + 1 a sink for lifted boundary
+// columns, so pass 2 adds without branching).  max_e is even, so every
+// double2 array is 16-byte aligned.
+template <int NQ, bool MASS, bool COMPACT>
 __global__ void __launch_bounds__(AS_NT)
-    assemble_kernel(int n, int npad, QuadRule rule, const double *__restrict__ x,
-                    const double *__restrict__ y, const int *__restrict__ blk_eptr,
-                    const int4 *__restrict__ blk_rec, const int *__restrict__ pair_ptr,
-                    const uint32_t *__restrict__ pairs, const int *__restrict__ row_bd,
-                    const int *__restrict__ slice_ptr, admfem_problem P, SellOut o, int max_e,
-                    int wmax) {
+    assemble_kernel(int n, int npad, QuadRule rule, const double2 *__restrict__ xy,
+                    const int *__restrict__ blk_eptr, const int2 *__restrict__ blk_rec,
+                    const int *__restrict__ blk_vptr, const int *__restrict__ blk_vert,
+                    const int *__restrict__ pair_ptr, const uint32_t *__restrict__ pairs,
+                    const int *__restrict__ row_bd, const int *__restrict__ slice_ptr,
+                    admfem_problem P, SellOut o, int max_e, int max_v, int wmax) {
   extern __shared__ double lds[];
-  double *ae = lds;                     // [max_e][6] element matrices
-  double *rl = ae + 6 * (size_t)max_e;  // [max_e][3] element load terms
-  // [wmax + 2][AS_ROWS] row accumulators, slot-major: slots [0, wmax) are
-  // the row's SELL positions, slot wmax its diagonal, wmax + 1 a sink for
-  // lifted (boundary) columns, so pass 2 adds without branching
-  double *vl = rl + 3 * (size_t)max_e;
+  double *ae = lds;                                      // full only
+  double *rl = COMPACT ? lds : ae + 6 * (size_t)max_e;
+  double *ec = rl + 3 * (size_t)max_e;                   // compact only
+  VertLds V;
+  double *region;
+  int *eloc = nullptr;
+  if (COMPACT) {
+    V.xy = reinterpret_cast<double2 *>(ec + max_e);
+    eloc = reinterpret_cast<int *>(V.xy + max_v);
+    region = reinterpret_cast<double *>(eloc + max_e);
+    V.up = region;
+  } else {
+    region = rl + 3 * (size_t)max_e;
+    V.xy = reinterpret_cast<double2 *>(region);
+    V.up = region + 2 * (size_t)max_v;
+  }
+  V.bv = V.up + (MASS ? (size_t)max_v : 0);
+  double *vl = region;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int row = blk * AS_ROWS + (int)threadIdx.x;  // phase 2: threads < AS_ROWS
+  const int row = blk * AS_ROWS + (int)threadIdx.x;  // pass 2: threads < AS_ROWS
   const int e0 = blk_eptr[blk];
   const int ne = blk_eptr[blk + 1] - e0;
+  const int v0 = blk_vptr[blk];
+  const int nv = blk_vptr[blk + 1] - v0;
   // the row's pair words, fetched now so that their latency overlaps pass 1
   int bi = -1, t0 = 0, t1 = 0;
   uint32_t wb[PF];
@@ -176,26 +227,73 @@ __global__ void __launch_bounds__(AS_NT)
 #pragma unroll
     for (int k = 0; k < PF; k++) wb[k] = t0 + k < t1 ? pairs[t0 + k] : 0u;
   }
-  for (int k = threadIdx.x; k < AS_ROWS * (wmax + 2); k += AS_NT) vl[k] = 0.0;
-  // pass 1: elements (MASS: a mass term or u_prev is present), up to three
-  // per thread and step with every load issued before any computes (a
-  // 64-row block of a triangulation touches ~260 elements: one step)
-  for (int t = threadIdx.x; t < ne; t += 3 * AS_NT) {
-    ElemIn<NQ> in[3];
-    const bool b1 = t + AS_NT < ne, b2 = t + 2 * AS_NT < ne;
-    in[0].template load<MASS>(blk_rec[e0 + t], x, y, P);
-    if (b1) in[1].template load<MASS>(blk_rec[e0 + t + AS_NT], x, y, P);
-    if (b2) in[2].template load<MASS>(blk_rec[e0 + t + 2 * AS_NT], x, y, P);
-    element<NQ, MASS>(in[0], rule, P, row_bd, ae + 6 * t, rl + 3 * t);
-    if (b1) element<NQ, MASS>(in[1], rule, P, row_bd, ae + 6 * (t + AS_NT), rl + 3 * (t + AS_NT));
-    if (b2)
-      element<NQ, MASS>(in[2], rule, P, row_bd, ae + 6 * (t + 2 * AS_NT),
-                        rl + 3 * (t + 2 * AS_NT));
+  // pass 1 (MASS: a mass term or u_prev is present).  The first 3 * AS_NT
+  // elements' records and coefficients are loaded before the vertices are
+  // staged, so both latencies overlap (a 64-row block of a triangulation
+  // touches ~260 elements); the vertices are consecutive rows in runs, so
+  // their loads coalesce.
+  ElemIn<NQ> in[3];
+  const bool b0 = (int)threadIdx.x < ne, b1 = (int)threadIdx.x + AS_NT < ne,
+             b2 = (int)threadIdx.x + 2 * AS_NT < ne;
+  if (b0) in[0].template load<MASS>(blk_rec[e0 + threadIdx.x], P);
+  if (b1) in[1].template load<MASS>(blk_rec[e0 + threadIdx.x + AS_NT], P);
+  if (b2) in[2].template load<MASS>(blk_rec[e0 + threadIdx.x + 2 * AS_NT], P);
+  for (int k = threadIdx.x; k < nv; k += AS_NT) {
+    const int v = blk_vert[v0 + k];
+    V.xy[k] = xy[v];
+    if (MASS && P.u_prev) V.up[k] = P.u_prev[v];
+    if (P.bdval) {
+      const int b = row_bd[v];
+      V.bv[k] = b >= 0 ? P.bdval[b] : 0.0;
+    }
   }
   __syncthreads();
-  // pass 2: rows
+  auto store = [&](const ElemIn<NQ> &e, int t) {
+    double a[6], c, r[3];
+    element<NQ, MASS>(e, V, rule, P, a, c, r);
+    if (COMPACT) {
+      ec[t] = c;
+      eloc[t] = e.loc;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; k++) ae[6 * t + k] = a[k];
+    }
+#pragma unroll
+    for (int p = 0; p < 3; p++) rl[3 * t + p] = r[p];
+  };
+  if (b0) store(in[0], threadIdx.x);
+  if (b1) store(in[1], threadIdx.x + AS_NT);
+  if (b2) store(in[2], threadIdx.x + 2 * AS_NT);
+  for (int t = threadIdx.x + 3 * AS_NT; t < ne; t += AS_NT) {  // unusually dense blocks
+    ElemIn<NQ> e1;
+    e1.template load<MASS>(blk_rec[e0 + t], P);
+    store(e1, t);
+  }
+  __syncthreads();
+  // pass 2: rows; each row's thread owns its slots of the accumulators and
+  // clears them itself (the staged pass-1 values are no longer read)
   if (threadIdx.x >= AS_ROWS || row >= npad) return;
   double *my = vl + threadIdx.x;  // slot k at my[AS_ROWS * k]: no bank conflicts
+  for (int k = 0; k < wmax + 2; k++) my[AS_ROWS * k] = 0.0;
+  // compact, mass term: sum_k ipw_k h_pk h_qk of the rule (rho = 1)
+  double mref[3][3];
+  if (COMPACT && MASS) {
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int q = 0; q < 3; q++) mref[p][q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      const double h[3] = {shape(0, rule.xi[k], rule.eta[k]), shape(1, rule.xi[k], rule.eta[k]),
+                           shape(2, rule.xi[k], rule.eta[k])};
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        const double wr = rule.w[k] * 1.0 * h[p];
+#pragma unroll
+        for (int q = p; q < 3; q++) mref[p][q] += wr * h[q];
+      }
+    }
+  }
   double d = 1.0, acc = 0.0;  // padding rows: identity, rhs 0
   if (row < n) {
     if (bi >= 0) {
@@ -203,12 +301,38 @@ __global__ void __launch_bounds__(AS_NT)
     } else {
       auto add = [&](uint32_t w) {
         const int p = (int)(w & 3u), le = (int)(w >> 20);
-        const double *a = ae + 6 * le;
+        double arow[3];
+        if (COMPACT) {
+          const int loc = eloc[le];
+          double ex[3], ey[3];
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            const double2 c2 = V.xy[(loc >> (BLK_VERT_BITS * q)) & (BLK_MAX_VERTS - 1)];
+            ex[q] = c2.x;
+            ey[q] = c2.y;
+          }
+          const ElemGeo G = elem_geo(ex, ey);
+          const double c = ec[le], dxp = pick3(G.dx, p), dyp = pick3(G.dy, p);
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            double val = c * (dxp * G.dx[q] + dyp * G.dy[q]);
+            if (MASS && P.mass_coef != 0.0) {
+              const int lo = min(p, q), hi = max(p, q);
+              const double mr =
+                  lo == 0 ? pick3(mref[0], hi) : (lo == 1 ? pick3(mref[1], hi) : mref[2][2]);
+              val += P.mass_coef * (mr * G.ad);
+            }
+            arow[q] = val;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 3; q++) arow[q] = ae[6 * le + sym3(p, q)];
+        }
 #pragma unroll
         for (int q = 0; q < 3; q++) {
           const uint32_t code = (w >> (2 + SLOT_BITS * q)) & 63u;
           const int sl = code < PAIR_DIAG ? (int)code : wmax + (int)(code - PAIR_DIAG);
-          my[AS_ROWS * sl] += a[sym3(p, q)];
+          my[AS_ROWS * sl] += arow[q];
         }
         acc += rl[3 * le + p];
       };
@@ -265,8 +389,7 @@ __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
 // ImposeDirichlet.h:63-93).  One thread per element, nq outputs each.
 __global__ void kappa_grad_kernel(int nelem, QuadRule rule,
                                   const int *__restrict__ conn,
-                                  const double *__restrict__ x,
-                                  const double *__restrict__ y,
+                                  const double2 *__restrict__ xy,
                                   const int *__restrict__ row_bd,
                                   const double *__restrict__ u,
                                   const double *__restrict__ lam, double scale,
@@ -278,8 +401,9 @@ __global__ void kappa_grad_kernel(int nelem, QuadRule rule,
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     v[k] = conn[(size_t)k * nelem + e];
-    ex[k] = x[v[k]];
-    ey[k] = y[v[k]];
+    const double2 c = xy[v[k]];
+    ex[k] = c.x;
+    ey[k] = c.y;
     uu[k] = u[v[k]];
     ll[k] = row_bd[v[k]] >= 0 ? 0.0 : lam[v[k]];
   }
@@ -336,37 +460,45 @@ inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 
 }  // namespace
 
-void launch_assemble_to(const System &s, const double *x, const double *y,
-                        const admfem_problem &p, const SellOut &o, hipStream_t st) {
+void launch_assemble_to(const System &s, const double2 *xy, const admfem_problem &p,
+                        const SellOut &o, hipStream_t st) {
   const Mesh &m = *s.mesh;
-  const int me = std::max(s.asm_max_e, 1), w = std::max(s.sell_wmax, 1);
-  const size_t lds = (size_t)me * 9 * sizeof(double) + (size_t)AS_ROWS * (w + 2) * sizeof(double);
+  const bool mass = p.mass_coef != 0.0 || p.u_prev;
+  const bool compact = p.rho == nullptr;
+  const int me = (std::max(s.asm_max_e, 1) + 1) / 2 * 2, mv = std::max(s.asm_max_v, 1);
+  const int w = std::max(s.sell_wmax, 1);
+  // doubles: per-element arrays, staged vertex values read by pass 1 only,
+  // row accumulators (the last two share one region)
+  const size_t per_e = compact ? (size_t)me * 4 + (size_t)mv * 2 + (size_t)me / 2
+                               : (size_t)me * 9;
+  const size_t verts = (size_t)mv * ((compact ? 0 : 2) + (mass ? 1 : 0) + (p.bdval ? 1 : 0));
+  const size_t rows = (size_t)AS_ROWS * (w + 2);
+  const size_t lds = (per_e + std::max(verts, rows)) * sizeof(double);
   REQUIRE(lds <= 160 * 1024, ADMFEM_ERR_UNSUPPORTED, "assembly block too dense for LDS");
   const unsigned grid = nblk(s.npad, AS_ROWS);
   static_assert(AS_ROWS == 64, "an assembly block is one SELL slice");
-  const bool mass = p.mass_coef != 0.0 || p.u_prev;
   auto go = [&](auto kern) {
-    kern<<<grid, AS_NT, lds, st>>>(s.n, s.npad, m.rule, x, y, s.d_blk_eptr.p, s.d_blk_rec.p,
-                                   m.d_n2e_ptr.p, s.d_pairs.p, s.d_row_bd.p, s.d_slice_ptr.p, p,
-                                   o, me, w);
+    kern<<<grid, AS_NT, lds, st>>>(s.n, s.npad, m.rule, xy, s.d_blk_eptr.p, s.d_blk_rec.p,
+                                   s.d_blk_vptr.p, s.d_blk_vert.p, m.d_n2e_ptr.p, s.d_pairs.p,
+                                   s.d_row_bd.p, s.d_slice_ptr.p, p, o, me, mv, w);
   };
   if (m.rule.nq == 3) {
-    if (mass) go(assemble_kernel<3, true>);
-    else go(assemble_kernel<3, false>);
+    if (mass) compact ? go(assemble_kernel<3, true, true>) : go(assemble_kernel<3, true, false>);
+    else compact ? go(assemble_kernel<3, false, true>) : go(assemble_kernel<3, false, false>);
   } else if (m.rule.nq == 6) {
-    if (mass) go(assemble_kernel<6, true>);
-    else go(assemble_kernel<6, false>);
+    if (mass) compact ? go(assemble_kernel<6, true, true>) : go(assemble_kernel<6, true, false>);
+    else compact ? go(assemble_kernel<6, false, true>) : go(assemble_kernel<6, false, false>);
   } else {
     REQUIRE(m.rule.nq == 1, ADMFEM_ERR_UNSUPPORTED, "quadrature rule not built");
-    if (mass) go(assemble_kernel<1, true>);
-    else go(assemble_kernel<1, false>);
+    if (mass) compact ? go(assemble_kernel<1, true, true>) : go(assemble_kernel<1, true, false>);
+    else compact ? go(assemble_kernel<1, false, true>) : go(assemble_kernel<1, false, false>);
   }
   HIPCHECK(hipGetLastError());
 }
 
 void launch_assemble(System &s, const admfem_problem &p, hipStream_t st) {
   SellOut o{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p};
-  launch_assemble_to(s, s.mesh->x.p, s.mesh->y.p, p, o, st);
+  launch_assemble_to(s, s.mesh->xy.p, p, o, st);
 }
 
 void launch_to_sell(System &s, hipStream_t st) {
@@ -376,19 +508,18 @@ void launch_to_sell(System &s, hipStream_t st) {
   HIPCHECK(hipGetLastError());
 }
 
-void launch_kappa_gradient_to(const System &s, const double *x, const double *y,
-                              const double *u_int, const double *lam_int, double scale,
-                              double *gk, hipStream_t st) {
+void launch_kappa_gradient_to(const System &s, const double2 *xy, const double *u_int,
+                              const double *lam_int, double scale, double *gk, hipStream_t st) {
   const Mesh &m = *s.mesh;
   kappa_grad_kernel<<<nblk(m.nelem, 256), 256, 0, st>>>(
-      m.nelem, m.rule, m.conn.p, x, y, s.d_row_bd.p, u_int, lam_int, scale, gk);
+      m.nelem, m.rule, m.conn.p, xy, s.d_row_bd.p, u_int, lam_int, scale, gk);
   HIPCHECK(hipGetLastError());
 }
 
 void launch_kappa_gradient(const System &s, const double *u_int,
                            const double *lam_int, double scale, double *gk,
                            hipStream_t st) {
-  launch_kappa_gradient_to(s, s.mesh->x.p, s.mesh->y.p, u_int, lam_int, scale, gk, st);
+  launch_kappa_gradient_to(s, s.mesh->xy.p, u_int, lam_int, scale, gk, st);
 }
 
 // Loss and its gradient without a host round trip: g (npad, internal order)

@@ -52,6 +52,6 @@ void batch_final(const BSys *D, const BScal *S, int sys, int npad, double *out,
                  hipStream_t st);
 void synth_inputs(const System &s, const double *base_x, const double *base_y, uint64_t seed,
                   double amp, const SynField &kf, const SynField *uf, double *x, double *y,
-                  double *kappa, double *f, double *u_prev_int, hipStream_t st);
+                  double2 *xy, double *kappa, double *f, double *u_prev_int, hipStream_t st);
 
 }  // namespace admfem

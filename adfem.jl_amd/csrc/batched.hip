@@ -365,7 +365,7 @@ __device__ __forceinline__ double unif(uint64_t seed, uint64_t k) {  // [0, 1)
 __global__ void syn_coords(int n, const double *__restrict__ bx, const double *__restrict__ by,
                            const int *__restrict__ row_bd, const int *__restrict__ iperm,
                            uint64_t seed, double amp, double *__restrict__ x,
-                           double *__restrict__ y) {
+                           double *__restrict__ y, double2 *__restrict__ xy) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   double dx = 0, dy = 0;
@@ -376,6 +376,7 @@ __global__ void syn_coords(int n, const double *__restrict__ bx, const double *_
   }
   x[r] = bx[r] + dx;
   y[r] = by[r] + dy;
+  xy[r] = make_double2(bx[r] + dx, by[r] + dy);
 }
 
 // per Gauss point: kappa = exp(GRF) and f ~ U[-1, 1]
@@ -414,14 +415,17 @@ __global__ void syn_nodal(int n, const double *__restrict__ x, const double *__r
   u[r] = F.scale * g;
 }
 
-// user-order interleaved coordinates -> internal-order x, y
+// user-order interleaved coordinates -> internal-order x, y and xy
 __global__ void perm_coords(int n, const int *__restrict__ iperm, const double *__restrict__ c,
-                            double *__restrict__ x, double *__restrict__ y) {
+                            double *__restrict__ x, double *__restrict__ y,
+                            double2 *__restrict__ xy) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int u = iperm[r];
-  x[r] = c[2 * (size_t)u];
-  y[r] = c[2 * (size_t)u + 1];
+  const double cx = c[2 * (size_t)u], cy = c[2 * (size_t)u + 1];
+  x[r] = cx;
+  y[r] = cy;
+  xy[r] = make_double2(cx, cy);
 }
 
 inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -429,8 +433,8 @@ inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 }  // namespace
 
 void launch_perm_coords(int n, const int *iperm, const double *coords_user, double *x,
-                        double *y, hipStream_t st) {
-  perm_coords<<<nb(n, 256), 256, 0, st>>>(n, iperm, coords_user, x, y);
+                        double *y, double2 *xy, hipStream_t st) {
+  perm_coords<<<nb(n, 256), 256, 0, st>>>(n, iperm, coords_user, x, y, xy);
   HIPCHECK(hipGetLastError());
 }
 
@@ -478,10 +482,10 @@ void batch_final(const BSys *D, const BScal *S, int sys, int npad, double *out,
 
 void synth_inputs(const System &s, const double *base_x, const double *base_y, uint64_t seed,
                   double amp, const SynField &kf, const SynField *uf, double *x, double *y,
-                  double *kappa, double *f, double *u_prev_int, hipStream_t st) {
+                  double2 *xy, double *kappa, double *f, double *u_prev_int, hipStream_t st) {
   const Mesh &m = *s.mesh;
   syn_coords<<<nb(s.n, 256), 256, 0, st>>>(s.n, base_x, base_y, s.d_row_bd.p, m.d_iperm.p,
-                                            seed, amp, x, y);
+                                            seed, amp, x, y, xy);
   syn_gauss<<<nb(m.nelem, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p, x, y, kf,
                                                seed ^ 0x5A5A5A5A5A5A5A5Aull, kappa, f);
   if (uf && u_prev_int) syn_nodal<<<nb(s.n, 256), 256, 0, st>>>(s.n, x, y, *uf, u_prev_int);

@@ -79,6 +79,10 @@ __host__ __device__ inline uint32_t slot_of(uint32_t w, int q) {
 // index in its assembly block's element list.
 constexpr uint32_t PAIR_DIAG = 62;
 constexpr int PAIR_MAX_ELEMS = 4096;
+// An assembly block's element record is {element id, its 3 vertices as
+// 10-bit indices into the block's sorted list of distinct vertices}.
+constexpr int BLK_VERT_BITS = 10;
+constexpr int BLK_MAX_VERTS = 1 << BLK_VERT_BITS;
 // Mixed-precision CG: default residual replacement factor (a replacement
 // when the FP32 residual fell by this factor since the last one) and the
 // stall guard (a replacement when the FP32 residual has not fallen by 1%
@@ -149,6 +153,7 @@ struct Mesh {
   std::vector<int> dofs;         // P2: nelem x 6 user dof ids
   // device
   DevBuf<double> x, y;          // internal order
+  DevBuf<double2> xy;           // internal order, interleaved: one 16-byte gather per vertex
   DevBuf<int> conn;             // [3][nelem] internal ids (SoA)
   DevBuf<int> conn_user;        // [3][nelem] user ids (SoA)
   DevBuf<int> d_perm, d_iperm;
@@ -178,12 +183,15 @@ struct System {
   // device: pattern + assembly maps
   DevBuf<int> d_rp, d_col, d_row_bd;
   // assembly: per block of AS_ROWS rows the distinct elements incident to
-  // its interior rows (ascending) as {element, 3 internal vertex ids}, and a
-  // pair word per incidence (row-major, parallel to the mesh's n2e lists)
+  // its interior rows (ascending) as {element, 3 local vertex indices}, the
+  // distinct vertices of those elements (ascending internal ids), and a pair
+  // word per incidence (row-major, parallel to the mesh's n2e lists)
   DevBuf<int> d_blk_eptr;
-  DevBuf<int4> d_blk_rec;
+  DevBuf<int2> d_blk_rec;
+  DevBuf<int> d_blk_vptr, d_blk_vert;
   DevBuf<uint32_t> d_pairs;
   int asm_max_e = 0;   // most elements of one block
+  int asm_max_v = 0;   // most vertices of one block
   int sell_wmax = 0;   // widest SELL slice
   // CSR values (general sparse systems only) and the right-hand side
   DevBuf<double> val, rhs;
@@ -256,16 +264,15 @@ struct SellOut {
   double *sv, *diag, *dinv, *rhs;
 };
 void launch_assemble(System &s, const admfem_problem &p, hipStream_t st);
-void launch_assemble_to(const System &s, const double *x, const double *y,
-                        const admfem_problem &p, const SellOut &o, hipStream_t st);
+void launch_assemble_to(const System &s, const double2 *xy, const admfem_problem &p,
+                        const SellOut &o, hipStream_t st);
 // CSR values (s.val, general sparse systems) -> solver format
 void launch_to_sell(System &s, hipStream_t st);
 void launch_kappa_gradient(const System &s, const double *u_int,
                            const double *lam_int, double scale, double *gk,
                            hipStream_t st);
-void launch_kappa_gradient_to(const System &s, const double *x, const double *y,
-                              const double *u_int, const double *lam_int, double scale,
-                              double *gk, hipStream_t st);
+void launch_kappa_gradient_to(const System &s, const double2 *xy, const double *u_int,
+                              const double *lam_int, double scale, double *gk, hipStream_t st);
 void launch_loss(const System &s, const double *u_int, const int *obs_idx,
                  const double *obs_val, int nobs, double *g_int, double *loss2,
                  hipStream_t st);

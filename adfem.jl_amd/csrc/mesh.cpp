@@ -59,7 +59,7 @@ int quad_rule(int order, QuadRule &r) {
 }
 
 size_t Mesh::device_bytes() const {
-  return x.bytes() + y.bytes() + conn.bytes() + conn_user.bytes() +
+  return x.bytes() + y.bytes() + xy.bytes() + conn.bytes() + conn_user.bytes() +
          d_perm.bytes() + d_iperm.bytes() + d_n2e_ptr.bytes() +
          d_n2e_elem.bytes() + d_slots_full.bytes() + d_rp.bytes() +
          d_col.bytes() + gx.bytes() + gy.bytes() + gw.bytes() + area.bytes() +
@@ -147,6 +147,36 @@ static std::vector<int> rcm_order(int n, const std::vector<int> &rp,
 
 
 
+// Element order: nodes numbered by first appearance in the element list
+// (vertices in their order within each element; nodes of no element last,
+// ascending).  order[k] = node placed at position k.
+static std::vector<int> element_order(int n, const std::vector<int> &elems) {
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<char> seen(n, 0);
+  for (int v : elems)
+    if (!seen[v]) {
+      seen[v] = 1;
+      order.push_back(v);
+    }
+  for (int v = 0; v < n; v++)
+    if (!seen[v]) order.push_back(v);
+  return order;
+}
+
+// Bandwidth of the node graph under `order`: max |pos(u) - pos(v)| over edges.
+static int64_t order_bandwidth(const std::vector<int> &order, const std::vector<int> &rp,
+                               const std::vector<int> &col) {
+  const int n = (int)order.size();
+  std::vector<int> pos(n);
+  for (int k = 0; k < n; k++) pos[order[k]] = k;
+  int64_t bw = 0;
+  for (int u = 0; u < n; u++)
+    for (int k = rp[u]; k < rp[u + 1]; k++)
+      bw = std::max<int64_t>(bw, std::abs(pos[u] - pos[col[k]]));
+  return bw;
+}
+
 Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
                  int order, int degree) {
   REQUIRE(nnode > 0 && nelem > 0, ADMFEM_ERR_ARG, "empty mesh");
@@ -226,10 +256,14 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
         std::copy(part[c].begin(), part[c].end(), ucol.begin() + urp[b]);
       });
     }
-    // internal numbering: reverse Cuthill-McKee over the node graph.  Rows
-    // then run along BFS levels: the k-th neighbours of consecutive rows are
-    // consecutive nodes of the adjacent levels, so the solver's column
-    // gathers coalesce within a wavefront.
+    // internal numbering: reverse Cuthill-McKee over the node graph, or the
+    // element order when that is banded too (bandwidth within 2.5x of
+    // RCM's).  Either way rows run along fronts: the k-th neighbours of
+    // consecutive rows are consecutive nodes of the adjacent fronts, so the
+    // solver's column gathers coalesce within a wavefront.  The element
+    // order also makes the elements of a block of rows consecutive in the
+    // user's element order, so the assembly's per-element inputs (kappa, f,
+    // given in that order) share cache lines.
     {
       std::vector<int> arp(nnode + 1, 0), acol;
       acol.reserve(ucol.size());
@@ -238,7 +272,11 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
           if (ucol[k] != i) acol.push_back(ucol[k]);
         arp[i + 1] = (int)acol.size();
       }
-      m->iperm = rcm_order(nnode, arp, acol);
+      std::vector<int> rcm = rcm_order(nnode, arp, acol);
+      std::vector<int> eo = element_order(nnode, m->elems);
+      const int64_t bw_rcm = order_bandwidth(rcm, arp, acol),
+                    bw_eo = order_bandwidth(eo, arp, acol);
+      m->iperm = 2 * bw_eo <= 5 * bw_rcm ? std::move(eo) : std::move(rcm);
       m->perm.assign(nnode, 0);
       for (int k = 0; k < nnode; k++) m->perm[m->iperm[k]] = k;
     }
@@ -296,12 +334,15 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
     });
     // device copies
     std::vector<double> xs(nnode), ys(nnode);
+    std::vector<double2> xys(nnode);
     for (int r = 0; r < nnode; r++) {
       xs[r] = coords[2 * (size_t)IP[r]];
       ys[r] = coords[2 * (size_t)IP[r] + 1];
+      xys[r] = make_double2(xs[r], ys[r]);
     }
     m->x.upload(xs);
     m->y.upload(ys);
+    m->xy.upload(xys);
     m->conn.upload(conn_int);
     m->conn_user.upload(conn_usr);
     m->d_perm.upload(m->perm);
@@ -453,7 +494,7 @@ void mesh_edges(Mesh &m) {
 
 size_t System::device_bytes() const {
   return d_rp.bytes() + d_col.bytes() + d_row_bd.bytes() + d_blk_eptr.bytes() +
-         d_blk_rec.bytes() + d_pairs.bytes() + val.bytes() + rhs.bytes() +
+         d_blk_rec.bytes() + d_blk_vptr.bytes() + d_blk_vert.bytes() + d_pairs.bytes() + val.bytes() + rhs.bytes() +
          d_slice_ptr.bytes() + d_sell_col.bytes() +
          d_sell_val.bytes() + d_diag.bytes() + d_dinv.bytes();
 }
@@ -546,12 +587,12 @@ System *system_build(Mesh *m, const long long *bdnode, int nbd) {
     // one pair word per (row, element) incidence; host threads over chunks
     // of blocks, element lists concatenated in block order
     const int nblk = (s->npad + AS_ROWS - 1) / AS_ROWS;
-    std::vector<int> beptr(nblk + 1, 0), belem;
+    std::vector<int> beptr(nblk + 1, 0), bvptr(nblk + 1, 0);
     std::vector<uint32_t> pairs(m->n2e_elem.size(), 0);
-    std::vector<std::vector<int>> part(16);
-    std::vector<int> part_max(16, 0);
+    std::vector<std::vector<int>> part(16), part_v(16), part_loc(16);
+    std::vector<int> part_max(16, 0), part_maxv(16, 0);
     auto blocks = [&](int64_t bb, int64_t be, int c) {
-      std::vector<int> &bl = part[c];
+      std::vector<int> &bl = part[c], &bv = part_v[c], &bloc = part_loc[c];
       std::vector<int> offd;
       for (int b = (int)bb; b < (int)be; b++) {
         const int r0 = b * AS_ROWS, r1 = std::min(r0 + AS_ROWS, n);
@@ -566,6 +607,28 @@ System *system_build(Mesh *m, const long long *bdnode, int nbd) {
                 "assembly block has too many elements");
         part_max[c] = std::max(part_max[c], ne);
         beptr[b + 1] = ne;
+        // the block's distinct vertices (ascending internal ids: a few runs
+        // of consecutive rows under RCM) and each element's local indices
+        const size_t vfirst = bv.size();
+        for (size_t k = first; k < bl.size(); k++)
+          for (int q = 0; q < 3; q++) bv.push_back(m->perm[m->elems[3 * (size_t)bl[k] + q]]);
+        std::sort(bv.begin() + vfirst, bv.end());
+        bv.erase(std::unique(bv.begin() + vfirst, bv.end()), bv.end());
+        const int nv = (int)(bv.size() - vfirst);
+        REQUIRE(nv <= BLK_MAX_VERTS, ADMFEM_ERR_UNSUPPORTED,
+                "assembly block touches more than 1024 vertices");
+        part_maxv[c] = std::max(part_maxv[c], nv);
+        bvptr[b + 1] = nv;
+        for (size_t k = first; k < bl.size(); k++) {
+          int loc = 0;
+          for (int q = 0; q < 3; q++) {
+            const int v = m->perm[m->elems[3 * (size_t)bl[k] + q]];
+            const int l = (int)(std::lower_bound(bv.begin() + vfirst, bv.end(), v) -
+                                (bv.begin() + vfirst));
+            loc |= l << (BLK_VERT_BITS * q);
+          }
+          bloc.push_back(loc);
+        }
         for (int r = r0; r < r1; r++) {
           if (s->row_bd[r] >= 0) continue;
           offd.clear();
@@ -590,23 +653,29 @@ System *system_build(Mesh *m, const long long *bdnode, int nbd) {
       }
     };
     parallel_chunks(nblk, 64, blocks);
-    for (int b = 0; b < nblk; b++) beptr[b + 1] += beptr[b];
-    for (int c = 0; c < 16; c++) s->asm_max_e = std::max(s->asm_max_e, part_max[c]);
-    belem.resize(beptr[nblk]);
-    parallel_chunks(nblk, 64, [&](int64_t bb, int64_t, int c) {
-      std::copy(part[c].begin(), part[c].end(), belem.begin() + beptr[bb]);
-    });
+    for (int b = 0; b < nblk; b++) {
+      beptr[b + 1] += beptr[b];
+      bvptr[b + 1] += bvptr[b];
+    }
+    for (int c = 0; c < 16; c++) {
+      s->asm_max_e = std::max(s->asm_max_e, part_max[c]);
+      s->asm_max_v = std::max(s->asm_max_v, part_maxv[c]);
+    }
     REQUIRE(s->sell_wmax < (int)PAIR_DIAG, ADMFEM_ERR_UNSUPPORTED, "row wider than 61");
-    std::vector<int4> brec(std::max<size_t>(1, belem.size()), make_int4(0, 0, 0, 0));
-    parallel_chunks((int64_t)belem.size(), PAR_GRAIN, [&](int64_t kb, int64_t ke, int) {
-      for (int64_t k = kb; k < ke; k++) {
-        const int *v = &m->elems[3 * (size_t)belem[k]];
-        brec[k] = make_int4(belem[k], m->perm[v[0]], m->perm[v[1]], m->perm[v[2]]);
-      }
+    // chunk c covers a contiguous range of blocks: its lists start at the
+    // offsets of its first block
+    std::vector<int2> brec(std::max<int>(1, beptr[nblk]), make_int2(0, 0));
+    std::vector<int> bvert(std::max<int>(1, bvptr[nblk]), 0);
+    parallel_chunks(nblk, 64, [&](int64_t bb, int64_t, int c) {
+      for (size_t k = 0; k < part[c].size(); k++)
+        brec[beptr[bb] + k] = make_int2(part[c][k], part_loc[c][k]);
+      std::copy(part_v[c].begin(), part_v[c].end(), bvert.begin() + bvptr[bb]);
     });
     s->d_row_bd.upload(s->row_bd);
     s->d_blk_eptr.upload(beptr);
     s->d_blk_rec.upload(brec);
+    s->d_blk_vptr.upload(bvptr);
+    s->d_blk_vert.upload(bvert);
     s->d_pairs.upload(pairs.empty() ? std::vector<uint32_t>(1, 0) : pairs);
   } catch (...) {
     system_free(s);

@@ -89,6 +89,7 @@ struct Req {
   int slot = -1;
   size_t off = 0;
   // device buffers (inside the arena)
+  double2 *xy;
   double *x, *y, *kappa, *f, *uprev, *b, *sv64, *diag64, *dsq, *xa, *xb, *u64, *lam64,
       *grad, *obsv, *loss2, *part, *stage;
   float *sv32, *ysa, *ysb, *r32, *pa, *pb, *q32;
@@ -104,7 +105,7 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 
 // small kernels of the engine (batched.hip)
 void launch_perm_coords(int n, const int *iperm, const double *coords_user, double *x,
-                        double *y, hipStream_t st);
+                        double *y, double2 *xy, hipStream_t st);
 
 struct Server {
   admfem_server_opts o{};
@@ -157,11 +158,12 @@ struct Server {
            *sv64 = D(ns), *diag64 = D(np), *dsq = D(np), *xa = D(np), *xb = D(np),
            *u64 = D(np), *lam64 = D(np), *grad = D(ng), *obsv = D(nobs + 1), *loss2 = D(2),
            *part = D(4 * nblk), *stage = D(2 * np);
+    double2 *xy = (double2 *)take(np * sizeof(double2));
     float *sv32 = F(ns), *ysa = F(np), *ysb = F(np), *r32 = F(np), *pa = F(np), *pb = F(np),
           *q32 = F(np);
     int *obsi = (int *)take((nobs + 1) * sizeof(int));
     if (q) {
-      q->x = x; q->y = y; q->kappa = kappa; q->f = f; q->uprev = uprev; q->b = b;
+      q->x = x; q->y = y; q->xy = xy; q->kappa = kappa; q->f = f; q->uprev = uprev; q->b = b;
       q->sv64 = sv64; q->diag64 = diag64; q->dsq = dsq; q->xa = xa; q->xb = xb;
       q->u64 = u64; q->lam64 = lam64; q->grad = grad; q->obsv = obsv; q->loss2 = loss2;
       q->part = part; q->stage = stage; q->sv32 = sv32; q->ysa = ysa; q->ysb = ysb;
@@ -230,7 +232,7 @@ struct Server {
         uf.phi[j] = ph(rng);
       }
       synth_inputs(s, m.x.p, m.y.p, r.seed, T.jitter, kf, heat ? &uf : nullptr, q.x, q.y,
-                   q.kappa, q.f, q.uprev, st);
+                   q.xy, q.kappa, q.f, q.uprev, st);
       // observations: 1% of the interior nodes, obs_val = 0
       std::vector<int> idx;
       std::unordered_set<int> seen;
@@ -253,10 +255,11 @@ struct Server {
     // user-order inputs are staged, then permuted to the internal order
     if (r.coords) {
       HIPCHECK(hipMemcpyAsync(q.stage, r.coords, sizeof(double) * 2 * s.n, kind, st));
-      launch_perm_coords(s.n, m.d_iperm.p, q.stage, q.x, q.y, st);
+      launch_perm_coords(s.n, m.d_iperm.p, q.stage, q.x, q.y, q.xy, st);
     } else {
       HIPCHECK(hipMemcpyAsync(q.x, m.x.p, sizeof(double) * s.n, hipMemcpyDeviceToDevice, st));
       HIPCHECK(hipMemcpyAsync(q.y, m.y.p, sizeof(double) * s.n, hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipMemcpyAsync(q.xy, m.xy.p, sizeof(double2) * s.n, hipMemcpyDeviceToDevice, st));
     }
     HIPCHECK(hipMemcpyAsync(q.kappa, r.kappa, sizeof(double) * m.ngauss, kind, st));
     HIPCHECK(hipMemcpyAsync(q.f, r.f, sizeof(double) * m.ngauss, kind, st));
@@ -288,7 +291,7 @@ struct Server {
     inputs(q);
     admfem_problem p = problem(q);
     // FP64 matrix and rhs; the FP32 scaled matrix of the inner iteration
-    launch_assemble_to(s, q.x, q.y, p, SellOut{q.sv64, q.diag64, nullptr, q.b}, st);
+    launch_assemble_to(s, q.xy, p, SellOut{q.sv64, q.diag64, nullptr, q.b}, st);
     batch_scale(s, q.sv64, q.diag64, q.dsq, q.sv32, st);
   }
 
@@ -347,7 +350,7 @@ struct Server {
     q.res.iters_adj = sc.it;
     batch_final(d_desc.p, d_scal.p, q.slot, s.npad, q.lam64, st);
     admfem_problem p = problem(q);
-    launch_kappa_gradient_to(s, q.x, q.y, q.u64, q.lam64, p.lap_coef, q.grad, st);
+    launch_kappa_gradient_to(s, q.xy, q.u64, q.lam64, p.lap_coef, q.grad, st);
     if (q.r.u_out) {
       launch_gather(m.d_perm.p, q.u64, q.stage, s.n, st);  // user order
       HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
@@ -387,6 +390,28 @@ struct Server {
   }
 
   // ------------------------------------------------------------ batched run
+  // Fills h_map[0, gmap) with the (slot, block) pairs of `slots` (each
+  // system's blocks in order) so that each XCD runs one contiguous eighth of
+  // that list: blocks are dispatched round-robin over the 8 XCDs (grid block
+  // b to XCD b % 8), so entry j of XCD x's share goes to grid block 8 j + x.
+  // Neighbouring blocks of a system then gather from the same L2.  Grid
+  // blocks past an XCD's share get system -1 (no work).
+  void fill_map(const std::vector<std::pair<int, int>> &slots, size_t nmap, size_t gmap) {
+    size_t sys = 0, blk = 0;  // cursor into the logical list
+    for (int x = 0; x < 8; x++) {
+      const size_t share = nmap / 8 + ((size_t)x < nmap % 8 ? 1 : 0);
+      size_t b = (size_t)x;
+      for (size_t j = 0; j < share; j++, b += 8) {
+        while (blk >= (size_t)slots[sys].second) {
+          sys++;
+          blk = 0;
+        }
+        h_map[b] = make_int2(slots[sys].first, (int)blk++);
+      }
+      for (; b < gmap; b += 8) h_map[b] = make_int2(-1, 0);
+    }
+  }
+
   void ensure_map(size_t nmap, size_t nlist) {
     if (nmap > map_cap) {
       if (h_map) HIPCHECK(hipHostFree(h_map));
@@ -471,13 +496,13 @@ struct Server {
       const size_t gmap = use_graph ? (size_t)bucket(nmap) : nmap;
       const size_t glist = use_graph ? (size_t)bucket(active.size()) : active.size();
       ensure_map(gmap, glist);
-      size_t k = 0;
+      std::vector<std::pair<int, int>> slots(active.size());
       for (size_t a = 0; a < active.size(); a++) {
         const Req &q = reqs[active[a]];
-        for (int b = 0; b < topos[q.r.topo].nblk; b++) h_map[k++] = make_int2(q.slot, b);
+        slots[a] = std::make_pair(q.slot, topos[q.r.topo].nblk);
         h_list[a] = q.slot;
       }
-      for (; k < gmap; k++) h_map[k] = make_int2(-1, 0);
+      fill_map(slots, nmap, gmap);
       for (size_t a = active.size(); a < glist; a++) h_list[a] = max_slots;  // dummy slot
       HIPCHECK(hipMemcpyAsync(d_map.p, h_map, sizeof(int2) * gmap, hipMemcpyHostToDevice, st));
       HIPCHECK(hipMemcpyAsync(d_list.p, h_list, sizeof(int) * glist, hipMemcpyHostToDevice, st));
@@ -555,7 +580,7 @@ struct Server {
       // FP64 single-system path on the topology's own solver buffers
       inputs(q);
       admfem_problem p = problem(q);
-      launch_assemble_to(s, q.x, q.y, p, SellOut{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p},
+      launch_assemble_to(s, q.xy, p, SellOut{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p},
                          st);
       admfem_solve_opts so{ADMFEM_PREC_FP64, o.max_iter, o.rtol, 0, 0.0};
       admfem_solve_stats s0{}, s1{};
@@ -575,7 +600,7 @@ struct Server {
         continue;
       }
       const Mesh &m = *s.mesh;
-      launch_kappa_gradient_to(s, q.x, q.y, q.u64, q.lam64, p.lap_coef, q.grad, st);
+      launch_kappa_gradient_to(s, q.xy, q.u64, q.lam64, p.lap_coef, q.grad, st);
       if (q.r.u_out) {
         launch_gather(m.d_perm.p, q.u64, q.stage, s.n, st);
         HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
@@ -642,10 +667,10 @@ double server_spmv_repeat(Server *sv, const int *topos, int nsys, int repeat,
       bytes += (double)s.col.size() * 8.0 + 4.0 * (s.n + 1) + 2.0 * s.n * 4.0;
     }
     sv->ensure_map(nmap, (size_t)nsys);
-    size_t k = 0;
+    std::vector<std::pair<int, int>> slots(nsys);
     for (int i = 0; i < nsys; i++)
-      for (int b = 0; b < sv->topos[qs[i].r.topo].nblk; b++)
-        sv->h_map[k++] = make_int2(qs[i].slot, b);
+      slots[i] = std::make_pair(qs[i].slot, sv->topos[qs[i].r.topo].nblk);
+    sv->fill_map(slots, nmap, nmap);
     HIPCHECK(hipMemcpyAsync(sv->d_map.p, sv->h_map, sizeof(int2) * nmap, hipMemcpyHostToDevice,
                             sv->st));
     HIPCHECK(hipStreamSynchronize(sv->st));

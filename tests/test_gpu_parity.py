@@ -237,7 +237,10 @@ def _oracle_system(om, kap, f, bd, bdval, dt=None, u_prev=None, rho=None):
     return O.csr_matrix(oi, ov, om.nnode), orhs
 
 
-@pytest.mark.parametrize("m,heat", [(12, False), (12, True), (260, True)])
+# heat: False (Poisson), "rho" (a mass density field: the per-element matrix
+# layout), "unit" (rho = 1: the matrix rows rebuilt from the coordinates)
+@pytest.mark.parametrize("m,heat", [(12, False), (12, "rho"), (260, "rho"), (12, "unit"),
+                                    (260, "unit"), (260, False)])
 def test_system_assembly_matches_oracle(m, heat):
     print(f"system assembly m={m} heat={heat}", flush=True)
     gm, om = pair(m, seed=7)
@@ -249,7 +252,7 @@ def test_system_assembly_matches_oracle(m, heat):
     S = admfem.DirichletSystem(gm, bd)
     if heat:
         dt, up = 0.05, rng.standard_normal(om.nnode)
-        rho = rng.random(om.ngauss) + 0.5
+        rho = rng.random(om.ngauss) + 0.5 if heat == "rho" else None
         S.assemble(kappa=kap, f=f, bdval=bdval, rho=rho, u_prev=up, lap_coef=dt, mass_coef=1.0,
                    src_coef=dt)
         A0, r0 = _oracle_system(om, kap, f, bd, bdval, dt=dt, u_prev=up, rho=rho)
