@@ -11,9 +11,13 @@
 //
 // A block owns one 64-row SELL slice (blocks mapped XCD-contiguously) and
 // works in two passes over LDS:
+//  0. the block's distinct vertices (a few runs of consecutive rows) are
+//     staged in LDS -- coordinates, and u_prev / Dirichlet values when used
+//     -- while the elements' coefficients are being fetched;
 //  1. one thread per element incident to the block's interior rows computes
-//     the element's symmetric 3x3 matrix and its 3 lifted load terms -- each
-//     element once per block, its gathers independent of each other;
+//     the element's symmetric 3x3 matrix (or, with unit density, only its
+//     Laplace factor) and its 3 lifted load terms -- each element once per
+//     block, its only gathers its Gauss-point coefficients;
 //  2. one thread per row adds, in ascending element order, its pairs' matrix
 //     row into its SELL slots (LDS) and diagonal, and their load terms.
 // Each row's thread then writes its slice lanes: consecutive rows are
