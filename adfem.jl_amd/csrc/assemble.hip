@@ -231,27 +231,11 @@ __global__ void __launch_bounds__(AS_NT)
 #pragma unroll
     for (int k = 0; k < PF; k++) wb[k] = t0 + k < t1 ? pairs[t0 + k] : 0u;
   }
-  // pass 1 (MASS: a mass term or u_prev is present).  The first 3 * AS_NT
-  // elements' records and coefficients are loaded before the vertices are
-  // staged, so both latencies overlap (a 64-row block of a triangulation
-  // touches ~260 elements); the vertices are consecutive rows in runs, so
-  // their loads coalesce.
-  ElemIn<NQ> in[3];
-  const bool b0 = (int)threadIdx.x < ne, b1 = (int)threadIdx.x + AS_NT < ne,
-             b2 = (int)threadIdx.x + 2 * AS_NT < ne;
-  if (b0) in[0].template load<MASS>(blk_rec[e0 + threadIdx.x], P);
-  if (b1) in[1].template load<MASS>(blk_rec[e0 + threadIdx.x + AS_NT], P);
-  if (b2) in[2].template load<MASS>(blk_rec[e0 + threadIdx.x + 2 * AS_NT], P);
-  for (int k = threadIdx.x; k < nv; k += AS_NT) {
-    const int v = blk_vert[v0 + k];
-    V.xy[k] = xy[v];
-    if (MASS && P.u_prev) V.up[k] = P.u_prev[v];
-    if (P.bdval) {
-      const int b = row_bd[v];
-      V.bv[k] = b >= 0 ? P.bdval[b] : 0.0;
-    }
-  }
-  __syncthreads();
+  // pass 1 (MASS: a mass term or u_prev is present), in chunks of up to
+  // three elements per thread (a 64-row block of a triangulation touches
+  // ~260 elements: one chunk).  The first chunk's records and coefficients
+  // are loaded before the vertices are staged, so both latencies overlap;
+  // the vertices are consecutive rows in runs, so their loads coalesce.
   auto store = [&](const ElemIn<NQ> &e, int t) {
     double a[6], c, r[3];
     element<NQ, MASS>(e, V, rule, P, a, c, r);
@@ -265,13 +249,28 @@ __global__ void __launch_bounds__(AS_NT)
 #pragma unroll
     for (int p = 0; p < 3; p++) rl[3 * t + p] = r[p];
   };
-  if (b0) store(in[0], threadIdx.x);
-  if (b1) store(in[1], threadIdx.x + AS_NT);
-  if (b2) store(in[2], threadIdx.x + 2 * AS_NT);
-  for (int t = threadIdx.x + 3 * AS_NT; t < ne; t += AS_NT) {  // unusually dense blocks
-    ElemIn<NQ> e1;
-    e1.template load<MASS>(blk_rec[e0 + t], P);
-    store(e1, t);
+  for (int c0 = 0; c0 < max(ne, 1); c0 += 3 * AS_NT) {  // uniform over the block
+    ElemIn<NQ> in[3];
+    const int t = c0 + (int)threadIdx.x;
+    const bool b0 = t < ne, b1 = t + AS_NT < ne, b2 = t + 2 * AS_NT < ne;
+    if (b0) in[0].template load<MASS>(blk_rec[e0 + t], P);
+    if (b1) in[1].template load<MASS>(blk_rec[e0 + t + AS_NT], P);
+    if (b2) in[2].template load<MASS>(blk_rec[e0 + t + 2 * AS_NT], P);
+    if (c0 == 0) {
+      for (int k = threadIdx.x; k < nv; k += AS_NT) {
+        const int v = blk_vert[v0 + k];
+        V.xy[k] = xy[v];
+        if (MASS && P.u_prev) V.up[k] = P.u_prev[v];
+        if (P.bdval) {
+          const int b = row_bd[v];
+          V.bv[k] = b >= 0 ? P.bdval[b] : 0.0;
+        }
+      }
+      __syncthreads();
+    }
+    if (b0) store(in[0], t);
+    if (b1) store(in[1], t + AS_NT);
+    if (b2) store(in[2], t + 2 * AS_NT);
   }
   __syncthreads();
   // pass 2: rows; each row's thread owns its slots of the accumulators and

@@ -177,6 +177,32 @@ static int64_t order_bandwidth(const std::vector<int> &order, const std::vector<
   return bw;
 }
 
+// Widest level of a breadth-first search of the node graph from s (its
+// component only): the yardstick for a banded order, whose bandwidth is a
+// small multiple of it (reverse Cuthill-McKee's is at most about twice the
+// widest level of its own level structure).
+static int64_t bfs_max_level(int n, const std::vector<int> &rp, const std::vector<int> &col,
+                             int s) {
+  std::vector<int> level(n, -1), q;
+  q.reserve(n);
+  q.push_back(s);
+  level[s] = 0;
+  std::vector<int64_t> width(1, 1);
+  for (size_t h = 0; h < q.size(); h++) {
+    const int u = q[h];
+    for (int k = rp[u]; k < rp[u + 1]; k++) {
+      const int v = col[k];
+      if (level[v] < 0) {
+        level[v] = level[u] + 1;
+        if ((size_t)level[v] >= width.size()) width.push_back(0);
+        width[level[v]]++;
+        q.push_back(v);
+      }
+    }
+  }
+  return *std::max_element(width.begin(), width.end());
+}
+
 Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
                  int order, int degree) {
   REQUIRE(nnode > 0 && nelem > 0, ADMFEM_ERR_ARG, "empty mesh");
@@ -256,9 +282,10 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
         std::copy(part[c].begin(), part[c].end(), ucol.begin() + urp[b]);
       });
     }
-    // internal numbering: reverse Cuthill-McKee over the node graph, or the
-    // element order when that is banded too (bandwidth within 2.5x of
-    // RCM's).  Either way rows run along fronts: the k-th neighbours of
+    // internal numbering: the element order when it is banded (bandwidth
+    // at most 5x the widest level of a breadth-first search from the first
+    // element's first vertex), else reverse Cuthill-McKee over the node
+    // graph.  Either way rows run along fronts: the k-th neighbours of
     // consecutive rows are consecutive nodes of the adjacent fronts, so the
     // solver's column gathers coalesce within a wavefront.  The element
     // order also makes the elements of a block of rows consecutive in the
@@ -272,11 +299,12 @@ Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
           if (ucol[k] != i) acol.push_back(ucol[k]);
         arp[i + 1] = (int)acol.size();
       }
-      std::vector<int> rcm = rcm_order(nnode, arp, acol);
       std::vector<int> eo = element_order(nnode, m->elems);
-      const int64_t bw_rcm = order_bandwidth(rcm, arp, acol),
-                    bw_eo = order_bandwidth(eo, arp, acol);
-      m->iperm = 2 * bw_eo <= 5 * bw_rcm ? std::move(eo) : std::move(rcm);
+      const int64_t bw_eo = order_bandwidth(eo, arp, acol);
+      if (bw_eo <= 5 * bfs_max_level(nnode, arp, acol, m->elems[0]))
+        m->iperm = std::move(eo);
+      else
+        m->iperm = rcm_order(nnode, arp, acol);
       m->perm.assign(nnode, 0);
       for (int k = 0; k < nnode; k++) m->perm[m->iperm[k]] = k;
     }

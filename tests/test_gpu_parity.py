@@ -268,6 +268,40 @@ def test_system_assembly_matches_oracle(m, heat):
     assert relerr(r, r0) < 1e-13
 
 
+@pytest.mark.parametrize("heat", [False, True])
+def test_shuffled_elements_system_matches_oracle(heat):
+    """Elements listed in random order: the element order is not banded, so
+    the nodes are numbered by reverse Cuthill-McKee instead."""
+    print(f"shuffled elements heat={heat}", flush=True)
+    c, e = meshgen.jittered_square(40, seed=11)
+    e = e[np.random.default_rng(4).permutation(e.shape[0])]
+    gm, om = admfem.Mesh(c, e, zero_based=True), O.OracleMesh(c, e)
+    rng = np.random.default_rng(8)
+    kap = np.exp(meshgen.grf(np.stack([om.gx, om.gy], 1), 4))
+    f = rng.uniform(-1, 1, om.ngauss)
+    bd = om.bcnode()
+    bdval = rng.standard_normal(bd.size)
+    S = admfem.DirichletSystem(gm, bd)
+    if heat:
+        up = rng.standard_normal(om.nnode)
+        S.assemble(kappa=kap, f=f, bdval=bdval, u_prev=up, lap_coef=0.05, mass_coef=1.0,
+                   src_coef=0.05)
+        A0, r0 = _oracle_system(om, kap, f, bd, bdval, dt=0.05, u_prev=up)
+    else:
+        S.assemble(kappa=kap, f=f, bdval=bdval)
+        A0, r0 = _oracle_system(om, kap, f, bd, bdval)
+    A, r = S.csr()
+    A0 = A0.tocsr()
+    A0.sort_indices()
+    np.testing.assert_array_equal(A.indptr, A0.indptr)
+    np.testing.assert_array_equal(A.indices, A0.indices)
+    assert relerr(A.data, A0.data) < 1e-13
+    assert relerr(r, r0) < 1e-13
+    u, st = S.solve(precision="fp64", rtol=1e-13)
+    assert st["status"] == 0
+    assert relerr(host(u), O.DirectSolver(A0).solve(r0)) < 1e-10
+
+
 @pytest.mark.parametrize("m,prec,tol", [(16, "fp64", 1e-10), (40, "fp64", 1e-10),
                                         (40, "mixed", 1e-8)])
 def test_solve_matches_direct(m, prec, tol):
