@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 
 #include "geom.h"
@@ -201,69 +202,102 @@ void bd_keys(const long long *bd_dev, int bdN, int N, hipStream_t st,
   }
 }
 
+// Per-device workspace of the op, grown on demand and kept until the process
+// ends (never freed at exit): a call allocates nothing once it has seen its
+// sizes.  Calls on one device are serialised by its lock.
+struct DirWork {
+  std::mutex mu;
+  DevBuf<int> keys, win, bdidx, keep, lift, err, lkey, skey;
+  DevBuf<int64_t> pos, lpos, lval, sval, cnt;
+  DevBuf<char> tmp;
+};
+
+template <class T>
+void ensure(DevBuf<T> &b, size_t n) {
+  if (b.n < n) b.alloc(std::max(n, b.n + b.n / 4));
+}
+
+DirWork &dir_work() {
+  static std::mutex mu;
+  static DirWork *w[64] = {};
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  REQUIRE(dev >= 0 && dev < 64, ADMFEM_ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> g(mu);
+  if (!w[dev]) w[dev] = new DirWork();
+  return *w[dev];
+}
+
 struct DirPrep {
-  DevBuf<int> keys, win, bdidx, keep, lift;
-  DevBuf<int64_t> pos, lpos;
+  DirWork &w;
   int nu = 0;
   int64_t nkept = 0, nlift = 0;
 };
 
-void exclusive_scan(const int *in, int64_t *out, int64_t n, hipStream_t st) {
+void exclusive_scan(DirWork &w, const int *in, int64_t *out, int64_t n, hipStream_t st) {
   size_t tmp = 0;
   HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, st));
-  DevBuf<char> t(std::max<size_t>(tmp, 1));
-  HIPCHECK(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, in, out, (int)n, st));
+  ensure(w.tmp, std::max<size_t>(tmp, 1));
+  HIPCHECK(hipcub::DeviceScan::ExclusiveSum(w.tmp.p, tmp, in, out, (int)n, st));
 }
+
+__global__ void set_i(int *__restrict__ a, int v) { *a = v; }
 
 void prepare(DirPrep &d, const long long *ind, int64_t sN, const long long *bd, int bdN,
              int N, hipStream_t st) {
   REQUIRE(sN >= 0 && sN < (int64_t)INT32_MAX, ADMFEM_ERR_ARG, "triplet count out of range");
+  DirWork &w = d.w;
   std::vector<int> keys, win;
   bd_keys(bd, bdN, N, st, keys, win);
   d.nu = (int)keys.size();
-  d.keys.upload(keys);
-  d.win.upload(win);
-  d.bdidx.alloc(N);
-  fill_int<<<nblk(N, 256), 256, 0, st>>>(d.bdidx.p, N, -1);
-  if (d.nu) set_bdidx<<<nblk(d.nu, 256), 256, 0, st>>>(d.keys.p, d.win.p, d.nu, d.bdidx.p);
-  d.keep.alloc(sN + 1);
-  d.lift.alloc(sN + 1);
-  DevBuf<int> err(1);
-  HIPCHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
-  d.pos.alloc(sN + 1);
-  d.lpos.alloc(sN + 1);
-  HIPCHECK(hipMemsetAsync(d.keep.p, 0, sizeof(int) * (sN + 1), st));
-  HIPCHECK(hipMemsetAsync(d.lift.p, 0, sizeof(int) * (sN + 1), st));
+  ensure(w.keys, std::max<size_t>(keys.size(), 1));
+  ensure(w.win, std::max<size_t>(win.size(), 1));
+  if (d.nu) {
+    HIPCHECK(hipMemcpyAsync(w.keys.p, keys.data(), sizeof(int) * d.nu, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(w.win.p, win.data(), sizeof(int) * d.nu, hipMemcpyHostToDevice, st));
+  }
+  ensure(w.bdidx, N);
+  fill_int<<<nblk(N, 256), 256, 0, st>>>(w.bdidx.p, N, -1);
+  if (d.nu) set_bdidx<<<nblk(d.nu, 256), 256, 0, st>>>(w.keys.p, w.win.p, d.nu, w.bdidx.p);
+  ensure(w.keep, sN + 1);
+  ensure(w.lift, sN + 1);
+  ensure(w.pos, sN + 1);
+  ensure(w.lpos, sN + 1);
+  ensure(w.err, 1);
+  set_i<<<1, 1, 0, st>>>(w.err.p, 0);
+  set_i<<<1, 1, 0, st>>>(w.keep.p + sN, 0);  // the scans' last entry: their totals
+  set_i<<<1, 1, 0, st>>>(w.lift.p + sN, 0);
   if (sN)
-    classify<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, N, d.bdidx.p, d.keep.p, d.lift.p, err.p);
+    classify<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, N, w.bdidx.p, w.keep.p, w.lift.p, w.err.p);
   HIPCHECK(hipGetLastError());
+  exclusive_scan(w, w.keep.p, w.pos.p, sN + 1, st);
+  exclusive_scan(w, w.lift.p, w.lpos.p, sN + 1, st);
   int herr = 0;
-  HIPCHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  exclusive_scan(d.keep.p, d.pos.p, sN + 1, st);
-  exclusive_scan(d.lift.p, d.lpos.p, sN + 1, st);
-  HIPCHECK(hipMemcpyAsync(&d.nkept, d.pos.p + sN, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  HIPCHECK(hipMemcpyAsync(&d.nlift, d.lpos.p + sN, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(&herr, w.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(&d.nkept, w.pos.p + sN, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(&d.nlift, w.lpos.p + sN, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   REQUIRE(!herr, ADMFEM_ERR_ARG, "triplet index out of range [0, N)");
 }
 
-// stably sorted (key, triplet) list of the lifted triplets
+// stably sorted (key, triplet) list of the lifted triplets, in w.skey / w.sval
 void sorted_lift(const DirPrep &d, const long long *ind, int64_t sN, int use_col,
-                 DevBuf<int> &skey, DevBuf<int64_t> &sval, hipStream_t st) {
-  int64_t nl = d.nlift;
-  DevBuf<int> lkey(std::max<int64_t>(nl, 1));
-  DevBuf<int64_t> lval(std::max<int64_t>(nl, 1));
-  skey.alloc(std::max<int64_t>(nl, 1));
-  sval.alloc(std::max<int64_t>(nl, 1));
+                 hipStream_t st) {
+  DirWork &w = d.w;
+  const int64_t nl = d.nlift;
+  ensure(w.lkey, std::max<int64_t>(nl, 1));
+  ensure(w.lval, std::max<int64_t>(nl, 1));
+  ensure(w.skey, std::max<int64_t>(nl, 1));
+  ensure(w.sval, std::max<int64_t>(nl, 1));
   if (nl == 0) return;
-  scatter_lift<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, d.lift.p, d.lpos.p, use_col, lkey.p,
-                                              lval.p);
+  scatter_lift<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, w.lift.p, w.lpos.p, use_col, w.lkey.p,
+                                              w.lval.p);
   size_t tmp = 0;
-  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, lkey.p, skey.p, lval.p, sval.p,
-                                              (int)nl, 0, 32, st));
-  DevBuf<char> t(std::max<size_t>(tmp, 1));
-  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, lkey.p, skey.p, lval.p, sval.p,
-                                              (int)nl, 0, 32, st));
+  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, w.lkey.p, w.skey.p, w.lval.p,
+                                              w.sval.p, (int)nl, 0, 32, st));
+  ensure(w.tmp, std::max<size_t>(tmp, 1));
+  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(w.tmp.p, tmp, w.lkey.p, w.skey.p, w.lval.p,
+                                              w.sval.p, (int)nl, 0, 32, st));
 }
 
 }  // namespace
@@ -273,18 +307,18 @@ int64_t run_impose_dirichlet(const long long *ind, const double *vv, int64_t sN,
                              const double *rhs, int N, long long *oind, double *ov,
                              double *orhs, hipStream_t st) {
   REQUIRE(N > 0 && bdN >= 0, ADMFEM_ERR_ARG, "bad sizes");
-  DirPrep d;
+  DirWork &w = dir_work();
+  std::lock_guard<std::mutex> g(w.mu);
+  DirPrep d{w};
   prepare(d, ind, sN, bd, bdN, N, st);
-  if (sN) scatter_kept<<<nblk(sN, 256), 256, 0, st>>>(ind, vv, sN, d.keep.p, d.pos.p, oind, ov);
-  if (d.nu) append_diag<<<nblk(d.nu, 256), 256, 0, st>>>(d.keys.p, d.nu, d.nkept, oind, ov);
+  if (sN) scatter_kept<<<nblk(sN, 256), 256, 0, st>>>(ind, vv, sN, w.keep.p, w.pos.p, oind, ov);
+  if (d.nu) append_diag<<<nblk(d.nu, 256), 256, 0, st>>>(w.keys.p, d.nu, d.nkept, oind, ov);
   copy_rhs<<<nblk(N, 256), 256, 0, st>>>(rhs, orhs, N);
-  DevBuf<int> skey;
-  DevBuf<int64_t> sval;
-  sorted_lift(d, ind, sN, 0, skey, sval, st);
+  sorted_lift(d, ind, sN, 0, st);
   if (d.nlift)
-    lift_rows<<<nblk(d.nlift, 256), 256, 0, st>>>(skey.p, sval.p, d.nlift, ind, vv, d.bdidx.p,
-                                                  bdval, orhs);
-  if (d.nu) set_bd_rhs<<<nblk(d.nu, 256), 256, 0, st>>>(d.keys.p, d.win.p, d.nu, bdval, orhs);
+    lift_rows<<<nblk(d.nlift, 256), 256, 0, st>>>(w.skey.p, w.sval.p, d.nlift, ind, vv,
+                                                  w.bdidx.p, bdval, orhs);
+  if (d.nu) set_bd_rhs<<<nblk(d.nu, 256), 256, 0, st>>>(w.keys.p, w.win.p, d.nu, bdval, orhs);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));
   return d.nkept + d.nu;
@@ -295,21 +329,21 @@ void run_impose_dirichlet_grad(const double *gov, const double *gorhs,
                                const long long *bd, const double *bdval, int bdN, int N,
                                double *gvv, double *grhs, double *gbd, hipStream_t st) {
   REQUIRE(N > 0 && bdN >= 0, ADMFEM_ERR_ARG, "bad sizes");
-  DirPrep d;
+  DirWork &w = dir_work();
+  std::lock_guard<std::mutex> g(w.mu);
+  DirPrep d{w};
   prepare(d, ind, sN, bd, bdN, N, st);
   if (sN)
-    grad_vv_rhs<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, d.keep.p, d.pos.p, d.bdidx.p, bdval,
+    grad_vv_rhs<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, w.keep.p, w.pos.p, w.bdidx.p, bdval,
                                                gov, gorhs, gvv);
-  grad_rhs_k<<<nblk(N, 256), 256, 0, st>>>(d.bdidx.p, gorhs, grhs, N);
+  grad_rhs_k<<<nblk(N, 256), 256, 0, st>>>(w.bdidx.p, gorhs, grhs, N);
   if (bdN) zero_d<<<nblk(bdN, 256), 256, 0, st>>>(gbd, bdN);
-  DevBuf<int> skey;
-  DevBuf<int64_t> sval;
-  sorted_lift(d, ind, sN, 1, skey, sval, st);
+  sorted_lift(d, ind, sN, 1, st);
   if (d.nlift)
-    grad_bdval_k<<<nblk(d.nlift, 256), 256, 0, st>>>(skey.p, sval.p, d.nlift, ind, vv,
-                                                     d.bdidx.p, gorhs, gbd);
+    grad_bdval_k<<<nblk(d.nlift, 256), 256, 0, st>>>(w.skey.p, w.sval.p, d.nlift, ind, vv,
+                                                     w.bdidx.p, gorhs, gbd);
   if (d.nu)
-    grad_bdval_diag<<<nblk(d.nu, 256), 256, 0, st>>>(d.keys.p, d.win.p, d.nu, gorhs, gbd);
+    grad_bdval_diag<<<nblk(d.nu, 256), 256, 0, st>>>(w.keys.p, w.win.p, d.nu, gorhs, gbd);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));
 }

@@ -70,7 +70,7 @@ __global__ void fem_grad_kernel(int nelem, QuadRule rule, const int *__restrict_
 // Per node: the transposes of the two maps above, and K(nu) u.
 enum { NODE_TO_GAUSS_T = 0, NODE_GRAD_T = 1, NODE_LAPLACE = 2 };
 
-template <int OP>
+template <int OP, int NQ>
 __global__ void node_gather_kernel(int n, int nelem, QuadRule rule, const int *__restrict__ conn,
                                    const int *__restrict__ conn_user,
                                    const double *__restrict__ x, const double *__restrict__ y,
@@ -78,38 +78,44 @@ __global__ void node_gather_kernel(int n, int nelem, QuadRule rule, const int *_
                                    const int *__restrict__ n2e_elem,
                                    const uint32_t *__restrict__ slots,
                                    const int *__restrict__ iperm,
+                                   const double *__restrict__ gw,   // mesh Gauss weights
                                    const double *__restrict__ in,   // Gauss array or field
                                    const double *__restrict__ nu,   // NODE_LAPLACE
                                    double *__restrict__ out_user) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   double acc = 0.0;
-  for (int t = n2e_ptr[r]; t < n2e_ptr[r + 1]; t++) {
+  const int t1 = n2e_ptr[r + 1];
+#pragma unroll 4
+  for (int t = n2e_ptr[r]; t < t1; t++) {  // unrolled: the incidences' loads overlap
     const int e = n2e_elem[t];
     const int p = (int)(slots[t] & 3u);
     if (OP == NODE_TO_GAUSS_T) {
-      for (int k = 0; k < rule.nq; k++)
-        acc += in[(size_t)rule.nq * e + k] * shape(p, rule.xi[k], rule.eta[k]);
+#pragma unroll
+      for (int k = 0; k < NQ; k++)
+        acc += in[(size_t)NQ * e + k] * shape(p, rule.xi[k], rule.eta[k]);
     } else {
       const Tri tr = load_tri(conn, nelem, e, x, y);
-      const ElemGeom G = elem_geom(tr.x[0], tr.y[0], tr.x[1], tr.y[1], tr.x[2], tr.y[2]);
+      double gx[3], gy[3];
+      elem_grads(tr.x[0], tr.y[0], tr.x[1], tr.y[1], tr.x[2], tr.y[2], gx, gy);
       if (OP == NODE_GRAD_T) {
-        for (int k = 0; k < rule.nq; k++) {
-          const size_t g = (size_t)rule.nq * e + k;
-          acc += in[2 * g] * G.gx[p] + in[2 * g + 1] * G.gy[p];
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+          const size_t g = (size_t)NQ * e + k;
+          acc += in[2 * g] * gx[p] + in[2 * g + 1] * gy[p];
         }
       } else {  // K(nu) u: sum_k nu_k w_k grad phi_p . grad u_e
         double ux = 0.0, uy = 0.0;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
           const double uq = in[conn_user[(size_t)q * nelem + e]];
-          ux += G.gx[q] * uq;
-          uy += G.gy[q] * uq;
+          ux += gx[q] * uq;
+          uy += gy[q] * uq;
         }
         double cw = 0.0;
-        for (int k = 0; k < rule.nq; k++)
-          cw += nu[(size_t)rule.nq * e + k] * gauss_weight(rule.w[k], G.area);
-        acc += cw * (G.gx[p] * ux + G.gy[p] * uy);
+#pragma unroll
+        for (int k = 0; k < NQ; k++) cw += nu[(size_t)NQ * e + k] * gw[(size_t)NQ * e + k];
+        acc += cw * (gx[p] * ux + gy[p] * uy);
       }
     }
   }
@@ -147,9 +153,15 @@ inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 template <int OP>
 void node_gather(const Mesh &m, const double *in, const double *nu, double *out_user,
                  hipStream_t st) {
-  node_gather_kernel<OP><<<nb(m.nnode, 256), 256, 0, st>>>(
-      m.nnode, m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, m.d_n2e_ptr.p,
-      m.d_n2e_elem.p, m.d_slots_full.p, m.d_iperm.p, in, nu, out_user);
+  auto go = [&](auto kern) {
+    kern<<<nb(m.nnode, 256), 256, 0, st>>>(m.nnode, m.nelem, m.rule, m.conn.p, m.conn_user.p,
+                                           m.x.p, m.y.p, m.d_n2e_ptr.p, m.d_n2e_elem.p,
+                                           m.d_slots_full.p, m.d_iperm.p, m.gw.p, in, nu,
+                                           out_user);
+  };
+  if (m.rule.nq == 3) go(node_gather_kernel<OP, 3>);
+  else if (m.rule.nq == 6) go(node_gather_kernel<OP, 6>);
+  else go(node_gather_kernel<OP, 1>);
   HIPCHECK(hipGetLastError());
 }
 

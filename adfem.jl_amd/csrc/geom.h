@@ -28,17 +28,22 @@ __device__ __forceinline__ double heron_area(double x0, double y0, double x1,
 // grad phi_k = (y_{k+1} - y_{k+2}, x_{k+2} - x_{k+1}) / det, the closed form
 // of CalcPhysDShape (dshape_ref * J^{-1}) for the affine P1 map
 // (deps/MFEM/Common.cpp:119-126).
+__device__ __forceinline__ void elem_grads(double x0, double y0, double x1, double y1,
+                                           double x2, double y2, double (&gx)[3],
+                                           double (&gy)[3]) {
+  double det = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
+  double id = 1.0 / det;
+  gx[0] = (y1 - y2) * id;
+  gy[0] = (x2 - x1) * id;
+  gx[1] = (y2 - y0) * id;
+  gy[1] = (x0 - x2) * id;
+  gx[2] = (y0 - y1) * id;
+  gy[2] = (x1 - x0) * id;
+}
 __device__ __forceinline__ ElemGeom elem_geom(double x0, double y0, double x1,
                                               double y1, double x2, double y2) {
   ElemGeom g;
-  double det = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
-  double id = 1.0 / det;
-  g.gx[0] = (y1 - y2) * id;
-  g.gy[0] = (x2 - x1) * id;
-  g.gx[1] = (y2 - y0) * id;
-  g.gy[1] = (x0 - x2) * id;
-  g.gx[2] = (y0 - y1) * id;
-  g.gy[2] = (x1 - x0) * id;
+  elem_grads(x0, y0, x1, y1, x2, y2, g.gx, g.gy);
   g.area = heron_area(x0, y0, x1, y1, x2, y2);
   return g;
 }

@@ -52,32 +52,63 @@ __global__ void geometry_kernel(int nelem, QuadRule rule,
   }
 }
 
-// FemLaplaceScalar_forward (deps/MFEM/FemLaplace1/FemLaplaceScalar.h:3-26):
-// triplet t = 9*g + 3*p + q of Gauss point g = nq*e + k.  One thread per
-// triplet so the 24-byte outputs are written coalesced.
-__global__ void laplace_coo_kernel(int nelem, QuadRule rule,
-                                   const int *__restrict__ conn,
-                                   const int *__restrict__ conn_user,
-                                   const double *__restrict__ x,
-                                   const double *__restrict__ y,
-                                   const double *__restrict__ kappa,
-                                   long long *__restrict__ ind,
-                                   double *__restrict__ vv) {
+// FemLaplaceScalar_forward (deps/MFEM/FemLaplace1/FemLaplaceScalar.h:3-26)
+// and ComputeFemMassMatrix1_forward (ComputeFemMassMatrixMfem.h:4-29):
+// triplet t = 9*g + 3*p + q of Gauss point g = nq*e + k, value
+// D_pq * coef[g] * w[g] with D_pq = grad phi_p . grad phi_q (Laplace) or
+// h_p h_q (mass), w the mesh's Gauss weight (ip.weight * area / 0.5, the
+// same bits as the per-element computation).  A block owns CO_E consecutive
+// elements: one thread per element puts its 6 distinct D_pq and its user
+// vertex ids in LDS, then all threads write the block's contiguous run of
+// 9 * nq * CO_E triplets, coalesced.
+constexpr int CO_NT = 256, CO_E = 64;
+
+template <bool MASS>
+__global__ void __launch_bounds__(CO_NT)
+    coo_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
+               const int *__restrict__ conn_user, const double *__restrict__ x,
+               const double *__restrict__ y, const double *__restrict__ gw,
+               const double *__restrict__ coef, long long *__restrict__ ind,
+               double *__restrict__ vv) {
 #pragma clang fp contract(off)
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t total = (int64_t)9 * rule.nq * nelem;
-  if (t >= total) return;
-  int64_t g = t / 9;
-  int pq = (int)(t - 9 * g);
-  int p = pq / 3, q = pq - 3 * p;
-  int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
-  ElemCoords c = load_elem(conn, nelem, e, x, y);
-  ElemGeom G = elem_geom(c.x[0], c.y[0], c.x[1], c.y[1], c.x[2], c.y[2]);
-  double d = G.gx[p] * G.gx[q] + G.gy[p] * G.gy[q];
-  double w = gauss_weight(rule.w[k], G.area);
-  ind[2 * t] = conn_user[(size_t)p * nelem + e];
-  ind[2 * t + 1] = conn_user[(size_t)q * nelem + e];
-  vv[t] = d * kappa[g] * w;
+  __shared__ double dd[CO_E][6];        // D_pq, packed upper triangle (Laplace)
+  __shared__ int cu[CO_E][3];
+  const int e0 = blockIdx.x * CO_E;
+  const int ne = min(CO_E, nelem - e0);
+  if ((int)threadIdx.x < ne) {
+    const int e = e0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 3; k++) cu[threadIdx.x][k] = conn_user[(size_t)k * nelem + e];
+    if (!MASS) {
+      const ElemCoords c = load_elem(conn, nelem, e, x, y);
+      const ElemGeom G = elem_geom(c.x[0], c.y[0], c.x[1], c.y[1], c.x[2], c.y[2]);
+      int i = 0;
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = p; q < 3; q++) dd[threadIdx.x][i++] = G.gx[p] * G.gx[q] + G.gy[p] * G.gy[q];
+    }
+  }
+  __syncthreads();
+  const int nq = rule.nq;
+  const int64_t t0 = (int64_t)9 * nq * e0;
+  const int nt = 9 * nq * ne;
+  for (int l = threadIdx.x; l < nt; l += CO_NT) {
+    const int gl = l / 9, pq = l - 9 * gl;
+    const int p = pq / 3, q = pq - 3 * p;
+    const int el = gl / nq, k = gl - nq * el;
+    const int64_t g = (int64_t)nq * e0 + gl, t = t0 + l;
+    double d;
+    if (MASS) {
+      d = shape(p, rule.xi[k], rule.eta[k]) * shape(q, rule.xi[k], rule.eta[k]);
+    } else {
+      const int lo = min(p, q), hi = max(p, q);
+      d = dd[el][lo == 0 ? hi : (lo == 1 ? 2 + hi : 5)];
+    }
+    ind[2 * t] = cu[el][p];
+    ind[2 * t + 1] = cu[el][q];
+    vv[t] = d * coef[g] * gw[g];
+  }
 }
 
 // FemLaplaceScalar_backward (FemLaplaceScalar.h:28-54): one thread per Gauss
@@ -102,32 +133,6 @@ __global__ void laplace_coo_grad_kernel(int nelem, QuadRule rule,
       v += gvv[9 * g + 3 * p + q] * d;
     }
   gk[g] = v;
-}
-
-// ComputeFemMassMatrix1_forward (ComputeFemMassMatrixMfem.h:4-29).
-__global__ void mass_coo_kernel(int nelem, QuadRule rule,
-                                const int *__restrict__ conn,
-                                const int *__restrict__ conn_user,
-                                const double *__restrict__ x,
-                                const double *__restrict__ y,
-                                const double *__restrict__ rho,
-                                long long *__restrict__ ind,
-                                double *__restrict__ vv) {
-#pragma clang fp contract(off)
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t total = (int64_t)9 * rule.nq * nelem;
-  if (t >= total) return;
-  int64_t g = t / 9;
-  int pq = (int)(t - 9 * g);
-  int p = pq / 3, q = pq - 3 * p;
-  int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
-  ElemCoords c = load_elem(conn, nelem, e, x, y);
-  double A = heron_area(c.x[0], c.y[0], c.x[1], c.y[1], c.x[2], c.y[2]);
-  double w = gauss_weight(rule.w[k], A);
-  double hp = shape(p, rule.xi[k], rule.eta[k]), hq = shape(q, rule.xi[k], rule.eta[k]);
-  ind[2 * t] = conn_user[(size_t)p * nelem + e];
-  ind[2 * t + 1] = conn_user[(size_t)q * nelem + e];
-  vv[t] = hp * hq * rho[g] * w;
 }
 
 // ComputeFemMassMatrix1_backward (ComputeFemMassMatrixMfem.h:31-56), output
@@ -158,29 +163,27 @@ __global__ void mass_coo_grad_kernel(int nelem, QuadRule rule,
 // node r sums f[g]*h_p(g)*w[g] over its incident elements in ascending
 // element order and their Gauss points in order -- the same sequence of
 // additions the reference's element loop applies to rhs[dof].
-__global__ void source_kernel(int n, QuadRule rule, int nelem,
-                              const int *__restrict__ conn,
-                              const double *__restrict__ x,
-                              const double *__restrict__ y,
-                              const int *__restrict__ n2e_ptr,
+template <int NQ>
+__global__ void source_kernel(int n, QuadRule rule, const int *__restrict__ n2e_ptr,
                               const int *__restrict__ n2e_elem,
                               const uint32_t *__restrict__ slots,
-                              const int *__restrict__ iperm,
-                              const double *__restrict__ f,
-                              double *__restrict__ rhs_user, int accumulate) {
+                              const int *__restrict__ iperm, const double *__restrict__ gw,
+                              const double *__restrict__ f, double *__restrict__ rhs_user,
+                              int accumulate) {
 #pragma clang fp contract(off)
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   int u = iperm[r];
   double acc = accumulate ? rhs_user[u] : 0.0;
-  for (int t = n2e_ptr[r]; t < n2e_ptr[r + 1]; t++) {
-    int e = n2e_elem[t];
-    int p = slots[t] & 3u;
-    ElemCoords c = load_elem(conn, nelem, e, x, y);
-    double A = heron_area(c.x[0], c.y[0], c.x[1], c.y[1], c.x[2], c.y[2]);
-    for (int k = 0; k < rule.nq; k++) {
-      double w = gauss_weight(rule.w[k], A);
-      acc += f[(size_t)rule.nq * e + k] * shape(p, rule.xi[k], rule.eta[k]) * w;
+  const int t1 = n2e_ptr[r + 1];
+#pragma unroll 4
+  for (int t = n2e_ptr[r]; t < t1; t++) {  // unrolled: the incidences' loads overlap
+    const int e = n2e_elem[t];
+    const int p = slots[t] & 3u;
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      const size_t g = (size_t)NQ * e + k;
+      acc += f[g] * shape(p, rule.xi[k], rule.eta[k]) * gw[g];
     }
   }
   rhs_user[u] = acc;
@@ -188,19 +191,15 @@ __global__ void source_kernel(int n, QuadRule rule, int nelem,
 
 // FemSourceScalar_backward (FemSourceScalar.h:17-32).
 __global__ void source_grad_kernel(int nelem, QuadRule rule,
-                                   const int *__restrict__ conn,
                                    const int *__restrict__ conn_user,
-                                   const double *__restrict__ x,
-                                   const double *__restrict__ y,
+                                   const double *__restrict__ gw,
                                    const double *__restrict__ grhs,
                                    double *__restrict__ gf) {
 #pragma clang fp contract(off)
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)rule.nq * nelem) return;
   int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
-  ElemCoords c = load_elem(conn, nelem, e, x, y);
-  double A = heron_area(c.x[0], c.y[0], c.x[1], c.y[1], c.x[2], c.y[2]);
-  double w = gauss_weight(rule.w[k], A);
+  const double w = gw[g];
   double v = 0.0;
   for (int r = 0; r < 3; r++)
     v += shape(r, rule.xi[k], rule.eta[k]) * w * grhs[conn_user[(size_t)r * nelem + e]];
@@ -220,9 +219,8 @@ void launch_geometry(const Mesh &m, hipStream_t st) {
 void launch_laplace_coo(const Mesh &m, const double *kappa, long long *ind,
                         double *vv, hipStream_t st) {
   if (m.degree == 2) return launch_p2_coo(m, false, kappa, ind, vv, st);
-  int64_t total = 9 * m.ngauss;
-  laplace_coo_kernel<<<nblk(total, 256), 256, 0, st>>>(
-      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, kappa, ind, vv);
+  coo_kernel<false><<<nblk(m.nelem, CO_E), CO_NT, 0, st>>>(
+      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, m.gw.p, kappa, ind, vv);
   HIPCHECK(hipGetLastError());
 }
 
@@ -237,9 +235,8 @@ void launch_laplace_coo_grad(const Mesh &m, const double *gvv, double *gk,
 void launch_mass_coo(const Mesh &m, const double *rho, long long *ind,
                      double *vv, hipStream_t st) {
   if (m.degree == 2) return launch_p2_coo(m, true, rho, ind, vv, st);
-  int64_t total = 9 * m.ngauss;
-  mass_coo_kernel<<<nblk(total, 256), 256, 0, st>>>(
-      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, rho, ind, vv);
+  coo_kernel<true><<<nblk(m.nelem, CO_E), CO_NT, 0, st>>>(
+      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, m.gw.p, rho, ind, vv);
   HIPCHECK(hipGetLastError());
 }
 
@@ -254,18 +251,22 @@ void launch_mass_coo_grad(const Mesh &m, const double *gvv, double *gr,
 void launch_source(const Mesh &m, const double *f, double *rhs_user,
                    bool accumulate, hipStream_t st) {
   if (m.degree == 2) return launch_p2_source(m, f, rhs_user, accumulate, st);
-  source_kernel<<<nblk(m.nnode, 256), 256, 0, st>>>(
-      m.nnode, m.rule, m.nelem, m.conn.p, m.x.p, m.y.p, m.d_n2e_ptr.p,
-      m.d_n2e_elem.p, m.d_slots_full.p, m.d_iperm.p, f, rhs_user,
-      accumulate ? 1 : 0);
+  auto go = [&](auto kern) {
+    kern<<<nblk(m.nnode, 256), 256, 0, st>>>(m.nnode, m.rule, m.d_n2e_ptr.p, m.d_n2e_elem.p,
+                                             m.d_slots_full.p, m.d_iperm.p, m.gw.p, f, rhs_user,
+                                             accumulate ? 1 : 0);
+  };
+  if (m.rule.nq == 3) go(source_kernel<3>);
+  else if (m.rule.nq == 6) go(source_kernel<6>);
+  else go(source_kernel<1>);
   HIPCHECK(hipGetLastError());
 }
 
 void launch_source_grad(const Mesh &m, const double *grhs_user, double *gf,
                         hipStream_t st) {
   if (m.degree == 2) return launch_p2_source_grad(m, grhs_user, gf, st);
-  source_grad_kernel<<<nblk(m.ngauss, 256), 256, 0, st>>>(
-      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, grhs_user, gf);
+  source_grad_kernel<<<nblk(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn_user.p,
+                                                         m.gw.p, grhs_user, gf);
   HIPCHECK(hipGetLastError());
 }
 

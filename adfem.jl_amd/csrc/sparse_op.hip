@@ -6,8 +6,18 @@
 // Duplicates are summed in triplet order (Eigen's setFromTriplets order);
 // the matrix must be symmetric positive definite, since the solve is the
 // library's Jacobi-PCG (then A^-T = A^-1).
+//
+// The pattern is built on the device: a stable radix sort of the keys
+// row * n + col (equal keys keep their triplet order), one slot per run of
+// equal keys.  The last pattern is kept (per device): a call whose indices
+// equal it -- checked exactly on the device -- reuses its pattern, solver
+// layout and CG workspace and only sums the values again, as an optimisation
+// loop or the gradient after its forward solve does.  Calls are serialised
+// by a lock (the kept pattern and its workspace are shared).
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
-#include <numeric>
+#include <mutex>
 
 #include "internal.h"
 
@@ -17,8 +27,53 @@ System *system_from_csr(int n, std::vector<int> rp, std::vector<int> col);
 
 namespace {
 
+inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// key = row * n + col, value = triplet index; bad indices raise the flag
+// (and get key 0, never used: the call fails)
+__global__ void coo_keys(int64_t nnz, const long long *__restrict__ ind, int n,
+                         unsigned long long *__restrict__ key, int *__restrict__ val,
+                         int *__restrict__ err) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const long long i = ind[2 * k], j = ind[2 * k + 1];
+  const bool ok = i >= 0 && i < n && j >= 0 && j < n;
+  if (!ok) *err = 1;
+  key[k] = ok ? (unsigned long long)i * (unsigned long long)n + (unsigned long long)j : 0ull;
+  val[k] = (int)k;
+}
+
+__global__ void coo_fresh(int64_t nnz, const unsigned long long *__restrict__ skey,
+                          int *__restrict__ fresh) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nnz) fresh[t] = (t == 0 || skey[t] != skey[t - 1]) ? 1 : 0;
+}
+
+// slot s starts at sorted position start[s]; its row and column
+__global__ void coo_slots(int64_t nnz, const unsigned long long *__restrict__ skey,
+                          const int *__restrict__ fresh, const int *__restrict__ pos, int n,
+                          int *__restrict__ start, int *__restrict__ srow,
+                          int *__restrict__ scol) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnz || !fresh[t]) return;
+  const int s = pos[t];
+  start[s] = (int)t;
+  srow[s] = (int)(skey[t] / (unsigned long long)n);
+  scol[s] = (int)(skey[t] % (unsigned long long)n);
+}
+
+// any index differing from the kept pattern's raises the flag (plain stores
+// of the same value: no atomics needed)
+__global__ void coo_differs(int64_t m, const long long *__restrict__ a,
+                            const long long *__restrict__ b, int *__restrict__ flag) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m && a[k] != b[k]) *flag = 1;
+}
+
+__global__ void set_int(int *__restrict__ a, int v) { *a = v; }
+
 // val[slot] = sum of vv over the slot's triplets, in triplet order
-__global__ void coo_sum(int nslot, const int *__restrict__ start, const int64_t *__restrict__ trip,
+__global__ void coo_sum(int nslot, const int *__restrict__ start, const int *__restrict__ trip,
                         const double *__restrict__ vv, double *__restrict__ val) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslot) return;
@@ -34,53 +89,129 @@ __global__ void coo_grad(int64_t nnz, const long long *__restrict__ ind,
   if (k < nnz) gvv[k] = -lam[ind[2 * k]] * u[ind[2 * k + 1]];
 }
 
-inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
-
-struct CooSystem {
+// A COO matrix's pattern, ready to receive values and solve.
+struct CooPattern {
+  int device = -1, n = 0, nslot = 0;
+  int64_t nnz = -1;
+  DevBuf<long long> ind;  // the indices it was built from
+  DevBuf<int> start;      // nslot + 1 sorted positions
+  DevBuf<int> trip;       // triplet index at each sorted position
   System *s = nullptr;
-  ~CooSystem() { system_free(s); }
 };
 
-// Pattern + summed values of the COO matrix, ready to solve.
-void build(CooSystem &cs, const long long *ind_dev, const double *vv, int64_t nnz, int n,
-           hipStream_t st) {
-  REQUIRE(n > 0 && nnz >= 0 && nnz < INT32_MAX, ADMFEM_ERR_ARG, "bad sizes");
-  std::vector<long long> ind(2 * (size_t)nnz);
-  if (nnz)
-    HIPCHECK(hipMemcpyAsync(ind.data(), ind_dev, sizeof(long long) * 2 * nnz,
-                            hipMemcpyDeviceToHost, st));
+std::mutex g_lock;
+CooPattern *g_kept = nullptr;  // kept until the process ends (never freed at exit)
+
+bool same_pattern(const CooPattern &p, const long long *ind, int64_t nnz, int n, int dev,
+                  hipStream_t st) {
+  if (p.device != dev || p.n != n || p.nnz != nnz) return false;
+  if (nnz == 0) return true;
+  DevBuf<int> flag(1);
+  set_int<<<1, 1, 0, st>>>(flag.p, 0);
+  coo_differs<<<nb(2 * nnz, 256), 256, 0, st>>>(2 * nnz, ind, p.ind.p, flag.p);
+  HIPCHECK(hipGetLastError());
+  int h = 1;
+  HIPCHECK(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
-  for (int64_t k = 0; k < 2 * nnz; k++)
-    REQUIRE(ind[k] >= 0 && ind[k] < n, ADMFEM_ERR_ARG, "triplet index out of range [0, n)");
-  std::vector<int64_t> ord(nnz);
-  std::iota(ord.begin(), ord.end(), 0);
-  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
-    return ind[2 * a] < ind[2 * b] || (ind[2 * a] == ind[2 * b] && ind[2 * a + 1] < ind[2 * b + 1]);
-  });
-  std::vector<int> rp(n + 1, 0), col, start;
-  col.reserve(nnz);
-  start.reserve(nnz + 1);
-  for (int64_t t = 0; t < nnz; t++) {
-    const int64_t k = ord[t];
-    const bool fresh = t == 0 || ind[2 * k] != ind[2 * ord[t - 1]] ||
-                       ind[2 * k + 1] != ind[2 * ord[t - 1] + 1];
-    if (fresh) {
-      col.push_back((int)ind[2 * k + 1]);
-      start.push_back((int)t);
-      rp[ind[2 * k] + 1]++;
-    }
+  return h == 0;
+}
+
+void build_pattern(CooPattern &p, const long long *ind, int64_t nnz, int n, int dev,
+                   hipStream_t st) {
+  p.device = dev;
+  p.n = n;
+  p.nnz = nnz;
+  const int64_t m = std::max<int64_t>(nnz, 1);
+  DevBuf<unsigned long long> key(m), skey(m);
+  DevBuf<int> val(m), fresh(m), pos(m), err(1);
+  p.trip.alloc(m);
+  set_int<<<1, 1, 0, st>>>(err.p, 0);
+  int nslot = 0;
+  std::vector<int> rp(n + 1, 0), col;
+  if (nnz) {
+    coo_keys<<<nb(nnz, 256), 256, 0, st>>>(nnz, ind, n, key.p, val.p, err.p);
+    HIPCHECK(hipGetLastError());
+    int herr = 0;
+    HIPCHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    REQUIRE(!herr, ADMFEM_ERR_ARG, "triplet index out of range [0, n)");
+    const unsigned long long nn = (unsigned long long)n * (unsigned long long)n;
+    int bits = 1;
+    while (bits < 64 && (nn - 1) >> bits) bits++;
+    size_t tmp = 0;
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key.p, skey.p, val.p, p.trip.p,
+                                                (int)nnz, 0, bits, st));
+    DevBuf<char> t(std::max<size_t>(tmp, 1));
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, key.p, skey.p, val.p, p.trip.p,
+                                                (int)nnz, 0, bits, st));
+    coo_fresh<<<nb(nnz, 256), 256, 0, st>>>(nnz, skey.p, fresh.p);
+    tmp = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, fresh.p, pos.p, (int)nnz, st));
+    DevBuf<char> t2(std::max<size_t>(tmp, 1));
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(t2.p, tmp, fresh.p, pos.p, (int)nnz, st));
+    int last[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(&last[0], pos.p + nnz - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&last[1], fresh.p + nnz - 1, sizeof(int), hipMemcpyDeviceToHost,
+                            st));
+    HIPCHECK(hipStreamSynchronize(st));
+    nslot = last[0] + last[1];
+    p.start.alloc(nslot + 1);
+    DevBuf<int> srow(nslot), scol(nslot);
+    coo_slots<<<nb(nnz, 256), 256, 0, st>>>(nnz, skey.p, fresh.p, pos.p, n, p.start.p, srow.p,
+                                            scol.p);
+    HIPCHECK(hipGetLastError());
+    const int end = (int)nnz;
+    HIPCHECK(hipMemcpyAsync(p.start.p + nslot, &end, sizeof(int), hipMemcpyHostToDevice, st));
+    std::vector<int> hrow(nslot);
+    col.resize(nslot);
+    HIPCHECK(hipMemcpyAsync(hrow.data(), srow.p, sizeof(int) * nslot, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(col.data(), scol.p, sizeof(int) * nslot, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int s = 0; s < nslot; s++) rp[hrow[s] + 1]++;
+    for (int i = 0; i < n; i++) rp[i + 1] += rp[i];
+  } else {
+    p.start.alloc(1);
+    const int zero = 0;
+    HIPCHECK(hipMemcpyAsync(p.start.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
   }
-  start.push_back((int)nnz);
-  for (int i = 0; i < n; i++) rp[i + 1] += rp[i];
-  const int nslot = (int)col.size();
-  cs.s = system_from_csr(n, std::move(rp), std::move(col));
-  DevBuf<int> d_start;
-  DevBuf<int64_t> d_trip;
-  d_start.upload(start);
-  d_trip.upload(ord);
-  if (nslot) coo_sum<<<nb(nslot, 256), 256, 0, st>>>(nslot, d_start.p, d_trip.p, vv, cs.s->val.p);
-  launch_to_sell(*cs.s, st);
-  HIPCHECK(hipStreamSynchronize(st));  // d_start / d_trip go out of scope
+  p.nslot = nslot;
+  p.s = system_from_csr(n, std::move(rp), std::move(col));
+  p.ind.alloc(std::max<int64_t>(2 * nnz, 1));
+  if (nnz)
+    HIPCHECK(hipMemcpyAsync(p.ind.p, ind, sizeof(long long) * 2 * nnz, hipMemcpyDeviceToDevice,
+                            st));
+  HIPCHECK(hipStreamSynchronize(st));  // temporaries go out of scope
+}
+
+// The kept pattern for these indices (rebuilt when they differ), with the
+// values summed into its solver format.  Caller holds g_lock.
+System &pattern_with_values(const long long *ind, const double *vv, int64_t nnz, int n,
+                            hipStream_t st) {
+  REQUIRE(n > 0 && nnz >= 0 && nnz < INT32_MAX, ADMFEM_ERR_ARG, "bad sizes");
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (!g_kept || !same_pattern(*g_kept, ind, nnz, n, dev, st)) {
+    if (g_kept) {
+      system_free(g_kept->s);
+      delete g_kept;
+      g_kept = nullptr;
+    }
+    CooPattern *p = new CooPattern();
+    try {
+      build_pattern(*p, ind, nnz, n, dev, st);
+    } catch (...) {
+      system_free(p->s);
+      delete p;
+      throw;
+    }
+    g_kept = p;
+  }
+  CooPattern &p = *g_kept;
+  if (p.nslot)
+    coo_sum<<<nb(p.nslot, 256), 256, 0, st>>>(p.nslot, p.start.p, p.trip.p, vv, p.s->val.p);
+  HIPCHECK(hipGetLastError());
+  launch_to_sell(*p.s, st);
+  return *p.s;
 }
 
 }  // namespace
@@ -88,18 +219,18 @@ void build(CooSystem &cs, const long long *ind_dev, const double *vv, int64_t nn
 void sparse_solve(const long long *ind, const double *vv, int64_t nnz, int n, const double *rhs,
                   double *u, const admfem_solve_opts &o, admfem_solve_stats &stats,
                   hipStream_t st) {
-  CooSystem cs;
-  build(cs, ind, vv, nnz, n, st);
-  cg_solve(*cs.s, rhs, u, o, stats, st);
+  std::lock_guard<std::mutex> g(g_lock);
+  System &s = pattern_with_values(ind, vv, nnz, n, st);
+  cg_solve(s, rhs, u, o, stats, st);
 }
 
 void sparse_solve_grad(const long long *ind, const double *vv, int64_t nnz, int n,
                        const double *u, const double *grad_u, double *grad_vv,
                        double *grad_rhs, const admfem_solve_opts &o, admfem_solve_stats &stats,
                        hipStream_t st) {
-  CooSystem cs;
-  build(cs, ind, vv, nnz, n, st);
-  cg_solve(*cs.s, grad_u, grad_rhs, o, stats, st);  // lambda = A^-1 dL/du = dL/drhs
+  std::lock_guard<std::mutex> g(g_lock);
+  System &s = pattern_with_values(ind, vv, nnz, n, st);
+  cg_solve(s, grad_u, grad_rhs, o, stats, st);  // lambda = A^-1 dL/du = dL/drhs
   if (nnz) coo_grad<<<nb(nnz, 256), 256, 0, st>>>(nnz, ind, grad_rhs, u, grad_vv);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));

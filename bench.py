@@ -28,6 +28,11 @@ request queued at t = 0 (throughput at capacity).  After the timed steps:
   * precision_form: one 2048x2048-quad system (8.4M triangles), mixed vs plain
     FP64 forward+adjoint, errors against an FP64 CG solve to 1e-13, the
     single-system SpMV kernels and the assembly timed with HIP events;
+  * operators: each reference operator's kernel through its C-ABI entry
+    point (P1 at 2048x2048, P2 at 1024x1024 quads: the COO Laplace op and
+    its gradient, the load vector, the maps to Gauss points, FemGrad, the
+    matrix-free Laplace term; Dirichlet on a COO and the general K\\F op at
+    512x512), HIP events, against algorithmic bytes;
   * cpu_baseline: the CPU oracle (C restatement of the reference loops + SciPy
     SuperLU) on the first requests of the stream, one core.
 
@@ -340,6 +345,115 @@ def precision_form(m, rtol, kernels_only=False):
     return out
 
 
+# ------------------------------------------------------ operator kernels
+def ops_roofline(m1=2048, m2=1024, solve_m=512, repeat=5):
+    """Each reference operator's kernel(s) through its C-ABI entry point on
+    preallocated device buffers (HIP events on the stream the library is
+    handed), against algorithmic bytes: inputs read once, outputs written
+    once, with 12 B of connectivity per element and 16 B of coordinates per
+    node where the op needs geometry.  P1 at m1 x m1 quads, P2 at m2 x m2;
+    the general K\\F op on the Dirichlet-eliminated Poisson COO at solve_m."""
+    import ctypes
+    import torch
+    import admfem
+    from admfem import _lib, meshgen
+    L, st = _lib.lib(), _lib.stream_ptr
+    res = {}
+
+    def run(name, fn, nbytes, **extra):
+        fn()
+        avg = hip_time(lambda: [fn() for _ in range(repeat)], repeat)
+        res[name] = roof(nbytes, avg, name, **extra)
+
+    for deg, m in ((1, m1), (2, m2)):
+        c, e = meshgen.jittered_square(m, seed=SEEDS["mesh"])
+        mesh = admfem.Mesh(c, e, zero_based=True, degree=deg)
+        h, ng, nd, ne, N = mesh.handle, mesh.ngauss, mesh.elem_ndof, mesh.nelem, mesh.ndof
+        nt = nd * nd * ng
+        geo = 12 * ne + 16 * mesh.nnode
+        tag = f"P{deg} {m}x{m}"
+        g = torch.rand(ng, dtype=torch.float64, device="cuda") + 0.5
+        u = torch.rand(N, dtype=torch.float64, device="cuda")
+        ind = torch.empty((nt, 2), dtype=torch.int64, device="cuda")
+        vv = torch.empty(nt, dtype=torch.float64, device="cuda")
+        P = _lib.ptr
+        run(f"{tag} laplace COO (FemLaplaceScalar)",
+            lambda: _lib.check(L.admfem_fem_laplace_scalar(h, P(g), P(ind), P(vv), st())),
+            nt * 24 + ng * 8 + geo, triplets=nt)
+        gk = torch.empty(ng, dtype=torch.float64, device="cuda")
+        run(f"{tag} laplace COO grad (FemLaplaceScalarGrad)",
+            lambda: _lib.check(L.admfem_fem_laplace_scalar_grad(h, P(vv), P(gk), st())),
+            nt * 8 + ng * 8 + geo)
+        del ind, vv
+        rhs = torch.empty(N, dtype=torch.float64, device="cuda")
+        run(f"{tag} source (FemSourceScalar)",
+            lambda: _lib.check(L.admfem_fem_source_scalar(h, P(g), P(rhs), st())),
+            ng * 8 + N * 8 + geo)
+        run(f"{tag} source grad (FemSourceScalarGrad)",
+            lambda: _lib.check(L.admfem_fem_source_scalar_grad(h, P(u), P(gk), st())),
+            N * 8 + ng * 8 + geo)
+        run(f"{tag} dof_to_gauss_points (DofToGaussPoints)",
+            lambda: _lib.check(L.admfem_dof_to_gauss_points(h, P(u), P(gk), st())),
+            N * 8 + ng * 8 + 4 * nd * ne)
+        run(f"{tag} dof_to_gauss_points grad",
+            lambda: _lib.check(L.admfem_dof_to_gauss_points_grad(h, P(g), P(rhs), st())),
+            ng * 8 + N * 8 + 4 * nd * ne)
+        gg = torch.empty(2 * ng, dtype=torch.float64, device="cuda")
+        run(f"{tag} eval_grad_on_gauss_pts1 (FemGrad)",
+            lambda: _lib.check(L.admfem_fem_grad(h, P(u), P(gg), st())),
+            N * 8 + 2 * ng * 8 + geo)
+        run(f"{tag} laplace term, matrix-free K(nu) u (ComputeLaplaceTerm)",
+            lambda: _lib.check(L.admfem_fem_laplace_term(h, P(u), P(g), P(rhs), st())),
+            N * 8 + ng * 8 + N * 8 + geo)
+        run(f"{tag} laplace term grad (ComputeLaplaceTermGrad)",
+            lambda: _lib.check(L.admfem_fem_laplace_term_grad(h, P(u), P(u), P(g), P(rhs),
+                                                               P(gk), st())),
+            2 * N * 8 + ng * 8 + N * 8 + ng * 8 + geo)
+        mesh.close()
+        del g, u, gk, gg, rhs
+        torch.cuda.empty_cache()
+    # Dirichlet on the COO, then the general sparse solve (ADCME SparseSolver)
+    c, e = meshgen.jittered_square(solve_m, seed=SEEDS["mesh"])
+    mesh = admfem.Mesh(c, e, zero_based=True)
+    kap = torch.rand(mesh.ngauss, dtype=torch.float64, device="cuda") + 0.5
+    f = torch.rand(mesh.ngauss, dtype=torch.float64, device="cuda") - 0.5
+    A = admfem.compute_fem_laplace_matrix1(kap, mesh)
+    F = admfem.compute_fem_source_term1(f, mesh)
+    bd = admfem.bcnode(mesh)
+    sN, N, nb = A.values.numel(), mesh.nnode, bd.size
+    bdt = torch.as_tensor(bd.astype(np.int64), device="cuda")
+    bdv = torch.zeros(nb, dtype=torch.float64, device="cuda")
+    oi = torch.empty((sN + nb, 2), dtype=torch.int64, device="cuda")
+    ov = torch.empty(sN + nb, dtype=torch.float64, device="cuda")
+    orhs = torch.empty(N, dtype=torch.float64, device="cuda")
+    cnt = ctypes.c_int64()
+    ind = A.indices.contiguous()
+    P = _lib.ptr
+
+    def dirichlet():
+        _lib.check(L.admfem_impose_dirichlet(P(ind), P(A.values), sN, P(bdt), P(bdv), nb, P(F),
+                                             N, P(oi), P(ov), P(orhs), ctypes.byref(cnt), st()))
+    dirichlet()
+    run(f"P1 {solve_m}x{solve_m} impose_Dirichlet_boundary_conditions (COO)", dirichlet,
+        sN * 24 + cnt.value * 24 + 2 * N * 8 + nb * 16, triplets_in=sN,
+        triplets_out=int(cnt.value))
+    K = admfem.SparseTensor(oi[:cnt.value], ov[:cnt.value], N)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        admfem.solve(K, orhs, precision=prec, rtol=1e-10)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        uu = admfem.solve(K, orhs, precision=prec, rtol=1e-10)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        S = admfem.DirichletSystem(mesh, bd).assemble(kappa=kap, f=f)
+        u0, _ = S.solve(precision="fp64", rtol=1e-13)
+        out[prec] = dict(seconds=round(dt, 4), rel_err_u=relerr(uu, u0))
+    res[f"P1 {solve_m}x{solve_m} K\\F general op (COO in, rtol 1e-10)"] = out
+    mesh.close()
+    return res
+
+
 # -------------------------------------------------------------- CPU baseline
 def cpu_baseline(sides, heat, dts, budget_s=20.0):
     """The CPU oracle on the first requests of the stream, pinned to one
@@ -384,6 +498,9 @@ def main():
                     help="set up the stream and time only the batched SpMV kernel")
     ap.add_argument("--precision-only", action="store_true",
                     help="run only the single-system precision form (for profiling)")
+    ap.add_argument("--ops-only", action="store_true",
+                    help="time only the reference operators' kernels (P1, P2, Dirichlet, K\\F)")
+    ap.add_argument("--no-ops", action="store_true", help="skip the operator-kernel leg")
     ap.add_argument("--kernels-only", action="store_true",
                     help="with --precision-only: time the SpMV and assembly kernels, no solves")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -408,6 +525,9 @@ def main():
     if a.precision_only:
         print(json.dumps(precision_form(a.precision_m, a.precision_rtol, a.kernels_only)),
               flush=True)
+        return
+    if a.ops_only:
+        print(json.dumps(ops_roofline()), flush=True)
         return
     n = a.requests
     sides, heat, dts = make_stream(n)
@@ -487,6 +607,12 @@ def main():
     }
     if a.precision_m > 0:
         line["precision_form"] = precision_form(a.precision_m, a.precision_rtol)
+    if not a.no_ops:
+        log("operator kernels")
+        try:
+            line["operators"] = ops_roofline()
+        except Exception as exc:  # reported, never hides the headline line
+            line["operators"] = {"error": repr(exc)}
     if not a.no_cpu:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(sides, heat, dts)

@@ -9,7 +9,7 @@ import scipy.sparse.linalg as spl
 import torch
 
 import admfem
-from admfem import meshgen
+from admfem import _lib, meshgen
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -151,3 +151,43 @@ def test_sparse_solve_far_coupling():
                      torch.tensor(b, device="cuda"), rtol=1e-13)
     ref = spl.spsolve(sp.csr_matrix((V, (I, J)), shape=(n, n)), b)
     assert np.abs(u.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-10
+
+
+def test_sparse_solve_pattern_kept_between_calls():
+    """The op keeps the last pattern: the same indices with new values, then
+    indices of the same sizes but another pattern, then an out-of-range
+    index (an error that leaves the op usable) -- each against SciPy."""
+    rng = np.random.default_rng(21)
+    n = 300
+
+    def spd(seed):
+        R = sp.random(n, n, density=0.02, random_state=seed)
+        A = (R @ R.T + sp.eye(n) * 3.0).tocoo()
+        return A.row.astype(np.int64), A.col.astype(np.int64), A.data
+
+    I, J, V = spd(1)
+    b = rng.standard_normal(n)
+    ind = torch.as_tensor(np.stack([I, J], 1), device="cuda")
+    for scale in (1.0, 2.5):  # same pattern, new values
+        u = admfem.solve(admfem.SparseTensor(ind, torch.tensor(scale * V, device="cuda"), n),
+                         torch.tensor(b, device="cuda"), rtol=1e-14)
+        ref = spl.spsolve(sp.csc_matrix((scale * V, (I, J)), shape=(n, n)), b)
+        assert rel(u.cpu().numpy(), ref) < 1e-10
+    # another pattern with the same n and nnz: a permuted copy of the matrix
+    p = rng.permutation(n)
+    I2, J2 = p[I], p[J]
+    ind2 = torch.as_tensor(np.stack([I2, J2], 1), device="cuda")
+    u = admfem.solve(admfem.SparseTensor(ind2, torch.tensor(V, device="cuda"), n),
+                     torch.tensor(b, device="cuda"), rtol=1e-14)
+    ref = spl.spsolve(sp.csc_matrix((V, (I2, J2)), shape=(n, n)), b)
+    assert rel(u.cpu().numpy(), ref) < 1e-10
+    bad = ind2.clone()
+    bad[5, 1] = n
+    with pytest.raises(_lib.AdmfemError) as ei:
+        admfem.solve(admfem.SparseTensor(bad, torch.tensor(V, device="cuda"), n),
+                     torch.tensor(b, device="cuda"), rtol=1e-14)
+    assert ei.value.code == _lib.ERR_ARG
+    u = admfem.solve(admfem.SparseTensor(ind, torch.tensor(V, device="cuda"), n),
+                     torch.tensor(b, device="cuda"), rtol=1e-14)
+    ref = spl.spsolve(sp.csc_matrix((V, (I, J)), shape=(n, n)), b)
+    assert rel(u.cpu().numpy(), ref) < 1e-10
