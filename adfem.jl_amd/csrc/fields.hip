@@ -68,6 +68,9 @@ __global__ void fem_grad_kernel(int nelem, QuadRule rule, const int *__restrict_
 }
 
 // Per node: the transposes of the two maps above, and K(nu) u.
+// NODE_LAPLACE reads its field in the internal node order (permuted once by
+// the launcher), so each incidence's three values are local gathers through
+// the element's internal vertex ids instead of random ones in user order.
 enum { NODE_TO_GAUSS_T = 0, NODE_GRAD_T = 1, NODE_LAPLACE = 2 };
 
 template <int OP, int NQ>
@@ -108,7 +111,7 @@ __global__ void node_gather_kernel(int n, int nelem, QuadRule rule, const int *_
         double ux = 0.0, uy = 0.0;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-          const double uq = in[conn_user[(size_t)q * nelem + e]];
+          const double uq = in[conn[(size_t)q * nelem + e]];
           ux += gx[q] * uq;
           uy += gy[q] * uq;
         }
@@ -149,6 +152,13 @@ __global__ void laplace_term_nu_grad_kernel(int nelem, QuadRule rule, const int 
 }
 
 inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// dst[r] = src[iperm[r]]: a user-order field in the internal order
+__global__ void permute_kernel(int n, const int *__restrict__ iperm,
+                               const double *__restrict__ src, double *__restrict__ dst) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[r] = src[iperm[r]];
+}
 
 template <int OP>
 void node_gather(const Mesh &m, const double *in, const double *nu, double *out_user,
@@ -201,10 +211,16 @@ void launch_fem_grad_grad(const Mesh &m, const double *grad_out, double *grad_u,
   node_gather<NODE_GRAD_T>(m, grad_out, nullptr, grad_u, st);
 }
 
+// u (user order) -> internal order in a stream-ordered temporary, then the
+// per-node gather
 void launch_laplace_term(const Mesh &m, const double *u, const double *nu, double *out,
                          hipStream_t st) {
   if (m.degree == 2) return launch_p2_laplace_term(m, u, nu, out, st);
-  node_gather<NODE_LAPLACE>(m, u, nu, out, st);
+  double *u_int = nullptr;
+  HIPCHECK(hipMallocAsync((void **)&u_int, sizeof(double) * std::max(m.nnode, 1), st));
+  permute_kernel<<<nb(m.nnode, 256), 256, 0, st>>>(m.nnode, m.d_iperm.p, u, u_int);
+  node_gather<NODE_LAPLACE>(m, u_int, nu, out, st);
+  HIPCHECK(hipFreeAsync(u_int, st));
 }
 
 void launch_laplace_term_nu_grad(const Mesh &m, const double *grad_out, const double *u,

@@ -35,7 +35,10 @@ __device__ __forceinline__ ElemGeom p2_geom(const int *__restrict__ conn, int ne
     ex[k] = x[v];
     ey[k] = y[v];
   }
-  return elem_geom(ex[0], ey[0], ex[1], ey[1], ex[2], ey[2]);
+  ElemGeom G;
+  elem_grads(ex[0], ey[0], ex[1], ey[1], ex[2], ey[2], G.gx, G.gy);
+  G.area = 0.0;  // unused: Gauss weights come from the mesh's array (same bits)
+  return G;
 }
 
 // P2 basis values at (xi, eta).
@@ -68,36 +71,62 @@ __device__ __forceinline__ void p2_grad(double xi, double eta, const ElemGeom &G
   }
 }
 
-// FemLaplaceScalar_forward / ComputeFemMassMatrix1_forward, P2: one thread
-// per triplet (FemLaplaceScalar.h:3-26, ComputeFemMassMatrixMfem.h:4-29).
+// FemLaplaceScalar_forward / ComputeFemMassMatrix1_forward, P2
+// (FemLaplaceScalar.h:3-26, ComputeFemMassMatrixMfem.h:4-29): triplet
+// t = 36 g + 6 p + q, value D_pq * coef[g] * w[g].  A block owns P2_E
+// consecutive elements: one thread per (element, Gauss point) puts the six
+// basis gradients in LDS (Laplace), then all threads write the block's
+// contiguous run of triplets, coalesced.
+constexpr int P2_NT = 256, P2_E = 16;
+
 template <bool MASS>
-__global__ void p2_coo_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
-                              const int *__restrict__ conn_dof, const double *__restrict__ x,
-                              const double *__restrict__ y, const double *__restrict__ coef,
-                              long long *__restrict__ ind, double *__restrict__ vv) {
+__global__ void __launch_bounds__(P2_NT)
+    p2_coo_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
+                  const int *__restrict__ conn_dof, const double *__restrict__ x,
+                  const double *__restrict__ y, const double *__restrict__ gw,
+                  const double *__restrict__ coef, long long *__restrict__ ind,
+                  double *__restrict__ vv) {
 #pragma clang fp contract(off)
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t total = (int64_t)36 * rule.nq * nelem;
-  if (t >= total) return;
-  const int64_t g = t / 36;
-  const int pq = (int)(t - 36 * g);
-  const int p = pq / 6, q = pq - 6 * p;
-  const int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
-  const ElemGeom G = p2_geom(conn, nelem, e, x, y);
-  const double w = gauss_weight(rule.w[k], G.area);
-  double v;
-  if (MASS) {
-    double h[6];
-    p2_shape(rule.xi[k], rule.eta[k], h);
-    v = h[p] * h[q] * coef[g] * w;
-  } else {
+  __shared__ double lgx[P2_E][MAXQ][6], lgy[P2_E][MAXQ][6];
+  __shared__ int cd[P2_E][6];
+  const int nq = rule.nq;
+  const int e0 = blockIdx.x * P2_E;
+  const int ne = min(P2_E, nelem - e0);
+  for (int i = threadIdx.x; i < ne * 6; i += P2_NT) {
+    const int el = i / 6, r = i - 6 * el;
+    cd[el][r] = conn_dof[(size_t)r * nelem + e0 + el];
+  }
+  if (!MASS && (int)threadIdx.x < ne * nq) {
+    const int el = threadIdx.x / nq, k = threadIdx.x - nq * el;
+    const ElemGeom G = p2_geom(conn, nelem, e0 + el, x, y);
     double gx[6], gy[6];
     p2_grad(rule.xi[k], rule.eta[k], G, gx, gy);
-    v = (gx[p] * gx[q] + gy[p] * gy[q]) * coef[g] * w;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      lgx[el][k][r] = gx[r];
+      lgy[el][k][r] = gy[r];
+    }
   }
-  ind[2 * t] = conn_dof[(size_t)p * nelem + e];
-  ind[2 * t + 1] = conn_dof[(size_t)q * nelem + e];
-  vv[t] = v;
+  __syncthreads();
+  const int64_t t0 = (int64_t)36 * nq * e0;
+  const int nt = 36 * nq * ne;
+  for (int l = threadIdx.x; l < nt; l += P2_NT) {
+    const int gl = l / 36, pq = l - 36 * gl;
+    const int p = pq / 6, q = pq - 6 * p;
+    const int el = gl / nq, k = gl - nq * el;
+    const int64_t g = (int64_t)nq * e0 + gl, t = t0 + l;
+    double v;
+    if (MASS) {
+      double h[6];
+      p2_shape(rule.xi[k], rule.eta[k], h);
+      v = h[p] * h[q] * coef[g] * gw[g];
+    } else {
+      v = (lgx[el][k][p] * lgx[el][k][q] + lgy[el][k][p] * lgy[el][k][q]) * coef[g] * gw[g];
+    }
+    ind[2 * t] = cd[el][p];
+    ind[2 * t + 1] = cd[el][q];
+    vv[t] = v;
+  }
 }
 
 // The two backward ops (FemLaplaceScalar.h:28-54, ComputeFemMassMatrixMfem.h:
@@ -105,13 +134,14 @@ __global__ void p2_coo_kernel(int nelem, QuadRule rule, const int *__restrict__ 
 template <bool MASS>
 __global__ void p2_coo_grad_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
                                    const double *__restrict__ x, const double *__restrict__ y,
+                                   const double *__restrict__ gw,
                                    const double *__restrict__ gvv, double *__restrict__ out) {
 #pragma clang fp contract(off)
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)rule.nq * nelem) return;
   const int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
   const ElemGeom G = p2_geom(conn, nelem, e, x, y);
-  const double w = gauss_weight(rule.w[k], G.area);
+  const double w = gw[g];
   double a[6], b[6];
   if (MASS) {
     p2_shape(rule.xi[k], rule.eta[k], a);
@@ -138,7 +168,8 @@ enum { G_SOURCE_T = 0, G_DOF_TO_GAUSS = 1, G_FEM_GRAD = 2, G_LAP_NU = 3 };
 template <int OP>
 __global__ void p2_gauss_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
                                 const int *__restrict__ conn_dof, const double *__restrict__ x,
-                                const double *__restrict__ y, const double *__restrict__ a,
+                                const double *__restrict__ y, const double *__restrict__ gw,
+                                const double *__restrict__ a,
                                 const double *__restrict__ b, double *__restrict__ out) {
 #pragma clang fp contract(off)
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,8 +187,7 @@ __global__ void p2_gauss_kernel(int nelem, QuadRule rule, const int *__restrict_
     out[g] = v;
     return;
   }
-  const ElemGeom G = p2_geom(conn, nelem, e, x, y);
-  const double w = gauss_weight(rule.w[k], G.area);
+  const double w = gw[g];
   if (OP == G_SOURCE_T) {
     double h[6];
     p2_shape(rule.xi[k], rule.eta[k], h);
@@ -167,6 +197,7 @@ __global__ void p2_gauss_kernel(int nelem, QuadRule rule, const int *__restrict_
     out[g] = v;
     return;
   }
+  const ElemGeom G = p2_geom(conn, nelem, e, x, y);
   double gx[6], gy[6];
   p2_grad(rule.xi[k], rule.eta[k], G, gx, gy);
   if (OP == G_FEM_GRAD) {
@@ -198,47 +229,52 @@ enum { D_SOURCE = 0, D_DOF_TO_GAUSS_T = 1, D_FEM_GRAD_T = 2, D_LAPLACE = 3 };
 //   D_FEM_GRAD_T      FemGradMfem_backward: sum_k in[2g] dh_p/dx + in[2g+1] dh_p/dy
 //   D_LAPLACE         ComputeLaplaceTermMfem_forward (= K(nu) u):
 //                     sum_k nu_k w_k grad h_p . grad u_e(k)
-template <int OP>
+template <int OP, int NQ>
 __global__ void p2_dof_kernel(int ndof, int nelem, QuadRule rule, const int *__restrict__ conn,
                               const int *__restrict__ conn_dof, const double *__restrict__ x,
-                              const double *__restrict__ y, const int *__restrict__ d2e_ptr,
-                              const int *__restrict__ d2e, const double *__restrict__ in,
-                              const double *__restrict__ nu, double *__restrict__ out,
-                              int accumulate) {
+                              const double *__restrict__ y, const double *__restrict__ gw,
+                              const int *__restrict__ d2e_ptr, const int *__restrict__ d2e,
+                              const double *__restrict__ in, const double *__restrict__ nu,
+                              double *__restrict__ out, int accumulate) {
 #pragma clang fp contract(off)
   const int dof = blockIdx.x * blockDim.x + threadIdx.x;
   if (dof >= ndof) return;
   double acc = accumulate ? out[dof] : 0.0;
-  for (int t = d2e_ptr[dof]; t < d2e_ptr[dof + 1]; t++) {
+  const int t1 = d2e_ptr[dof + 1];
+#pragma unroll 2
+  for (int t = d2e_ptr[dof]; t < t1; t++) {  // unrolled: the incidences' loads overlap
     const int e = d2e[t] >> 3, p = d2e[t] & 7;
     if (OP == D_DOF_TO_GAUSS_T) {
-      for (int k = 0; k < rule.nq; k++) {
+#pragma unroll
+      for (int k = 0; k < NQ; k++) {
         double h[6];
         p2_shape(rule.xi[k], rule.eta[k], h);
-        acc += in[(size_t)rule.nq * e + k] * h[p];
+        acc += in[(size_t)NQ * e + k] * h[p];
       }
-      continue;
-    }
-    const ElemGeom G = p2_geom(conn, nelem, e, x, y);
-    if (OP == D_SOURCE) {
-      for (int k = 0; k < rule.nq; k++) {
+    } else if (OP == D_SOURCE) {
+#pragma unroll
+      for (int k = 0; k < NQ; k++) {
         double h[6];
         p2_shape(rule.xi[k], rule.eta[k], h);
-        const double w = gauss_weight(rule.w[k], G.area);
-        acc += in[(size_t)rule.nq * e + k] * h[p] * w;
+        const size_t g = (size_t)NQ * e + k;
+        acc += in[g] * h[p] * gw[g];
       }
     } else if (OP == D_FEM_GRAD_T) {
-      for (int k = 0; k < rule.nq; k++) {
+      const ElemGeom G = p2_geom(conn, nelem, e, x, y);
+#pragma unroll
+      for (int k = 0; k < NQ; k++) {
         double gx[6], gy[6];
         p2_grad(rule.xi[k], rule.eta[k], G, gx, gy);
-        const size_t g = (size_t)rule.nq * e + k;
+        const size_t g = (size_t)NQ * e + k;
         acc += in[2 * g] * gx[p] + in[2 * g + 1] * gy[p];
       }
     } else {  // D_LAPLACE
+      const ElemGeom G = p2_geom(conn, nelem, e, x, y);
       double ue[6];
 #pragma unroll
       for (int q = 0; q < 6; q++) ue[q] = in[conn_dof[(size_t)q * nelem + e]];
-      for (int k = 0; k < rule.nq; k++) {
+#pragma unroll
+      for (int k = 0; k < NQ; k++) {
         double gx[6], gy[6];
         p2_grad(rule.xi[k], rule.eta[k], G, gx, gy);
         double ux = 0.0, uy = 0.0;
@@ -247,8 +283,8 @@ __global__ void p2_dof_kernel(int ndof, int nelem, QuadRule rule, const int *__r
           ux += gx[q] * ue[q];
           uy += gy[q] * ue[q];
         }
-        const double w = gauss_weight(rule.w[k], G.area);
-        acc += nu[(size_t)rule.nq * e + k] * w * (gx[p] * ux + gy[p] * uy);
+        const size_t g = (size_t)NQ * e + k;
+        acc += nu[g] * gw[g] * (gx[p] * ux + gy[p] * uy);
       }
     }
   }
@@ -259,17 +295,22 @@ inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <int OP>
 void gauss_op(const Mesh &m, const double *a, const double *b, double *out, hipStream_t st) {
-  p2_gauss_kernel<OP><<<nb(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p,
-                                                          m.conn_dof.p, m.x.p, m.y.p, a, b, out);
+  p2_gauss_kernel<OP><<<nb(m.ngauss, 256), 256, 0, st>>>(
+      m.nelem, m.rule, m.conn.p, m.conn_dof.p, m.x.p, m.y.p, m.gw.p, a, b, out);
   HIPCHECK(hipGetLastError());
 }
 
 template <int OP>
 void dof_op(const Mesh &m, const double *in, const double *nu, double *out, bool accumulate,
             hipStream_t st) {
-  p2_dof_kernel<OP><<<nb(m.ndof, 256), 256, 0, st>>>(
-      (int)m.ndof, m.nelem, m.rule, m.conn.p, m.conn_dof.p, m.x.p, m.y.p, m.d_d2e_ptr.p,
-      m.d_d2e.p, in, nu, out, accumulate ? 1 : 0);
+  auto go = [&](auto kern) {
+    kern<<<nb(m.ndof, 256), 256, 0, st>>>((int)m.ndof, m.nelem, m.rule, m.conn.p, m.conn_dof.p,
+                                          m.x.p, m.y.p, m.gw.p, m.d_d2e_ptr.p, m.d_d2e.p, in, nu,
+                                          out, accumulate ? 1 : 0);
+  };
+  if (m.rule.nq == 6) go(p2_dof_kernel<OP, 6>);
+  else if (m.rule.nq == 3) go(p2_dof_kernel<OP, 3>);
+  else go(p2_dof_kernel<OP, 1>);
   HIPCHECK(hipGetLastError());
 }
 
@@ -277,14 +318,13 @@ void dof_op(const Mesh &m, const double *in, const double *nu, double *out, bool
 
 void launch_p2_coo(const Mesh &m, bool mass, const double *coef, long long *ind, double *vv,
                    hipStream_t st) {
-  const int64_t total = 36 * m.ngauss;
+  const unsigned grid = nb(m.nelem, P2_E);
   if (mass)
-    p2_coo_kernel<true><<<nb(total, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p, m.conn_dof.p,
-                                                         m.x.p, m.y.p, coef, ind, vv);
+    p2_coo_kernel<true><<<grid, P2_NT, 0, st>>>(m.nelem, m.rule, m.conn.p, m.conn_dof.p, m.x.p,
+                                                m.y.p, m.gw.p, coef, ind, vv);
   else
-    p2_coo_kernel<false><<<nb(total, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p,
-                                                          m.conn_dof.p, m.x.p, m.y.p, coef, ind,
-                                                          vv);
+    p2_coo_kernel<false><<<grid, P2_NT, 0, st>>>(m.nelem, m.rule, m.conn.p, m.conn_dof.p, m.x.p,
+                                                 m.y.p, m.gw.p, coef, ind, vv);
   HIPCHECK(hipGetLastError());
 }
 
@@ -292,10 +332,10 @@ void launch_p2_coo_grad(const Mesh &m, bool mass, const double *gvv, double *out
                         hipStream_t st) {
   if (mass)
     p2_coo_grad_kernel<true><<<nb(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p,
-                                                                 m.x.p, m.y.p, gvv, out);
+                                                                 m.x.p, m.y.p, m.gw.p, gvv, out);
   else
     p2_coo_grad_kernel<false><<<nb(m.ngauss, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p,
-                                                                  m.x.p, m.y.p, gvv, out);
+                                                                  m.x.p, m.y.p, m.gw.p, gvv, out);
   HIPCHECK(hipGetLastError());
 }
 
