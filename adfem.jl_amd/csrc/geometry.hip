@@ -1,8 +1,10 @@
 // Element-level kernels: mesh geometry (Gauss points, weights, areas), the
 // reference-layout COO operators (FemLaplaceScalar, ComputeFemMassMatrixMfem)
 // with their gradient ops, and the load vector (FemSourceScalar) with its
-// gradient.  All atomic-free: COO ops write one triplet per thread; the
-// load vector gathers per node over its incident elements.
+// gradient.  All atomic-free: a COO op's block stages its elements' factors
+// in LDS and writes its contiguous run of triplets; the load vector gathers
+// per node over its incident elements.  Gauss weights come from the mesh's
+// array (geometry_kernel), the same bits as recomputing them per use.
 #include "geom.h"
 
 namespace admfem {

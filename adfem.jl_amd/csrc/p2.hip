@@ -15,10 +15,12 @@
 //
 // Layouts as the P1 kernels: Gauss arrays in user element order
 // (g = nq e + k), dof arrays in user dof order, COO triplet
-// t = 36 g + 6 p + q.  Maps to Gauss points run one thread per Gauss point
-// (or triplet); maps to dofs gather per dof over its incident (element,
-// local dof) pairs in ascending element order -- the reference's addition
-// order -- with one writer per output and no atomics.
+// t = 36 g + 6 p + q.  Maps to Gauss points run one thread per Gauss point;
+// the COO ops stage each block's basis gradients in LDS and write its run of
+// triplets; maps to dofs gather per dof over its incident (element, local
+// dof) pairs in ascending element order -- the reference's addition order --
+// with one writer per output and no atomics.  Gauss weights come from the
+// mesh's array (the same bits as recomputing them).
 #include "geom.h"
 
 namespace admfem {

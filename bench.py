@@ -501,6 +501,8 @@ def main():
     ap.add_argument("--ops-only", action="store_true",
                     help="time only the reference operators' kernels (P1, P2, Dirichlet, K\\F)")
     ap.add_argument("--no-ops", action="store_true", help="skip the operator-kernel leg")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the timed passes of the stream at capacity (for profiling)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="with --precision-only: time the SpMV and assembly kernels, no solves")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -550,6 +552,9 @@ def main():
         walls.append(wall)
         last = summary(n, wall, rs, st)
         log(f"step {k}: {n / wall:.1f} solves/s, failed {last['failed']}")
+    if a.serve_only:
+        print(json.dumps(last), flush=True)
+        return
     total = float(np.sum(walls))
     value = n * a.steps / total
     rate = 0.8 * value
