@@ -229,18 +229,23 @@ enum { D_SOURCE = 0, D_DOF_TO_GAUSS_T = 1, D_FEM_GRAD_T = 2, D_LAPLACE = 3 };
 //   D_SOURCE          FemSourceScalar_forward: sum_k f[g] h_p(k) w_k
 //   D_DOF_TO_GAUSS_T  DofToGaussPointsMfem_backward: sum_k in[g] h_p(k)
 //   D_FEM_GRAD_T      FemGradMfem_backward: sum_k in[2g] dh_p/dx + in[2g+1] dh_p/dy
-//   D_LAPLACE         ComputeLaplaceTermMfem_forward (= K(nu) u):
+//   D_LAPLACE         ComputeLaplaceTermMfem_forward (= K(nu) u), with u and
+//                     the thread order in the internal dof order (vertex
+//                     dofs permuted like the nodes, edge dofs as they are),
+//                     so an incidence's six values are local gathers:
 //                     sum_k nu_k w_k grad h_p . grad u_e(k)
 template <int OP, int NQ>
-__global__ void p2_dof_kernel(int ndof, int nelem, QuadRule rule, const int *__restrict__ conn,
+__global__ void p2_dof_kernel(int ndof, int nnode, const int *__restrict__ iperm, int nelem,
+                              QuadRule rule, const int *__restrict__ conn,
                               const int *__restrict__ conn_dof, const double *__restrict__ x,
                               const double *__restrict__ y, const double *__restrict__ gw,
                               const int *__restrict__ d2e_ptr, const int *__restrict__ d2e,
                               const double *__restrict__ in, const double *__restrict__ nu,
                               double *__restrict__ out, int accumulate) {
 #pragma clang fp contract(off)
-  const int dof = blockIdx.x * blockDim.x + threadIdx.x;
-  if (dof >= ndof) return;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ndof) return;
+  const int dof = (OP == D_LAPLACE && r < nnode) ? iperm[r] : r;  // user dof id
   double acc = accumulate ? out[dof] : 0.0;
   const int t1 = d2e_ptr[dof + 1];
 #pragma unroll 2
@@ -274,7 +279,8 @@ __global__ void p2_dof_kernel(int ndof, int nelem, QuadRule rule, const int *__r
       const ElemGeom G = p2_geom(conn, nelem, e, x, y);
       double ue[6];
 #pragma unroll
-      for (int q = 0; q < 6; q++) ue[q] = in[conn_dof[(size_t)q * nelem + e]];
+      for (int q = 0; q < 6; q++)
+        ue[q] = in[q < 3 ? conn[(size_t)q * nelem + e] : conn_dof[(size_t)q * nelem + e]];
 #pragma unroll
       for (int k = 0; k < NQ; k++) {
         double gx[6], gy[6];
@@ -295,6 +301,14 @@ __global__ void p2_dof_kernel(int ndof, int nelem, QuadRule rule, const int *__r
 
 inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// dst = src in the internal dof order: vertex dof r <- src[iperm[r]], edge
+// dofs unchanged
+__global__ void p2_permute_dofs(int ndof, int nnode, const int *__restrict__ iperm,
+                                const double *__restrict__ src, double *__restrict__ dst) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < ndof) dst[r] = src[r < nnode ? iperm[r] : r];
+}
+
 template <int OP>
 void gauss_op(const Mesh &m, const double *a, const double *b, double *out, hipStream_t st) {
   p2_gauss_kernel<OP><<<nb(m.ngauss, 256), 256, 0, st>>>(
@@ -306,7 +320,8 @@ template <int OP>
 void dof_op(const Mesh &m, const double *in, const double *nu, double *out, bool accumulate,
             hipStream_t st) {
   auto go = [&](auto kern) {
-    kern<<<nb(m.ndof, 256), 256, 0, st>>>((int)m.ndof, m.nelem, m.rule, m.conn.p, m.conn_dof.p,
+    kern<<<nb(m.ndof, 256), 256, 0, st>>>((int)m.ndof, m.nnode, m.d_iperm.p, m.nelem, m.rule,
+                                          m.conn.p, m.conn_dof.p,
                                           m.x.p, m.y.p, m.gw.p, m.d_d2e_ptr.p, m.d_d2e.p, in, nu,
                                           out, accumulate ? 1 : 0);
   };
@@ -368,7 +383,11 @@ void launch_p2_fem_grad_grad(const Mesh &m, const double *g, double *gu, hipStre
 
 void launch_p2_laplace_term(const Mesh &m, const double *u, const double *nu, double *out,
                             hipStream_t st) {
-  dof_op<D_LAPLACE>(m, u, nu, out, false, st);
+  double *u_int = nullptr;  // u in the internal dof order, a stream-ordered temporary
+  HIPCHECK(hipMallocAsync((void **)&u_int, sizeof(double) * m.ndof, st));
+  p2_permute_dofs<<<nb(m.ndof, 256), 256, 0, st>>>((int)m.ndof, m.nnode, m.d_iperm.p, u, u_int);
+  dof_op<D_LAPLACE>(m, u_int, nu, out, false, st);
+  HIPCHECK(hipFreeAsync(u_int, st));
 }
 
 void launch_p2_laplace_term_nu_grad(const Mesh &m, const double *go, const double *u,
