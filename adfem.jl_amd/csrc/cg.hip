@@ -149,11 +149,19 @@ __device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
   const int w = min(wfull, 8);
   int c[8];
   VT v[8];
+  // FP64 values: the matrix is streamed once per product, so its loads are
+  // non-temporal (measured on the FP64 SpMV: 99 -> 82 us at 2048^2; FP32
+  // values lose with them, 49 -> 56 us, and keep ordinary loads)
 #pragma unroll
   for (int k = 0; k < 8; k++)
     if (k < w) {
-      c[k] = sc[off + 64 * k];
-      v[k] = sv[off + 64 * k];
+      if constexpr (sizeof(VT) == 8) {
+        c[k] = __builtin_nontemporal_load(sc + off + 64 * k);
+        v[k] = __builtin_nontemporal_load(sv + off + 64 * k);
+      } else {
+        c[k] = sc[off + 64 * k];
+        v[k] = sv[off + 64 * k];
+      }
     }
   double s = 0.0;
 #pragma unroll
