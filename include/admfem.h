@@ -190,7 +190,8 @@ int admfem_fem_grad_grad(admfem_mesh m, const double *grad_out, double *grad_u, 
 /* ComputeLaplaceTermMfem (deps/MFEM/ComputeLaplaceTermMfem/
  * ComputeLaplaceTermMfem.h:4-17), the op behind compute_fem_laplace_term1
  * (src/MFEM/MCore.jl:740-763): out [ndof] = K(nu) u without forming K
- * (overwritten); u [ndof], nu [ngauss]. */
+ * (overwritten); u [ndof], nu [ngauss].  Takes a stream-ordered temporary
+ * of ndof doubles (u in the internal order). */
 int admfem_fem_laplace_term(admfem_mesh m, const double *u, const double *nu, double *out,
                             void *stream);
 /* ComputeLaplaceTermMfemGrad (h:19-39): grad_u = K(nu) grad_out (K is
@@ -207,7 +208,9 @@ int admfem_fem_laplace_term_grad(admfem_mesh m, const double *grad_out, const do
  * (explicit zeros and duplicates too), then one (b,b,1.0) per distinct
  * boundary node in ascending order; duplicated bd entries: the last one
  * wins, as with the reference's std::map assignment.  Synchronises the
- * stream (the count is data dependent). */
+ * stream (the count is data dependent).  Works in a per-device workspace
+ * grown to the largest call so far and kept until the process ends; calls
+ * on one device (this op and its gradient) are serialised by a lock. */
 int admfem_impose_dirichlet(const long long *indices, const double *vv,
                             int64_t sN, const long long *bd,
                             const double *bdval, int bdN, const double *rhs,
@@ -322,7 +325,13 @@ int admfem_system_forward_adjoint(admfem_system s, const admfem_problem *p,
  * triplets indices [nnz x 2] (0-based int64) / vv [nnz], duplicates summed
  * in triplet order; rhs, u [n].  A must be symmetric positive definite
  * (Jacobi-PCG; ADMFEM_ERR_BREAKDOWN otherwise).  Device pointers;
- * synchronises the stream. */
+ * synchronises the stream.  The last pattern built (per device) is kept: a
+ * call whose indices equal it (compared on the device) only sums the new
+ * values into it, so the gradient call after a forward solve, and an
+ * optimisation loop on a fixed mesh, skip the pattern build.  The kept
+ * pattern holds about 24 B per triplet plus the solver's arrays until the
+ * next pattern or the process end; calls of this op and its gradient are
+ * serialised by a lock. */
 int admfem_sparse_solve(const long long *indices, const double *vv, int64_t nnz,
                         int n, const double *rhs, double *u,
                         const admfem_solve_opts *o, admfem_solve_stats *st,
