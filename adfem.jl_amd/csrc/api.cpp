@@ -448,8 +448,8 @@ int admfem_system_kappa_gradient(admfem_system h, const double *u, const double 
     System &s = SY(h);
     REQUIRE(u && lam && grad_kappa, ADMFEM_ERR_ARG, "null pointer");
     hipStream_t str = S(stream);
-    launch_gather(s.mesh->d_iperm.p, u, h->u_int.p, s.n, str);
-    launch_gather(s.mesh->d_iperm.p, lam, h->lam_int.p, s.n, str);
+    launch_scatter(s.mesh->d_perm.p, u, h->u_int.p, s.n, str);  // user -> internal order
+    launch_scatter(s.mesh->d_perm.p, lam, h->lam_int.p, s.n, str);
     launch_kappa_gradient(s, h->u_int.p, h->lam_int.p, scale, grad_kappa, str);
   });
 }

@@ -543,6 +543,20 @@ void launch_loss(const System &s, const double *u_int, const int *obs_idx,
   HIPCHECK(hipGetLastError());
 }
 
+namespace {
+__global__ void scatter_kernel(const int *__restrict__ idx, const double *__restrict__ src,
+                               double *__restrict__ dst, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[idx[i]] = src[i];
+}
+}  // namespace
+
+void launch_scatter(const int *idx, const double *src, double *dst, int n, hipStream_t st) {
+  if (n <= 0) return;
+  scatter_kernel<<<nblk(n, 256), 256, 0, st>>>(idx, src, dst, n);
+  HIPCHECK(hipGetLastError());
+}
+
 void launch_gather(const int *idx, const double *src, double *dst, int n,
                    hipStream_t st) {
   if (n <= 0) return;

@@ -277,6 +277,11 @@ void launch_loss(const System &s, const double *u_int, const int *obs_idx,
                  const double *obs_val, int nobs, double *g_int, double *loss2,
                  hipStream_t st);
 // permutation helpers (internal <-> user order)
+// dst[idx[i]] = src[i] (idx a permutation): the same data movement as a
+// gather with the inverse permutation, with coalesced reads and scattered
+// writes, which do not stall the wave (measured faster for user <-> internal
+// node order)
+void launch_scatter(const int *idx, const double *src, double *dst, int n, hipStream_t st);
 void launch_gather(const int *idx, const double *src, double *dst, int n,
                    hipStream_t st);  // dst[i] = src[idx[i]]
 // dirichlet.hip

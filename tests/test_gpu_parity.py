@@ -345,6 +345,35 @@ def test_forward_adjoint_gradient_matches_oracle_chain(prec, tol):
     assert out["loss"] == pytest.approx(((ref["u"][obs] - uobs) ** 2).sum(), rel=tol)
 
 
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_kappa_gradient_entry_point_matches_oracle(shuffle):
+    """admfem_system_kappa_gradient on the oracle's own u and lambda (user
+    order) gives the oracle's gradient chain SparseSolverGrad ->
+    ImposeDirichletGrad -> FemLaplaceScalarGrad; shuffled elements take the
+    RCM node order."""
+    c, e = meshgen.jittered_square(30, seed=13)
+    if shuffle:
+        e = e[np.random.default_rng(6).permutation(e.shape[0])]
+    gm, om = admfem.Mesh(c, e, zero_based=True), O.OracleMesh(c, e)
+    rng = np.random.default_rng(14)
+    kap = np.exp(meshgen.grf(np.stack([om.gx, om.gy], 1), 5))
+    f = rng.uniform(-1, 1, om.ngauss)
+    bd = om.bcnode()
+    interior = np.setdiff1d(np.arange(om.nnode), bd - 1)
+    obs = rng.choice(interior, size=40, replace=False)
+    uobs = rng.standard_normal(obs.size) * 1e-2
+
+    def dl(u):
+        g = np.zeros_like(u)
+        g[obs] = 2 * (u[obs] - uobs)
+        return g
+
+    ref = O.forward_adjoint(om, kap, f, bd, np.zeros(bd.size), dl)
+    S = admfem.DirichletSystem(gm, bd).assemble(kappa=kap, f=f)
+    g = S.kappa_gradient(ref["u"], ref["lam"])
+    assert relerr(host(g), ref["grad_kappa"]) < 1e-12
+
+
 def fan_mesh(k=12):
     """Centre node, a ring of k nodes and an outer ring of 2k boundary nodes:
     the centre row has k interior neighbours (wider than the 8-entry fast
