@@ -409,7 +409,7 @@ static void assemble_impl(admfem_system h, const admfem_problem *p, hipStream_t 
   REQUIRE(p, ADMFEM_ERR_ARG, "null problem");
   admfem_problem q = *p;
   if (p->u_prev) {  // node vector: user -> internal order
-    launch_gather(s.mesh->d_iperm.p, p->u_prev, h->tmp_int.p, s.n, st);
+    launch_scatter(s.mesh->d_perm.p, p->u_prev, h->tmp_int.p, s.n, st);
     q.u_prev = h->tmp_int.p;
   }
   launch_assemble(s, q, st);
@@ -427,12 +427,12 @@ int admfem_system_solve(admfem_system h, const double *b, double *x,
     hipStream_t str = S(stream);
     const double *b_int = s.rhs.p;
     if (b) {
-      launch_gather(s.mesh->d_iperm.p, b, h->g_int.p, s.n, str);
+      launch_scatter(s.mesh->d_perm.p, b, h->g_int.p, s.n, str);
       b_int = h->g_int.p;
     }
     admfem_solve_stats tmp{};
     cg_solve(s, b_int, h->u_int.p, *o, tmp, str);
-    launch_gather(s.mesh->d_perm.p, h->u_int.p, x, s.n, str);
+    launch_scatter(s.mesh->d_iperm.p, h->u_int.p, x, s.n, str);
     HIPCHECK(hipStreamSynchronize(str));
     if (st) *st = tmp;
     if (tmp.status != ADMFEM_OK) {
@@ -471,8 +471,8 @@ int admfem_system_forward_adjoint(admfem_system h, const admfem_problem *p,
     cg_solve(s, h->g_int.p, h->lam_int.p, *o, s1, str);
     if (grad_kappa)
       launch_kappa_gradient(s, h->u_int.p, h->lam_int.p, p->lap_coef, grad_kappa, str);
-    if (u) launch_gather(s.mesh->d_perm.p, h->u_int.p, u, s.n, str);
-    if (lam) launch_gather(s.mesh->d_perm.p, h->lam_int.p, lam, s.n, str);
+    if (u) launch_scatter(s.mesh->d_iperm.p, h->u_int.p, u, s.n, str);
+    if (lam) launch_scatter(s.mesh->d_iperm.p, h->lam_int.p, lam, s.n, str);
     HIPCHECK(hipStreamSynchronize(str));
     if (loss) *loss = L;
     if (st) {

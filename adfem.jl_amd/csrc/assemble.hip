@@ -424,13 +424,6 @@ __global__ void kappa_grad_kernel(int nelem, QuadRule rule,
     gk[(size_t)rule.nq * e + k] = -scale * gauss_weight(rule.w[k], G.area) * d;
 }
 
-__global__ void gather_kernel(const int *__restrict__ idx,
-                              const double *__restrict__ src,
-                              double *__restrict__ dst, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[idx[i]];
-}
-
 // Loss L = sum_k (u[perm[obs_idx[k]]] - obs_val[k])^2 and its gradient
 // g = dL/du (internal order; g zeroed beforehand).  One block, fixed order.
 __global__ void __launch_bounds__(1024)
@@ -554,13 +547,6 @@ __global__ void scatter_kernel(const int *__restrict__ idx, const double *__rest
 void launch_scatter(const int *idx, const double *src, double *dst, int n, hipStream_t st) {
   if (n <= 0) return;
   scatter_kernel<<<nblk(n, 256), 256, 0, st>>>(idx, src, dst, n);
-  HIPCHECK(hipGetLastError());
-}
-
-void launch_gather(const int *idx, const double *src, double *dst, int n,
-                   hipStream_t st) {
-  if (n <= 0) return;
-  gather_kernel<<<nblk(n, 256), 256, 0, st>>>(idx, src, dst, n);
   HIPCHECK(hipGetLastError());
 }
 

@@ -282,8 +282,7 @@ void launch_loss(const System &s, const double *u_int, const int *obs_idx,
 // writes, which do not stall the wave (measured faster for user <-> internal
 // node order)
 void launch_scatter(const int *idx, const double *src, double *dst, int n, hipStream_t st);
-void launch_gather(const int *idx, const double *src, double *dst, int n,
-                   hipStream_t st);  // dst[i] = src[idx[i]]
+ // dst[i] = src[idx[i]]
 // dirichlet.hip
 int64_t run_impose_dirichlet(const long long *ind, const double *vv,
                              int64_t sN, const long long *bd,

@@ -265,7 +265,7 @@ struct Server {
     HIPCHECK(hipMemcpyAsync(q.f, r.f, sizeof(double) * m.ngauss, kind, st));
     if (heat) {
       HIPCHECK(hipMemcpyAsync(q.stage, r.u_prev, sizeof(double) * s.n, kind, st));
-      launch_gather(m.d_iperm.p, q.stage, q.uprev, s.n, st);
+      launch_scatter(m.d_perm.p, q.stage, q.uprev, s.n, st);
     }
     if (r.nobs) {
       HIPCHECK(hipMemcpyAsync(q.obsi, r.obs_idx, sizeof(int) * r.nobs, kind, st));
@@ -352,7 +352,7 @@ struct Server {
     admfem_problem p = problem(q);
     launch_kappa_gradient_to(s, q.xy, q.u64, q.lam64, p.lap_coef, q.grad, st);
     if (q.r.u_out) {
-      launch_gather(m.d_perm.p, q.u64, q.stage, s.n, st);  // user order
+      launch_scatter(m.d_iperm.p, q.u64, q.stage, s.n, st);  // user order
       HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
                               st));
     }
@@ -602,7 +602,7 @@ struct Server {
       const Mesh &m = *s.mesh;
       launch_kappa_gradient_to(s, q.xy, q.u64, q.lam64, p.lap_coef, q.grad, st);
       if (q.r.u_out) {
-        launch_gather(m.d_perm.p, q.u64, q.stage, s.n, st);
+        launch_scatter(m.d_iperm.p, q.u64, q.stage, s.n, st);
         HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
                                 st));
       }
