@@ -511,8 +511,8 @@ def main():
                          "counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {a.gpus}): this build is "
-              "single-GPU; run with --gpus 1", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}): the serving engine "
+              "and its kernels run on one GPU; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "no HIP device"
