@@ -324,14 +324,14 @@ def precision_form(m, rtol, kernels_only=False):
                                      "FP64) and rhs written: the kernel computes K and F",
                           survey_bytes=int(b_asm),
                           survey_frac=round(b_asm / avg / 1e9 / HBM_PEAK_GBS, 4))
-    # adjoint gradient dL/dkappa at every Gauss point (the API call: two
-    # user->internal gathers of u and lam, then kappa_grad_kernel)
+    # adjoint gradient dL/dkappa at every Gauss point (the API call: u and
+    # lam permuted to the internal node order, then kappa_grad_kernel)
     ug = torch.rand(N, dtype=torch.float64, device="cuda")
     lg = torch.rand(N, dtype=torch.float64, device="cuda")
     S.kappa_gradient(ug, lg)
     avg = hip_time(lambda: [S.kappa_gradient(ug, lg) for _ in range(20)], 20)
     b_grad = 12 * ne + 16 * N + 2 * N * 8 + ng * 8
-    rl["kappa_gradient"] = roof(b_grad, avg, "kappa_grad_kernel (+ 2 gathers of u, lam)",
+    rl["kappa_gradient"] = roof(b_grad, avg, "kappa_grad_kernel (+ permuting u, lam to the internal order)",
                                 bytes_note="connectivity, coordinates, u and lam read once, "
                                            "dL/dkappa written")
     out = dict(workload=f"{m}x{m}-quad jittered node-permuted mesh ({ne} triangles, {N} "
