@@ -331,7 +331,8 @@ def precision_form(m, rtol, kernels_only=False):
     S.kappa_gradient(ug, lg)
     avg = hip_time(lambda: [S.kappa_gradient(ug, lg) for _ in range(20)], 20)
     b_grad = 12 * ne + 16 * N + 2 * N * 8 + ng * 8
-    rl["kappa_gradient"] = roof(b_grad, avg, "kappa_grad_kernel (+ permuting u, lam to the internal order)",
+    rl["kappa_gradient"] = roof(b_grad, avg,
+                                "kappa_grad_kernel (+ permuting u, lam to the internal order)",
                                 bytes_note="connectivity, coordinates, u and lam read once, "
                                            "dL/dkappa written")
     out = dict(workload=f"{m}x{m}-quad jittered node-permuted mesh ({ne} triangles, {N} "
