@@ -74,7 +74,7 @@ __global__ void fem_grad_kernel(int nelem, QuadRule rule, const int *__restrict_
 enum { NODE_TO_GAUSS_T = 0, NODE_GRAD_T = 1, NODE_LAPLACE = 2 };
 
 template <int OP, int NQ>
-__global__ void node_gather_kernel(int n, int nelem, QuadRule rule, const int *__restrict__ conn,
+__global__ void __launch_bounds__(256) node_gather_kernel(int n, int nelem, QuadRule rule, const int *__restrict__ conn,
                                    const int *__restrict__ conn_user,
                                    const double *__restrict__ x, const double *__restrict__ y,
                                    const int *__restrict__ n2e_ptr,
@@ -89,8 +89,9 @@ __global__ void node_gather_kernel(int n, int nelem, QuadRule rule, const int *_
   if (r >= n) return;
   double acc = 0.0;
   const int t1 = n2e_ptr[r + 1];
+  // unrolled over incidences so that their loads overlap
 #pragma unroll 4
-  for (int t = n2e_ptr[r]; t < t1; t++) {  // unrolled: the incidences' loads overlap
+  for (int t = n2e_ptr[r]; t < t1; t++) {
     const int e = n2e_elem[t];
     const int p = (int)(slots[t] & 3u);
     if (OP == NODE_TO_GAUSS_T) {

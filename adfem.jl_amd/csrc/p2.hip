@@ -235,7 +235,7 @@ enum { D_SOURCE = 0, D_DOF_TO_GAUSS_T = 1, D_FEM_GRAD_T = 2, D_LAPLACE = 3 };
 //                     so an incidence's six values are local gathers:
 //                     sum_k nu_k w_k grad h_p . grad u_e(k)
 template <int OP, int NQ>
-__global__ void p2_dof_kernel(int ndof, int nnode, const int *__restrict__ iperm, int nelem,
+__global__ void __launch_bounds__(256) p2_dof_kernel(int ndof, int nnode, const int *__restrict__ iperm, int nelem,
                               QuadRule rule, const int *__restrict__ conn,
                               const int *__restrict__ conn_dof, const double *__restrict__ x,
                               const double *__restrict__ y, const double *__restrict__ gw,
@@ -243,13 +243,16 @@ __global__ void p2_dof_kernel(int ndof, int nnode, const int *__restrict__ iperm
                               const double *__restrict__ in, const double *__restrict__ nu,
                               double *__restrict__ out, int accumulate) {
 #pragma clang fp contract(off)
+  constexpr bool CHEAP = OP == D_DOF_TO_GAUSS_T || OP == D_SOURCE;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= ndof) return;
   const int dof = (OP == D_LAPLACE && r < nnode) ? iperm[r] : r;  // user dof id
   double acc = accumulate ? out[dof] : 0.0;
   const int t1 = d2e_ptr[dof + 1];
-#pragma unroll 2
-  for (int t = d2e_ptr[dof]; t < t1; t++) {  // unrolled: the incidences' loads overlap
+  // the cheap ops are unrolled over incidences so that their loads overlap;
+  // the gradient ops keep one incidence in flight (registers, no spills)
+#pragma unroll(CHEAP ? 2 : 1)
+  for (int t = d2e_ptr[dof]; t < t1; t++) {
     const int e = d2e[t] >> 3, p = d2e[t] & 7;
     if (OP == D_DOF_TO_GAUSS_T) {
 #pragma unroll
