@@ -69,7 +69,18 @@ struct CGWork {
   DevBuf<CGScalars> sc;
   CGScalars *h_sc = nullptr;  // pinned
   std::vector<double> h_part;
+  // GRAPH_ITERS iterations (from an iteration index divisible by
+  // GRAPH_ITERS) captured once as a graph of kernel nodes and replayed: the
+  // launches of a small system cost more than its kernels.  Captured on a
+  // private stream (the caller's may be the null stream, which cannot be
+  // captured), replayed on the caller's; `gkey` holds every pointer and
+  // setting baked into the nodes.
+  hipStream_t cap = nullptr;
+  hipGraphExec_t graph = nullptr;
+  std::vector<const void *> gkey;
   ~CGWork() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
     if (h_sc) (void)hipHostFree(h_sc);
   }
 };
@@ -523,6 +534,75 @@ void spmv_repeat(System &s, int precision, int repeat, hipStream_t st) {
   HIPCHECK(hipGetLastError());
 }
 
+namespace {
+
+constexpr int GRAPH_ITERS = 8;  // a multiple of the replacement interval (8)
+
+// One iteration with index `it` (its parity picks the direction buffers;
+// the mixed form checks for a replacement after every eighth iteration and
+// after the last one allowed).
+void enqueue_iteration(System &s, CGWork *w, bool mixed, const double *b, int max_iter, int it,
+                       hipStream_t st) {
+  const int nb = w->nblocks, np = s.npad;
+  const int *sp = s.d_slice_ptr.p, *scol = s.d_sell_col.p;
+  CGScalars *S = w->sc.p;
+  double *part = w->part.p;
+  const bool odd = it & 1;
+  if (mixed) {
+    float *po = odd ? w->pb32.p : w->pa32.p, *pn = odd ? w->pa32.p : w->pb32.p;
+    k_mspmv<<<nb, NT, 0, st>>>(np, sp, scol, w->sv32s.p, w->r32.p, po, pn, w->q32.p, part, S);
+    s_alpha<<<1, RT, 0, st>>>(nb, part, S);
+    k_mupdate<<<nb, NT, 0, st>>>(np, pn, w->q32.p, w->xs32.p, w->r32.p, part, S);
+    s_beta<<<1, RT, 0, st>>>(nb, part, S, max_iter);
+    if ((it & 7) == 7 || it + 1 == max_iter) {  // replacement check every 8
+      k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, w->dsq.p, S, 0);
+      k_mreplace<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b, w->x.p,
+                                    w->dsq.p, w->r32.p, part, S);
+      s_replace<<<1, RT, 0, st>>>(nb, part, S, max_iter);
+    }
+  } else {
+    double *po = odd ? w->pb64.p : w->pa64.p, *pn = odd ? w->pa64.p : w->pb64.p;
+    k_spmv<double><<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val.p, s.d_diag.p, w->z64.p, po,
+                                      pn, w->q64.p, part, S);
+    s_alpha<<<1, RT, 0, st>>>(nb, part, S);
+    k_update<double, double><<<nb, NT, 0, st>>>(np, pn, w->q64.p, w->x.p, w->r64.p, w->z64.p,
+                                                s.d_dinv.p, part, S);
+    s_beta<<<1, RT, 0, st>>>(nb, part, S, max_iter);
+  }
+}
+
+// Iterations it .. it + GRAPH_ITERS - 1 for an `it` divisible by
+// GRAPH_ITERS with it + GRAPH_ITERS <= max_iter: the nodes are those of
+// iterations 0 .. GRAPH_ITERS - 1 (same parities, and the last one allowed,
+// if among them, is a replacement point anyway).
+void launch_graph(System &s, CGWork *w, bool mixed, const double *b, int max_iter,
+                  hipStream_t st) {
+  const std::vector<const void *> key = {
+      (const void *)(intptr_t)mixed, (const void *)(intptr_t)max_iter,
+      mixed ? (const void *)b : nullptr, s.d_slice_ptr.p, s.d_sell_col.p, s.d_sell_val.p,
+      s.d_diag.p, s.d_dinv.p, (const void *)(intptr_t)s.n};
+  if (!w->graph || w->gkey != key) {
+    if (w->graph) {
+      HIPCHECK(hipGraphExecDestroy(w->graph));
+      w->graph = nullptr;
+    }
+    if (!w->cap) HIPCHECK(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
+    hipGraph_t g;
+    HIPCHECK(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < GRAPH_ITERS; k++) enqueue_iteration(s, w, mixed, b, max_iter, k, w->cap);
+    HIPCHECK(hipStreamEndCapture(w->cap, &g));
+    hipGraphExec_t ex;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHECK(e);
+    w->graph = ex;
+    w->gkey = key;
+  }
+  HIPCHECK(hipGraphLaunch(w->graph, st));
+}
+
+}  // namespace
+
 void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts &o,
               admfem_solve_stats &stats, hipStream_t st) {
   auto t0 = std::chrono::steady_clock::now();
@@ -550,35 +630,19 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
   }
   s_init<<<1, RT, 0, st>>>(nb, part, S);
   HIPCHECK(hipGetLastError());
-  int chunk = o.check_every > 0 ? o.check_every : 16;
+  int chunk = o.check_every > 0 ? o.check_every : 2 * GRAPH_ITERS;
   int it = 0;  // iterations enqueued
   double rr_prev = -1;
   int it_prev = 0;
   for (;;) {
     const int stop = std::min(it + chunk, o.max_iter);
-    for (; it < stop; it++) {
-      const bool odd = it & 1;
-      if (mixed) {
-        float *po = odd ? w->pb32.p : w->pa32.p, *pn = odd ? w->pa32.p : w->pb32.p;
-        k_mspmv<<<nb, NT, 0, st>>>(np, sp, scol, w->sv32s.p, w->r32.p, po, pn, w->q32.p, part,
-                                   S);
-        s_alpha<<<1, RT, 0, st>>>(nb, part, S);
-        k_mupdate<<<nb, NT, 0, st>>>(np, pn, w->q32.p, w->xs32.p, w->r32.p, part, S);
-        s_beta<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
-        if ((it & 7) == 7 || it + 1 == o.max_iter) {  // replacement check every 8
-          k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, w->dsq.p, S, 0);
-          k_mreplace<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b,
-                                        w->x.p, w->dsq.p, w->r32.p, part, S);
-          s_replace<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
-        }
+    while (it < stop) {
+      if (it % GRAPH_ITERS == 0 && it + GRAPH_ITERS <= stop) {
+        launch_graph(s, w, mixed, b, o.max_iter, st);
+        it += GRAPH_ITERS;
       } else {
-        double *po = odd ? w->pb64.p : w->pa64.p, *pn = odd ? w->pa64.p : w->pb64.p;
-        k_spmv<double><<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val.p, s.d_diag.p, w->z64.p,
-                                          po, pn, w->q64.p, part, S);
-        s_alpha<<<1, RT, 0, st>>>(nb, part, S);
-        k_update<double, double><<<nb, NT, 0, st>>>(np, pn, w->q64.p, w->x.p, w->r64.p,
-                                                    w->z64.p, s.d_dinv.p, part, S);
-        s_beta<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
+        enqueue_iteration(s, w, mixed, b, o.max_iter, it, st);
+        it++;
       }
     }
     HIPCHECK(hipGetLastError());
@@ -599,7 +663,9 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
     }
     rr_prev = mon;
     it_prev = h.it;
-    chunk = o.check_every > 0 ? o.check_every : (next + 1) / 2 * 2;
+    chunk = o.check_every > 0 ? o.check_every
+                              : std::max(GRAPH_ITERS, (next + GRAPH_ITERS / 2) / GRAPH_ITERS *
+                                                          GRAPH_ITERS);
   }
   const bool converged = w->h_sc->status == ST_CONV;
   if (mixed && !converged)  // stopped without a final replacement
