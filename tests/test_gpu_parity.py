@@ -6,6 +6,8 @@ relative (the same formulas, FMA contraction aside); FP64 solves and
 gradients <= 1e-10 relative to the oracle's direct solve; the mixed-precision
 path <= 1e-8.
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -318,6 +320,38 @@ def test_solve_matches_direct(m, prec, tol):
     print(st, flush=True)
     assert st["status"] == 0
     assert relerr(host(u), u0) < tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_graph_replayed_iterations_match_one_by_one(prec):
+    """The solver replays groups of 8 iterations from a captured HIP graph;
+    with check_every=1 every iteration is launched on its own.  Same kernels
+    in the same order, so the solutions are bitwise equal: converged, again
+    (the cached graph replayed), with another right-hand side (mixed: a new
+    graph, it holds b), and stopped at a max_iter that is no multiple of 8."""
+    print(f"graph vs one-by-one {prec}", flush=True)
+    gm, om = pair(40, seed=8)
+    rng = np.random.default_rng(7)
+    kap = np.exp(meshgen.grf(np.stack([om.gx, om.gy], 1), 2))
+    S = admfem.DirichletSystem(gm, om.bcnode()).assemble(kappa=kap, f=rng.uniform(-1, 1,
+                                                                                  om.ngauss))
+    b2 = dev(rng.uniform(-1, 1, gm.nnode))
+
+    def run(b, max_iter, every):  # through the C-ABI: a stopped solve returns its code
+        x = torch.empty(gm.nnode, dtype=torch.float64, device="cuda")
+        o = admfem.solve_opts(prec, 1e-10, max_iter, check_every=every)
+        st = _lib.SolveStats()
+        rc = _lib.lib().admfem_system_solve(S._h, _lib.ptr(b), _lib.ptr(x), ctypes.byref(o),
+                                            ctypes.byref(st), _lib.stream_ptr())
+        return rc, host(x), st.as_dict()
+
+    for b, max_iter in ((None, 100000), (None, 100000), (b2, 100000), (b2, 21)):
+        rg, ug, sg = run(b, max_iter, 0)
+        re_, ue, se = run(b, max_iter, 1)
+        print(max_iter, rg, sg, se, flush=True)
+        assert rg == re_ == (_lib.OK if max_iter > 21 else _lib.ERR_NOT_CONVERGED)
+        assert sg["iterations"] == se["iterations"]
+        assert np.array_equal(ug, ue)
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("mixed", 1e-8)])
