@@ -68,8 +68,7 @@ size_t Mesh::device_bytes() const {
 
 // Reverse Cuthill-McKee order of the node graph (adjacency = pattern rows
 // without the diagonal).  Returns order[k] = node placed at position k.
-static std::vector<int> rcm_order(int n, const std::vector<int> &rp,
-                                  const std::vector<int> &col) {
+std::vector<int> rcm_order(int n, const std::vector<int> &rp, const std::vector<int> &col) {
   std::vector<int> deg(n);
   for (int i = 0; i < n; i++) deg[i] = rp[i + 1] - rp[i];
   std::vector<int> order;

@@ -14,16 +14,27 @@
 // layout and CG workspace and only sums the values again, as an optimisation
 // loop or the gradient after its forward solve does.  Calls are serialised
 // by a lock (the kept pattern and its workspace are shared).
+//
+// The solver works in its own row order: reverse Cuthill-McKee of the
+// pattern when that narrows its bandwidth (the caller's numbering may be
+// anything -- a randomly numbered mesh makes every SpMV gather at random),
+// else the caller's.  The right-hand side is scattered into that order and
+// the solution back; the matrix values are the same sums, stored at their
+// rows' new places.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
+#include <numeric>
+#include <utility>
 
 #include "internal.h"
 
 namespace admfem {
 
 System *system_from_csr(int n, std::vector<int> rp, std::vector<int> col);
+std::vector<int> rcm_order(int n, const std::vector<int> &rp, const std::vector<int> &col);
 
 namespace {
 
@@ -72,14 +83,16 @@ __global__ void coo_differs(int64_t m, const long long *__restrict__ a,
 
 __global__ void set_int(int *__restrict__ a, int v) { *a = v; }
 
-// val[slot] = sum of vv over the slot's triplets, in triplet order
+// val[pos[slot]] (pos null: val[slot]) = sum of vv over the slot's
+// triplets, in triplet order
 __global__ void coo_sum(int nslot, const int *__restrict__ start, const int *__restrict__ trip,
-                        const double *__restrict__ vv, double *__restrict__ val) {
+                        const int *__restrict__ pos, const double *__restrict__ vv,
+                        double *__restrict__ val) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslot) return;
   double a = 0.0;
   for (int t = start[s]; t < start[s + 1]; t++) a += vv[trip[t]];
-  val[s] = a;
+  val[pos ? pos[s] : s] = a;
 }
 
 __global__ void coo_grad(int64_t nnz, const long long *__restrict__ ind,
@@ -96,8 +109,54 @@ struct CooPattern {
   DevBuf<long long> ind;  // the indices it was built from
   DevBuf<int> start;      // nslot + 1 sorted positions
   DevBuf<int> trip;       // triplet index at each sorted position
+  // solver row order (empty when it is the caller's): order[k] = caller's
+  // row at position k, inv its inverse, pos[slot] = the slot's place in the
+  // reordered CSR; b, x: right-hand side and solution in that order
+  bool reordered = false;
+  DevBuf<int> order, inv, pos;
+  DevBuf<double> b, x;
   System *s = nullptr;
 };
+
+int64_t bandwidth(const std::vector<int> &rp, const std::vector<int> &col,
+                  const std::vector<int> &inv) {
+  int64_t bw = 0;
+  for (int i = 0; i + 1 < (int)rp.size(); i++)
+    for (int k = rp[i]; k < rp[i + 1]; k++)
+      bw = std::max<int64_t>(bw, std::abs((int64_t)inv[i] - inv[col[k]]));
+  return bw;
+}
+
+// RCM order of the CSR pattern (rp, col) if it has a smaller bandwidth than
+// the given order; then (rp, col) become the reordered pattern and the
+// order, its inverse and each slot's new place are uploaded to p.
+void reorder(CooPattern &p, int n, std::vector<int> &rp, std::vector<int> &col) {
+  std::vector<int> ord = rcm_order(n, rp, col), inv(n), id(n);
+  for (int k = 0; k < n; k++) inv[ord[k]] = k;
+  std::iota(id.begin(), id.end(), 0);
+  if (bandwidth(rp, col, inv) >= bandwidth(rp, col, id)) return;
+  std::vector<int> nrp(n + 1, 0), ncol(col.size()), pos(col.size());
+  std::vector<std::pair<int, int>> row;
+  for (int k = 0; k < n; k++) {
+    const int r = ord[k];
+    row.clear();
+    for (int j = rp[r]; j < rp[r + 1]; j++) row.emplace_back(inv[col[j]], j);
+    std::sort(row.begin(), row.end());
+    nrp[k + 1] = nrp[k] + (int)row.size();
+    for (size_t t = 0; t < row.size(); t++) {
+      ncol[nrp[k] + t] = row[t].first;
+      pos[row[t].second] = nrp[k] + (int)t;
+    }
+  }
+  rp.swap(nrp);
+  col.swap(ncol);
+  p.order.upload(ord);
+  p.inv.upload(inv);
+  p.pos.upload(pos);
+  p.b.alloc(n);
+  p.x.alloc(n);
+  p.reordered = true;
+}
 
 std::mutex g_lock;
 CooPattern *g_kept = nullptr;  // kept until the process ends (never freed at exit)
@@ -175,6 +234,7 @@ void build_pattern(CooPattern &p, const long long *ind, int64_t nnz, int n, int 
     HIPCHECK(hipMemcpyAsync(p.start.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
   }
   p.nslot = nslot;
+  if (nslot) reorder(p, n, rp, col);
   p.s = system_from_csr(n, std::move(rp), std::move(col));
   p.ind.alloc(std::max<int64_t>(2 * nnz, 1));
   if (nnz)
@@ -208,10 +268,21 @@ System &pattern_with_values(const long long *ind, const double *vv, int64_t nnz,
   }
   CooPattern &p = *g_kept;
   if (p.nslot)
-    coo_sum<<<nb(p.nslot, 256), 256, 0, st>>>(p.nslot, p.start.p, p.trip.p, vv, p.s->val.p);
+    coo_sum<<<nb(p.nslot, 256), 256, 0, st>>>(p.nslot, p.start.p, p.trip.p,
+                                               p.reordered ? p.pos.p : nullptr, vv, p.s->val.p);
   HIPCHECK(hipGetLastError());
   launch_to_sell(*p.s, st);
   return *p.s;
+}
+
+// x = A^-1 b for the kept pattern p, b and x in the caller's order
+void solve_kept(CooPattern &p, const double *b, double *x, const admfem_solve_opts &o,
+                admfem_solve_stats &stats, hipStream_t st) {
+  if (!p.reordered) return cg_solve(*p.s, b, x, o, stats, st);
+  launch_scatter(p.inv.p, b, p.b.p, p.n, st);  // b_int[inv[i]] = b[i]
+  cg_solve(*p.s, p.b.p, p.x.p, o, stats, st);
+  launch_scatter(p.order.p, p.x.p, x, p.n, st);  // x[order[k]] = x_int[k]
+  HIPCHECK(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -220,8 +291,8 @@ void sparse_solve(const long long *ind, const double *vv, int64_t nnz, int n, co
                   double *u, const admfem_solve_opts &o, admfem_solve_stats &stats,
                   hipStream_t st) {
   std::lock_guard<std::mutex> g(g_lock);
-  System &s = pattern_with_values(ind, vv, nnz, n, st);
-  cg_solve(s, rhs, u, o, stats, st);
+  pattern_with_values(ind, vv, nnz, n, st);
+  solve_kept(*g_kept, rhs, u, o, stats, st);
 }
 
 void sparse_solve_grad(const long long *ind, const double *vv, int64_t nnz, int n,
@@ -229,8 +300,8 @@ void sparse_solve_grad(const long long *ind, const double *vv, int64_t nnz, int 
                        double *grad_rhs, const admfem_solve_opts &o, admfem_solve_stats &stats,
                        hipStream_t st) {
   std::lock_guard<std::mutex> g(g_lock);
-  System &s = pattern_with_values(ind, vv, nnz, n, st);
-  cg_solve(s, grad_u, grad_rhs, o, stats, st);  // lambda = A^-1 dL/du = dL/drhs
+  pattern_with_values(ind, vv, nnz, n, st);
+  solve_kept(*g_kept, grad_u, grad_rhs, o, stats, st);  // lambda = A^-1 dL/du = dL/drhs
   if (nnz) coo_grad<<<nb(nnz, 256), 256, 0, st>>>(nnz, ind, grad_rhs, u, grad_vv);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));

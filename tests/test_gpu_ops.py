@@ -135,9 +135,8 @@ def test_sparse_solve_op_and_gradient_general_spd():
 
 
 def test_sparse_solve_far_coupling():
-    """A chain with one coupling 69,999 rows away (a wide SELL gather, in the
-    caller's numbering since a general sparse system keeps it) against
-    SciPy."""
+    """A chain closed into a cycle by one coupling 69,999 rows away (the solver
+    renumbers it by reverse Cuthill-McKee, bandwidth 2) against SciPy."""
     n = 70000
     main = np.full(n, 4.0)
     off = np.full(n - 1, -1.0)
@@ -151,6 +150,36 @@ def test_sparse_solve_far_coupling():
                      torch.tensor(b, device="cuda"), rtol=1e-13)
     ref = spl.spsolve(sp.csr_matrix((V, (I, J)), shape=(n, n)), b)
     assert np.abs(u.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-10
+
+
+@pytest.mark.parametrize("permute", [False, True])
+def test_sparse_solve_grid_laplacian_any_numbering(permute):
+    """The shifted 5-point Laplacian of a 150 x 150 grid, row-major (banded:
+    solved in the caller's order) or with its unknowns numbered at random
+    (solved in reverse Cuthill-McKee order, the solution scattered back),
+    against SciPy, with the gradient of the op."""
+    m = 150
+    n = m * m
+    T = sp.diags([-np.ones(m - 1), 2.0 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+    A = (sp.kron(sp.eye(m), T) + sp.kron(T, sp.eye(m)) + 0.05 * sp.eye(n)).tocoo()
+    I, J, V = A.row.astype(np.int64), A.col.astype(np.int64), A.data
+    rng = np.random.default_rng(11)
+    if permute:
+        p = rng.permutation(n)
+        I, J = p[I], p[J]
+    b = rng.standard_normal(n)
+    w = rng.standard_normal(n)
+    ind = torch.as_tensor(np.stack([I, J], 1), device="cuda")
+    vv = torch.tensor(V, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, device="cuda", requires_grad=True)
+    u = admfem.solve(admfem.SparseTensor(ind, vv, n), bt, rtol=1e-13)
+    (u * torch.as_tensor(w, device="cuda")).sum().backward()
+    As = sp.csc_matrix((V, (I, J)), shape=(n, n))
+    u0 = spl.spsolve(As, b)
+    lam = spl.spsolve(As.T.tocsc(), w)
+    assert rel(u.detach().cpu().numpy(), u0) < 1e-10
+    assert rel(bt.grad.cpu().numpy(), lam) < 1e-10
+    assert rel(vv.grad.cpu().numpy(), -lam[I] * u0[J]) < 1e-10
 
 
 def test_sparse_solve_pattern_kept_between_calls():
