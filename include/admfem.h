@@ -331,7 +331,9 @@ int admfem_system_forward_adjoint(admfem_system s, const admfem_problem *p,
  * optimisation loop on a fixed mesh, skip the pattern build.  The kept
  * pattern holds about 24 B per triplet plus the solver's arrays until the
  * next pattern or the process end; calls of this op and its gradient are
- * serialised by a lock. */
+ * serialised by a lock.  The solve runs in reverse Cuthill-McKee order of
+ * the pattern when that narrows its bandwidth (rhs and u stay in the
+ * caller's numbering). */
 int admfem_sparse_solve(const long long *indices, const double *vv, int64_t nnz,
                         int n, const double *rhs, double *u,
                         const admfem_solve_opts *o, admfem_solve_stats *st,
