@@ -66,7 +66,23 @@ __device__ __forceinline__ void put_partials2(double a, double b, double *part, 
 __device__ __forceinline__ void sum2(const double *part, int G, double &o0, double &o1) {
   __shared__ double sh[2][NT / 64];
   double a = 0.0, b = 0.0;
-  for (int k = threadIdx.x; k < G; k += NT) {
+  // part[k], part[k + NT], ... added in that order, their loads issued 4 at
+  // a time so that they are in flight together
+  int k = threadIdx.x;
+  for (; k + 3 * NT < G; k += 4 * NT) {
+    double va[4], vb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      va[u] = part[k + u * NT];
+      vb[u] = part[G + k + u * NT];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      a += va[u];
+      b += vb[u];
+    }
+  }
+  for (; k < G; k += NT) {
     a += part[k];
     b += part[G + k];
   }

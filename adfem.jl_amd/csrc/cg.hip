@@ -126,7 +126,22 @@ __device__ __forceinline__ void sum_partials(const double *part, int G, double (
   double a[NV];
 #pragma unroll
   for (int j = 0; j < NV; j++) a[j] = 0.0;
-  for (int k = threadIdx.x; k < G; k += RT) {
+  // each thread adds part[k], part[k + RT], ... in that order; the loads
+  // are issued 8 at a time before their adds, so they are in flight
+  // together (a plain loop waits for every load before the next)
+  int k = threadIdx.x;
+  for (; k + 7 * RT < G; k += 8 * RT) {
+    double v[NV][8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+#pragma unroll
+      for (int j = 0; j < NV; j++) v[j][u] = part[(size_t)j * G + k + u * RT];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+#pragma unroll
+      for (int j = 0; j < NV; j++) a[j] += v[j][u];
+  }
+  for (; k < G; k += RT) {
 #pragma unroll
     for (int j = 0; j < NV; j++) a[j] += part[(size_t)j * G + k];
   }
