@@ -293,7 +293,12 @@ typedef struct {
 } admfem_solve_stats;
 
 /* Solves A x = b with the system's assembled A.  b, x: device [n]; b NULL =
- * the assembled rhs.  x is the output (its input value is ignored). */
+ * the assembled rhs.  x is the output (its input value is ignored).  The
+ * solver replays its iterations from a HIP graph (8 iterations, kernel nodes
+ * only), captured on the first solve of a setting on a stream of its own
+ * and replayed on the caller's; the system holds both until it is
+ * destroyed.  check_every = 1 launches every iteration by itself (same
+ * kernels and order, the same result bit for bit). */
 int admfem_system_solve(admfem_system s, const double *b, double *x,
                         const admfem_solve_opts *o, admfem_solve_stats *st,
                         void *stream);
