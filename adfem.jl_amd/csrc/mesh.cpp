@@ -180,8 +180,7 @@ static int64_t order_bandwidth(const std::vector<int> &order, const std::vector<
 // component only): the yardstick for a banded order, whose bandwidth is a
 // small multiple of it (reverse Cuthill-McKee's is at most about twice the
 // widest level of its own level structure).
-static int64_t bfs_max_level(int n, const std::vector<int> &rp, const std::vector<int> &col,
-                             int s) {
+int64_t bfs_max_level(int n, const std::vector<int> &rp, const std::vector<int> &col, int s) {
   std::vector<int> level(n, -1), q;
   q.reserve(n);
   q.push_back(s);

@@ -35,6 +35,7 @@ namespace admfem {
 
 System *system_from_csr(int n, std::vector<int> rp, std::vector<int> col);
 std::vector<int> rcm_order(int n, const std::vector<int> &rp, const std::vector<int> &col);
+int64_t bfs_max_level(int n, const std::vector<int> &rp, const std::vector<int> &col, int s);
 
 namespace {
 
@@ -127,14 +128,19 @@ int64_t bandwidth(const std::vector<int> &rp, const std::vector<int> &col,
   return bw;
 }
 
-// RCM order of the CSR pattern (rp, col) if it has a smaller bandwidth than
-// the given order; then (rp, col) become the reordered pattern and the
-// order, its inverse and each slot's new place are uploaded to p.
+// RCM order of the CSR pattern (rp, col) if the caller's order is not
+// banded already (bandwidth at most 5x the widest level of a breadth-first
+// search from row 0) and RCM has the smaller bandwidth; then (rp, col)
+// become the reordered pattern and the order, its inverse and each slot's
+// new place are uploaded to p.
 void reorder(CooPattern &p, int n, std::vector<int> &rp, std::vector<int> &col) {
-  std::vector<int> ord = rcm_order(n, rp, col), inv(n), id(n);
-  for (int k = 0; k < n; k++) inv[ord[k]] = k;
+  std::vector<int> id(n);
   std::iota(id.begin(), id.end(), 0);
-  if (bandwidth(rp, col, inv) >= bandwidth(rp, col, id)) return;
+  const int64_t bw_id = bandwidth(rp, col, id);
+  if (bw_id <= 5 * bfs_max_level(n, rp, col, 0)) return;
+  std::vector<int> ord = rcm_order(n, rp, col), inv(n);
+  for (int k = 0; k < n; k++) inv[ord[k]] = k;
+  if (bandwidth(rp, col, inv) >= bw_id) return;
   std::vector<int> nrp(n + 1, 0), ncol(col.size()), pos(col.size());
   std::vector<std::pair<int, int>> row;
   for (int k = 0; k < n; k++) {
