@@ -13,6 +13,7 @@ struct admfem_mesh_s {
 struct admfem_system_s {
   admfem::System *s;
   admfem::DevBuf<double> u_int, lam_int, g_int, tmp_int;
+  admfem::DevBuf<double2> ul_int;  // (u, lam) pairs of the gradient call
 };
 struct admfem_server_s {
   admfem::Server *sv;
@@ -325,6 +326,7 @@ int admfem_system_create(admfem_mesh h, const long long *bdnode, int nbd,
       hs->lam_int.alloc(s->npad);
       hs->g_int.alloc(s->npad);
       hs->tmp_int.alloc(s->npad);
+      hs->ul_int.alloc(s->npad);
     } catch (...) {
       system_free(s);
       delete hs;
@@ -349,7 +351,7 @@ int admfem_system_info(admfem_system h, int64_t info[4]) {
     info[1] = (int64_t)s.col.size();
     info[2] = s.nbd_unique;
     info[3] = (int64_t)(s.device_bytes() + s.mesh->device_bytes() +
-                        4 * sizeof(double) * (size_t)s.npad);
+                        6 * sizeof(double) * (size_t)s.npad);  // + the handle's vectors
   });
 }
 
@@ -448,9 +450,7 @@ int admfem_system_kappa_gradient(admfem_system h, const double *u, const double 
     System &s = SY(h);
     REQUIRE(u && lam && grad_kappa, ADMFEM_ERR_ARG, "null pointer");
     hipStream_t str = S(stream);
-    launch_scatter(s.mesh->d_perm.p, u, h->u_int.p, s.n, str);  // user -> internal order
-    launch_scatter(s.mesh->d_perm.p, lam, h->lam_int.p, s.n, str);
-    launch_kappa_gradient(s, h->u_int.p, h->lam_int.p, scale, grad_kappa, str);
+    launch_kappa_gradient_user(s, u, lam, h->ul_int.p, scale, grad_kappa, str);
   });
 }
 

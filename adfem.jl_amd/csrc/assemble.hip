@@ -390,12 +390,16 @@ __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
 // dL/dkappa_{e,k} = -scale * w_k * grad(lam~) . grad(u) on element e, with
 // lam~ = lam zeroed on boundary rows (the gradient ImposeDirichletGrad drops,
 // ImposeDirichlet.h:63-93).  One thread per element, nq outputs each.
+// PAIR: u and lam come interleaved, ul[v] = (u_v, lam_v): one 16-byte
+// gather per vertex instead of two 8-byte ones.
+template <bool PAIR>
 __global__ void kappa_grad_kernel(int nelem, QuadRule rule,
                                   const int *__restrict__ conn,
                                   const double2 *__restrict__ xy,
                                   const int *__restrict__ row_bd,
                                   const double *__restrict__ u,
-                                  const double *__restrict__ lam, double scale,
+                                  const double *__restrict__ lam,
+                                  const double2 *__restrict__ ul, double scale,
                                   double *__restrict__ gk) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
@@ -407,8 +411,14 @@ __global__ void kappa_grad_kernel(int nelem, QuadRule rule,
     const double2 c = xy[v[k]];
     ex[k] = c.x;
     ey[k] = c.y;
-    uu[k] = u[v[k]];
-    ll[k] = row_bd[v[k]] >= 0 ? 0.0 : lam[v[k]];
+    if (PAIR) {
+      const double2 w = ul[v[k]];
+      uu[k] = w.x;
+      ll[k] = row_bd[v[k]] >= 0 ? 0.0 : w.y;
+    } else {
+      uu[k] = u[v[k]];
+      ll[k] = row_bd[v[k]] >= 0 ? 0.0 : lam[v[k]];
+    }
   }
   ElemGeom G = elem_geom(ex[0], ey[0], ex[1], ey[1], ex[2], ey[2]);
   double dux = 0, duy = 0, dlx = 0, dly = 0;
@@ -507,8 +517,28 @@ void launch_to_sell(System &s, hipStream_t st) {
 void launch_kappa_gradient_to(const System &s, const double2 *xy, const double *u_int,
                               const double *lam_int, double scale, double *gk, hipStream_t st) {
   const Mesh &m = *s.mesh;
-  kappa_grad_kernel<<<nblk(m.nelem, 256), 256, 0, st>>>(
-      m.nelem, m.rule, m.conn.p, xy, s.d_row_bd.p, u_int, lam_int, scale, gk);
+  kappa_grad_kernel<false><<<nblk(m.nelem, 256), 256, 0, st>>>(
+      m.nelem, m.rule, m.conn.p, xy, s.d_row_bd.p, u_int, lam_int, nullptr, scale, gk);
+  HIPCHECK(hipGetLastError());
+}
+
+namespace {
+// dst[idx[i]] = (a[i], b[i]): one 16-byte write per entry
+__global__ void scatter_pair_kernel(const int *__restrict__ idx, const double *__restrict__ a,
+                                    const double *__restrict__ b, double2 *__restrict__ dst,
+                                    int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[idx[i]] = make_double2(a[i], b[i]);
+}
+}  // namespace
+
+void launch_kappa_gradient_user(const System &s, const double *u, const double *lam,
+                                double2 *ul_int, double scale, double *gk, hipStream_t st) {
+  const Mesh &m = *s.mesh;
+  if (s.n > 0)
+    scatter_pair_kernel<<<nblk(s.n, 256), 256, 0, st>>>(m.d_perm.p, u, lam, ul_int, s.n);
+  kappa_grad_kernel<true><<<nblk(m.nelem, 256), 256, 0, st>>>(
+      m.nelem, m.rule, m.conn.p, m.xy.p, s.d_row_bd.p, nullptr, nullptr, ul_int, scale, gk);
   HIPCHECK(hipGetLastError());
 }
 

@@ -273,6 +273,10 @@ void launch_kappa_gradient(const System &s, const double *u_int,
                            hipStream_t st);
 void launch_kappa_gradient_to(const System &s, const double2 *xy, const double *u_int,
                               const double *lam_int, double scale, double *gk, hipStream_t st);
+// the same from u and lam in the user's node order: both scattered into
+// ul_int [n] as (u, lam) pairs in the internal order, then the gradient
+void launch_kappa_gradient_user(const System &s, const double *u, const double *lam,
+                                double2 *ul_int, double scale, double *gk, hipStream_t st);
 void launch_loss(const System &s, const double *u_int, const int *obs_idx,
                  const double *obs_val, int nobs, double *g_int, double *loss2,
                  hipStream_t st);
