@@ -14,6 +14,7 @@ struct admfem_system_s {
   admfem::System *s;
   admfem::DevBuf<double> u_int, lam_int, g_int, tmp_int;
   admfem::DevBuf<double2> ul_int;  // (u, lam) pairs of the gradient call
+  admfem::DevBuf<double> loss2;    // loss and index-error flag of forward_adjoint
 };
 struct admfem_server_s {
   admfem::Server *sv;
@@ -327,6 +328,7 @@ int admfem_system_create(admfem_mesh h, const long long *bdnode, int nbd,
       hs->g_int.alloc(s->npad);
       hs->tmp_int.alloc(s->npad);
       hs->ul_int.alloc(s->npad);
+      hs->loss2.alloc(2);
     } catch (...) {
       system_free(s);
       delete hs;
@@ -467,7 +469,8 @@ int admfem_system_forward_adjoint(admfem_system h, const admfem_problem *p,
     assemble_impl(h, p, str);
     admfem_solve_stats s0{}, s1{};
     cg_solve(s, s.rhs.p, h->u_int.p, *o, s0, str);
-    double L = loss_and_grad(s, h->u_int.p, obs_idx, obs_val, nobs, h->g_int.p, str);
+    double L = loss_and_grad(s, h->u_int.p, obs_idx, obs_val, nobs, h->g_int.p, h->loss2.p,
+                             str);
     cg_solve(s, h->g_int.p, h->lam_int.p, *o, s1, str);
     if (grad_kappa)
       launch_kappa_gradient(s, h->u_int.p, h->lam_int.p, p->lap_coef, grad_kappa, str);

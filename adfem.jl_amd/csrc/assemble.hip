@@ -581,15 +581,11 @@ void launch_scatter(const int *idx, const double *src, double *dst, int n, hipSt
 }
 
 double loss_and_grad(const System &s, const double *u_int, const int *obs_idx,
-                     const double *obs_val, int nobs, double *g_int,
+                     const double *obs_val, int nobs, double *g_int, double *loss2,
                      hipStream_t st) {
-  HIPCHECK(hipMemsetAsync(g_int, 0, sizeof(double) * s.npad, st));
-  DevBuf<double> d_loss(2);
-  loss_kernel<<<1, 1024, 0, st>>>(s.n, s.mesh->d_perm.p, obs_idx, obs_val, nobs,
-                                  u_int, g_int, d_loss.p);
-  HIPCHECK(hipGetLastError());
+  launch_loss(s, u_int, obs_idx, obs_val, nobs, g_int, loss2, st);
   double h[2] = {0.0, 0.0};
-  HIPCHECK(hipMemcpyAsync(h, d_loss.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(h, loss2, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   REQUIRE(h[1] == 0.0, ADMFEM_ERR_ARG, "observation index out of range [0, n)");
   return h[0];

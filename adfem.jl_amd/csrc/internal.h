@@ -325,8 +325,9 @@ void server_stats(Server *sv, double *stats);
 double server_spmv_repeat(Server *sv, const int *topos, int nsys, int repeat,
                           hipStream_t st, double *seconds);
 // loss gradient: g_int = 0; g[perm[idx]] = 2 (u - obs); returns loss on host
+// (loss2: 2 device doubles of the caller's, loss and an index-error flag)
 double loss_and_grad(const System &s, const double *u_int, const int *obs_idx,
-                     const double *obs_val, int nobs, double *g_int,
+                     const double *obs_val, int nobs, double *g_int, double *loss2,
                      hipStream_t st);
 
 }  // namespace admfem
