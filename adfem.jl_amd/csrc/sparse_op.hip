@@ -116,6 +116,7 @@ struct CooPattern {
   bool reordered = false;
   DevBuf<int> order, inv, pos;
   DevBuf<double> b, x;
+  DevBuf<int> flag;  // set when a call's indices differ from `ind`
   System *s = nullptr;
 };
 
@@ -167,16 +168,15 @@ void reorder(CooPattern &p, int n, std::vector<int> &rp, std::vector<int> &col) 
 std::mutex g_lock;
 CooPattern *g_kept = nullptr;  // kept until the process ends (never freed at exit)
 
-bool same_pattern(const CooPattern &p, const long long *ind, int64_t nnz, int n, int dev,
+bool same_pattern(CooPattern &p, const long long *ind, int64_t nnz, int n, int dev,
                   hipStream_t st) {
   if (p.device != dev || p.n != n || p.nnz != nnz) return false;
   if (nnz == 0) return true;
-  DevBuf<int> flag(1);
-  set_int<<<1, 1, 0, st>>>(flag.p, 0);
-  coo_differs<<<nb(2 * nnz, 256), 256, 0, st>>>(2 * nnz, ind, p.ind.p, flag.p);
+  set_int<<<1, 1, 0, st>>>(p.flag.p, 0);
+  coo_differs<<<nb(2 * nnz, 256), 256, 0, st>>>(2 * nnz, ind, p.ind.p, p.flag.p);
   HIPCHECK(hipGetLastError());
   int h = 1;
-  HIPCHECK(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(&h, p.flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   return h == 0;
 }
@@ -186,6 +186,7 @@ void build_pattern(CooPattern &p, const long long *ind, int64_t nnz, int n, int 
   p.device = dev;
   p.n = n;
   p.nnz = nnz;
+  p.flag.alloc(1);
   const int64_t m = std::max<int64_t>(nnz, 1);
   DevBuf<unsigned long long> key(m), skey(m);
   DevBuf<int> val(m), fresh(m), pos(m), err(1);
