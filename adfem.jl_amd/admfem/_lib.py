@@ -54,6 +54,7 @@ _SIGS = {
     "admfem_sparse_solve_grad": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp,
                                          ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats),
                                          c_vp]),
+    "admfem_sparse_solve_info": (c_int, [c_vp]),
     # A. reference-named
     "init_nnfem_mesh": (c_ll_p, [c_vp, c_int, c_vp, c_int, c_int, c_int, c_int, c_vp]),
     "mfem_get_ngauss": (c_int, []),
@@ -69,6 +70,8 @@ _SIGS = {
     "FemToGaussPointsMfem_Julia": (None, [c_vp, c_vp]),
     "DofToGaussPointsMfem_forward_Julia": (None, [c_vp, c_vp]),
     "ComputeLaplaceTermMfem_forward_Julia": (None, [c_vp, c_vp, c_vp]),
+    "pcl_FemLaplaceScalar_Jacobian": (None, [c_vp]),
+    "pcl_ImposeDirichlet": (None, [c_vp, c_vp, c_vp, c_int, c_int]),
     "admfem_default_mesh": (c_vp, []),
     # B. meshes
     "admfem_mesh_create": (c_int, [c_vp, c_int, c_vp, c_int, c_int, c_int, c_vp]),

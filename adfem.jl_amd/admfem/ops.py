@@ -360,3 +360,13 @@ def solve(A, rhs, precision="fp64", rtol=1e-12, max_iter=1000000):
     gradient SparseSolverGrad)."""
     opts = solve_opts(precision, rtol, max_iter)
     return _Solve.apply(A.values, rhs, A.indices.contiguous(), A.n, opts)
+
+
+def solve_pattern_info():
+    """The pattern the general solve keeps (the last one built):
+    dict(n, triplets, slots, reordered), ``reordered`` True when the solver
+    works in reverse Cuthill-McKee order of it rather than the caller's."""
+    info = np.zeros(4, dtype=np.int64)
+    _lib.check(_lib.lib().admfem_sparse_solve_info(_lib.ptr(info)))
+    return dict(n=int(info[0]), triplets=int(info[1]), slots=int(info[2]),
+                reordered=bool(info[3]))

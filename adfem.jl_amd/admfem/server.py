@@ -70,6 +70,10 @@ def exported_symbols():
     return list(_SIGS)
 
 
+def _on_device(a):
+    return hasattr(a, "is_cuda") and a.is_cuda
+
+
 def _f64(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
 
@@ -117,6 +121,7 @@ class Server:
                kappa=None, f=None, u_prev=None, obs_idx=None, obs_val=None, want_u=True,
                want_grad=True):
         """Synthetic inputs when ``seed`` is given (generated on the device),
+        torch device tensors (kept alive here until the request is done) or
         host arrays otherwise.  Returns the request id."""
         S = self._topos[topo]
         k = {"poisson": 0, "heat": 1}[kind]
@@ -124,6 +129,17 @@ class Server:
         if seed is not None:
             src, nobs = SRC_SYNTH, 0
             ci = ki = fi = ui = oi = ov = None
+        elif _on_device(kappa):
+            import torch
+            src = SRC_DEVICE
+
+            def dev(t, dtype):
+                return None if t is None else t.to(device="cuda", dtype=dtype).contiguous()
+            ci, ki, fi, ui = (dev(t, torch.float64) for t in (coords, kappa, f, u_prev))
+            oi, ov = dev(obs_idx, torch.int32), dev(obs_val, torch.float64)
+            nobs = 0 if oi is None else oi.numel()
+            keep = [ci, ki, fi, ui, oi, ov]
+            torch.cuda.current_stream().synchronize()  # the engine reads them on its stream
         else:
             src = SRC_HOST
             ci, ki, fi, ui = _f64(coords), _f64(kappa), _f64(f), _f64(u_prev)

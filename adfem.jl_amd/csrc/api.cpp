@@ -513,6 +513,13 @@ int admfem_sparse_solve_grad(const long long *indices, const double *vv, int64_t
   });
 }
 
+int admfem_sparse_solve_info(int64_t info[4]) {
+  return guarded([&] {
+    REQUIRE(info, ADMFEM_ERR_ARG, "null pointer");
+    sparse_solve_info(info);
+  });
+}
+
 int admfem_system_spmv_repeat(admfem_system h, int precision, int repeat, double *bytes,
                               void *stream) {
   return guarded([&] {
@@ -522,7 +529,9 @@ int admfem_system_spmv_repeat(admfem_system h, int precision, int repeat, double
     if (bytes) {
       const double vb = precision == ADMFEM_PREC_MIXED ? 4.0 : 8.0;
       const double nnz = (double)s.col.size(), n = s.n;
-      *bytes = nnz * (vb + 4.0) + 4.0 * (n + 1) + 2.0 * n * vb;
+      // one CG iteration of the canonical CSR form: the SpMV plus the
+      // iteration's vector updates (6 n s)
+      *bytes = nnz * (vb + 4.0) + 4.0 * (n + 1) + 2.0 * n * vb + 6.0 * n * vb;
     }
   });
 }
@@ -708,6 +717,45 @@ void FemLaplaceScalar_forward_Julia(long long *indices, double *vv, const double
     d2h(vv, dv.p, nt);
   });
   if (rc) fprintf(stderr, "FemLaplaceScalar_forward_Julia: %s\n", admfem_last_error());
+}
+
+void pcl_FemLaplaceScalar_Jacobian(double *H) {
+  Mesh *m = default_or_warn("pcl_FemLaplaceScalar_Jacobian");
+  if (!m) return;
+  int rc = guarded([&] {
+    // the triplet values are linear in kappa, each in one Gauss point's:
+    // d vv[t] / d kappa[t / nd^2] = vv[t] at kappa = 1
+    const size_t nd2 = (size_t)m->nd * m->nd, ng = (size_t)m->ngauss, nt = nd2 * ng;
+    std::vector<double> one(ng, 1.0), vv(nt);
+    DevBuf<double> dk, dv(nt);
+    DevBuf<long long> di(2 * nt);
+    dk.upload(one);
+    launch_laplace_coo(*m, dk.p, di.p, dv.p, nullptr);
+    d2h(vv.data(), dv.p, nt);
+    for (size_t t = 0; t < nt; t++) H[t / nd2 + t * ng] = vv[t];
+  });
+  if (rc) fprintf(stderr, "pcl_FemLaplaceScalar_Jacobian: %s\n", admfem_last_error());
+}
+
+void pcl_ImposeDirichlet(double *J, const long long *indices, const long long *bd, int bdN,
+                         int sN) {
+  int rc = guarded([&] {
+    REQUIRE(sN >= 0 && bdN >= 0 && (sN == 0 || (J && indices)) && (bdN == 0 || bd),
+            ADMFEM_ERR_ARG, "bad arguments");
+    long long nmax = 0;
+    for (int k = 0; k < 2 * sN; k++) nmax = std::max(nmax, indices[k]);
+    for (int k = 0; k < bdN; k++) nmax = std::max(nmax, bd[k]);
+    std::vector<char> isbd((size_t)nmax + 1, 0);
+    for (int k = 0; k < bdN; k++)
+      if (bd[k] >= 1) isbd[bd[k]] = 1;  // 1-based
+    size_t s = 0;
+    for (int k = 0; k < sN; k++) {
+      const long long i = indices[k], j = indices[k + sN];
+      const bool ib = i >= 1 && isbd[i], jb = j >= 1 && isbd[j];
+      if (!ib && !jb) J[(size_t)k + s++ * (size_t)sN] = 1.0;
+    }
+  });
+  if (rc) fprintf(stderr, "pcl_ImposeDirichlet: %s\n", admfem_last_error());
 }
 
 void FemSourceScalar_forward_Julia(double *rhs, const double *f) {

@@ -436,6 +436,10 @@ __global__ void kappa_grad_kernel(int nelem, QuadRule rule,
 
 // Loss L = sum_k (u[perm[obs_idx[k]]] - obs_val[k])^2 and its gradient
 // g = dL/du (internal order; g zeroed beforehand).  One block, fixed order.
+// loss[1] flags an index outside [0, n) or one that occurs twice: the
+// observations first claim their rows in g (g[r] = k + 1), and one whose
+// claim did not stick shares its row with another.  (A shared row would
+// need its terms summed; the entry points require distinct indices.)
 __global__ void __launch_bounds__(1024)
     loss_kernel(int n, const int *__restrict__ perm, const int *__restrict__ idx,
                 const double *__restrict__ obs, int nobs,
@@ -444,13 +448,24 @@ __global__ void __launch_bounds__(1024)
   __shared__ double sh[16];
   double acc = 0.0, bad = 0.0;
   for (int k = threadIdx.x; k < nobs; k += blockDim.x) {
-    int id = idx[k];
+    const int id = idx[k];
     if (id < 0 || id >= n) {  // caller data: never index with it unchecked
       bad = 1.0;
       continue;
     }
-    int r = perm[id];
-    double d = u[r] - obs[k];
+    g[perm[id]] = (double)(k + 1);
+  }
+  __syncthreads();  // the claims are visible to the whole block
+  for (int k = threadIdx.x; k < nobs; k += blockDim.x) {
+    const int id = idx[k];
+    if (id >= 0 && id < n && g[perm[id]] != (double)(k + 1)) bad = 1.0;
+  }
+  __syncthreads();  // every check has read its claim
+  for (int k = threadIdx.x; k < nobs; k += blockDim.x) {
+    const int id = idx[k];
+    if (id < 0 || id >= n) continue;
+    const int r = perm[id];
+    const double d = u[r] - obs[k];
     g[r] = 2.0 * d;
     acc += d * d;
   }
@@ -587,7 +602,8 @@ double loss_and_grad(const System &s, const double *u_int, const int *obs_idx,
   double h[2] = {0.0, 0.0};
   HIPCHECK(hipMemcpyAsync(h, loss2, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
-  REQUIRE(h[1] == 0.0, ADMFEM_ERR_ARG, "observation index out of range [0, n)");
+  REQUIRE(h[1] == 0.0, ADMFEM_ERR_ARG,
+          "observation index out of range [0, n) or repeated");
   return h[0];
 }
 

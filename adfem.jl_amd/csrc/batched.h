@@ -20,15 +20,17 @@ struct BSys {
   const double *b;          // right-hand side
   double *x[2];             // FP64 solution, two buffers (swapped on replacement)
   float *ys[2];             // FP32 scaled correction since the last replacement
-  float *r, *p[2], *q;      // scaled residual, directions, A p
+  float *r[2], *p[2], *q[2];  // scaled residual, direction, A p: the step reads
+                              // one buffer of each and writes the other
   double *part;             // 4 * nblk block partials
   int n, npad, nblk;
 };
 
-// CG state of one system: rz (= scaled r.r) drives the iteration, rr is the
-// true FP64 ||b - A x||^2 of the last replacement.
+// CG state of one system: rz (= scaled r.r of the current residual) drives
+// the iteration, rr is the true FP64 ||b - A x||^2 of the last replacement;
+// alpha and beta are the pending step's.
 struct BScal {
-  double rtol2, delta2, bb, tol2, rz, rz_prev, alpha, beta, rr, rr_ref, rz_ref, rz_min;
+  double rtol2, delta2, bb, tol2, rz, alpha, beta, rr, rr_ref, rz_ref, rz_min;
   int status, it, maxit, replace, nrepl, sel, stall;
 };
 

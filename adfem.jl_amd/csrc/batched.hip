@@ -14,19 +14,30 @@
 // -- the same iterates in exact arithmetic, but the FP32 inner iteration
 // reads no diagonal, no inverse diagonal and keeps no separate z = D^-1 r.
 //
-// Per iteration (all active systems), FP32 storage, FP64 arithmetic:
-//   bk_spmv   p = r + beta p_old, q = Â p
-//   bs_alpha  one block per system: alpha = r.r / p.q
-//   bk_update y += alpha p, r -= alpha q
-//   bs_beta   beta; a converged-looking residual freezes the system
-// and every 8th iteration of a chunk (a replacement point):
+// One CG iteration is one wide kernel and one single-block-per-system
+// kernel (FP32 storage, FP64 arithmetic):
+//   bk_step  applies the pending step and forms the next direction,
+//              r' = r - alpha q,  y += alpha p,  p' = r' + beta p,  q' = Â p'
+//            (a neighbour's p' is formed on the fly from its r, q, p), and
+//            the partials of r'.r', p'.q', r'.q', q'.q'
+//   bs_step  alpha' = r'.r' / p'.q' and beta' = |r' - alpha' q'|^2 / r'.r',
+//            the numerator expanded as r'.r' - 2 alpha' r'.q' + alpha'^2 q'.q'
+//            (exact for the stored vectors up to rounding, one step ahead,
+//            so no second reduction is needed before p'' is formed);
+//            a converged-looking residual freezes the system
+// so r.r is reduced directly every iteration (never carried by a
+// recurrence), and the iteration is standard CG's (Hestenes-Stiefel) with
+// both reductions of an iteration done in one pass.  Every 8th iteration of
+// a chunk (a replacement point):
 //   bk_replace (frozen systems, residual fell by delta since the last
 //              replacement, or stalled): x' = x + S y in FP64, the true
 //              residual b - A x' with the FP64 matrix, r = S (b - A x'), y' = 0
 //   bs_replace true-residual convergence test, swap the x / y buffers
-// Systems start at chunk boundaries, so a system's replacement points fall
-// on the same iterations of its own count whatever else is in the batch.
-// No atomics, no fences (see cg.hip).
+// The pending alpha and beta carry over a replacement (the direction is
+// kept; reliable updates, van der Vorst & Ye 2000).  Systems start at chunk
+// boundaries, so a system's replacement points fall on the same iterations
+// of its own count whatever else is in the batch.  No atomics, no fences
+// (see cg.hip).
 #include <cmath>
 
 #include "batched.h"
@@ -39,68 +50,72 @@ namespace {
 constexpr int NT = BATCH_NT;
 constexpr int ST_RUN = 0, ST_CONV = 1, ST_MAXIT = 2, ST_NAN = 3, ST_BREAK = 4;
 
-__device__ __forceinline__ void put_partials2(double a, double b, double *part, int G,
-                                              int lb) {
-  __shared__ double sh[2][NT / 64];
+// This block's sums of NV values, stored to part[j * G + lb].
+template <int NV>
+__device__ __forceinline__ void put_partials(const double (&v)[NV], double *part, int G,
+                                             int lb) {
+  __shared__ double sh[NV][NT / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  a = wave_sum(a);
-  b = wave_sum(b);
+  double t[NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) t[j] = wave_sum(v[j]);
   __syncthreads();  // sh may still be read by a previous call
   if (lane == 0) {
-    sh[0][wv] = a;
-    sh[1][wv] = b;
+#pragma unroll
+    for (int j = 0; j < NV; j++) sh[j][wv] = t[j];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-    for (int k = 0; k < NT / 64; k++) {
-      s0 += sh[0][k];
-      s1 += sh[1][k];
+    for (int j = 0; j < NV; j++) {
+      double a = 0.0;
+#pragma unroll
+      for (int k = 0; k < NT / 64; k++) a += sh[j][k];
+      part[j * G + lb] = a;
     }
-    part[lb] = s0;
-    part[G + lb] = s1;
   }
 }
 
-__device__ __forceinline__ void sum2(const double *part, int G, double &o0, double &o1) {
-  __shared__ double sh[2][NT / 64];
-  double a = 0.0, b = 0.0;
+// Sums of NV partial arrays part[j * G .. j * G + G) over one block (fixed
+// order), valid in every thread.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *part, int G, double (&o)[NV]) {
+  __shared__ double sh[NV][NT / 64];
+  double a[NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) a[j] = 0.0;
   // part[k], part[k + NT], ... added in that order, their loads issued 4 at
   // a time so that they are in flight together
   int k = threadIdx.x;
   for (; k + 3 * NT < G; k += 4 * NT) {
-    double va[4], vb[4];
+    double v[NV][4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      va[u] = part[k + u * NT];
-      vb[u] = part[G + k + u * NT];
-    }
+    for (int u = 0; u < 4; u++)
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      a += va[u];
-      b += vb[u];
-    }
+      for (int j = 0; j < NV; j++) v[j][u] = part[j * G + k + u * NT];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int j = 0; j < NV; j++) a[j] += v[j][u];
   }
   for (; k < G; k += NT) {
-    a += part[k];
-    b += part[G + k];
+#pragma unroll
+    for (int j = 0; j < NV; j++) a[j] += part[j * G + k];
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  a = wave_sum(a);
-  b = wave_sum(b);
+#pragma unroll
+  for (int j = 0; j < NV; j++) a[j] = wave_sum(a[j]);
   __syncthreads();  // sh may still be read by a previous call
   if (lane == 0) {
-    sh[0][wv] = a;
-    sh[1][wv] = b;
+#pragma unroll
+    for (int j = 0; j < NV; j++) sh[j][wv] = a[j];
   }
   __syncthreads();
-  o0 = 0.0;
-  o1 = 0.0;
 #pragma unroll
-  for (int k = 0; k < NT / 64; k++) {
-    o0 += sh[0][k];
-    o1 += sh[1][k];
+  for (int j = 0; j < NV; j++) {
+    o[j] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NT / 64; k++) o[j] += sh[j][k];
   }
 }
 
@@ -140,13 +155,13 @@ __device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
 // --------------------------------------------------------------- CG kernels
 // init of ONE system (grid = its nblk): x0 = b on identity rows, 0 elsewhere;
 // r0 = S (b - A x0) (b on the other rows, 0 on identity rows: their columns
-// are eliminated); p_old = 0, y = 0; partials of r.r (scaled), ||b - A x0||^2
-// and, into part[2G..4G), b.b
+// are eliminated); p = q = 0 and y = 0 (with alpha = beta = 0 the first step
+// forms p = r0, q = Â r0); partials of the scaled r.r, ||b - A x0||^2 and b.b
 __global__ void __launch_bounds__(NT)
     bk_init(const BSys *__restrict__ D, int sys) {
   const BSys &d = D[sys];
   const int i = blockIdx.x * NT + threadIdx.x;
-  double rz = 0, rr = 0, bb = 0;
+  double v[3] = {0.0, 0.0, 0.0};
   if (i < d.npad) {
     const double bi = i < d.n ? d.b[i] : 0.0;
     const bool ident = i >= d.n || d.row_bd[i] >= 0;
@@ -154,24 +169,23 @@ __global__ void __launch_bounds__(NT)
     d.x[0][i] = (ident && i < d.n) ? bi : 0.0;
     d.ys[0][i] = 0.0f;
     const float rv = (float)(ri * d.dsq[i]);
-    d.r[i] = rv;
+    d.r[0][i] = rv;
     d.p[0][i] = 0.0f;
-    rz = (double)rv * (double)rv;
-    rr = ri * ri;
-    bb = bi * bi;
+    d.q[0][i] = 0.0f;
+    v[0] = (double)rv * (double)rv;
+    v[1] = ri * ri;
+    v[2] = bi * bi;
   }
-  put_partials2(rz, rr, d.part, d.nblk, blockIdx.x);
-  __syncthreads();
-  put_partials2(bb, 0.0, d.part + 2 * d.nblk, d.nblk, blockIdx.x);
+  put_partials<3>(v, d.part, d.nblk, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(NT) bs_init(const BSys *__restrict__ D, BScal *S, int sys,
                                               double rtol2, double delta2, int maxit) {
   const BSys &d = D[sys];
-  double rz, rr, bb, unused;
-  sum2(d.part, d.nblk, rz, rr);
-  sum2(d.part + 2 * d.nblk, d.nblk, bb, unused);
+  double t[3];
+  sum_partials<3>(d.part, d.nblk, t);
   if (threadIdx.x != 0) return;
+  const double rz = t[0], rr = t[1], bb = t[2];
   BScal &s = S[sys];
   s.rtol2 = rtol2;
   s.delta2 = delta2;
@@ -179,7 +193,6 @@ __global__ void __launch_bounds__(NT) bs_init(const BSys *__restrict__ D, BScal 
   s.bb = bb;
   s.tol2 = rtol2 * bb;
   s.rz = rz;
-  s.rz_prev = rz;
   s.rz_ref = rz;
   s.rz_min = rz;
   s.rr = rr;
@@ -194,84 +207,68 @@ __global__ void __launch_bounds__(NT) bs_init(const BSys *__restrict__ D, BScal 
   s.status = (!(rr == rr) || !isfinite(rr)) ? ST_NAN : (rr <= s.tol2 ? ST_CONV : ST_RUN);
 }
 
+// The pending step's residual and the next direction of one row:
+// r' = r - alpha q and p' = r' + beta p in FP64, p' rounded as it is stored.
+// Rows and their neighbours evaluate it with the same explicit fused
+// multiply-adds, so q' = Â p' is the product with the stored p' exactly.
+struct Step {
+  double r, p;
+};
+__device__ __forceinline__ Step step_of(float r, float q, float p, double alpha, double beta) {
+  const double rn = fma(-alpha, (double)q, (double)r);
+  return Step{rn, (double)(float)fma(beta, (double)p, rn)};
+}
+
 __global__ void __launch_bounds__(NT)
-    bk_spmv(const int2 *__restrict__ map, const BSys *__restrict__ D, const BScal *S) {
+    bk_step(const int2 *__restrict__ map, const BSys *__restrict__ D, const BScal *S) {
   const int2 m = map[blockIdx.x];
   if (m.x < 0) return;
   const BScal &s = S[m.x];
   if (s.status != ST_RUN || s.replace) return;
   const BSys &d = D[m.x];
   const int i = m.y * NT + threadIdx.x;
-  const double beta = s.beta;
-  const float *__restrict__ po = d.p[s.it & 1];
-  float *__restrict__ pn = d.p[(s.it + 1) & 1];
-  const float *__restrict__ r = d.r;
-  double pq = 0;
+  const double alpha = s.alpha, beta = s.beta;
+  const int cur = s.it & 1;
+  const float *__restrict__ r = d.r[cur];
+  const float *__restrict__ p = d.p[cur];
+  const float *__restrict__ q = d.q[cur];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < d.npad) {
-    const double p_i = (double)r[i] + beta * (double)po[i];
-    const double a = p_i + sell_dot<float>(i, d.sp, d.sc, d.sv32, [&](int c) {
-                       return (double)r[c] + beta * (double)po[c];
+    const float pi = p[i];
+    const Step own = step_of(r[i], q[i], pi, alpha, beta);
+    float *__restrict__ ys = d.ys[s.sel];
+    ys[i] = (float)fma(alpha, (double)pi, (double)ys[i]);
+    const double a = own.p + sell_dot<float>(i, d.sp, d.sc, d.sv32, [&](int c) {
+                       return step_of(r[c], q[c], p[c], alpha, beta).p;
                      });
-    const float pv = (float)p_i, qv = (float)a;
-    pn[i] = pv;
-    d.q[i] = qv;
-    pq = (double)pv * (double)qv;
+    const float rv = (float)own.r, pv = (float)own.p, qv = (float)a;
+    d.r[cur ^ 1][i] = rv;
+    d.p[cur ^ 1][i] = pv;
+    d.q[cur ^ 1][i] = qv;
+    v[0] = (double)rv * (double)rv;
+    v[1] = (double)pv * (double)qv;
+    v[2] = (double)rv * (double)qv;
+    v[3] = (double)qv * (double)qv;
   }
-  put_partials2(pq, 0.0, d.part, d.nblk, m.y);
+  put_partials<4>(v, d.part, d.nblk, m.y);
 }
 
 __global__ void __launch_bounds__(NT)
-    bs_alpha(const int *__restrict__ list, const BSys *__restrict__ D, BScal *S) {
+    bs_step(const int *__restrict__ list, const BSys *__restrict__ D, BScal *S) {
   const int sys = list[blockIdx.x];
   BScal &s = S[sys];
   if (s.status != ST_RUN || s.replace) return;
-  double pq, unused;
-  sum2(D[sys].part, D[sys].nblk, pq, unused);
+  double t[4];
+  sum_partials<4>(D[sys].part, D[sys].nblk, t);
+  __syncthreads();  // every thread has read the scalars it tests
   if (threadIdx.x != 0) return;
-  if (!(pq > 0.0)) {
-    s.status = (pq != pq) ? ST_NAN : ST_BREAK;
-    return;
-  }
-  s.alpha = s.rz / pq;
-}
-
-__global__ void __launch_bounds__(NT)
-    bk_update(const int2 *__restrict__ map, const BSys *__restrict__ D, const BScal *S) {
-  const int2 m = map[blockIdx.x];
-  if (m.x < 0) return;
-  const BScal &s = S[m.x];
-  if (s.status != ST_RUN || s.replace) return;
-  const BSys &d = D[m.x];
-  const int i = m.y * NT + threadIdx.x;
-  const double alpha = s.alpha;
-  const float *__restrict__ p = d.p[(s.it + 1) & 1];
-  float *__restrict__ ys = d.ys[s.sel];
-  double rz = 0;
-  if (i < d.npad) {
-    ys[i] = (float)((double)ys[i] + alpha * (double)p[i]);
-    const float rv = (float)((double)d.r[i] - alpha * (double)d.q[i]);
-    d.r[i] = rv;
-    rz = (double)rv * (double)rv;
-  }
-  put_partials2(rz, 0.0, d.part, d.nblk, m.y);
-}
-
-__global__ void __launch_bounds__(NT)
-    bs_beta(const int *__restrict__ list, const BSys *__restrict__ D, BScal *S) {
-  const int sys = list[blockIdx.x];
-  BScal &s = S[sys];
-  if (s.status != ST_RUN || s.replace) return;
-  double rz, unused;
-  sum2(D[sys].part, D[sys].nblk, rz, unused);
-  if (threadIdx.x != 0) return;
-  s.rz_prev = s.rz;
-  s.rz = rz;
-  s.beta = rz / s.rz_prev;
+  const double rz = t[0], pq = t[1], rq = t[2], qq = t[3];
   s.it += 1;
-  if (!(rz == rz) || !isfinite(rz)) {
+  if (!(rz == rz) || !isfinite(rz) || pq != pq) {
     s.status = ST_NAN;
     return;
   }
+  s.rz = rz;
   if (rz < 0.98 * s.rz_min) {
     s.rz_min = rz;
     s.stall = 0;
@@ -283,6 +280,16 @@ __global__ void __launch_bounds__(NT)
   // the next replacement point, where the FP64 residual decides
   if (rz * s.rr_ref <= s.tol2 * s.rz_ref) s.replace = 1;
   else if (s.it >= s.maxit) s.status = ST_MAXIT;
+  if (!(pq > 0.0)) {  // not SPD (or p = 0 with r = 0: the replacement decides)
+    if (!s.replace) s.status = ST_BREAK;
+    s.alpha = 0.0;
+    s.beta = 0.0;
+    return;
+  }
+  const double alpha = rz / pq;
+  const double next = fmax(rz - 2.0 * alpha * rq + alpha * alpha * qq, 0.0);
+  s.alpha = alpha;
+  s.beta = rz > 0.0 ? next / rz : 0.0;
 }
 
 // A system is replaced at a replacement point when it is frozen (converged
@@ -307,7 +314,7 @@ __global__ void __launch_bounds__(NT)
   const double *__restrict__ x = d.x[s.sel];
   const float *__restrict__ ys = d.ys[s.sel];
   const double *__restrict__ dsq = d.dsq;
-  double rz = 0, rr = 0;
+  double v[2] = {0.0, 0.0};
   if (i < d.npad) {
     const double xi = x[i] + dsq[i] * (double)ys[i];
     const double ax = d.diag64[i] * xi + sell_dot<double>(i, d.sp, d.sc, d.sv64, [&](int c) {
@@ -317,11 +324,11 @@ __global__ void __launch_bounds__(NT)
     d.ys[s.sel ^ 1][i] = 0.0f;
     const double ri = (i < d.n ? d.b[i] : 0.0) - ax;
     const float rf = (float)(ri * dsq[i]);
-    d.r[i] = rf;
-    rz = (double)rf * (double)rf;
-    rr = ri * ri;
+    d.r[s.it & 1][i] = rf;
+    v[0] = (double)rf * (double)rf;
+    v[1] = ri * ri;
   }
-  put_partials2(rz, rr, d.part, d.nblk, m.y);
+  put_partials<2>(v, d.part, d.nblk, m.y);
 }
 
 __global__ void __launch_bounds__(NT)
@@ -329,12 +336,12 @@ __global__ void __launch_bounds__(NT)
   const int sys = list[blockIdx.x];
   BScal &s = S[sys];
   if (!replace_due(s)) return;
-  double rz, rr;
-  sum2(D[sys].part, D[sys].nblk, rz, rr);
+  double t[2];
+  sum_partials<2>(D[sys].part, D[sys].nblk, t);
   __syncthreads();  // every thread has read the decision's inputs
   if (threadIdx.x != 0) return;
+  const double rz = t[0], rr = t[1];
   s.rz = rz;
-  s.beta = rz / s.rz_prev;
   s.rr = rr;
   s.rr_ref = rr;
   s.rz_ref = rz;
@@ -473,10 +480,8 @@ void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, con
                       BScal *S, int iters, hipStream_t st) {
   REQUIRE(iters % REPLACE_EVERY == 0, ADMFEM_ERR_ARG, "chunk must be a multiple of 8");
   for (int k = 0; k < iters; k++) {
-    bk_spmv<<<nmap, NT, 0, st>>>(map, D, S);
-    bs_alpha<<<nlist, NT, 0, st>>>(list, D, S);
-    bk_update<<<nmap, NT, 0, st>>>(map, D, S);
-    bs_beta<<<nlist, NT, 0, st>>>(list, D, S);
+    bk_step<<<nmap, NT, 0, st>>>(map, D, S);
+    bs_step<<<nlist, NT, 0, st>>>(list, D, S);
     if (k % REPLACE_EVERY == REPLACE_EVERY - 1) {  // replacement point
       bk_replace<<<nmap, NT, 0, st>>>(map, D, S);
       bs_replace<<<nlist, NT, 0, st>>>(list, D, S);
@@ -487,7 +492,7 @@ void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, con
 
 void batch_spmv_repeat(const int2 *map, int nmap, const BSys *D, const BScal *S, int repeat,
                        hipStream_t st) {
-  for (int k = 0; k < repeat; k++) bk_spmv<<<nmap, NT, 0, st>>>(map, D, S);
+  for (int k = 0; k < repeat; k++) bk_step<<<nmap, NT, 0, st>>>(map, D, S);
   HIPCHECK(hipGetLastError());
 }
 

@@ -2,23 +2,37 @@
 // Dirichlet-eliminated system -- the GPU replacement of the reference's
 // direct sparse solve K\F (ADCME SparseSolver, Eigen SparseLU [ext]).
 //
-// Two precisions share the kernels:
+// Two precisions:
 //  * FP64: matrix, vectors and reductions in FP64.
-//  * mixed: matrix and the inner vectors (r, z, p, q) stored in FP32 (FP64
-//    arithmetic), the solution correction accumulated in FP32; whenever the
-//    FP32 residual has fallen by `replace_delta` (or looks converged) the
-//    correction is folded into the FP64 solution and the residual is
-//    replaced by the true FP64 residual b - A x, computed with the FP64
-//    matrix ("reliable updates", van der Vorst & Ye 2000; Clark et al.
-//    2010).  Convergence is declared only on a true FP64 residual.
+//  * mixed: CG on the symmetrically scaled system S A S (S = diag^-1/2,
+//    unit diagonal; the same iterates as Jacobi-PCG in exact arithmetic)
+//    with the matrix and the inner vectors stored in FP32 (FP64
+//    arithmetic) and the solution correction y accumulated in FP32
+//    (x = x64 + S y); whenever the FP32 residual has fallen by
+//    `replace_delta` (or looks converged, or has stalled) the correction is
+//    folded into the FP64 solution and the residual is replaced by the true
+//    FP64 residual b - A x, computed with the FP64 matrix ("reliable
+//    updates", van der Vorst & Ye 2000; Clark et al. 2010).  Convergence is
+//    declared only on a true FP64 residual.
 //
-// Per iteration:
-//   spmv:    p = z + beta p_old (own row, and on the fly for each gathered
-//            neighbour), q = A p, per-block partials of p.q
-//   scalar:  one block sums the partials in fixed order -> alpha
-//   update:  x += alpha p, r -= alpha q, z = D^-1 r, partials of r.z, r.r
-//   scalar:  -> beta, convergence / replacement decision
-// (mixed, every eighth iteration, when flagged: fold x, replace r, scalar).
+// One iteration is one wide kernel and one single-block kernel:
+//   step:    applies the pending step and forms the next direction,
+//            r' = r - alpha q, x += alpha p, p' = M^-1 r' + beta p, q' = A p'
+//            (a neighbour's p' formed on the fly from its own vectors), and
+//            the block partials of every dot product the next scalars need
+//   scalar:  alpha' = r'.M^-1 r' / p'.q' and beta' = the same quadratic form
+//            of r' - alpha' q' over r'.M^-1 r', its numerator expanded as
+//            r'.M^-1 r' - 2 alpha' q'.M^-1 r' + alpha'^2 q'.M^-1 q' (exact for
+//            the stored vectors up to rounding, one step ahead, so no second
+//            reduction is needed before p'' is formed)
+// so r.M^-1 r is reduced directly every iteration (never carried by a
+// recurrence) and the iterates are standard PCG's (Hestenes-Stiefel) with
+// both reductions of an iteration in one pass over the vectors.  The FP64
+// form keeps z = D^-1 r and w = D^-1 q instead of r and q (r = D z), so a
+// neighbour's p' = z - alpha w + beta p needs no diagonal; the mixed form's
+// unit diagonal needs none either.  The mixed form checks for a
+// replacement every eighth iteration (fold x, replace r, scalar); the
+// pending alpha and beta carry over it (the direction is kept).
 //
 // No atomics and no fences anywhere: on this part a cross-block atomic or a
 // device-scope fence costs tens of ns per block, serialised, which dwarfs a
@@ -41,18 +55,22 @@ enum { ST_RUN = 0, ST_CONV = 1, ST_MAXIT = 2, ST_NAN = 3, ST_BREAK = 4 };
 struct CGScalars {
   double rtol2, delta2;  // set by the host
   double bb, tol2;       // ||b||^2, rtol^2 ||b||^2
-  double rz, rz_prev;    // r.z of the current / previous residual
-  double alpha, beta;
-  double rr;             // ||r||^2 of the current residual (recursive)
+  double rz;             // r.D^-1 r of the current residual (mixed: scaled r.r)
+  double alpha, beta;    // the pending step's
+  double rr;             // FP64: ||r||^2 of the current residual (recursive);
+                         // mixed: true ||b - A x||^2 at the last replacement
   double rr_ref;         // mixed: true ||b - A x||^2 at the last replacement
   double rz_ref;         // mixed: scaled r.r at the last replacement
   double rz_min;         // mixed: smallest scaled r.r since then
   int mixed;
   int status;            // ST_*
-  int it;                // completed iterations
-  int replace;           // mixed: replacement pending
+  int it;                // step kernels run (each one SpMV)
+  int replace;           // mixed: frozen (converged-looking) until a replacement
   int nrepl;
   int stall;             // mixed: iterations without a new rz_min
+  const double *b;       // right-hand side of the solve (read through here by
+                         // the replacement kernel, so replayed graphs do not
+                         // depend on which b a solve is given)
 };
 
 
@@ -61,11 +79,12 @@ constexpr int RT = 1024;  // threads of the single-block scalar kernels
 
 struct CGWork {
   int npad = 0, nblocks = 0;
-  DevBuf<double> x, r64, z64, pa64, pb64, q64;
-  DevBuf<float> xs32, r32, pa32, pb32, q32;
+  DevBuf<double> x;                         // FP64 solution
+  DevBuf<double> z64[2], w64[2], p64[2];    // FP64: D^-1 r, D^-1 q, direction
+  DevBuf<float> ys32, r32[2], p32[2], q32[2];  // mixed: correction, r, p, q
   DevBuf<double> dsq;    // mixed: 1 / sqrt(a_ii)
   DevBuf<float> sv32s;   // mixed: FP32 scaled off-diagonals a_ij dsq_i dsq_j
-  DevBuf<double> part;  // 3 x nblocks block partials
+  DevBuf<double> part;  // 5 x nblocks block partials
   DevBuf<CGScalars> sc;
   CGScalars *h_sc = nullptr;  // pinned
   std::vector<double> h_part;
@@ -197,15 +216,16 @@ __device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
   return s;
 }
 
+// ---- FP64 Jacobi-PCG
+
 // x0 = b on identity (boundary) rows, 0 elsewhere; r0 = b - A x0 (0 on those
 // rows, b on the rest: interior rows of the eliminated system have no
-// boundary columns); z0 = D^-1 r0; p_old = 0 (with beta = 0 the first spmv
-// forms p = z0); partials of r.z, r.r, b.b.
-template <typename VT>
+// boundary columns) kept as z0 = D^-1 r0; w = p = 0 (with alpha = beta = 0
+// the first step forms p = z0, q = A z0); partials of r.z, r.r, b.b.
 __global__ void __launch_bounds__(NT)
     k_init(int n, int npad, const int *__restrict__ row_bd, const double *__restrict__ b,
-           const VT *__restrict__ dinv, double *__restrict__ x, float *__restrict__ xs,
-           VT *__restrict__ r, VT *__restrict__ z, VT *__restrict__ po, double *part) {
+           const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ z,
+           double *__restrict__ w, double *__restrict__ p, double *part) {
   const int i = my_row();
   double v[3] = {0.0, 0.0, 0.0};
   if (i < npad) {
@@ -213,13 +233,11 @@ __global__ void __launch_bounds__(NT)
     const bool ident = i >= n || row_bd[i] >= 0;
     const double ri = ident ? 0.0 : bi;
     x[i] = (ident && i < n) ? bi : 0.0;
-    if (xs) xs[i] = 0.0f;
-    const VT rv = (VT)ri;
-    const VT zv = (VT)((double)rv * (double)dinv[i]);
-    r[i] = rv;
-    z[i] = zv;
-    po[i] = (VT)0;
-    v[0] = (double)rv * (double)zv;
+    const double zi = ri * dinv[i];
+    z[i] = zi;
+    w[i] = 0.0;
+    p[i] = 0.0;
+    v[0] = ri * zi;
     v[1] = ri * ri;
     v[2] = bi * bi;
   }
@@ -238,6 +256,7 @@ __global__ void __launch_bounds__(RT) s_init(int G, const double *part, CGScalar
   S->stall = 0;
   S->bb = t[2];
   S->tol2 = S->rtol2 * t[2];
+  S->alpha = 0.0;
   S->beta = 0.0;
   S->it = 0;
   S->nrepl = 0;
@@ -247,109 +266,92 @@ __global__ void __launch_bounds__(RT) s_init(int G, const double *part, CGScalar
                                                    : ST_RUN;
 }
 
-// p = z + beta p_old; q = A p; partials of p.q
-template <typename VT>
-__global__ void __launch_bounds__(NT)
-    k_spmv(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
-           const VT *__restrict__ sv, const VT *__restrict__ diag, const VT *__restrict__ z,
-           const VT *__restrict__ po, VT *__restrict__ pn, VT *__restrict__ q, double *part,
-           const CGScalars *S) {
-  if (S->status != ST_RUN) return;
-  const double beta = S->beta;
-  const int i = my_row();
-  double v[1] = {0.0};
-  if (i < npad) {
-    const double p_i = (double)z[i] + beta * (double)po[i];
-    const double s = (double)diag[i] * p_i +
-                     sell_dot<VT>(i, sp, sc, sv, [&](int c) {
-                       return (double)z[c] + beta * (double)po[c];
-                     });
-    const VT pv = (VT)p_i, qv = (VT)s;
-    pn[i] = pv;
-    q[i] = qv;
-    v[0] = (double)pv * (double)qv;
-  }
-  put_partials<1>(v, part);
+// The pending step and the next direction of one row (FP64 form):
+// z' = z - alpha w, p' = z' + beta p, with explicit fused multiply-adds so
+// that a row and its neighbours evaluate it identically.
+__device__ __forceinline__ double dir64(double z, double w, double p, double alpha,
+                                        double beta, double &zn) {
+  zn = fma(-alpha, w, z);
+  return fma(beta, p, zn);
 }
 
-__global__ void __launch_bounds__(RT) s_alpha(int G, const double *part, CGScalars *S) {
+// x += alpha p; z' = z - alpha w; p' = z' + beta p; q = A p'; w' = D^-1 q;
+// partials of r'.z', p'.q, r'.w' (= q.z'), q.w', r'.r' with r' = D z'
+__global__ void __launch_bounds__(NT)
+    k_step(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
+           const double *__restrict__ sv, const double *__restrict__ diag,
+           const double *__restrict__ dinv, const double *__restrict__ z,
+           const double *__restrict__ w, const double *__restrict__ p, double *__restrict__ zn,
+           double *__restrict__ wn, double *__restrict__ pn, double *__restrict__ x,
+           double *part, const CGScalars *S) {
   if (S->status != ST_RUN) return;
-  double t[1];
-  sum_partials<1>(part, G, t);
+  const double alpha = S->alpha, beta = S->beta;
+  const int i = my_row();
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < npad) {
+    const double pi = p[i];
+    double zi;
+    const double pni = dir64(z[i], w[i], pi, alpha, beta, zi);
+    x[i] = fma(alpha, pi, x[i]);
+    const double di = diag[i];
+    const double q = di * pni + sell_dot<double>(i, sp, sc, sv, [&](int c) {
+                       double unused;
+                       return dir64(z[c], w[c], p[c], alpha, beta, unused);
+                     });
+    const double wi = q * dinv[i];
+    zn[i] = zi;
+    pn[i] = pni;
+    wn[i] = wi;
+    const double ri = di * zi;
+    v[0] = ri * zi;
+    v[1] = pni * q;
+    v[2] = ri * wi;
+    v[3] = q * wi;
+    v[4] = ri * ri;
+  }
+  put_partials<5>(v, part);
+}
+
+__global__ void __launch_bounds__(RT) s_step(int G, const double *part, CGScalars *S,
+                                             int maxit) {
+  if (S->status != ST_RUN) return;
+  double t[5];
+  sum_partials<5>(part, G, t);
   if (threadIdx.x != 0) return;
-  if (!(t[0] > 0.0)) {  // not SPD, or NaN
-    S->status = (t[0] != t[0]) ? ST_NAN : ST_BREAK;
+  const double rz = t[0], pq = t[1], zq = t[2], qw = t[3], rr = t[4];
+  S->it += 1;
+  if (!(rr == rr) || !isfinite(rr) || !(rz == rz) || pq != pq) {
+    S->status = ST_NAN;
     return;
   }
-  S->alpha = S->rz / t[0];
-}
-
-// x += alpha p (FP64 x, or the FP32 correction xs in the mixed solve);
-// r -= alpha q; z = D^-1 r; partials of r.z, r.r
-template <typename VT, typename XT>
-__global__ void __launch_bounds__(NT)
-    k_update(int npad, const VT *__restrict__ p, const VT *__restrict__ q,
-             XT *__restrict__ x, VT *__restrict__ r, VT *__restrict__ z,
-             const VT *__restrict__ dinv, double *part, const CGScalars *S) {
-  if (S->status != ST_RUN) return;
-  const double alpha = S->alpha;
-  const int i = my_row();
-  double v[2] = {0.0, 0.0};
-  if (i < npad) {
-    const double pi = (double)p[i];
-    x[i] = (XT)((double)x[i] + alpha * pi);
-    const VT rv = (VT)((double)r[i] - alpha * (double)q[i]);
-    const VT zv = (VT)((double)rv * (double)dinv[i]);
-    r[i] = rv;
-    z[i] = zv;
-    v[0] = (double)rv * (double)zv;
-    v[1] = (double)rv * (double)rv;
+  S->rz = rz;
+  S->rr = rr;
+  if (rr <= S->tol2) {
+    S->status = ST_CONV;
+    return;
   }
-  put_partials<2>(v, part);
-}
-
-__global__ void __launch_bounds__(RT) s_beta(int G, const double *part, CGScalars *S,
-                                              int maxit) {
-  if (S->status != ST_RUN) return;
-  double t[2];
-  sum_partials<2>(part, G, t);
-  if (threadIdx.x != 0) return;
-  S->rz_prev = S->rz;
-  S->rz = t[0];
-  S->beta = t[0] / S->rz_prev;
-  S->it += 1;
-  if (!(t[0] == t[0]) || !isfinite(t[0])) {
-    S->status = ST_NAN;
-  } else if (S->mixed) {
-    // scaled form: r.r of the scaled residual is the monitor; a replacement
-    // when it fell by delta, stalled, or looks converged (below the
-    // tolerance times the scaled-to-true ratio of the last replacement)
-    if (t[0] < 0.98 * S->rz_min) {
-      S->rz_min = t[0];
-      S->stall = 0;
-    } else {
-      S->stall += 1;
-    }
-    if (t[0] * S->rr_ref <= S->tol2 * S->rz_ref || t[0] <= S->delta2 * S->rz_ref ||
-        S->stall >= STALL_ITERS)
-      S->replace = 1;
-    if (S->it >= maxit && !S->replace) S->status = ST_MAXIT;
-  } else {
-    S->rr = t[1];
-    if (t[1] <= S->tol2) S->status = ST_CONV;
-    else if (S->it >= maxit) S->status = ST_MAXIT;
+  if (S->it >= maxit) {
+    S->status = ST_MAXIT;
+    return;
   }
+  if (!(pq > 0.0)) {  // not SPD
+    S->status = ST_BREAK;
+    return;
+  }
+  const double alpha = rz / pq;
+  S->alpha = alpha;
+  S->beta = rz > 0.0 ? fmax(rz - 2.0 * alpha * zq + alpha * alpha * qw, 0.0) / rz : 0.0;
 }
 
 // ---- mixed precision, scaled form: plain CG on S A S (S = diag^-1/2,
 // unit diagonal) with FP32 storage, the solution x = x64 + S y
 
-// x0 = b on identity rows; r = S r0 (FP32); y = 0; p_old = 0; partials of
+// x0 = b on identity rows; r = S r0 (FP32); y = 0; p = q = 0; partials of
 // the scaled r.r, the true ||r0||^2 and b.b
 __global__ void __launch_bounds__(NT)
     k_minit(int n, int npad, const int *__restrict__ row_bd, const double *__restrict__ b,
             const double *__restrict__ dsq, double *__restrict__ x, float *__restrict__ ys,
-            float *__restrict__ r, float *__restrict__ po, double *part) {
+            float *__restrict__ r, float *__restrict__ p, float *__restrict__ q, double *part) {
   const int i = my_row();
   double v[3] = {0.0, 0.0, 0.0};
   if (i < npad) {
@@ -360,7 +362,8 @@ __global__ void __launch_bounds__(NT)
     ys[i] = 0.0f;
     const float rv = (float)(ri * dsq[i]);
     r[i] = rv;
-    po[i] = 0.0f;
+    p[i] = 0.0f;
+    q[i] = 0.0f;
     v[0] = (double)rv * (double)rv;
     v[1] = ri * ri;
     v[2] = bi * bi;
@@ -368,51 +371,96 @@ __global__ void __launch_bounds__(NT)
   put_partials<3>(v, part);
 }
 
-// p = r + beta p_old; q = (S A S) p (unit diagonal); partials of p.q
+// The pending step and the next direction of one row (scaled form):
+// r' = r - alpha q, p' = r' + beta p rounded as it is stored, with explicit
+// fused multiply-adds, so q' = Â p' is the product with the stored p'.
+__device__ __forceinline__ double dir32(float r, float q, float p, double alpha, double beta,
+                                        double &rn) {
+  rn = fma(-alpha, (double)q, (double)r);
+  return (double)(float)fma(beta, (double)p, rn);
+}
+
+// y += alpha p; r' = r - alpha q; p' = r' + beta p; q' = (S A S) p';
+// partials of r'.r', p'.q', r'.q', q'.q'
 __global__ void __launch_bounds__(NT)
-    k_mspmv(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
+    k_mstep(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
             const float *__restrict__ sv, const float *__restrict__ r,
-            const float *__restrict__ po, float *__restrict__ pn, float *__restrict__ q,
+            const float *__restrict__ q, const float *__restrict__ p, float *__restrict__ rn,
+            float *__restrict__ qn, float *__restrict__ pn, float *__restrict__ ys,
             double *part, const CGScalars *S) {
-  if (S->status != ST_RUN) return;
-  const double beta = S->beta;
+  if (S->status != ST_RUN || S->replace) return;
+  const double alpha = S->alpha, beta = S->beta;
   const int i = my_row();
-  double v[1] = {0.0};
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < npad) {
-    const double p_i = (double)r[i] + beta * (double)po[i];
-    const double a = p_i + sell_dot<float>(i, sp, sc, sv, [&](int c) {
-                       return (double)r[c] + beta * (double)po[c];
+    const float pi = p[i];
+    double ri;
+    const double pni = dir32(r[i], q[i], pi, alpha, beta, ri);
+    ys[i] = (float)fma(alpha, (double)pi, (double)ys[i]);
+    const double a = pni + sell_dot<float>(i, sp, sc, sv, [&](int c) {
+                       double unused;
+                       return dir32(r[c], q[c], p[c], alpha, beta, unused);
                      });
-    const float pv = (float)p_i, qv = (float)a;
+    const float rv = (float)ri, pv = (float)pni, qv = (float)a;
+    rn[i] = rv;
     pn[i] = pv;
-    q[i] = qv;
-    v[0] = (double)pv * (double)qv;
-  }
-  put_partials<1>(v, part);
-}
-
-// y += alpha p; r -= alpha q; partials of the scaled r.r
-__global__ void __launch_bounds__(NT)
-    k_mupdate(int npad, const float *__restrict__ p, const float *__restrict__ q,
-              float *__restrict__ ys, float *__restrict__ r, double *part, const CGScalars *S) {
-  if (S->status != ST_RUN) return;
-  const double alpha = S->alpha;
-  const int i = my_row();
-  double v[2] = {0.0, 0.0};
-  if (i < npad) {
-    ys[i] = (float)((double)ys[i] + alpha * (double)p[i]);
-    const float rv = (float)((double)r[i] - alpha * (double)q[i]);
-    r[i] = rv;
+    qn[i] = qv;
     v[0] = (double)rv * (double)rv;
+    v[1] = (double)pv * (double)qv;
+    v[2] = (double)rv * (double)qv;
+    v[3] = (double)qv * (double)qv;
   }
-  put_partials<2>(v, part);
+  put_partials<4>(v, part);
 }
 
-// when a replacement is pending (or forced at the end): x += S y, y = 0
+__global__ void __launch_bounds__(RT) s_mstep(int G, const double *part, CGScalars *S,
+                                              int maxit) {
+  if (S->status != ST_RUN || S->replace) return;
+  double t[4];
+  sum_partials<4>(part, G, t);
+  if (threadIdx.x != 0) return;
+  const double rz = t[0], pq = t[1], rq = t[2], qq = t[3];
+  S->it += 1;
+  if (!(rz == rz) || !isfinite(rz) || pq != pq) {
+    S->status = ST_NAN;
+    return;
+  }
+  S->rz = rz;
+  if (rz < 0.98 * S->rz_min) {
+    S->rz_min = rz;
+    S->stall = 0;
+  } else {
+    S->stall += 1;
+  }
+  // a scaled residual that looks converged -- below the tolerance times the
+  // scaled-to-true ratio of the last replacement -- freezes the solve until
+  // the next replacement point, where the FP64 residual decides
+  if (rz * S->rr_ref <= S->tol2 * S->rz_ref) S->replace = 1;
+  else if (S->it >= maxit) S->status = ST_MAXIT;
+  if (!(pq > 0.0)) {  // not SPD (or p = 0 with r = 0: the replacement decides)
+    if (!S->replace) S->status = ST_BREAK;
+    S->alpha = 0.0;
+    S->beta = 0.0;
+    return;
+  }
+  const double alpha = rz / pq;
+  S->alpha = alpha;
+  S->beta = rz > 0.0 ? fmax(rz - 2.0 * alpha * rq + alpha * alpha * qq, 0.0) / rz : 0.0;
+}
+
+// A replacement at a replacement point: when frozen (converged-looking),
+// when the scaled residual fell by delta since the last one, or when it
+// stalled.  The point's kernels read the same, unchanged scalars.
+__device__ __forceinline__ bool replace_due(const CGScalars *S) {
+  return S->status == ST_RUN &&
+         (S->replace || S->rz <= S->delta2 * S->rz_ref || S->stall >= STALL_ITERS);
+}
+
+// when a replacement is due (or forced at the end): x += S y, y = 0
 __global__ void __launch_bounds__(NT)
     k_mfold(int npad, double *__restrict__ x, float *__restrict__ ys,
             const double *__restrict__ dsq, const CGScalars *S, int force) {
-  if (!force && (S->status != ST_RUN || !S->replace)) return;
+  if (!force && !replace_due(S)) return;
   const int i = my_row();
   if (i < npad) {
     x[i] += dsq[i] * (double)ys[i];
@@ -420,16 +468,17 @@ __global__ void __launch_bounds__(NT)
   }
 }
 
-// when a replacement is pending: the true residual b - A x in FP64 (FP64
-// matrix), handed to the FP32 iteration as r = S (b - A x); partials of the
-// scaled r.r and the true ||b - A x||^2
+// when a replacement is due: the true residual b - A x in FP64 (FP64
+// matrix), handed to the FP32 iteration as r = S (b - A x) (the buffer the
+// next step reads); partials of the scaled r.r and the true ||b - A x||^2
 __global__ void __launch_bounds__(NT)
     k_mreplace(int n, int npad, const int *__restrict__ sp, const int *__restrict__ sc,
                const double *__restrict__ sv, const double *__restrict__ diag,
-               const double *__restrict__ b, const double *__restrict__ x,
-               const double *__restrict__ dsq, float *__restrict__ r, double *part,
-               const CGScalars *S) {
-  if (S->status != ST_RUN || !S->replace) return;
+               const double *__restrict__ x, const double *__restrict__ dsq, float *r0,
+               float *r1, double *part, const CGScalars *S) {
+  if (!replace_due(S)) return;
+  const double *__restrict__ b = S->b;
+  float *__restrict__ r = (S->it & 1) ? r1 : r0;
   const int i = my_row();
   double v[2] = {0.0, 0.0};
   if (i < npad) {
@@ -445,12 +494,12 @@ __global__ void __launch_bounds__(NT)
 
 __global__ void __launch_bounds__(RT) s_replace(int G, const double *part, CGScalars *S,
                                                  int maxit) {
-  if (S->status != ST_RUN || !S->replace) return;
+  if (!replace_due(S)) return;
   double t[2];
   sum_partials<2>(part, G, t);
+  __syncthreads();  // every thread has read the decision's inputs
   if (threadIdx.x != 0) return;
   S->rz = t[0];
-  S->beta = t[0] / S->rz_prev;
   S->rr = t[1];
   S->rr_ref = t[1];
   S->rz_ref = t[0];
@@ -496,19 +545,18 @@ static CGWork *ensure_work(System &s) {
     nb = (nb + 7) / 8 * 8;  // XCD mapping needs a multiple of 8
     w->nblocks = nb;
     w->x.alloc(s.npad);
-    w->r64.alloc(s.npad);
-    w->z64.alloc(s.npad);
-    w->pa64.alloc(s.npad);
-    w->pb64.alloc(s.npad);
-    w->q64.alloc(s.npad);
-    w->xs32.alloc(s.npad);
-    w->r32.alloc(s.npad);
+    for (int k = 0; k < 2; k++) {
+      w->z64[k].alloc(s.npad);
+      w->w64[k].alloc(s.npad);
+      w->p64[k].alloc(s.npad);
+      w->r32[k].alloc(s.npad);
+      w->p32[k].alloc(s.npad);
+      w->q32[k].alloc(s.npad);
+    }
+    w->ys32.alloc(s.npad);
     w->dsq.alloc(s.npad);
     w->sv32s.alloc(std::max(1, s.slice_ptr.back()));
-    w->pa32.alloc(s.npad);
-    w->pb32.alloc(s.npad);
-    w->q32.alloc(s.npad);
-    w->part.alloc((size_t)3 * nb);
+    w->part.alloc((size_t)5 * nb);
     w->h_part.resize(nb);
     w->sc.alloc(1);
     HIPCHECK(hipMemset(w->sc.p, 0, sizeof(CGScalars)));
@@ -521,31 +569,50 @@ static CGWork *ensure_work(System &s) {
   return w;
 }
 
-// Instrumentation: the solver's SpMV kernel launched `repeat` times on the
-// system's work vectors (z = 1, p_old = 0, status running): it runs its full
-// body every launch and changes none of its inputs.
+namespace {
+
+// One step kernel of the given precision reading buffer `cur` and writing
+// the other one.
+void launch_step(System &s, CGWork *w, bool mixed, int cur, hipStream_t st) {
+  const int nb = w->nblocks, np = s.npad, nx = cur ^ 1;
+  const int *sp = s.d_slice_ptr.p, *scol = s.d_sell_col.p;
+  if (mixed)
+    k_mstep<<<nb, NT, 0, st>>>(np, sp, scol, w->sv32s.p, w->r32[cur].p, w->q32[cur].p,
+                               w->p32[cur].p, w->r32[nx].p, w->q32[nx].p, w->p32[nx].p,
+                               w->ys32.p, w->part.p, w->sc.p);
+  else
+    k_step<<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val.p, s.d_diag.p, s.d_dinv.p,
+                              w->z64[cur].p, w->w64[cur].p, w->p64[cur].p, w->z64[nx].p,
+                              w->w64[nx].p, w->p64[nx].p, w->x.p, w->part.p, w->sc.p);
+}
+
+}  // namespace
+
+// Instrumentation: the solver's step kernel (the SpMV with the iteration's
+// vector updates) launched `repeat` times on the system's work vectors
+// (r or z = 1, everything else 0, alpha = beta = 0, status running): every
+// launch reads the same buffers, runs its full body and writes the same
+// values to the other buffers.
 void spmv_repeat(System &s, int precision, int repeat, hipStream_t st) {
   CGWork *w = ensure_work(s);
-  const int nb = w->nblocks;
   CGScalars init{};
   HIPCHECK(hipMemcpyAsync(w->sc.p, &init, sizeof(CGScalars), hipMemcpyHostToDevice, st));
   HIPCHECK(hipStreamSynchronize(st));  // init lives on this stack frame
   const unsigned fb = (unsigned)((s.npad + 255) / 256);
-  if (precision == ADMFEM_PREC_MIXED) {  // the scaled-form kernel of the mixed solve
+  const bool mixed = precision == ADMFEM_PREC_MIXED;
+  if (mixed) {  // the scaled-form kernel of the mixed solve
     batch_scale(s, s.d_sell_val.p, s.d_diag.p, w->dsq.p, w->sv32s.p, st);
-    k_fill<float><<<fb, 256, 0, st>>>(w->r32.p, s.npad, 1.0f);
-    k_fill<float><<<fb, 256, 0, st>>>(w->pa32.p, s.npad, 0.0f);
-    for (int k = 0; k < repeat; k++)
-      k_mspmv<<<nb, NT, 0, st>>>(s.npad, s.d_slice_ptr.p, s.d_sell_col.p, w->sv32s.p,
-                                 w->r32.p, w->pa32.p, w->pb32.p, w->q32.p, w->part.p, w->sc.p);
+    k_fill<float><<<fb, 256, 0, st>>>(w->r32[0].p, s.npad, 1.0f);
+    k_fill<float><<<fb, 256, 0, st>>>(w->q32[0].p, s.npad, 0.0f);
+    k_fill<float><<<fb, 256, 0, st>>>(w->p32[0].p, s.npad, 0.0f);
+    k_fill<float><<<fb, 256, 0, st>>>(w->ys32.p, s.npad, 0.0f);
   } else {
-    k_fill<double><<<fb, 256, 0, st>>>(w->z64.p, s.npad, 1.0);
-    k_fill<double><<<fb, 256, 0, st>>>(w->pa64.p, s.npad, 0.0);
-    for (int k = 0; k < repeat; k++)
-      k_spmv<double><<<nb, NT, 0, st>>>(s.npad, s.d_slice_ptr.p, s.d_sell_col.p,
-                                        s.d_sell_val.p, s.d_diag.p, w->z64.p, w->pa64.p,
-                                        w->pb64.p, w->q64.p, w->part.p, w->sc.p);
+    k_fill<double><<<fb, 256, 0, st>>>(w->z64[0].p, s.npad, 1.0);
+    k_fill<double><<<fb, 256, 0, st>>>(w->w64[0].p, s.npad, 0.0);
+    k_fill<double><<<fb, 256, 0, st>>>(w->p64[0].p, s.npad, 0.0);
+    k_fill<double><<<fb, 256, 0, st>>>(w->x.p, s.npad, 0.0);
   }
+  for (int k = 0; k < repeat; k++) launch_step(s, w, mixed, 0, st);
   HIPCHECK(hipGetLastError());
 }
 
@@ -553,36 +620,26 @@ namespace {
 
 constexpr int GRAPH_ITERS = 8;  // a multiple of the replacement interval (8)
 
-// One iteration with index `it` (its parity picks the direction buffers;
-// the mixed form checks for a replacement after every eighth iteration and
-// after the last one allowed).
-void enqueue_iteration(System &s, CGWork *w, bool mixed, const double *b, int max_iter, int it,
+// One iteration with index `it` (its parity picks the buffers; the mixed
+// form has a replacement point after every eighth iteration and after the
+// last one allowed).
+void enqueue_iteration(System &s, CGWork *w, bool mixed, int max_iter, int it,
                        hipStream_t st) {
   const int nb = w->nblocks, np = s.npad;
-  const int *sp = s.d_slice_ptr.p, *scol = s.d_sell_col.p;
   CGScalars *S = w->sc.p;
   double *part = w->part.p;
-  const bool odd = it & 1;
+  launch_step(s, w, mixed, it & 1, st);
   if (mixed) {
-    float *po = odd ? w->pb32.p : w->pa32.p, *pn = odd ? w->pa32.p : w->pb32.p;
-    k_mspmv<<<nb, NT, 0, st>>>(np, sp, scol, w->sv32s.p, w->r32.p, po, pn, w->q32.p, part, S);
-    s_alpha<<<1, RT, 0, st>>>(nb, part, S);
-    k_mupdate<<<nb, NT, 0, st>>>(np, pn, w->q32.p, w->xs32.p, w->r32.p, part, S);
-    s_beta<<<1, RT, 0, st>>>(nb, part, S, max_iter);
-    if ((it & 7) == 7 || it + 1 == max_iter) {  // replacement check every 8
-      k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, w->dsq.p, S, 0);
-      k_mreplace<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b, w->x.p,
-                                    w->dsq.p, w->r32.p, part, S);
+    s_mstep<<<1, RT, 0, st>>>(nb, part, S, max_iter);
+    if ((it & 7) == 7 || it + 1 == max_iter) {
+      k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->ys32.p, w->dsq.p, S, 0);
+      k_mreplace<<<nb, NT, 0, st>>>(s.n, np, s.d_slice_ptr.p, s.d_sell_col.p, s.d_sell_val.p,
+                                    s.d_diag.p, w->x.p, w->dsq.p, w->r32[0].p, w->r32[1].p,
+                                    part, S);
       s_replace<<<1, RT, 0, st>>>(nb, part, S, max_iter);
     }
   } else {
-    double *po = odd ? w->pb64.p : w->pa64.p, *pn = odd ? w->pa64.p : w->pb64.p;
-    k_spmv<double><<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val.p, s.d_diag.p, w->z64.p, po,
-                                      pn, w->q64.p, part, S);
-    s_alpha<<<1, RT, 0, st>>>(nb, part, S);
-    k_update<double, double><<<nb, NT, 0, st>>>(np, pn, w->q64.p, w->x.p, w->r64.p, w->z64.p,
-                                                s.d_dinv.p, part, S);
-    s_beta<<<1, RT, 0, st>>>(nb, part, S, max_iter);
+    s_step<<<1, RT, 0, st>>>(nb, part, S, max_iter);
   }
 }
 
@@ -590,12 +647,10 @@ void enqueue_iteration(System &s, CGWork *w, bool mixed, const double *b, int ma
 // GRAPH_ITERS with it + GRAPH_ITERS <= max_iter: the nodes are those of
 // iterations 0 .. GRAPH_ITERS - 1 (same parities, and the last one allowed,
 // if among them, is a replacement point anyway).
-void launch_graph(System &s, CGWork *w, bool mixed, const double *b, int max_iter,
-                  hipStream_t st) {
+void launch_graph(System &s, CGWork *w, bool mixed, int max_iter, hipStream_t st) {
   const std::vector<const void *> key = {
-      (const void *)(intptr_t)mixed, (const void *)(intptr_t)max_iter,
-      mixed ? (const void *)b : nullptr, s.d_slice_ptr.p, s.d_sell_col.p, s.d_sell_val.p,
-      s.d_diag.p, s.d_dinv.p, (const void *)(intptr_t)s.n};
+      (const void *)(intptr_t)mixed, (const void *)(intptr_t)max_iter, s.d_slice_ptr.p,
+      s.d_sell_col.p, s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, (const void *)(intptr_t)s.n};
   if (!w->graph || w->gkey != key) {
     if (w->graph) {
       HIPCHECK(hipGraphExecDestroy(w->graph));
@@ -604,7 +659,7 @@ void launch_graph(System &s, CGWork *w, bool mixed, const double *b, int max_ite
     if (!w->cap) HIPCHECK(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
     hipGraph_t g;
     HIPCHECK(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < GRAPH_ITERS; k++) enqueue_iteration(s, w, mixed, b, max_iter, k, w->cap);
+    for (int k = 0; k < GRAPH_ITERS; k++) enqueue_iteration(s, w, mixed, max_iter, k, w->cap);
     HIPCHECK(hipStreamEndCapture(w->cap, &g));
     hipGraphExec_t ex;
     const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -631,17 +686,18 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
   const double delta = o.replace_delta > 0 ? o.replace_delta : REPLACE_DELTA;
   init.delta2 = delta * delta;
   init.mixed = mixed ? 1 : 0;
+  init.b = b;
   HIPCHECK(hipMemcpyAsync(w->sc.p, &init, sizeof(CGScalars), hipMemcpyHostToDevice, st));
   CGScalars *S = w->sc.p;
   double *part = w->part.p;
   const int *sp = s.d_slice_ptr.p, *scol = s.d_sell_col.p;
   if (mixed) {  // scaled form: S A S from the FP64 matrix, then the FP32 iteration
     batch_scale(s, s.d_sell_val.p, s.d_diag.p, w->dsq.p, w->sv32s.p, st);
-    k_minit<<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, w->dsq.p, w->x.p, w->xs32.p,
-                               w->r32.p, w->pa32.p, part);
+    k_minit<<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, w->dsq.p, w->x.p, w->ys32.p,
+                               w->r32[0].p, w->p32[0].p, w->q32[0].p, part);
   } else {
-    k_init<double><<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, s.d_dinv.p, w->x.p, nullptr,
-                                      w->r64.p, w->z64.p, w->pa64.p, part);
+    k_init<<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, s.d_dinv.p, w->x.p, w->z64[0].p,
+                              w->w64[0].p, w->p64[0].p, part);
   }
   s_init<<<1, RT, 0, st>>>(nb, part, S);
   HIPCHECK(hipGetLastError());
@@ -653,10 +709,10 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
     const int stop = std::min(it + chunk, o.max_iter);
     while (it < stop) {
       if (it % GRAPH_ITERS == 0 && it + GRAPH_ITERS <= stop) {
-        launch_graph(s, w, mixed, b, o.max_iter, st);
+        launch_graph(s, w, mixed, o.max_iter, st);
         it += GRAPH_ITERS;
       } else {
-        enqueue_iteration(s, w, mixed, b, o.max_iter, it, st);
+        enqueue_iteration(s, w, mixed, o.max_iter, it, st);
         it++;
       }
     }
@@ -684,7 +740,7 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
   }
   const bool converged = w->h_sc->status == ST_CONV;
   if (mixed && !converged)  // stopped without a final replacement
-    k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->xs32.p, w->dsq.p, S, 1);
+    k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->ys32.p, w->dsq.p, S, 1);
   k_true_residual<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b, w->x.p,
                                      part);
   HIPCHECK(hipGetLastError());

@@ -309,6 +309,7 @@ void spmv_repeat(System &s, int precision, int repeat, hipStream_t st);
 void sparse_solve(const long long *ind, const double *vv, int64_t nnz, int n, const double *rhs,
                   double *u, const admfem_solve_opts &o, admfem_solve_stats &stats,
                   hipStream_t st);
+void sparse_solve_info(int64_t info[4]);
 void sparse_solve_grad(const long long *ind, const double *vv, int64_t nnz, int n,
                        const double *u, const double *grad_u, double *grad_vv,
                        double *grad_rhs, const admfem_solve_opts &o, admfem_solve_stats &stats,

@@ -92,7 +92,7 @@ struct Req {
   double2 *xy;
   double *x, *y, *kappa, *f, *uprev, *b, *sv64, *diag64, *dsq, *xa, *xb, *u64, *lam64,
       *grad, *obsv, *loss2, *part, *stage;
-  float *sv32, *ysa, *ysb, *r32, *pa, *pb, *q32;
+  float *sv32, *ysa, *ysb, *ra, *rb, *pa, *pb, *qa, *qb;
   int *obsi;
   int nobs = 0;
 };
@@ -159,15 +159,15 @@ struct Server {
            *u64 = D(np), *lam64 = D(np), *grad = D(ng), *obsv = D(nobs + 1), *loss2 = D(2),
            *part = D(4 * nblk), *stage = D(2 * np);
     double2 *xy = (double2 *)take(np * sizeof(double2));
-    float *sv32 = F(ns), *ysa = F(np), *ysb = F(np), *r32 = F(np), *pa = F(np), *pb = F(np),
-          *q32 = F(np);
+    float *sv32 = F(ns), *ysa = F(np), *ysb = F(np), *ra = F(np), *rb = F(np), *pa = F(np),
+          *pb = F(np), *qa = F(np), *qb = F(np);
     int *obsi = (int *)take((nobs + 1) * sizeof(int));
     if (q) {
       q->x = x; q->y = y; q->xy = xy; q->kappa = kappa; q->f = f; q->uprev = uprev; q->b = b;
       q->sv64 = sv64; q->diag64 = diag64; q->dsq = dsq; q->xa = xa; q->xb = xb;
       q->u64 = u64; q->lam64 = lam64; q->grad = grad; q->obsv = obsv; q->loss2 = loss2;
       q->part = part; q->stage = stage; q->sv32 = sv32; q->ysa = ysa; q->ysb = ysb;
-      q->r32 = r32; q->pa = pa; q->pb = pb; q->q32 = q32; q->obsi = obsi;
+      q->ra = ra; q->rb = rb; q->pa = pa; q->pb = pb; q->qa = qa; q->qb = qb; q->obsi = obsi;
     }
     return off;
   }
@@ -197,10 +197,12 @@ struct Server {
     d.x[1] = q.xb;
     d.ys[0] = q.ysa;
     d.ys[1] = q.ysb;
-    d.r = q.r32;
+    d.r[0] = q.ra;
+    d.r[1] = q.rb;
     d.p[0] = q.pa;
     d.p[1] = q.pb;
-    d.q = q.q32;
+    d.q[0] = q.qa;
+    d.q[1] = q.qb;
     d.part = q.part;
     d.n = s.n;
     d.npad = s.npad;
@@ -250,8 +252,7 @@ struct Server {
     }
     const hipMemcpyKind kind =
         r.source == ADMFEM_SRC_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-    REQUIRE(r.kappa && r.f && (!heat || r.u_prev) && (r.nobs == 0 || (r.obs_idx && r.obs_val)),
-            ADMFEM_ERR_ARG, "request inputs missing");
+    // (the inputs' presence was checked by server_submit)
     // user-order inputs are staged, then permuted to the internal order
     if (r.coords) {
       HIPCHECK(hipMemcpyAsync(q.stage, r.coords, sizeof(double) * 2 * s.n, kind, st));
@@ -321,9 +322,18 @@ struct Server {
     q.slot = free_slots.back();
     free_slots.pop_back();
     q.res.t_admit = t;
-    assemble(q);
-    write_desc(q, q.b);
-    start_cg(q);
+    try {
+      assemble(q);
+      write_desc(q, q.b);
+      start_cg(q);
+    } catch (const Error &e) {
+      // the request fails alone: its memory and slot go back to the queue,
+      // the requests behind it are admitted as usual
+      (void)hipStreamSynchronize(st);
+      set_error(e.msg);
+      fail(q, e.code == ADMFEM_ERR_HIP ? ADMFEM_ERR_HIP : ADMFEM_ERR_ARG);
+      return true;
+    }
     q.state = Q_FWD;
     return true;
   }
@@ -577,42 +587,13 @@ struct Server {
       q.slot = 0;
       q.res.t_admit = t;
       peak_active = std::max(peak_active, 1);
-      // FP64 single-system path on the topology's own solver buffers
-      inputs(q);
-      admfem_problem p = problem(q);
-      launch_assemble_to(s, q.xy, p, SellOut{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p},
-                         st);
-      admfem_solve_opts so{ADMFEM_PREC_FP64, o.max_iter, o.rtol, 0, 0.0};
-      admfem_solve_stats s0{}, s1{};
-      cg_solve(s, s.rhs.p, q.u64, so, s0, st);
-      launch_loss(s, q.u64, q.obsi, q.obsv, q.nobs, q.b, q.loss2, st);
-      cg_solve(s, q.b, q.lam64, so, s1, st);
-      q.res.iters_fwd = s0.iterations;
-      q.res.iters_adj = s1.iterations;
-      q.res.relres_fwd = s0.rel_residual;
-      q.res.relres_adj = s1.rel_residual;
-      iters += s0.iterations + s1.iterations;
-      steps += 1;
-      batch_sum += 1;
-      q.res.status = s0.status != ADMFEM_OK ? s0.status : s1.status;
-      if (q.res.status != ADMFEM_OK) {
-        fail_seq(q);
-        continue;
+      try {
+        serve_seq(q, s);
+      } catch (const Error &e) {
+        (void)hipStreamSynchronize(st);
+        set_error(e.msg);
+        q.res.status = e.code == ADMFEM_ERR_HIP ? ADMFEM_ERR_HIP : ADMFEM_ERR_ARG;
       }
-      const Mesh &m = *s.mesh;
-      launch_kappa_gradient_to(s, q.xy, q.u64, q.lam64, p.lap_coef, q.grad, st);
-      if (q.r.u_out) {
-        launch_scatter(m.d_iperm.p, q.u64, q.stage, s.n, st);
-        HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
-                                st));
-      }
-      if (q.r.grad_out)
-        HIPCHECK(hipMemcpyAsync(q.r.grad_out, q.grad, sizeof(double) * m.ngauss,
-                                hipMemcpyDeviceToHost, st));
-      double l2[2];
-      HIPCHECK(hipMemcpyAsync(l2, q.loss2, sizeof(l2), hipMemcpyDeviceToHost, st));
-      HIPCHECK(hipStreamSynchronize(st));
-      q.res.loss = l2[0];
       q.res.t_done = now_s(t0);
       q.res.done = 1;
       q.state = Q_DONE;
@@ -621,19 +602,52 @@ struct Server {
     wall = now_s(t0);
   }
 
-  void fail_seq(Req &q) {
+  // One request on the FP64 single-system path, on the topology's own solver
+  // buffers (the request's arena range holds its inputs and outputs).
+  void serve_seq(Req &q, System &s) {
+    inputs(q);
+    admfem_problem p = problem(q);
+    launch_assemble_to(s, q.xy, p, SellOut{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p},
+                       st);
+    admfem_solve_opts so{ADMFEM_PREC_FP64, o.max_iter, o.rtol, 0, 0.0};
+    admfem_solve_stats s0{}, s1{};
+    cg_solve(s, s.rhs.p, q.u64, so, s0, st);
+    launch_loss(s, q.u64, q.obsi, q.obsv, q.nobs, q.b, q.loss2, st);
+    cg_solve(s, q.b, q.lam64, so, s1, st);
+    q.res.iters_fwd = s0.iterations;
+    q.res.iters_adj = s1.iterations;
+    q.res.relres_fwd = s0.rel_residual;
+    q.res.relres_adj = s1.rel_residual;
+    iters += s0.iterations + s1.iterations;
+    steps += 1;
+    batch_sum += 1;
+    q.res.status = s0.status != ADMFEM_OK ? s0.status : s1.status;
+    if (q.res.status != ADMFEM_OK) {
+      HIPCHECK(hipStreamSynchronize(st));
+      return;
+    }
+    const Mesh &m = *s.mesh;
+    launch_kappa_gradient_to(s, q.xy, q.u64, q.lam64, p.lap_coef, q.grad, st);
+    if (q.r.u_out) {
+      launch_scatter(m.d_iperm.p, q.u64, q.stage, s.n, st);
+      HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
+                              st));
+    }
+    if (q.r.grad_out)
+      HIPCHECK(hipMemcpyAsync(q.r.grad_out, q.grad, sizeof(double) * m.ngauss,
+                              hipMemcpyDeviceToHost, st));
+    double l2[2];
+    HIPCHECK(hipMemcpyAsync(l2, q.loss2, sizeof(l2), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    q.res.done = 1;
-    q.res.t_done = now_s(t0);
-    q.state = Q_DONE;
-    arena.release(q.off, layout(*topos[q.r.topo].sys, nullptr, nullptr, q.nobs));
+    q.res.loss = l2[0];
+    if (l2[1] != 0.0) q.res.status = ADMFEM_ERR_ARG;  // bad or repeated obs index
   }
 };
 
-// Instrumentation: the batched SpMV kernel over a batch of synthetic Poisson
-// systems.  bk_spmv reads z and p_old and writes p_new and q only, and the
-// scalars it reads (beta, it, status) stay as batch_init left them, so every
-// launch does the same work.
+// Instrumentation: the batched CG step kernel over a batch of synthetic
+// Poisson systems.  The scalars it reads (alpha = beta = 0, it, status) stay
+// as batch_init left them, so every launch reads the same buffers, does its
+// full work and writes the same values to the other buffers.
 double server_spmv_repeat(Server *sv, const int *topos, int nsys, int repeat,
                           hipStream_t st, double *seconds) {
   REQUIRE(nsys <= (int)sv->free_slots.size(), ADMFEM_ERR_ARG, "more systems than free slots");
@@ -664,7 +678,9 @@ double server_spmv_repeat(Server *sv, const int *topos, int nsys, int repeat,
       admitted++;
       nmap += sv->topos[q.r.topo].nblk;
       const System &s = *sv->topos[q.r.topo].sys;
-      bytes += (double)s.col.size() * 8.0 + 4.0 * (s.n + 1) + 2.0 * s.n * 4.0;
+      // one CG iteration of the canonical CSR form: the SpMV's
+      // nnz (s + 4) + 4 (n + 1) + 2 n s plus 6 n s of vector updates (s = 4)
+      bytes += (double)s.col.size() * 8.0 + 4.0 * (s.n + 1) + 8.0 * s.n * 4.0;
     }
     sv->ensure_map(nmap, (size_t)nsys);
     std::vector<std::pair<int, int>> slots(nsys);
@@ -747,6 +763,25 @@ int64_t server_submit(Server *sv, const admfem_request &r) {
   REQUIRE(r.source >= 0 && r.source <= 2, ADMFEM_ERR_ARG, "bad request source");
   REQUIRE(r.kind == 0 || r.dt > 0, ADMFEM_ERR_ARG, "heat step needs dt > 0");
   REQUIRE(r.nobs >= 0, ADMFEM_ERR_ARG, "bad nobs");
+  if (r.source != ADMFEM_SRC_SYNTH) {
+    // every input the request reads is checked here, before it can hold a
+    // slot or arena memory
+    REQUIRE(r.kappa && r.f, ADMFEM_ERR_ARG, "request needs kappa and f");
+    REQUIRE(r.kind == 0 || r.u_prev, ADMFEM_ERR_ARG, "heat step needs u_prev");
+    REQUIRE(r.nobs == 0 || (r.obs_idx && r.obs_val), ADMFEM_ERR_ARG,
+            "observations need obs_idx and obs_val");
+    if (r.source == ADMFEM_SRC_HOST && r.nobs > 0) {
+      // host indices: range and distinctness checked now (device indices
+      // are checked by the loss kernel, which fails the request)
+      const int n = sv->topos[r.topo].sys->n;
+      std::vector<int> idx(r.obs_idx, r.obs_idx + r.nobs);
+      for (int v : idx)
+        REQUIRE(v >= 0 && v < n, ADMFEM_ERR_ARG, "observation index out of range [0, n)");
+      std::sort(idx.begin(), idx.end());
+      REQUIRE(std::adjacent_find(idx.begin(), idx.end()) == idx.end(), ADMFEM_ERR_ARG,
+              "observation indices must be distinct");
+    }
+  }
   Req q;
   q.r = r;
   q.res.t_arrival = r.arrival;

@@ -131,14 +131,23 @@ int64_t bandwidth(const std::vector<int> &rp, const std::vector<int> &col,
 
 // RCM order of the CSR pattern (rp, col) if the caller's order is not
 // banded already (bandwidth at most 5x the widest level of a breadth-first
-// search from row 0) and RCM has the smaller bandwidth; then (rp, col)
-// become the reordered pattern and the order, its inverse and each slot's
-// new place are uploaded to p.
+// search from the first row coupled to another row) and RCM has the smaller
+// bandwidth; then (rp, col) become the reordered pattern and the order, its
+// inverse and each slot's new place are uploaded to p.  The search does not
+// start at row 0: after Dirichlet elimination that is usually a boundary
+// node holding only its diagonal, whose one-node "level" would make every
+// order look unbanded.
 void reorder(CooPattern &p, int n, std::vector<int> &rp, std::vector<int> &col) {
   std::vector<int> id(n);
   std::iota(id.begin(), id.end(), 0);
   const int64_t bw_id = bandwidth(rp, col, id);
-  if (bw_id <= 5 * bfs_max_level(n, rp, col, 0)) return;
+  int start = 0;
+  for (int i = 0; i < n; i++)
+    if (std::any_of(col.begin() + rp[i], col.begin() + rp[i + 1], [i](int c) { return c != i; })) {
+      start = i;
+      break;
+    }
+  if (bw_id <= 5 * bfs_max_level(n, rp, col, start)) return;
   std::vector<int> ord = rcm_order(n, rp, col), inv(n);
   for (int k = 0; k < n; k++) inv[ord[k]] = k;
   if (bandwidth(rp, col, inv) >= bw_id) return;
@@ -300,6 +309,14 @@ void sparse_solve(const long long *ind, const double *vv, int64_t nnz, int n, co
   std::lock_guard<std::mutex> g(g_lock);
   pattern_with_values(ind, vv, nnz, n, st);
   solve_kept(*g_kept, rhs, u, o, stats, st);
+}
+
+void sparse_solve_info(int64_t info[4]) {
+  std::lock_guard<std::mutex> g(g_lock);
+  info[0] = g_kept ? g_kept->n : 0;
+  info[1] = g_kept ? g_kept->nnz : 0;
+  info[2] = g_kept ? g_kept->nslot : 0;
+  info[3] = g_kept && g_kept->reordered ? 1 : 0;
 }
 
 void sparse_solve_grad(const long long *ind, const double *vv, int64_t nnz, int n,

@@ -97,6 +97,20 @@ void DofToGaussPointsMfem_forward_Julia(double *out, const double *u);
  * (src/MFEM/MCore.jl:744-745): out [ndof] += K(nu) u, ACCUMULATED (the
  * caller zeroes it). */
 void ComputeLaplaceTermMfem_forward_Julia(double *out, const double *nu, const double *u);
+/* deps/MFEM/FemLaplace1/FemLaplaceScalar.h:65-91 (src/pcl.jl:35-39): the
+ * Jacobian of FemLaplaceScalar's values with respect to kappa, H [ngauss x
+ * nd*nd*ngauss] column-major, H[s + t*ngauss] = d vv[t] / d kappa[s].  Only
+ * the nonzeros (t in the s-th run of nd*nd triplets) are written; the caller
+ * zeroes H, as the reference's does. */
+void pcl_FemLaplaceScalar_Jacobian(double *H);
+/* deps/MFEM/ImposeDirichlet/ImposeDirichlet.h:98-111 (src/pcl.jl:14-21):
+ * the Jacobian of ImposeDirichlet's output values with respect to its input
+ * values, J [sN x outdof] column-major: J[k + s*sN] = 1 for the s-th kept
+ * triplet k (neither index a boundary node).  indices: 1-based [sN x 2]
+ * column-major; bd: 1-based [bdN].  Only the ones are written (the caller
+ * zeroes J).  Host-side index bookkeeping: no device work. */
+void pcl_ImposeDirichlet(double *J, const long long *indices, const long long *bd, int bdN,
+                         int sN);
 /* The default mesh as a handle, for sections B-E (NULL if none). */
 admfem_mesh admfem_default_mesh(void);
 
@@ -315,7 +329,8 @@ int admfem_system_kappa_gradient(admfem_system s, const double *u,
 
 /* One forward + adjoint solve for L = sum_k (u[obs_idx[k]] - obs_val[k])^2:
  * assemble, solve A u = rhs, g = dL/du, solve A lam = g, grad_kappa.
- * obs_idx: device int32 [nobs], 0-based, distinct.  Outputs u [n], lam [n]
+ * obs_idx: device int32 [nobs], 0-based, distinct (an index outside [0, n)
+ * or repeated fails the call with ADMFEM_ERR_ARG).  Outputs u [n], lam [n]
  * (may be NULL), grad_kappa [ngauss]; *loss (host) receives L.  Stats for
  * the forward and adjoint solves in st[0], st[1]. */
 int admfem_system_forward_adjoint(admfem_system s, const admfem_problem *p,
@@ -351,11 +366,20 @@ int admfem_sparse_solve_grad(const long long *indices, const double *vv,
                              double *grad_rhs, const admfem_solve_opts *o,
                              admfem_solve_stats *st, void *stream);
 
-/* Instrumentation (no reference counterpart): launches the solver's SpMV
- * kernel (FP64 or FP32 matrix per `precision`) `repeat` times back to back on
- * `stream`, on the system's work vectors, for timing the dominant kernel.
- * *bytes (host, may be NULL) receives its algorithmic bytes per launch, the
- * canonical CSR figure nnz*(s+4) + 4*(n+1) + 2*n*s with s the value size. */
+/* Instrumentation (no reference counterpart): the pattern the general
+ * sparse solve keeps (the last one built on any device): info[0..3] = n,
+ * triplets it was built from, distinct (row, col) slots, and 1 when the
+ * solver works in reverse Cuthill-McKee order of it (0: the caller's order,
+ * kept when it is already banded).  Zeros before the first solve. */
+int admfem_sparse_solve_info(int64_t info[4]);
+
+/* Instrumentation (no reference counterpart): launches the solver's step
+ * kernel (the SpMV with the iteration's vector updates fused in; FP64 or
+ * FP32 matrix per `precision`) `repeat` times back to back on `stream`, on
+ * the system's work vectors, for timing the dominant kernel.  *bytes (host,
+ * may be NULL) receives its algorithmic bytes per launch, one canonical CSR
+ * CG iteration: nnz*(s+4) + 4*(n+1) + 2*n*s for the SpMV plus 6*n*s for the
+ * vector updates, with s the value size. */
 int admfem_system_spmv_repeat(admfem_system s, int precision, int repeat,
                               double *bytes, void *stream);
 
@@ -404,7 +428,10 @@ typedef struct {
   int source;              /* ADMFEM_SRC_* */
   /* HOST / DEVICE inputs (user node order; must stay valid until the
    * request is admitted): coords nnode x 2 (NULL: the topology's own),
-   * kappa/f [ngauss], u_prev [nnode] (heat), obs_idx (0-based) / obs_val */
+   * kappa/f [ngauss], u_prev [nnode] (heat), obs_idx (0-based, distinct) /
+   * obs_val.  Missing inputs and, for HOST inputs, indices outside
+   * [0, nnode) or repeated fail admfem_server_submit with ADMFEM_ERR_ARG;
+   * for DEVICE inputs the request finishes with status ADMFEM_ERR_ARG. */
   const double *coords, *kappa, *f, *u_prev;
   const int *obs_idx;
   const double *obs_val;
@@ -450,10 +477,12 @@ int admfem_server_stats(admfem_server sv, double stats[8]);
 
 /* Instrumentation (no reference counterpart): admits one synthetic Poisson
  * request (seed i) on topology topos[i] for i < nsys, as a batch, launches
- * the batched SpMV kernel over all of them `repeat` times back to back on
- * `stream`, then releases them.  *bytes (may be NULL) receives the
- * algorithmic bytes per launch: the sum over the systems of the canonical
- * CSR figure nnz*(4+4) + 4*(n+1) + 2*n*4 (FP32 values).  *seconds (may be
+ * the batched CG step kernel (the SpMV with the iteration's vector updates
+ * fused in) over all of them `repeat` times back to back on `stream`, then
+ * releases them.  *bytes (may be NULL) receives the algorithmic bytes per
+ * launch: the sum over the systems of one canonical CSR CG iteration,
+ * nnz*(4+4) + 4*(n+1) + 2*n*4 for the SpMV plus 6*n*4 for the vector
+ * updates (FP32 values).  *seconds (may be
  * NULL) receives the average time per launch, from HIP events recorded on
  * `stream` right before the first and after the last launch.  Not to be
  * called while admfem_server_run is running on the same server. */

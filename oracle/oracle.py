@@ -264,6 +264,37 @@ def dirichlet_backward(grad_ov, grad_orhs, indices, vv, bd, bdval, N):
     return gv, gr, gb
 
 
+def pcl_impose_dirichlet(indices, bd, outdof):
+    """pcl_ImposeDirichlet (deps/MFEM/ImposeDirichlet/ImposeDirichlet.h:98-111),
+    called as src/pcl.jl:14-21 does: ``indices`` 1-based [sN, 2], ``bd``
+    1-based.  J [sN, outdof]: J[k, s] = 1 for the s-th triplet k whose row
+    and column are both off the boundary."""
+    ind = _i64(indices).reshape(-1, 2)
+    bds = set(int(b) - 1 for b in _i64(bd))
+    J = np.zeros((ind.shape[0], outdof))
+    s = 0
+    for k in range(ind.shape[0]):
+        if int(ind[k, 0]) - 1 not in bds and int(ind[k, 1]) - 1 not in bds:
+            J[k, s] = 1.0
+            s += 1
+    return J
+
+
+def pcl_laplace_jacobian(mesh):
+    """pcl_FemLaplaceScalar_Jacobian (deps/MFEM/FemLaplace1/FemLaplaceScalar.h:65-91):
+    H [ngauss, nd^2 ngauss], H[s, t] = D D^T w of triplet t's Gauss point s
+    (t in the s-th run of nd^2 triplets), zero elsewhere.  The reference's
+    loop is the forward's (h:7-25) without the kappa factor; here that
+    forward is evaluated at kappa = 1."""
+    ng = mesh.ngauss
+    _, vv = mesh.laplace_coo(np.ones(ng))
+    nd2 = vv.size // ng
+    H = np.zeros((ng, vv.size))
+    t = np.arange(vv.size)
+    H[t // nd2, t] = vv
+    return H
+
+
 def coo_to_csr(indices, vv, N):
     """Sum duplicates in triplet order, keep explicit zeros: the matrix that
     ``sparse(I,J,V)`` (src/MFEM/MCore.jl:118-119) / Eigen ``setFromTriplets``

@@ -8,6 +8,11 @@
 * 512x512 differentiation form: the mixed-precision dL/dkappa against central
   finite differences (step 1e-6) in 20 seeded directions and against the
   oracle's gradient chain.
+* 2048x2048 precision form (8.4M triangles, 4.2M nodes): mixed forward +
+  adjoint at rtol 1e-10 checked through size-independent properties -- the
+  closed-form post-BC nnz, symmetry, the true FP64 residuals of u and lambda
+  recomputed on the host with SciPy from the assembled matrix, agreement
+  with the plain FP64 path, and central differences of the loss.
 """
 import os
 
@@ -204,3 +209,96 @@ def test_twoholes_poisson_example_matches_golden(prec, tol, tmp_path):
                             rtol=1e-13 if prec == "fp64" else 1e-11, kappa=kappa, f=f)
     assert rel(host(out["u"]), d["u"]) < tol
     assert rel(host(out["grad_kappa"]), d["grad_kappa"]) < tol
+
+
+def grf_device(points, seed, nmodes=32, sigma=1.0, ell=0.1):
+    """meshgen.grf's field (same modes from the same seed) evaluated with
+    torch on the GPU: 25M Gauss points x 32 modes is slow in numpy."""
+    rng = np.random.default_rng(seed)
+    w = torch.as_tensor(rng.normal(0.0, 1.0 / ell, size=(nmodes, 2)), device="cuda")
+    phi = torch.as_tensor(rng.uniform(0.0, 2 * np.pi, size=nmodes), device="cuda")
+    pts = torch.as_tensor(points, device="cuda")
+    g = torch.zeros(pts.shape[0], dtype=torch.float64, device="cuda")
+    for k in range(nmodes):
+        g += torch.cos(pts @ w[k] + phi[k])
+    return sigma * np.sqrt(2.0 / nmodes) * g
+
+
+def post_bc_nnz(m):
+    """Entries of the post-Dirichlet matrix of the m x m square (interior
+    pairs, explicit zeros kept, one diagonal per boundary node)."""
+    return (m - 1) ** 2 + 2 * (2 * (m - 1) * (m - 2) + (m - 2) ** 2) + 4 * m
+
+
+def test_c3_precision_form_m2048():
+    import time
+    t0 = time.perf_counter()
+    m = 2048
+    print(f"{m}x{m} precision form: set-up", flush=True)
+    c, e = meshgen.jittered_square(m, seed=41)
+    gm = admfem.Mesh(c, e, zero_based=True)
+    g = admfem.gauss_nodes(gm)
+    kappa = torch.exp(grf_device(g, 42))
+    kappa_obs = torch.exp(grf_device(g, 43))
+    f = torch.as_tensor(np.random.default_rng(44).uniform(-1, 1, gm.ngauss), device="cuda")
+    bd = admfem.bcnode(gm)
+    assert bd.size == 4 * m
+    interior = np.setdiff1d(np.arange(gm.nnode), bd - 1)
+    obs = np.sort(np.random.default_rng(45).choice(interior, size=gm.nnode // 100,
+                                                   replace=False))
+    S = admfem.DirichletSystem(gm, bd)
+    assert S.n == (m + 1) ** 2 and S.nnz == post_bc_nnz(m) == 29_323_281
+    S.assemble(kappa=kappa_obs, f=f)
+    u2, _ = S.solve(precision="fp64", rtol=1e-12)
+    uobs = host(u2)[obs]
+    print(f"set-up {time.perf_counter() - t0:.1f}s", flush=True)
+    out = {}
+    for prec in ("mixed", "fp64"):
+        out[prec] = S.forward_adjoint(obs, uobs, precision=prec, rtol=1e-10, kappa=kappa, f=f)
+        print(prec, out[prec]["stats"], flush=True)
+    # the assembled matrix (user order), its symmetry, and the true FP64
+    # residuals of the mixed solution and adjoint
+    A, rhs = S.csr()
+    assert A.nnz == S.nnz
+    asym = abs(A - A.T).max()
+    assert asym <= 1e-15 * abs(A).max(), asym
+    u, lam = host(out["mixed"]["u"]), host(out["mixed"]["lam"])
+    absA = abs(A)
+
+    def residual(x, b):
+        # ||b - A x|| / ||b|| and the rounding bound of evaluating it in FP64
+        # (8 eps |A||x| per row: at most 7 entries per row plus the subtraction)
+        r = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        return r, 8 * 2.0 ** -53 * np.linalg.norm(absA @ abs(x)) / np.linalg.norm(b)
+    ru, nu = residual(u, rhs)
+    gu = np.zeros(S.n)
+    gu[obs] = 2 * (u[obs] - uobs)
+    rl, nl = residual(lam, gu)
+    print(f"true residuals: u {ru:.2e} (rounding {nu:.1e}), lambda {rl:.2e} "
+          f"(rounding {nl:.1e})", flush=True)
+    assert ru <= 1e-10 + nu and rl <= 1e-10 + nl
+    assert abs(out["mixed"]["loss"] - float(((u[obs] - uobs) ** 2).sum())) <= \
+        1e-12 * out["mixed"]["loss"]
+    # the mixed and the plain FP64 path agree
+    eu = rel(u, host(out["fp64"]["u"]))
+    grad = host(out["mixed"]["grad_kappa"])
+    eg = rel(grad, host(out["fp64"]["grad_kappa"]))
+    print(f"mixed vs fp64: u {eu:.2e}, grad {eg:.2e}", flush=True)
+    assert eu <= 1e-8 and eg <= 1e-8
+
+    def loss(k):
+        return S.forward_adjoint(obs, uobs, precision="fp64", rtol=1e-13, kappa=k, f=f,
+                                 want_lam=False)["loss"]
+    # central differences of L in 3 seeded directions (relative perturbations
+    # of kappa at every Gauss point), steps 1e-3 and 1e-4
+    rng = np.random.default_rng(46)
+    worst = {1e-3: 0.0, 1e-4: 0.0}
+    for _ in range(3):
+        d = torch.as_tensor(rng.standard_normal(gm.ngauss), device="cuda") * kappa
+        an = float(torch.dot(out["mixed"]["grad_kappa"], d))
+        for eps in worst:
+            fd = (loss(kappa + eps * d) - loss(kappa - eps * d)) / (2 * eps)
+            worst[eps] = max(worst[eps], abs(fd - an) / abs(an))
+    print(f"worst relative FD mismatch {worst}; total {time.perf_counter() - t0:.1f}s",
+          flush=True)
+    assert worst[1e-3] < 1e-5 and worst[1e-4] < 1e-5

@@ -134,6 +134,30 @@ def test_sparse_solve_op_and_gradient_general_spd():
     assert rel(vv.grad.cpu().numpy(), -lam[I] * u0[J]) < 1e-10
 
 
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_sparse_solve_keeps_banded_post_dirichlet_order(shuffle):
+    """The reference's main use of the op: the Laplace COO of a row-major
+    rectangle mesh after Dirichlet elimination, whose first row is a corner
+    boundary node holding only its diagonal.  That order is banded, so the
+    solve keeps it; the same mesh with its nodes numbered at random is
+    solved in reverse Cuthill-McKee order.  Both against the oracle."""
+    m = 40
+    c, e = meshgen.jittered_square(m, seed=2, jitter=0.0, permute=shuffle)
+    gm = admfem.Mesh(c, e, zero_based=True)
+    om = O.OracleMesh(c, e)
+    kap = np.exp(meshgen.grf(np.stack([om.gx, om.gy], 1), 4))
+    f = np.random.default_rng(5).uniform(-1, 1, om.ngauss)
+    bd = om.bcnode()
+    K = admfem.compute_fem_laplace_matrix1(torch.tensor(kap, device="cuda"), gm)
+    F = admfem.compute_fem_source_term1(torch.tensor(f, device="cuda"), gm)
+    Kb, Fb = admfem.impose_Dirichlet_boundary_conditions(K, F, bd, np.zeros(bd.size))
+    u = admfem.solve(Kb, Fb, rtol=1e-13)
+    info = admfem.ops.solve_pattern_info()
+    assert info["n"] == om.nnode and info["reordered"] == shuffle
+    ref = O.forward_adjoint(om, kap, f, bd, np.zeros(bd.size), lambda u: np.zeros_like(u))
+    assert rel(u.cpu().numpy(), ref["u"]) < 1e-10
+
+
 def test_sparse_solve_far_coupling():
     """A chain closed into a cycle by one coupling 69,999 rows away (the solver
     renumbers it by reverse Cuthill-McKee, bandwidth 2) against SciPy."""
@@ -180,6 +204,7 @@ def test_sparse_solve_grid_laplacian_any_numbering(permute):
     assert rel(u.detach().cpu().numpy(), u0) < 1e-10
     assert rel(bt.grad.cpu().numpy(), lam) < 1e-10
     assert rel(vv.grad.cpu().numpy(), -lam[I] * u0[J]) < 1e-10
+    assert admfem.ops.solve_pattern_info()["reordered"] == permute
 
 
 def test_sparse_solve_pattern_kept_between_calls():

@@ -160,6 +160,12 @@ def test_legacy_reference_symbols_on_default_mesh():
     rhs = np.ones(om.nnode)  # accumulates
     L.FemSourceScalar_forward_Julia(_lib.ptr(rhs), _lib.ptr(kap))
     np.testing.assert_allclose(rhs, 1.0 + om.source(kap), rtol=1e-14)
+    # src/pcl.jl:35-39: zeros(ngauss, 9 ngauss), column-major, filled in place
+    H = np.zeros(om.ngauss * 9 * om.ngauss)
+    L.pcl_FemLaplaceScalar_Jacobian(_lib.ptr(H))
+    Href = O.pcl_laplace_jacobian(om)
+    np.testing.assert_allclose(H.reshape(9 * om.ngauss, om.ngauss).T, Href, rtol=1e-13,
+                               atol=1e-16)
 
 
 # ---------------------------------------------------------------- Dirichlet COO

@@ -143,6 +143,70 @@ def test_failure_statuses():
     assert few.result(rf)["status"] == admfem._lib.ERR_NOT_CONVERGED
 
 
+def test_largest_stream_requests_match_oracle():
+    """The serving stream's largest sides: a Poisson request at side 480 and
+    a heat step at side 511, batched together at the benchmark's rtol 1e-9,
+    each within 1e-8 of the oracle's direct solve (u and dL/dkappa)."""
+    cases = []
+    sv = Server(8 << 30, rtol=1e-9)
+    for m, heat, seed in ((480, False, 61), (511, True, 62)):
+        c, e, S = topo(m)
+        P = host_problem(c, e, seed, heat=heat)
+        if heat:
+            P["dt"] = 0.05
+        t = sv.add_topology(S, 0.2 / m)
+        cases.append((m, P, submit(sv, t, P)))
+    sv.run(timeout=120)
+    for m, P, rid in cases:
+        r = sv.result(rid)
+        assert r["status"] == 0, (m, r["status"])
+        ref = oracle_ref(P)
+        eu, eg = rel(r["u"], ref["u"]), rel(r["grad"], ref["grad_kappa"])
+        print(f"side {m} {'heat' if P['dt'] else 'poisson'}: rel err u {eu:.2e} grad {eg:.2e}, "
+              f"iterations {r['iters_fwd']}/{r['iters_adj']}", flush=True)
+        assert eu < 1e-8 and eg < 1e-8
+
+
+@pytest.mark.parametrize("mode", ["batched", "seq_fp64"])
+def test_bad_requests_fail_alone(mode):
+    """A host request without kappa, or with an observation index outside
+    [0, n) or repeated, is refused at submit; a device request with such an
+    index is admitted and finishes with ERR_ARG.  Either way it holds no slot
+    or memory afterwards: the valid requests around it complete and match the
+    oracle."""
+    import torch
+    c, e, S = topo(16)
+    P = host_problem(c, e, 9)
+    ref = oracle_ref(P)
+    sv = Server(1 << 30, mode=mode, rtol=1e-11, max_active=2)
+    t = sv.add_topology(S, 0.2 / 16)
+    for kw in (dict(kappa=None), dict(obs=np.append(P["obs"], S.n).astype(np.int32)),
+               dict(obs=np.append(P["obs"], P["obs"][0]).astype(np.int32))):
+        Q = dict(P)
+        Q.update(kw)
+        with pytest.raises(admfem.AdmfemError) as ei:
+            submit(sv, t, Q)
+        assert ei.value.code == admfem._lib.ERR_ARG
+    dev = lambda a: torch.as_tensor(a, device="cuda")  # noqa: E731
+    ids = []
+    for k, bad_obs in enumerate((None, np.append(P["obs"], -1), None,
+                                 np.append(P["obs"], P["obs"][1]), None)):
+        o = P["obs"] if bad_obs is None else bad_obs.astype(np.int32)
+        uo = np.append(P["uobs"], np.zeros(o.size - P["obs"].size))
+        ids.append((bad_obs is None, sv.submit(
+            t, coords=dev(P["coords"]), kappa=dev(P["kappa"]), f=dev(P["f"]),
+            obs_idx=dev(o), obs_val=dev(uo))))
+    sv.run(timeout=120)
+    for ok, rid in ids:
+        r = sv.result(rid)
+        assert r["done"]
+        if ok:
+            assert r["status"] == 0
+            assert rel(r["u"], ref["u"]) < 1e-8 and rel(r["grad"], ref["grad_kappa"]) < 1e-8
+        else:
+            assert r["status"] == admfem._lib.ERR_ARG
+
+
 def test_poisson_arrivals_complete_with_latency():
     tops = [topo(m) for m in (16, 32, 48)]
     sv = Server(4 << 30, rtol=1e-8)

@@ -391,3 +391,41 @@ def test_field_op_backwards_are_adjoints():
     out = mesh.laplace_term(u, nu)
     assert np.isclose(go @ out, gu @ u, rtol=1e-12)      # linear in u
     assert np.isclose(go @ out, gn @ nu, rtol=1e-12)     # linear in nu
+
+
+# ---------------------------------------------------------------- pcl Jacobians
+def test_pcl_jacobians_are_derivatives_of_the_forward_ops():
+    """src/pcl.jl's two Jacobians against the (already pinned) forward ops
+    they differentiate, both of which are linear in the differentiated
+    input: ImposeDirichlet's output values are J^T v plus the appended
+    (b,b,1) constants, and the Laplace COO values are H^T kappa."""
+    mesh = om(3)
+    rng = np.random.default_rng(12)
+    kap = rng.random(mesh.ngauss)
+    oi, vv = mesh.laplace_coo(kap)
+    H = O.pcl_laplace_jacobian(mesh)
+    assert H.shape == (mesh.ngauss, 9 * mesh.ngauss)
+    np.testing.assert_allclose(H.T @ kap, vv, rtol=1e-13, atol=1e-15)
+    bd = mesh.bcnode()
+    v = rng.standard_normal(vv.size)
+    pi, pv, _ = O.dirichlet(oi, v, bd, np.zeros(mesh.nnode), np.zeros(bd.size))
+    J = O.pcl_impose_dirichlet(oi + 1, bd, pv.size)
+    kept = pv.size - np.unique(bd).size
+    np.testing.assert_array_equal(J.T @ v, np.concatenate([pv[:kept], np.zeros(pv.size - kept)]))
+    assert J.sum() == kept
+
+
+def test_library_pcl_impose_dirichlet_matches_oracle():
+    """The library's pcl_ImposeDirichlet (host-side bookkeeping, no device
+    work, so callable here) against the oracle, with the reference's 1-based
+    column-major layouts (src/pcl.jl:14-21)."""
+    from admfem import _lib
+    mesh = om(4)
+    oi, _ = mesh.laplace_coo(np.ones(mesh.ngauss))
+    bd = mesh.bcnode()
+    sN = oi.shape[0]
+    outdof = sN  # wider than needed: the extra columns stay zero
+    ind = np.ascontiguousarray((oi + 1).T.ravel())  # [sN x 2] column-major, 1-based
+    J = np.zeros(sN * outdof)
+    _lib.lib().pcl_ImposeDirichlet(_lib.ptr(J), _lib.ptr(ind), _lib.ptr(bd), bd.size, sN)
+    np.testing.assert_array_equal(J.reshape(outdof, sN).T, O.pcl_impose_dirichlet(oi + 1, bd, outdof))
