@@ -114,6 +114,7 @@ _SIGS = {
     "admfem_system_solve": (c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(SolveOpts),
                                     ctypes.POINTER(SolveStats), c_vp]),
     "admfem_system_kappa_gradient": (c_int, [c_vp, c_vp, c_vp, c_double, c_vp, c_vp]),
+    "admfem_system_kappa_gradient_noalloc": (c_int, [c_vp, c_vp, c_vp, c_double, c_vp, c_vp]),
     "admfem_system_forward_adjoint": (c_int, [c_vp, ctypes.POINTER(Problem), c_vp, c_vp, c_int,
                                               ctypes.POINTER(SolveOpts), c_vp, c_vp, c_vp,
                                               c_vp, ctypes.POINTER(SolveStats), c_vp]),

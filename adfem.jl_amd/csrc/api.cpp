@@ -456,6 +456,15 @@ int admfem_system_kappa_gradient(admfem_system h, const double *u, const double 
   });
 }
 
+int admfem_system_kappa_gradient_noalloc(admfem_system h, const double *u, const double *lam,
+                                         double scale, double *grad_kappa, void *stream) {
+  return guarded([&] {
+    System &s = SY(h);
+    REQUIRE(u && lam && grad_kappa, ADMFEM_ERR_ARG, "null pointer");
+    kappa_gradient_noalloc(s, u, lam, scale, grad_kappa, S(stream));
+  });
+}
+
 int admfem_system_forward_adjoint(admfem_system h, const admfem_problem *p,
                                   const int *obs_idx, const double *obs_val, int nobs,
                                   const admfem_solve_opts *o, double *u, double *lam,

@@ -299,6 +299,10 @@ void run_impose_dirichlet_grad(const double *gov, const double *gorhs,
                                const double *bdval, int bdN, int N,
                                double *gvv, double *grhs, double *gbd,
                                hipStream_t st);
+// noalloc.hip: the gradient with its (u, lam) staging in a fixed device array
+size_t kappa_gradient_noalloc_max_nodes();
+void kappa_gradient_noalloc(const System &s, const double *u, const double *lam, double scale,
+                            double *gk, hipStream_t st);
 // cg.hip
 void cg_solve(System &s, const double *b_int, double *x_int,
               const admfem_solve_opts &o, admfem_solve_stats &stats,

@@ -327,6 +327,19 @@ int admfem_system_kappa_gradient(admfem_system s, const double *u,
                                  const double *lam, double scale,
                                  double *grad_kappa, void *stream);
 
+/* No counterpart in the reference: admfem_system_kappa_gradient for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Same arguments, kernels and result (bit
+ * for bit); it stages (u, lam) in a fixed 32 MiB device array instead of the
+ * system's buffer and calls no allocating routine.  Limit: systems of at
+ * most 2,097,152 nodes (larger ones return ADMFEM_ERR_UNSUPPORTED before any
+ * device work).  The array is one per GPU, held until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU (other
+ * streams or threads) race. */
+int admfem_system_kappa_gradient_noalloc(admfem_system s, const double *u,
+                                         const double *lam, double scale,
+                                         double *grad_kappa, void *stream);
+
 /* One forward + adjoint solve for L = sum_k (u[obs_idx[k]] - obs_val[k])^2:
  * assemble, solve A u = rhs, g = dL/du, solve A lam = g, grad_kappa.
  * obs_idx: device int32 [nobs], 0-based, distinct (an index outside [0, n)
