@@ -30,7 +30,9 @@
 // both reductions of an iteration in one pass over the vectors.  The FP64
 // form keeps z = D^-1 r and w = D^-1 q instead of r and q (r = D z), so a
 // neighbour's p' = z - alpha w + beta p needs no diagonal; the mixed form's
-// unit diagonal needs none either.  The mixed form checks for a
+// unit diagonal needs none either.  The vectors a step gathers are
+// double-buffered: it reads buffer it & 1 of the solver's own step count
+// and writes the other.  The mixed form checks for a
 // replacement every eighth iteration (fold x, replace r, scalar); the
 // pending alpha and beta carry over it (the direction is kept).
 //
@@ -79,8 +81,8 @@ constexpr int RT = 1024;  // threads of the single-block scalar kernels
 
 struct CGWork {
   int npad = 0, nblocks = 0;
-  DevBuf<double> x;                         // FP64 solution
-  DevBuf<double> z64[2], w64[2], p64[2];    // FP64: D^-1 r, D^-1 q, direction
+  DevBuf<double> x;                          // FP64 solution
+  DevBuf<double> z64[2], w64[2], p64[2];     // FP64: D^-1 r, D^-1 q, direction
   DevBuf<float> ys32, r32[2], p32[2], q32[2];  // mixed: correction, r, p, q
   DevBuf<double> dsq;    // mixed: 1 / sqrt(a_ii)
   DevBuf<float> sv32s;   // mixed: FP32 scaled off-diagonals a_ij dsq_i dsq_j
@@ -227,21 +229,21 @@ __global__ void __launch_bounds__(NT)
            const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ z,
            double *__restrict__ w, double *__restrict__ p, double *part) {
   const int i = my_row();
-  double v[3] = {0.0, 0.0, 0.0};
+  double t[3] = {0.0, 0.0, 0.0};
   if (i < npad) {
     const double bi = i < n ? b[i] : 0.0;
     const bool ident = i >= n || row_bd[i] >= 0;
     const double ri = ident ? 0.0 : bi;
-    x[i] = (ident && i < n) ? bi : 0.0;
     const double zi = ri * dinv[i];
+    x[i] = (ident && i < n) ? bi : 0.0;
     z[i] = zi;
     w[i] = 0.0;
     p[i] = 0.0;
-    v[0] = ri * zi;
-    v[1] = ri * ri;
-    v[2] = bi * bi;
+    t[0] = ri * zi;
+    t[1] = ri * ri;
+    t[2] = bi * bi;
   }
-  put_partials<3>(v, part);
+  put_partials<3>(t, part);
 }
 
 __global__ void __launch_bounds__(RT) s_init(int G, const double *part, CGScalars *S) {
@@ -280,14 +282,16 @@ __device__ __forceinline__ double dir64(double z, double w, double p, double alp
 __global__ void __launch_bounds__(NT)
     k_step(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
            const double *__restrict__ sv, const double *__restrict__ diag,
-           const double *__restrict__ dinv, const double *__restrict__ z,
-           const double *__restrict__ w, const double *__restrict__ p, double *__restrict__ zn,
-           double *__restrict__ wn, double *__restrict__ pn, double *__restrict__ x,
-           double *part, const CGScalars *S) {
+           const double *__restrict__ dinv, double *z0, double *z1, double *w0, double *w1,
+           double *p0, double *p1, double *__restrict__ x, double *part, const CGScalars *S) {
   if (S->status != ST_RUN) return;
   const double alpha = S->alpha, beta = S->beta;
+  const bool odd = S->it & 1;
+  const double *__restrict__ z = odd ? z1 : z0;
+  const double *__restrict__ w = odd ? w1 : w0;
+  const double *__restrict__ p = odd ? p1 : p0;
   const int i = my_row();
-  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < npad) {
     const double pi = p[i];
     double zi;
@@ -299,17 +303,17 @@ __global__ void __launch_bounds__(NT)
                        return dir64(z[c], w[c], p[c], alpha, beta, unused);
                      });
     const double wi = q * dinv[i];
-    zn[i] = zi;
-    pn[i] = pni;
-    wn[i] = wi;
+    (odd ? z0 : z1)[i] = zi;
+    (odd ? p0 : p1)[i] = pni;
+    (odd ? w0 : w1)[i] = wi;
     const double ri = di * zi;
-    v[0] = ri * zi;
-    v[1] = pni * q;
-    v[2] = ri * wi;
-    v[3] = q * wi;
-    v[4] = ri * ri;
+    t[0] = ri * zi;
+    t[1] = pni * q;
+    t[2] = ri * wi;
+    t[3] = q * wi;
+    t[4] = ri * ri;
   }
-  put_partials<5>(v, part);
+  put_partials<5>(t, part);
 }
 
 __global__ void __launch_bounds__(RT) s_step(int G, const double *part, CGScalars *S,
@@ -326,21 +330,52 @@ __global__ void __launch_bounds__(RT) s_step(int G, const double *part, CGScalar
   }
   S->rz = rz;
   S->rr = rr;
-  if (rr <= S->tol2) {
-    S->status = ST_CONV;
-    return;
+  if (pq > 0.0) {  // the pending step, also kept when the solve stops here
+    const double alpha = rz / pq;
+    S->alpha = alpha;
+    S->beta = rz > 0.0 ? fmax(rz - 2.0 * alpha * zq + alpha * alpha * qw, 0.0) / rz : 0.0;
   }
-  if (S->it >= maxit) {
-    S->status = ST_MAXIT;
-    return;
+  if (rr <= S->tol2) S->status = ST_CONV;
+  else if (S->it >= maxit) S->status = ST_MAXIT;
+  else if (!(pq > 0.0)) S->status = ST_BREAK;  // not SPD
+}
+
+// An FP64 solve whose recursive residual met the tolerance but whose true
+// residual did not: the true residual replaces the recursive one,
+// z = D^-1 (b - A x) in the current buffer, keeping the direction and the
+// pending step; partials of r.z and r.r (x: the solution taken out of the
+// vector buffer)
+__global__ void __launch_bounds__(NT)
+    k_restart(int n, int npad, const int *__restrict__ sp, const int *__restrict__ sc,
+              const double *__restrict__ sv, const double *__restrict__ diag,
+              const double *__restrict__ dinv, const double *__restrict__ x, double *z0,
+              double *z1, double *part, const CGScalars *S) {
+  const double *__restrict__ b = S->b;
+  double *__restrict__ z = (S->it & 1) ? z1 : z0;
+  const int i = my_row();
+  double t[2] = {0.0, 0.0};
+  if (i < npad) {
+    const double ax = diag[i] * x[i] + sell_dot<double>(i, sp, sc, sv, [&](int c) { return x[c]; });
+    const double ri = (i < n ? b[i] : 0.0) - ax;
+    const double zi = ri * dinv[i];
+    z[i] = zi;
+    t[0] = ri * zi;
+    t[1] = ri * ri;
   }
-  if (!(pq > 0.0)) {  // not SPD
-    S->status = ST_BREAK;
-    return;
-  }
-  const double alpha = rz / pq;
-  S->alpha = alpha;
-  S->beta = rz > 0.0 ? fmax(rz - 2.0 * alpha * zq + alpha * alpha * qw, 0.0) / rz : 0.0;
+  put_partials<2>(t, part);
+}
+
+__global__ void __launch_bounds__(RT) s_restart(int G, const double *part, CGScalars *S,
+                                                int maxit) {
+  double t[2];
+  sum_partials<2>(part, G, t);
+  if (threadIdx.x != 0) return;
+  S->rz = t[0];
+  S->rr = t[1];
+  S->nrepl += 1;
+  if (!(t[1] == t[1]) || !isfinite(t[1])) S->status = ST_NAN;
+  else if (t[1] <= S->tol2) S->status = ST_CONV;
+  else S->status = S->it >= maxit ? ST_MAXIT : ST_RUN;
 }
 
 // ---- mixed precision, scaled form: plain CG on S A S (S = diag^-1/2,
@@ -353,22 +388,22 @@ __global__ void __launch_bounds__(NT)
             const double *__restrict__ dsq, double *__restrict__ x, float *__restrict__ ys,
             float *__restrict__ r, float *__restrict__ p, float *__restrict__ q, double *part) {
   const int i = my_row();
-  double v[3] = {0.0, 0.0, 0.0};
+  double t[3] = {0.0, 0.0, 0.0};
   if (i < npad) {
     const double bi = i < n ? b[i] : 0.0;
     const bool ident = i >= n || row_bd[i] >= 0;
     const double ri = ident ? 0.0 : bi;
     x[i] = (ident && i < n) ? bi : 0.0;
-    ys[i] = 0.0f;
     const float rv = (float)(ri * dsq[i]);
+    ys[i] = 0.0f;
     r[i] = rv;
     p[i] = 0.0f;
     q[i] = 0.0f;
-    v[0] = (double)rv * (double)rv;
-    v[1] = ri * ri;
-    v[2] = bi * bi;
+    t[0] = (double)rv * (double)rv;
+    t[1] = ri * ri;
+    t[2] = bi * bi;
   }
-  put_partials<3>(v, part);
+  put_partials<3>(t, part);
 }
 
 // The pending step and the next direction of one row (scaled form):
@@ -384,14 +419,16 @@ __device__ __forceinline__ double dir32(float r, float q, float p, double alpha,
 // partials of r'.r', p'.q', r'.q', q'.q'
 __global__ void __launch_bounds__(NT)
     k_mstep(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
-            const float *__restrict__ sv, const float *__restrict__ r,
-            const float *__restrict__ q, const float *__restrict__ p, float *__restrict__ rn,
-            float *__restrict__ qn, float *__restrict__ pn, float *__restrict__ ys,
-            double *part, const CGScalars *S) {
+            const float *__restrict__ sv, float *r0, float *r1, float *q0, float *q1,
+            float *p0, float *p1, float *__restrict__ ys, double *part, const CGScalars *S) {
   if (S->status != ST_RUN || S->replace) return;
   const double alpha = S->alpha, beta = S->beta;
+  const bool odd = S->it & 1;
+  const float *__restrict__ r = odd ? r1 : r0;
+  const float *__restrict__ q = odd ? q1 : q0;
+  const float *__restrict__ p = odd ? p1 : p0;
   const int i = my_row();
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < npad) {
     const float pi = p[i];
     double ri;
@@ -402,15 +439,15 @@ __global__ void __launch_bounds__(NT)
                        return dir32(r[c], q[c], p[c], alpha, beta, unused);
                      });
     const float rv = (float)ri, pv = (float)pni, qv = (float)a;
-    rn[i] = rv;
-    pn[i] = pv;
-    qn[i] = qv;
-    v[0] = (double)rv * (double)rv;
-    v[1] = (double)pv * (double)qv;
-    v[2] = (double)rv * (double)qv;
-    v[3] = (double)qv * (double)qv;
+    (odd ? r0 : r1)[i] = rv;
+    (odd ? p0 : p1)[i] = pv;
+    (odd ? q0 : q1)[i] = qv;
+    t[0] = (double)rv * (double)rv;
+    t[1] = (double)pv * (double)qv;
+    t[2] = (double)rv * (double)qv;
+    t[3] = (double)qv * (double)qv;
   }
-  put_partials<4>(v, part);
+  put_partials<4>(t, part);
 }
 
 __global__ void __launch_bounds__(RT) s_mstep(int G, const double *part, CGScalars *S,
@@ -456,7 +493,8 @@ __device__ __forceinline__ bool replace_due(const CGScalars *S) {
          (S->replace || S->rz <= S->delta2 * S->rz_ref || S->stall >= STALL_ITERS);
 }
 
-// when a replacement is due (or forced at the end): x += S y, y = 0
+// when a replacement is due (or forced at the end): x += S y, y = 0 (each
+// row reads and writes only its own entries)
 __global__ void __launch_bounds__(NT)
     k_mfold(int npad, double *__restrict__ x, float *__restrict__ ys,
             const double *__restrict__ dsq, const CGScalars *S, int force) {
@@ -469,8 +507,9 @@ __global__ void __launch_bounds__(NT)
 }
 
 // when a replacement is due: the true residual b - A x in FP64 (FP64
-// matrix), handed to the FP32 iteration as r = S (b - A x) (the buffer the
-// next step reads); partials of the scaled r.r and the true ||b - A x||^2
+// matrix), handed to the FP32 iteration as r = S (b - A x) in the current
+// buffer (the direction is kept); partials of the scaled r.r and the true
+// ||b - A x||^2
 __global__ void __launch_bounds__(NT)
     k_mreplace(int n, int npad, const int *__restrict__ sp, const int *__restrict__ sc,
                const double *__restrict__ sv, const double *__restrict__ diag,
@@ -480,16 +519,16 @@ __global__ void __launch_bounds__(NT)
   const double *__restrict__ b = S->b;
   float *__restrict__ r = (S->it & 1) ? r1 : r0;
   const int i = my_row();
-  double v[2] = {0.0, 0.0};
+  double t[2] = {0.0, 0.0};
   if (i < npad) {
     const double ax = diag[i] * x[i] + sell_dot<double>(i, sp, sc, sv, [&](int c) { return x[c]; });
     const double ri = (i < n ? b[i] : 0.0) - ax;
     const float rf = (float)(ri * dsq[i]);
     r[i] = rf;
-    v[0] = (double)rf * (double)rf;
-    v[1] = ri * ri;
+    t[0] = (double)rf * (double)rf;
+    t[1] = ri * ri;
   }
-  put_partials<2>(v, part);
+  put_partials<2>(t, part);
 }
 
 __global__ void __launch_bounds__(RT) s_replace(int G, const double *part, CGScalars *S,
@@ -571,19 +610,19 @@ static CGWork *ensure_work(System &s) {
 
 namespace {
 
-// One step kernel of the given precision reading buffer `cur` and writing
-// the other one.
-void launch_step(System &s, CGWork *w, bool mixed, int cur, hipStream_t st) {
-  const int nb = w->nblocks, np = s.npad, nx = cur ^ 1;
+// One step kernel of the given precision (it reads the current vector
+// buffer, which the scalars name, and writes the other one).
+void launch_step(System &s, CGWork *w, bool mixed, hipStream_t st) {
+  const int nb = w->nblocks, np = s.npad;
   const int *sp = s.d_slice_ptr.p, *scol = s.d_sell_col.p;
   if (mixed)
-    k_mstep<<<nb, NT, 0, st>>>(np, sp, scol, w->sv32s.p, w->r32[cur].p, w->q32[cur].p,
-                               w->p32[cur].p, w->r32[nx].p, w->q32[nx].p, w->p32[nx].p,
-                               w->ys32.p, w->part.p, w->sc.p);
+    k_mstep<<<nb, NT, 0, st>>>(np, sp, scol, w->sv32s.p, w->r32[0].p, w->r32[1].p,
+                               w->q32[0].p, w->q32[1].p, w->p32[0].p, w->p32[1].p, w->ys32.p,
+                               w->part.p, w->sc.p);
   else
     k_step<<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val.p, s.d_diag.p, s.d_dinv.p,
-                              w->z64[cur].p, w->w64[cur].p, w->p64[cur].p, w->z64[nx].p,
-                              w->w64[nx].p, w->p64[nx].p, w->x.p, w->part.p, w->sc.p);
+                              w->z64[0].p, w->z64[1].p, w->w64[0].p, w->w64[1].p, w->p64[0].p,
+                              w->p64[1].p, w->x.p, w->part.p, w->sc.p);
 }
 
 }  // namespace
@@ -612,7 +651,7 @@ void spmv_repeat(System &s, int precision, int repeat, hipStream_t st) {
     k_fill<double><<<fb, 256, 0, st>>>(w->p64[0].p, s.npad, 0.0);
     k_fill<double><<<fb, 256, 0, st>>>(w->x.p, s.npad, 0.0);
   }
-  for (int k = 0; k < repeat; k++) launch_step(s, w, mixed, 0, st);
+  for (int k = 0; k < repeat; k++) launch_step(s, w, mixed, st);
   HIPCHECK(hipGetLastError());
 }
 
@@ -620,15 +659,14 @@ namespace {
 
 constexpr int GRAPH_ITERS = 8;  // a multiple of the replacement interval (8)
 
-// One iteration with index `it` (its parity picks the buffers; the mixed
-// form has a replacement point after every eighth iteration and after the
-// last one allowed).
+// One iteration with index `it` (the mixed form has a replacement point
+// after every eighth iteration and after the last one allowed).
 void enqueue_iteration(System &s, CGWork *w, bool mixed, int max_iter, int it,
                        hipStream_t st) {
   const int nb = w->nblocks, np = s.npad;
   CGScalars *S = w->sc.p;
   double *part = w->part.p;
-  launch_step(s, w, mixed, it & 1, st);
+  launch_step(s, w, mixed, st);
   if (mixed) {
     s_mstep<<<1, RT, 0, st>>>(nb, part, S, max_iter);
     if ((it & 7) == 7 || it + 1 == max_iter) {
@@ -645,8 +683,8 @@ void enqueue_iteration(System &s, CGWork *w, bool mixed, int max_iter, int it,
 
 // Iterations it .. it + GRAPH_ITERS - 1 for an `it` divisible by
 // GRAPH_ITERS with it + GRAPH_ITERS <= max_iter: the nodes are those of
-// iterations 0 .. GRAPH_ITERS - 1 (same parities, and the last one allowed,
-// if among them, is a replacement point anyway).
+// iterations 0 .. GRAPH_ITERS - 1 (the same replacement points; the last
+// iteration allowed, if among them, is a replacement point anyway).
 void launch_graph(System &s, CGWork *w, bool mixed, int max_iter, hipStream_t st) {
   const std::vector<const void *> key = {
       (const void *)(intptr_t)mixed, (const void *)(intptr_t)max_iter, s.d_slice_ptr.p,
@@ -705,54 +743,74 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
   int it = 0;  // iterations enqueued
   double rr_prev = -1;
   int it_prev = 0;
-  for (;;) {
-    const int stop = std::min(it + chunk, o.max_iter);
-    while (it < stop) {
-      if (it % GRAPH_ITERS == 0 && it + GRAPH_ITERS <= stop) {
-        launch_graph(s, w, mixed, o.max_iter, st);
-        it += GRAPH_ITERS;
-      } else {
-        enqueue_iteration(s, w, mixed, o.max_iter, it, st);
-        it++;
+  double rr_true = 0.0;  // ||b - A x||^2 of the returned x
+  for (int restarts = 0;; restarts++) {
+    for (;;) {
+      const int stop = std::min(it + chunk, o.max_iter);
+      while (it < stop) {
+        if (it % GRAPH_ITERS == 0 && it + GRAPH_ITERS <= stop) {
+          launch_graph(s, w, mixed, o.max_iter, st);
+          it += GRAPH_ITERS;
+        } else {
+          enqueue_iteration(s, w, mixed, o.max_iter, it, st);
+          it++;
+        }
       }
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(w->h_sc, S, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      const CGScalars &h = *w->h_sc;
+      if (h.status != ST_RUN || it >= o.max_iter) break;
+      // next chunk: the iterations left, estimated from the observed rate of
+      // the monitored residual (mixed: the scaled one, against the tolerance
+      // times the scaled-to-true ratio of the last replacement)
+      const double mon = mixed ? h.rz : h.rr;
+      const double target = mixed ? h.tol2 * h.rz_ref / h.rr_ref : h.tol2;
+      int next = 16;
+      if (rr_prev > 0 && mon > 0 && h.it > it_prev && mon < rr_prev) {
+        double rate = std::log(mon / rr_prev) / (h.it - it_prev);
+        double left = std::log(target / mon) / rate;
+        if (std::isfinite(left)) next = (int)std::min(1024.0, std::max(8.0, 0.9 * left));
+      }
+      rr_prev = mon;
+      it_prev = h.it;
+      chunk = o.check_every > 0 ? o.check_every
+                                : std::max(GRAPH_ITERS, (next + GRAPH_ITERS / 2) / GRAPH_ITERS *
+                                                            GRAPH_ITERS);
     }
+    const bool converged = w->h_sc->status == ST_CONV;
+    if (mixed && !converged)  // stopped without a final replacement
+      k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->ys32.p, w->dsq.p, S, 1);
+    k_true_residual<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b,
+                                       w->x.p, part);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(w->h_part.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                            st));
+    HIPCHECK(hipStreamSynchronize(st));
+    rr_true = 0.0;
+    for (int k = 0; k < nb; k++) rr_true += w->h_part[k];
+    // FP64: the recursive residual decides only when the true one agrees;
+    // otherwise it is replaced by the true one and the iteration goes on
+    if (mixed || !converged || rr_true <= w->h_sc->tol2 || it >= o.max_iter || restarts >= 8)
+      break;
+    k_restart<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, s.d_dinv.p,
+                                 w->x.p, w->z64[0].p, w->z64[1].p, part, S);
+    s_restart<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(w->h_sc, S, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    const CGScalars &h = *w->h_sc;
-    if (h.status != ST_RUN || it >= o.max_iter) break;
-    // next chunk: the iterations left, estimated from the observed rate of
-    // the monitored residual (mixed: the scaled one, against the tolerance
-    // times the scaled-to-true ratio of the last replacement)
-    const double mon = mixed ? h.rz : h.rr;
-    const double target = mixed ? h.tol2 * h.rz_ref / h.rr_ref : h.tol2;
-    int next = 16;
-    if (rr_prev > 0 && mon > 0 && h.it > it_prev && mon < rr_prev) {
-      double rate = std::log(mon / rr_prev) / (h.it - it_prev);
-      double left = std::log(target / mon) / rate;
-      if (std::isfinite(left)) next = (int)std::min(1024.0, std::max(8.0, 0.9 * left));
+    if (w->h_sc->status != ST_RUN) {  // x unchanged: its true residual is the restart's
+      rr_true = w->h_sc->rr;
+      break;
     }
-    rr_prev = mon;
-    it_prev = h.it;
-    chunk = o.check_every > 0 ? o.check_every
-                              : std::max(GRAPH_ITERS, (next + GRAPH_ITERS / 2) / GRAPH_ITERS *
-                                                          GRAPH_ITERS);
+    rr_prev = -1;
+    it = w->h_sc->it;  // the steps run so far (later ones were skipped)
   }
-  const bool converged = w->h_sc->status == ST_CONV;
-  if (mixed && !converged)  // stopped without a final replacement
-    k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->ys32.p, w->dsq.p, S, 1);
-  k_true_residual<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, b, w->x.p,
-                                     part);
-  HIPCHECK(hipGetLastError());
   if (x_out != w->x.p)
     HIPCHECK(hipMemcpyAsync(x_out, w->x.p, sizeof(double) * s.n, hipMemcpyDeviceToDevice, st));
-  HIPCHECK(hipMemcpyAsync(w->h_part.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost,
-                          st));
   HIPCHECK(hipMemcpyAsync(w->h_sc, S, sizeof(CGScalars), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   const CGScalars &h = *w->h_sc;
-  double rr_true = 0.0;
-  for (int k = 0; k < nb; k++) rr_true += w->h_part[k];
   stats.iterations = h.it;
   stats.replacements = h.nrepl;
   stats.rel_residual = h.bb > 0 ? std::sqrt(rr_true / h.bb) : 0.0;

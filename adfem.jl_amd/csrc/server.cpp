@@ -118,6 +118,7 @@ struct Server {
   DevBuf<BSys> d_desc;
   DevBuf<BScal> d_scal;
   BScal *h_scal = nullptr;       // pinned mirror
+  double *h_loss = nullptr;      // pinned, per slot: loss and bad-index flag
   BSys *h_desc = nullptr;        // pinned staging of the descriptors
   int2 *h_map = nullptr;         // pinned staging for the block map
   int *h_list = nullptr;
@@ -135,6 +136,7 @@ struct Server {
   ~Server() {
     drop_graphs();
     if (h_scal) (void)hipHostFree(h_scal);
+    if (h_loss) (void)hipHostFree(h_loss);
     if (h_desc) (void)hipHostFree(h_desc);
     if (h_map) (void)hipHostFree(h_map);
     if (h_list) (void)hipHostFree(h_list);
@@ -354,7 +356,11 @@ struct Server {
     q.state = Q_ADJ;
   }
 
-  void finish(Req &q, const BScal &sc, double t) {
+  // adjoint solve finished: the gradient and the outputs are enqueued, the
+  // loss goes to the slot's pinned staging; complete() finishes the request
+  // once the stream has drained (one synchronisation for every request that
+  // finished at the same chunk boundary)
+  void finish(Req &q, const BScal &sc) {
     const System &s = *topos[q.r.topo].sys;
     const Mesh &m = *s.mesh;
     q.res.iters_adj = sc.it;
@@ -369,13 +375,17 @@ struct Server {
     if (q.r.grad_out)
       HIPCHECK(hipMemcpyAsync(q.r.grad_out, q.grad, sizeof(double) * m.ngauss,
                               hipMemcpyDeviceToHost, st));
-    double l2[2];
-    HIPCHECK(hipMemcpyAsync(l2, q.loss2, sizeof(l2), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipMemcpyAsync(h_loss + 2 * q.slot, q.loss2, 2 * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+  }
+
+  // after the stream has drained: a finished request's loss and status, its
+  // memory and slot back to the queue
+  void complete(Req &q) {
+    const double *l2 = h_loss + 2 * q.slot;
     q.res.loss = l2[0];
     if (l2[1] != 0.0 && q.res.status == ADMFEM_OK) q.res.status = ADMFEM_ERR_ARG;
     q.res.t_done = now_s(t0);
-    (void)t;
     q.res.done = 1;
     q.state = Q_DONE;
     release(q);
@@ -527,8 +537,9 @@ struct Server {
       HIPCHECK(hipMemcpyAsync(h_scal, d_scal.p, sizeof(BScal) * max_slots,
                               hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
-      // phase changes
-      std::vector<int> still;
+      // phase changes; requests that finished or failed complete after one
+      // synchronisation
+      std::vector<int> still, done, failed;
       for (int i : active) {
         Req &q = reqs[i];
         const BScal sc = h_scal[q.slot];
@@ -539,7 +550,8 @@ struct Server {
         if (sc.status != 1) {
           if (q.state == Q_FWD) q.res.iters_fwd = sc.it;
           else q.res.iters_adj = sc.it;
-          fail(q, code_of(sc.status));
+          q.res.status = code_of(sc.status);
+          failed.push_back(i);
           continue;
         }
         if (q.state == Q_FWD) {
@@ -548,7 +560,19 @@ struct Server {
           still.push_back(i);
         } else {
           q.res.relres_adj = sc.bb > 0 ? std::sqrt(sc.rr / sc.bb) : 0.0;
-          finish(q, sc, now_s(t0));
+          finish(q, sc);
+          done.push_back(i);
+        }
+      }
+      if (!done.empty() || !failed.empty()) {
+        HIPCHECK(hipStreamSynchronize(st));
+        for (int i : done) complete(reqs[i]);
+        for (int i : failed) {
+          Req &q = reqs[i];
+          q.res.done = 1;
+          q.res.t_done = now_s(t0);
+          q.state = Q_DONE;
+          release(q);
         }
       }
       active.swap(still);
@@ -732,6 +756,8 @@ Server *server_create(const admfem_server_opts &o) {
     HIPCHECK(hipMemcpy(sv->d_scal.p + sv->max_slots, &dummy, sizeof(BScal),
                        hipMemcpyHostToDevice));
     HIPCHECK(hipHostMalloc((void **)&sv->h_scal, sizeof(BScal) * sv->max_slots,
+                           hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void **)&sv->h_loss, 2 * sizeof(double) * sv->max_slots,
                            hipHostMallocDefault));
     HIPCHECK(hipHostMalloc((void **)&sv->h_desc, sizeof(BSys) * sv->max_slots,
                            hipHostMallocDefault));

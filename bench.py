@@ -55,6 +55,7 @@ HBM_PEAK_GBS = 8000.0
 SEEDS = dict(stream=7, arrivals=8, mesh=20240601, kappa=11, kappa_obs=12, f=13, obs=14,
              host=5000)
 BUDGET = 64e9
+MAX_ACTIVE = 1024  # concurrently admitted requests (engine slots); --max-active
 
 
 def log(msg):
@@ -86,7 +87,7 @@ def serve(systems, sides, heat, dts, idx, arrivals, mode, rtol, want=False):
     (wall seconds of run(), results, stats)."""
     import torch
     from admfem.server import Server
-    sv = Server(BUDGET, mode=mode, rtol=rtol)
+    sv = Server(BUDGET, mode=mode, rtol=rtol, max_active=MAX_ACTIVE)
     tid = {s: sv.add_topology(S, 0.2 / s) for s, S in systems.items()}
     ids = [sv.submit(tid[int(sides[i])], kind="heat" if heat[i] else "poisson",
                      dt=float(dts[i]), seed=1000 + int(i), arrival=float(a), want_u=want,
@@ -624,7 +625,10 @@ def main():
                          "buffer once (a plain copy kernel of known size for scaling memory "
                          "counters)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--max-active", type=int, default=MAX_ACTIVE,
+                    help="engine slots: requests admitted at once (memory permitting)")
     a = ap.parse_args()
+    globals()["MAX_ACTIVE"] = a.max_active
     if a.cpu_worker:
         cpu_worker(a.cpu_worker)
         return
@@ -724,6 +728,7 @@ def main():
                         f"quads, jittered unit square, kappa=exp(GRF), f~U[-1,1], 1% observed "
                         f"interior nodes, all queued at t=0, {BUDGET / 1e9:g} GB HBM budget",
             "topologies": len(systems),
+            "max_active": MAX_ACTIVE,
             "rtol": a.rtol,
             "precision": "FP32 storage, FP64 arithmetic inner CG; FP64 residual replacement "
                          "and solution",

@@ -300,7 +300,9 @@ typedef struct {
 
 typedef struct {
   int iterations;
-  int replacements;   /* mixed: FP64 residual replacements done */
+  int replacements;   /* FP64 residual replacements done (mixed: at the
+                         replacement points; FP64: when the recursive
+                         residual met rtol but the true one did not) */
   int status;         /* ADMFEM_OK / NOT_CONVERGED / NAN / BREAKDOWN */
   double rel_residual; /* final true residual ||b-Ax||/||b|| in FP64 */
   double seconds;
