@@ -350,11 +350,10 @@ __global__ void __launch_bounds__(AS_NT)
   const int off = s0 + (row & 63);
   for (int k = 0; k < width; k++) o.sv[off + 64 * k] = my[AS_ROWS * k];
   o.diag[row] = d;
-  if (o.dinv) o.dinv[row] = d != 0.0 ? 1.0 / d : 0.0;
   o.rhs[row] = acc;
 }
 
-// CSR (post-BC) -> SELL-64 off-diagonals + diagonal and its inverse (FP64).
+// CSR (post-BC) -> SELL-64 off-diagonals + diagonal (FP64).
 // Every entry of the slice is written, the padding ones with 0, so the
 // destination needs no clearing beforehand (serving reuses device memory).
 __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
@@ -363,7 +362,7 @@ __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
                                const int *__restrict__ slice_ptr,
                                double *__restrict__ sv,
                                double *__restrict__ diag,
-                               double *__restrict__ dinv, int npad) {
+                               int npad) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npad) return;
   const int s0 = slice_ptr[r >> 6], width = (slice_ptr[(r >> 6) + 1] - s0) >> 6;
@@ -384,7 +383,6 @@ __global__ void to_sell_kernel(int n, const int *__restrict__ rp,
   }
   for (; kk < width; kk++) sv[off + 64 * kk] = 0.0;
   diag[r] = d;
-  dinv[r] = d != 0.0 ? 1.0 / d : 0.0;
 }
 
 // dL/dkappa_{e,k} = -scale * w_k * grad(lam~) . grad(u) on element e, with
@@ -518,14 +516,14 @@ void launch_assemble_to(const System &s, const double2 *xy, const admfem_problem
 }
 
 void launch_assemble(System &s, const admfem_problem &p, hipStream_t st) {
-  SellOut o{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p};
+  SellOut o{s.d_sell_val.p, s.d_diag.p, s.rhs.p};
   launch_assemble_to(s, s.mesh->xy.p, p, o, st);
 }
 
 void launch_to_sell(System &s, hipStream_t st) {
   to_sell_kernel<<<nblk(s.npad, 256), 256, 0, st>>>(s.n, s.d_rp.p, s.d_col.p, s.val.p,
                                                     s.d_slice_ptr.p, s.d_sell_val.p,
-                                                    s.d_diag.p, s.d_dinv.p, s.npad);
+                                                    s.d_diag.p, s.npad);
   HIPCHECK(hipGetLastError());
 }
 

@@ -226,7 +226,7 @@ __device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
 // the first step forms p = z0, q = A z0); partials of r.z, r.r, b.b.
 __global__ void __launch_bounds__(NT)
     k_init(int n, int npad, const int *__restrict__ row_bd, const double *__restrict__ b,
-           const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ z,
+           const double *__restrict__ diag, double *__restrict__ x, double *__restrict__ z,
            double *__restrict__ w, double *__restrict__ p, double *part) {
   const int i = my_row();
   double t[3] = {0.0, 0.0, 0.0};
@@ -234,7 +234,8 @@ __global__ void __launch_bounds__(NT)
     const double bi = i < n ? b[i] : 0.0;
     const bool ident = i >= n || row_bd[i] >= 0;
     const double ri = ident ? 0.0 : bi;
-    const double zi = ri * dinv[i];
+    const double di = diag[i];
+    const double zi = di != 0.0 ? ri / di : 0.0;
     x[i] = (ident && i < n) ? bi : 0.0;
     z[i] = zi;
     w[i] = 0.0;
@@ -282,7 +283,7 @@ __device__ __forceinline__ double dir64(double z, double w, double p, double alp
 __global__ void __launch_bounds__(NT)
     k_step(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
            const double *__restrict__ sv, const double *__restrict__ diag,
-           const double *__restrict__ dinv, double *z0, double *z1, double *w0, double *w1,
+           double *z0, double *z1, double *w0, double *w1,
            double *p0, double *p1, double *__restrict__ x, double *part, const CGScalars *S) {
   if (S->status != ST_RUN) return;
   const double alpha = S->alpha, beta = S->beta;
@@ -302,7 +303,7 @@ __global__ void __launch_bounds__(NT)
                        double unused;
                        return dir64(z[c], w[c], p[c], alpha, beta, unused);
                      });
-    const double wi = q * dinv[i];
+    const double wi = di != 0.0 ? q / di : 0.0;
     (odd ? z0 : z1)[i] = zi;
     (odd ? p0 : p1)[i] = pni;
     (odd ? w0 : w1)[i] = wi;
@@ -348,7 +349,7 @@ __global__ void __launch_bounds__(RT) s_step(int G, const double *part, CGScalar
 __global__ void __launch_bounds__(NT)
     k_restart(int n, int npad, const int *__restrict__ sp, const int *__restrict__ sc,
               const double *__restrict__ sv, const double *__restrict__ diag,
-              const double *__restrict__ dinv, const double *__restrict__ x, double *z0,
+              const double *__restrict__ x, double *z0,
               double *z1, double *part, const CGScalars *S) {
   const double *__restrict__ b = S->b;
   double *__restrict__ z = (S->it & 1) ? z1 : z0;
@@ -357,7 +358,8 @@ __global__ void __launch_bounds__(NT)
   if (i < npad) {
     const double ax = diag[i] * x[i] + sell_dot<double>(i, sp, sc, sv, [&](int c) { return x[c]; });
     const double ri = (i < n ? b[i] : 0.0) - ax;
-    const double zi = ri * dinv[i];
+    const double di = diag[i];
+    const double zi = di != 0.0 ? ri / di : 0.0;
     z[i] = zi;
     t[0] = ri * zi;
     t[1] = ri * ri;
@@ -620,7 +622,7 @@ void launch_step(System &s, CGWork *w, bool mixed, hipStream_t st) {
                                w->q32[0].p, w->q32[1].p, w->p32[0].p, w->p32[1].p, w->ys32.p,
                                w->part.p, w->sc.p);
   else
-    k_step<<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val.p, s.d_diag.p, s.d_dinv.p,
+    k_step<<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_val.p, s.d_diag.p,
                               w->z64[0].p, w->z64[1].p, w->w64[0].p, w->w64[1].p, w->p64[0].p,
                               w->p64[1].p, w->x.p, w->part.p, w->sc.p);
 }
@@ -688,7 +690,7 @@ void enqueue_iteration(System &s, CGWork *w, bool mixed, int max_iter, int it,
 void launch_graph(System &s, CGWork *w, bool mixed, int max_iter, hipStream_t st) {
   const std::vector<const void *> key = {
       (const void *)(intptr_t)mixed, (const void *)(intptr_t)max_iter, s.d_slice_ptr.p,
-      s.d_sell_col.p, s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, (const void *)(intptr_t)s.n};
+      s.d_sell_col.p, s.d_sell_val.p, s.d_diag.p, (const void *)(intptr_t)s.n};
   if (!w->graph || w->gkey != key) {
     if (w->graph) {
       HIPCHECK(hipGraphExecDestroy(w->graph));
@@ -734,7 +736,7 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
     k_minit<<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, w->dsq.p, w->x.p, w->ys32.p,
                                w->r32[0].p, w->p32[0].p, w->q32[0].p, part);
   } else {
-    k_init<<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, s.d_dinv.p, w->x.p, w->z64[0].p,
+    k_init<<<nb, NT, 0, st>>>(s.n, np, s.d_row_bd.p, b, s.d_diag.p, w->x.p, w->z64[0].p,
                               w->w64[0].p, w->p64[0].p, part);
   }
   s_init<<<1, RT, 0, st>>>(nb, part, S);
@@ -793,7 +795,7 @@ void cg_solve(System &s, const double *b, double *x_out, const admfem_solve_opts
     // otherwise it is replaced by the true one and the iteration goes on
     if (mixed || !converged || rr_true <= w->h_sc->tol2 || it >= o.max_iter || restarts >= 8)
       break;
-    k_restart<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p, s.d_dinv.p,
+    k_restart<<<nb, NT, 0, st>>>(s.n, np, sp, scol, s.d_sell_val.p, s.d_diag.p,
                                  w->x.p, w->z64[0].p, w->z64[1].p, part, S);
     s_restart<<<1, RT, 0, st>>>(nb, part, S, o.max_iter);
     HIPCHECK(hipGetLastError());

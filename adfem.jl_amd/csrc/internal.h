@@ -201,7 +201,7 @@ struct System {
   DevBuf<int> d_slice_ptr;
   DevBuf<int> d_sell_col;      // absolute internal column ids
   DevBuf<double> d_sell_val;   // FP64 values
-  DevBuf<double> d_diag, d_dinv;
+  DevBuf<double> d_diag;
   CGWork *work = nullptr;
   size_t device_bytes() const;
 };
@@ -261,7 +261,7 @@ void launch_p2_laplace_term_nu_grad(const Mesh &m, const double *go, const doubl
 // SELL values, diagonal and its inverse (skipped when null), rhs.  (The
 // mixed-precision solvers build their FP32 matrix from these, scaled.)
 struct SellOut {
-  double *sv, *diag, *dinv, *rhs;
+  double *sv, *diag, *rhs;
 };
 void launch_assemble(System &s, const admfem_problem &p, hipStream_t st);
 void launch_assemble_to(const System &s, const double2 *xy, const admfem_problem &p,

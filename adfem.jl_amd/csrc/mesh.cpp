@@ -522,7 +522,7 @@ size_t System::device_bytes() const {
   return d_rp.bytes() + d_col.bytes() + d_row_bd.bytes() + d_blk_eptr.bytes() +
          d_blk_rec.bytes() + d_blk_vptr.bytes() + d_blk_vert.bytes() + d_pairs.bytes() + val.bytes() + rhs.bytes() +
          d_slice_ptr.bytes() + d_sell_col.bytes() +
-         d_sell_val.bytes() + d_diag.bytes() + d_dinv.bytes();
+         d_sell_val.bytes() + d_diag.bytes();
 }
 
 // SELL-64 layout of the off-diagonal entries of the system's CSR pattern
@@ -571,7 +571,6 @@ static void build_solver_layout(System *s) {
   s->d_sell_val.alloc(sell_col.size());
   HIPCHECK(hipMemset(s->d_sell_val.p, 0, s->d_sell_val.bytes()));
   s->d_diag.alloc(s->npad);
-  s->d_dinv.alloc(s->npad);
 }
 
 System *system_build(Mesh *m, const long long *bdnode, int nbd) {

@@ -294,7 +294,7 @@ struct Server {
     inputs(q);
     admfem_problem p = problem(q);
     // FP64 matrix and rhs; the FP32 scaled matrix of the inner iteration
-    launch_assemble_to(s, q.xy, p, SellOut{q.sv64, q.diag64, nullptr, q.b}, st);
+    launch_assemble_to(s, q.xy, p, SellOut{q.sv64, q.diag64, q.b}, st);
     batch_scale(s, q.sv64, q.diag64, q.dsq, q.sv32, st);
   }
 
@@ -631,7 +631,7 @@ struct Server {
   void serve_seq(Req &q, System &s) {
     inputs(q);
     admfem_problem p = problem(q);
-    launch_assemble_to(s, q.xy, p, SellOut{s.d_sell_val.p, s.d_diag.p, s.d_dinv.p, s.rhs.p},
+    launch_assemble_to(s, q.xy, p, SellOut{s.d_sell_val.p, s.d_diag.p, s.rhs.p},
                        st);
     admfem_solve_opts so{ADMFEM_PREC_FP64, o.max_iter, o.rtol, 0, 0.0};
     admfem_solve_stats s0{}, s1{};
