@@ -360,13 +360,13 @@ int admfem_system_info(admfem_system h, int64_t info[4]) {
 int admfem_system_pattern(admfem_system h, int *row_ptr, int *col) {
   return guarded([&] {
     System &s = SY(h);
-    const Mesh &m = *s.mesh;
+    const std::vector<int> &P = s.uperm(), &IP = s.uiperm();
     int pos = 0;
     row_ptr[0] = 0;
     for (int u = 0; u < s.n; u++) {
-      int r = m.perm[u];
+      int r = P[u];
       int start = pos;
-      for (int k = s.rp[r]; k < s.rp[r + 1]; k++) col[pos++] = m.iperm[s.col[k]];
+      for (int k = s.rp[r]; k < s.rp[r + 1]; k++) col[pos++] = IP[s.col[k]];
       std::sort(col + start, col + pos);
       row_ptr[u + 1] = pos;
     }
@@ -376,7 +376,7 @@ int admfem_system_pattern(admfem_system h, int *row_ptr, int *col) {
 int admfem_system_csr(admfem_system h, int *row_ptr, int *col, double *val, double *rhs) {
   return guarded([&] {
     System &s = SY(h);
-    const Mesh &m = *s.mesh;
+    const std::vector<int> &P = s.uperm(), &IP = s.uiperm();
     // the assembled values live in the solver format: diagonal + SELL slices
     std::vector<double> sv(std::max(1, s.slice_ptr.back())), dg(s.npad), b(s.n);
     HIPCHECK(hipDeviceSynchronize());
@@ -387,12 +387,12 @@ int admfem_system_csr(admfem_system h, int *row_ptr, int *col, double *val, doub
     row_ptr[0] = 0;
     std::vector<std::pair<int, double>> row;
     for (int u = 0; u < s.n; u++) {
-      int r = m.perm[u];
+      int r = P[u];
       row.clear();
       const int off = s.slice_ptr[r >> 6] + (r & 63);
       int kk = 0;
       for (int k = s.rp[r]; k < s.rp[r + 1]; k++)
-        row.push_back({m.iperm[s.col[k]], s.col[k] == r ? dg[r] : sv[off + 64 * kk++]});
+        row.push_back({IP[s.col[k]], s.col[k] == r ? dg[r] : sv[off + 64 * kk++]});
       std::sort(row.begin(), row.end(),
                 [](const std::pair<int, double> &a, const std::pair<int, double> &c) {
                   return a.first < c.first;
@@ -413,7 +413,7 @@ static void assemble_impl(admfem_system h, const admfem_problem *p, hipStream_t 
   REQUIRE(p, ADMFEM_ERR_ARG, "null problem");
   admfem_problem q = *p;
   if (p->u_prev) {  // node vector: user -> internal order
-    launch_scatter(s.mesh->d_perm.p, p->u_prev, h->tmp_int.p, s.n, st);
+    launch_scatter(s.d_uperm(), p->u_prev, h->tmp_int.p, s.n, st);
     q.u_prev = h->tmp_int.p;
   }
   launch_assemble(s, q, st);
@@ -431,12 +431,12 @@ int admfem_system_solve(admfem_system h, const double *b, double *x,
     hipStream_t str = S(stream);
     const double *b_int = s.rhs.p;
     if (b) {
-      launch_scatter(s.mesh->d_perm.p, b, h->g_int.p, s.n, str);
+      launch_scatter(s.d_uperm(), b, h->g_int.p, s.n, str);
       b_int = h->g_int.p;
     }
     admfem_solve_stats tmp{};
     cg_solve(s, b_int, h->u_int.p, *o, tmp, str);
-    launch_scatter(s.mesh->d_iperm.p, h->u_int.p, x, s.n, str);
+    launch_scatter(s.d_uiperm(), h->u_int.p, x, s.n, str);
     HIPCHECK(hipStreamSynchronize(str));
     if (st) *st = tmp;
     if (tmp.status != ADMFEM_OK) {
@@ -483,8 +483,8 @@ int admfem_system_forward_adjoint(admfem_system h, const admfem_problem *p,
     cg_solve(s, h->g_int.p, h->lam_int.p, *o, s1, str);
     if (grad_kappa)
       launch_kappa_gradient(s, h->u_int.p, h->lam_int.p, p->lap_coef, grad_kappa, str);
-    if (u) launch_scatter(s.mesh->d_iperm.p, h->u_int.p, u, s.n, str);
-    if (lam) launch_scatter(s.mesh->d_iperm.p, h->lam_int.p, lam, s.n, str);
+    if (u) launch_scatter(s.d_uiperm(), h->u_int.p, u, s.n, str);
+    if (lam) launch_scatter(s.d_uiperm(), h->lam_int.p, lam, s.n, str);
     HIPCHECK(hipStreamSynchronize(str));
     if (loss) *loss = L;
     if (st) {

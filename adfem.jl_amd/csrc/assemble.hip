@@ -481,6 +481,10 @@ inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 
 void launch_assemble_to(const System &s, const double2 *xy, const admfem_problem &p,
                         const SellOut &o, hipStream_t st) {
+  if (s.degree == 2) {
+    launch_assemble_p2(s, xy, p, o, st);
+    return;
+  }
   const Mesh &m = *s.mesh;
   const bool mass = p.mass_coef != 0.0 || p.u_prev;
   const bool compact = p.rho == nullptr;
@@ -529,6 +533,10 @@ void launch_to_sell(System &s, hipStream_t st) {
 
 void launch_kappa_gradient_to(const System &s, const double2 *xy, const double *u_int,
                               const double *lam_int, double scale, double *gk, hipStream_t st) {
+  if (s.degree == 2) {
+    launch_kappa_gradient_p2(s, xy, u_int, lam_int, nullptr, scale, gk, st);
+    return;
+  }
   const Mesh &m = *s.mesh;
   kappa_grad_kernel<false><<<nblk(m.nelem, 256), 256, 0, st>>>(
       m.nelem, m.rule, m.conn.p, xy, s.d_row_bd.p, u_int, lam_int, nullptr, scale, gk);
@@ -549,7 +557,11 @@ void launch_kappa_gradient_user(const System &s, const double *u, const double *
                                 double2 *ul_int, double scale, double *gk, hipStream_t st) {
   const Mesh &m = *s.mesh;
   if (s.n > 0)
-    scatter_pair_kernel<<<nblk(s.n, 256), 256, 0, st>>>(m.d_perm.p, u, lam, ul_int, s.n);
+    scatter_pair_kernel<<<nblk(s.n, 256), 256, 0, st>>>(s.d_uperm(), u, lam, ul_int, s.n);
+  if (s.degree == 2) {
+    launch_kappa_gradient_p2(s, m.xy.p, nullptr, nullptr, ul_int, scale, gk, st);
+    return;
+  }
   kappa_grad_kernel<true><<<nblk(m.nelem, 256), 256, 0, st>>>(
       m.nelem, m.rule, m.conn.p, m.xy.p, s.d_row_bd.p, nullptr, nullptr, ul_int, scale, gk);
   HIPCHECK(hipGetLastError());
@@ -574,7 +586,7 @@ void launch_loss(const System &s, const double *u_int, const int *obs_idx,
                  const double *obs_val, int nobs, double *g_int, double *loss2,
                  hipStream_t st) {
   fill_zero<<<nblk(s.npad, 256), 256, 0, st>>>(g_int, s.npad);
-  loss_kernel<<<1, 1024, 0, st>>>(s.n, s.mesh->d_perm.p, obs_idx, obs_val, nobs, u_int, g_int,
+  loss_kernel<<<1, 1024, 0, st>>>(s.n, s.d_uperm(), obs_idx, obs_val, nobs, u_int, g_int,
                                   loss2);
   HIPCHECK(hipGetLastError());
 }

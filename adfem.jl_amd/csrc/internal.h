@@ -79,6 +79,11 @@ __host__ __device__ inline uint32_t slot_of(uint32_t w, int q) {
 // index in its assembly block's element list.
 constexpr uint32_t PAIR_DIAG = 62;
 constexpr int PAIR_MAX_ELEMS = 4096;
+// P2 pair word (64 bits): bits 0-2 the row dof's local index p in the
+// element, six 6-bit codes (as above) for the element's dofs q = 0..5 from
+// bit 3, and the element's index in its block's element list from bit 40.
+constexpr int P2_PAIR_CODE0 = 3;
+constexpr int P2_PAIR_ELEM = 40;
 // An assembly block's element record is {element id, its 3 vertices as
 // 10-bit indices into the block's sorted list of distinct vertices}.
 constexpr int BLK_VERT_BITS = 10;
@@ -174,7 +179,8 @@ void mesh_edges(Mesh &m);  // fills m.edges/m.nedges (MFEM numbering), m.bd_node
 struct CGWork;
 struct System {
   Mesh *mesh = nullptr;
-  int n = 0;       // rows (= nnode)
+  int degree = 1;  // the mesh's element degree
+  int n = 0;       // rows (P1: nnode, P2: ndof)
   int npad = 0;    // rows padded to a multiple of 64
   int nbd = 0;     // entries of bdnode as given
   int nbd_unique = 0;
@@ -202,8 +208,20 @@ struct System {
   DevBuf<int> d_sell_col;      // absolute internal column ids
   DevBuf<double> d_sell_val;   // FP64 values
   DevBuf<double> d_diag;
+  // P2: the internal dof order (each vertex dof at the mesh's node position,
+  // followed by the edge dofs whose lower-numbered end it is), the element
+  // dofs as internal rows ([6][nelem]), per row its incidences, and per
+  // assembly block its element ids and a 64-bit pair word per incidence
+  std::vector<int> perm2, iperm2;  // user dof id <-> internal row
+  DevBuf<int> d_perm2, d_iperm2, d_conn2, d_inc_ptr, d_blk_elem;
+  DevBuf<uint64_t> d_pairs2;
   CGWork *work = nullptr;
   size_t device_bytes() const;
+  // user node / dof id -> internal row, and back (mesh systems)
+  const std::vector<int> &uperm() const { return degree == 2 ? perm2 : mesh->perm; }
+  const std::vector<int> &uiperm() const { return degree == 2 ? iperm2 : mesh->iperm; }
+  const int *d_uperm() const { return degree == 2 ? d_perm2.p : mesh->d_perm.p; }
+  const int *d_uiperm() const { return degree == 2 ? d_iperm2.p : mesh->d_iperm.p; }
 };
 
 System *system_build(Mesh *m, const long long *bdnode, int nbd);
@@ -258,8 +276,9 @@ void launch_p2_laplace_term_nu_grad(const Mesh &m, const double *go, const doubl
                                     double *gnu, hipStream_t st);
 // assemble.hip
 // Fused assembly of a mesh system straight into the solver format: FP64
-// SELL values, diagonal and its inverse (skipped when null), rhs.  (The
-// mixed-precision solvers build their FP32 matrix from these, scaled.)
+// SELL values, diagonal, rhs.  (The mixed-precision solvers build their FP32
+// matrix from these, scaled.)  P1 and P2 systems (the P2 kernels are in
+// p2_system.hip).
 struct SellOut {
   double *sv, *diag, *rhs;
 };
@@ -299,6 +318,14 @@ void run_impose_dirichlet_grad(const double *gov, const double *gorhs,
                                const double *bdval, int bdN, int N,
                                double *gvv, double *grhs, double *gbd,
                                hipStream_t st);
+// p2_system.hip: assembly and adjoint gradient of a P2 system (the
+// launchers above dispatch to these); u and lam internal order, either
+// separate (u_int, lam_int) or as (u, lam) pairs (ul_int)
+void launch_assemble_p2(const System &s, const double2 *xy, const admfem_problem &p,
+                        const SellOut &o, hipStream_t st);
+void launch_kappa_gradient_p2(const System &s, const double2 *xy, const double *u_int,
+                              const double *lam_int, const double2 *ul_int, double scale,
+                              double *gk, hipStream_t st);
 // noalloc.hip: the gradient with its (u, lam) staging in a fixed device array
 size_t kappa_gradient_noalloc_max_nodes();
 void kappa_gradient_noalloc(const System &s, const double *u, const double *lam, double scale,

@@ -522,7 +522,8 @@ size_t System::device_bytes() const {
   return d_rp.bytes() + d_col.bytes() + d_row_bd.bytes() + d_blk_eptr.bytes() +
          d_blk_rec.bytes() + d_blk_vptr.bytes() + d_blk_vert.bytes() + d_pairs.bytes() + val.bytes() + rhs.bytes() +
          d_slice_ptr.bytes() + d_sell_col.bytes() +
-         d_sell_val.bytes() + d_diag.bytes();
+         d_sell_val.bytes() + d_diag.bytes() + d_perm2.bytes() + d_iperm2.bytes() +
+         d_conn2.bytes() + d_inc_ptr.bytes() + d_blk_elem.bytes() + d_pairs2.bytes();
 }
 
 // SELL-64 layout of the off-diagonal entries of the system's CSR pattern
@@ -573,12 +574,190 @@ static void build_solver_layout(System *s) {
   s->d_diag.alloc(s->npad);
 }
 
+// The fused system of a P2 mesh.  Rows are the ndof dofs (vertices, then
+// nnode + edge id, deps/MFEM/Common.cpp:63-72) in an internal order built
+// from the mesh's node order: each vertex dof at its node's position,
+// followed by the edge dofs whose lower-numbered end it is (ascending other
+// end), so the dof graph keeps the node graph's banded fronts.  Post-BC
+// pattern and Dirichlet bookkeeping as for P1 (ImposeDirichlet.h:27-51 on
+// the P2 triplets); the assembly blocks are 64-row SELL slices with the
+// distinct elements incident to their interior rows and one 64-bit pair word
+// per (interior row, incident element).
+static System *system_build_p2(Mesh *m, const long long *bdnode, int nbd) {
+  REQUIRE(m->ndof < (int64_t)1 << 30, ADMFEM_ERR_UNSUPPORTED,
+          "too many P2 dofs for the fused system");
+  System *s = new System();
+  try {
+    const int nn = m->nnode, ne = m->nelem, n = (int)m->ndof;
+    s->mesh = m;
+    s->degree = 2;
+    s->n = n;
+    s->npad = (n + 63) / 64 * 64;
+    s->nbd = nbd;
+    // internal dof order: edge dofs bucketed by their lower-numbered end
+    {
+      std::vector<int> lo(n - nn, -1), hi(n - nn, -1);
+      for (int e = 0; e < ne; e++)
+        for (int k = 0; k < 3; k++) {
+          const int d = m->dofs[6 * (size_t)e + 3 + k] - nn;
+          if (lo[d] >= 0) continue;
+          const int a = m->perm[m->elems[3 * (size_t)e + k]];
+          const int b = m->perm[m->elems[3 * (size_t)e + (k + 1) % 3]];
+          lo[d] = std::min(a, b);
+          hi[d] = std::max(a, b);
+        }
+      std::vector<int> bptr(nn + 1, 0), bk(n - nn);
+      for (int d = 0; d < n - nn; d++) bptr[lo[d] + 1]++;
+      for (int r = 0; r < nn; r++) bptr[r + 1] += bptr[r];
+      {
+        std::vector<int> pos(bptr.begin(), bptr.end() - 1);
+        for (int d = 0; d < n - nn; d++) bk[pos[lo[d]]++] = d;
+      }
+      s->iperm2.reserve(n);
+      for (int r = 0; r < nn; r++) {
+        s->iperm2.push_back(m->iperm[r]);
+        std::sort(bk.begin() + bptr[r], bk.begin() + bptr[r + 1],
+                  [&](int a, int b) { return hi[a] < hi[b]; });
+        for (int k = bptr[r]; k < bptr[r + 1]; k++) s->iperm2.push_back(nn + bk[k]);
+      }
+      s->perm2.assign(n, 0);
+      for (int k = 0; k < n; k++) s->perm2[s->iperm2[k]] = k;
+    }
+    // element dofs as internal rows, row -> (element, local dof) incidences
+    // in ascending element order
+    std::vector<int> conn2(6 * (size_t)ne), iptr(n + 1, 0), ie(6 * (size_t)ne);
+    std::vector<uint8_t> ip(6 * (size_t)ne);
+    for (int e = 0; e < ne; e++)
+      for (int q = 0; q < 6; q++) {
+        const int r = s->perm2[m->dofs[6 * (size_t)e + q]];
+        conn2[(size_t)q * ne + e] = r;
+        iptr[r + 1]++;
+      }
+    for (int r = 0; r < n; r++) iptr[r + 1] += iptr[r];
+    {
+      std::vector<int> pos(iptr.begin(), iptr.end() - 1);
+      for (int e = 0; e < ne; e++)
+        for (int q = 0; q < 6; q++) {
+          const int t = pos[conn2[(size_t)q * ne + e]]++;
+          ie[t] = e;
+          ip[t] = (uint8_t)q;
+        }
+    }
+    s->row_bd.assign(n, -1);
+    for (int i = 0; i < nbd; i++) {
+      const long long b = bdnode[i];
+      REQUIRE(b >= 1 && b <= n, ADMFEM_ERR_ARG, "bdnode entry out of range (1-based dof ids)");
+      s->row_bd[s->perm2[b - 1]] = i;  // later entries win
+    }
+    for (int r = 0; r < n; r++) s->nbd_unique += s->row_bd[r] >= 0;
+    // post-BC pattern: per interior row the interior dofs of its elements,
+    // per boundary row its diagonal; lengths, then columns, on host threads
+    auto row_cols = [&](int r, std::vector<int> &c) {
+      c.clear();
+      if (s->row_bd[r] >= 0) {
+        c.push_back(r);
+        return;
+      }
+      for (int t = iptr[r]; t < iptr[r + 1]; t++)
+        for (int q = 0; q < 6; q++) {
+          const int cq = conn2[(size_t)q * ne + ie[t]];
+          if (s->row_bd[cq] < 0) c.push_back(cq);
+        }
+      std::sort(c.begin(), c.end());
+      c.erase(std::unique(c.begin(), c.end()), c.end());
+      REQUIRE((int)c.size() <= MAX_ROW, ADMFEM_ERR_UNSUPPORTED,
+              "a P2 dof has more than 61 neighbours");
+    };
+    s->rp.assign(n + 1, 0);
+    parallel_chunks(n, PAR_GRAIN, [&](int64_t b, int64_t e1, int) {
+      std::vector<int> c;
+      for (int r = (int)b; r < (int)e1; r++) {
+        row_cols(r, c);
+        s->rp[r + 1] = (int)c.size();
+      }
+    });
+    for (int r = 0; r < n; r++) s->rp[r + 1] += s->rp[r];
+    s->col.resize(s->rp[n]);
+    parallel_chunks(n, PAR_GRAIN, [&](int64_t b, int64_t e1, int) {
+      std::vector<int> c;
+      for (int r = (int)b; r < (int)e1; r++) {
+        row_cols(r, c);
+        std::copy(c.begin(), c.end(), s->col.begin() + s->rp[r]);
+      }
+    });
+    build_solver_layout(s);
+    REQUIRE(s->sell_wmax < (int)PAIR_DIAG, ADMFEM_ERR_UNSUPPORTED, "row wider than 61");
+    // assembly blocks (host threads over chunks of blocks, element lists
+    // concatenated in block order)
+    const int nblk = s->npad / AS_ROWS;
+    std::vector<int> beptr(nblk + 1, 0);
+    std::vector<uint64_t> pairs(std::max<size_t>(1, ie.size()), 0);
+    std::vector<std::vector<int>> part(16);
+    std::vector<int> part_max(16, 0);
+    parallel_chunks(nblk, 64, [&](int64_t bb, int64_t be, int c) {
+      std::vector<int> &bl = part[c];
+      std::vector<int> offd;
+      for (int b = (int)bb; b < (int)be; b++) {
+        const int r0 = b * AS_ROWS, r1 = std::min(r0 + AS_ROWS, n);
+        const size_t first = bl.size();
+        for (int r = r0; r < r1; r++)
+          if (s->row_bd[r] < 0)
+            for (int t = iptr[r]; t < iptr[r + 1]; t++) bl.push_back(ie[t]);
+        std::sort(bl.begin() + first, bl.end());
+        bl.erase(std::unique(bl.begin() + first, bl.end()), bl.end());
+        const int nb = (int)(bl.size() - first);
+        REQUIRE(nb <= PAIR_MAX_ELEMS, ADMFEM_ERR_UNSUPPORTED,
+                "assembly block has too many elements");
+        part_max[c] = std::max(part_max[c], nb);
+        beptr[b + 1] = nb;
+        for (int r = r0; r < r1; r++) {
+          if (s->row_bd[r] >= 0) continue;
+          offd.clear();
+          for (int k = s->rp[r]; k < s->rp[r + 1]; k++)
+            if (s->col[k] != r) offd.push_back(s->col[k]);
+          for (int t = iptr[r]; t < iptr[r + 1]; t++) {
+            const int e = ie[t];
+            const uint64_t le = (uint64_t)(std::lower_bound(bl.begin() + first, bl.end(), e) -
+                                           (bl.begin() + first));
+            uint64_t w = (uint64_t)ip[t] | (le << P2_PAIR_ELEM);
+            for (int q = 0; q < 6; q++) {
+              const int cq = conn2[(size_t)q * ne + e];
+              uint64_t code;
+              if (s->row_bd[cq] >= 0) code = SLOT_LIFT;
+              else if (cq == r) code = PAIR_DIAG;
+              else code = (uint64_t)(std::lower_bound(offd.begin(), offd.end(), cq) - offd.begin());
+              w |= code << (P2_PAIR_CODE0 + SLOT_BITS * q);
+            }
+            pairs[t] = w;
+          }
+        }
+      }
+    });
+    for (int b = 0; b < nblk; b++) beptr[b + 1] += beptr[b];
+    for (int c = 0; c < 16; c++) s->asm_max_e = std::max(s->asm_max_e, part_max[c]);
+    std::vector<int> belem(std::max<int>(1, beptr[nblk]), 0);
+    parallel_chunks(nblk, 64, [&](int64_t bb, int64_t, int c) {
+      std::copy(part[c].begin(), part[c].end(), belem.begin() + beptr[bb]);
+    });
+    s->d_row_bd.upload(s->row_bd);
+    s->d_perm2.upload(s->perm2);
+    s->d_iperm2.upload(s->iperm2);
+    s->d_conn2.upload(conn2);
+    s->d_inc_ptr.upload(iptr);
+    s->d_blk_eptr.upload(beptr);
+    s->d_blk_elem.upload(belem);
+    s->d_pairs2.upload(pairs);
+  } catch (...) {
+    system_free(s);
+    throw;
+  }
+  return s;
+}
+
 System *system_build(Mesh *m, const long long *bdnode, int nbd) {
   REQUIRE(m, ADMFEM_ERR_ARG, "null mesh");
-  REQUIRE(m->degree == 1, ADMFEM_ERR_UNSUPPORTED,
-          "the fused system path is built for P1 meshes; for P2 assemble with the COO "
-          "ops and solve with admfem_sparse_solve");
   REQUIRE(nbd >= 0 && (nbd == 0 || bdnode), ADMFEM_ERR_ARG, "bad bdnode");
+  if (m->degree == 2) return system_build_p2(m, bdnode, nbd);
   System *s = new System();
   try {
     const int n = m->nnode;

@@ -771,6 +771,9 @@ Server *server_create(const admfem_server_opts &o) {
 void server_free(Server *sv) { delete sv; }
 
 int server_add_topology(Server *sv, System *s, double jitter) {
+  REQUIRE(s->degree == 1, ADMFEM_ERR_UNSUPPORTED,
+          "the serving engine takes P1 topologies; solve P2 systems with "
+          "admfem_system_forward_adjoint");
   Topo T;
   T.sys = s;
   T.jitter = jitter;

@@ -248,8 +248,9 @@ int admfem_impose_dirichlet_grad(const double *grad_ov, const double *grad_orhs,
 /* bdnode: host, 1-based (as bcnode returns), duplicates allowed (the last
  * bdval entry of a node wins).  The post-BC CSR pattern is the reference's:
  * interior-interior element pairs (explicit zeros kept) plus the boundary
- * diagonals.  P1 meshes only (ADMFEM_ERR_UNSUPPORTED for P2: assemble P2
- * with the section C ops, then admfem_sparse_solve). */
+ * diagonals.  P1 and P2 meshes (P2: rows are the ndof dofs, bdnode holds
+ * 1-based dof ids as bcnode returns them, every node vector below is ndof
+ * long). */
 int admfem_system_create(admfem_mesh m, const long long *bdnode, int nbd,
                          admfem_system *out);
 int admfem_system_destroy(admfem_system s);
