@@ -25,7 +25,7 @@ COMMON = [
 
 SOURCES = ["mesh.cpp", "api.cpp", "geometry.hip", "assemble.hip", "cg.hip",
            "dirichlet.hip", "batched.hip", "server.cpp", "sparse_op.hip", "fields.hip",
-           "p2.hip", "p2_system.hip", "noalloc.hip"]
+           "p2.hip", "p2_system.hip", "noalloc.hip", "topology.hip"]
 
 
 def _deps_mtime():

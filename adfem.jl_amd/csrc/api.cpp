@@ -107,7 +107,7 @@ int admfem_mesh_info(admfem_mesh h, int64_t info[8]) {
     info[3] = m.ndof;
     info[4] = m.nd;
     info[5] = m.nedges;
-    info[6] = (int64_t)m.col.size();
+    info[6] = m.nnz_full;
     info[7] = m.rule.nq;
   });
 }
@@ -147,7 +147,7 @@ int admfem_mesh_dofs(admfem_mesh h, int *dofs) {
 int admfem_mesh_edges(admfem_mesh h, long long *edges) {
   return guarded([&] {
     Mesh &m = M(h);
-    mesh_edges(m);
+    mesh_edges_host(m);
     std::copy(m.edges.begin(), m.edges.end(), edges);
   });
 }
@@ -171,6 +171,7 @@ int admfem_mesh_bcnode(admfem_mesh h, long long *out, int *count) {
 int admfem_mesh_pattern(admfem_mesh h, int *row_ptr, int *col) {
   return guarded([&] {
     Mesh &m = M(h);
+    mesh_user_pattern(m);
     std::copy(m.urp.begin(), m.urp.end(), row_ptr);
     std::copy(m.ucol.begin(), m.ucol.end(), col);
   });
@@ -350,7 +351,7 @@ int admfem_system_info(admfem_system h, int64_t info[4]) {
   return guarded([&] {
     System &s = SY(h);
     info[0] = s.n;
-    info[1] = (int64_t)s.col.size();
+    info[1] = s.nnz;
     info[2] = s.nbd_unique;
     info[3] = (int64_t)(s.device_bytes() + s.mesh->device_bytes() +
                         6 * sizeof(double) * (size_t)s.npad);  // + the handle's vectors
@@ -360,6 +361,7 @@ int admfem_system_info(admfem_system h, int64_t info[4]) {
 int admfem_system_pattern(admfem_system h, int *row_ptr, int *col) {
   return guarded([&] {
     System &s = SY(h);
+    system_host_pattern(s);
     const std::vector<int> &P = s.uperm(), &IP = s.uiperm();
     int pos = 0;
     row_ptr[0] = 0;
@@ -376,6 +378,7 @@ int admfem_system_pattern(admfem_system h, int *row_ptr, int *col) {
 int admfem_system_csr(admfem_system h, int *row_ptr, int *col, double *val, double *rhs) {
   return guarded([&] {
     System &s = SY(h);
+    system_host_pattern(s);
     const std::vector<int> &P = s.uperm(), &IP = s.uiperm();
     // the assembled values live in the solver format: diagonal + SELL slices
     std::vector<double> sv(std::max(1, s.slice_ptr.back())), dg(s.npad), b(s.n);
@@ -537,7 +540,7 @@ int admfem_system_spmv_repeat(admfem_system h, int precision, int repeat, double
     spmv_repeat(s, precision, repeat, S(stream));
     if (bytes) {
       const double vb = precision == ADMFEM_PREC_MIXED ? 4.0 : 8.0;
-      const double nnz = (double)s.col.size(), n = s.n;
+      const double nnz = (double)s.nnz, n = s.n;
       // one CG iteration of the canonical CSR form: the SpMV plus the
       // iteration's vector updates (6 n s)
       *bytes = nnz * (vb + 4.0) + 4.0 * (n + 1) + 2.0 * n * vb + 6.0 * n * vb;
@@ -642,7 +645,7 @@ long long *init_nnfem_mesh(double *vertices, int num_vertices, int *element_indi
       }
       g_default = h;
     }
-    mesh_edges(*m);
+    mesh_edges_host(*m);
     *nedges = m->nedges;
     ret = (long long *)malloc(sizeof(long long) * std::max<size_t>(1, m->edges.size()));
     REQUIRE(ret, ADMFEM_ERR_ARG, "host out of memory");

@@ -140,18 +140,15 @@ struct Mesh {
   QuadRule rule{};
   int64_t ngauss = 0;
   // host copies
-  std::vector<double> coords;   // user order, nnode x 2
   std::vector<int> elems;       // oriented, user node ids, nelem x 3
   std::vector<int> perm;        // user node id -> internal id
   std::vector<int> iperm;       // internal id -> user node id
-  std::vector<int> n2e_ptr;     // internal row -> incidences
-  std::vector<int> n2e_elem;    // incident element ids (ascending)
-  std::vector<uint8_t> n2e_p;   // local vertex of the row node
-  std::vector<int> rp;          // pre-BC pattern, internal numbering
-  std::vector<int> col;
-  std::vector<int> urp, ucol;   // pre-BC pattern, user numbering (rows sorted)
+  int64_t nnz_full = 0;         // entries of the pre-BC pattern
+  std::vector<int> urp, ucol;   // pre-BC pattern, user numbering (rows sorted),
+                                // made on request (mesh_user_pattern)
   int64_t nedges = -1;
-  std::vector<long long> edges;  // lazily built, 2*nedges, 1-based user ids
+  std::vector<long long> edges;  // host copy of d_edges, made on request (mesh_edges_host)
+  DevBuf<long long> d_edges;     // 2*nedges, 1-based user ids: [min ...; max ...]
   std::vector<char> bd_node;     // built with edges: node on a boundary edge
   std::vector<char> bd_edge;     // built with edges (P2): edge used by one element
   std::vector<int> elem_edge;    // P2: MFEM id of element edge j at 3e+j
@@ -162,9 +159,9 @@ struct Mesh {
   DevBuf<int> conn;             // [3][nelem] internal ids (SoA)
   DevBuf<int> conn_user;        // [3][nelem] user ids (SoA)
   DevBuf<int> d_perm, d_iperm;
-  DevBuf<int> d_n2e_ptr, d_n2e_elem;
+  DevBuf<int> d_n2e_ptr, d_n2e_elem;  // internal row -> incident elements (ascending)
   DevBuf<uint32_t> d_slots_full;  // per incidence, slots in the pre-BC row
-  DevBuf<int> d_rp, d_col;
+  DevBuf<int> d_rp, d_col;        // pre-BC pattern, internal numbering
   DevBuf<double> gx, gy, gw, area;  // geometry (user element order)
   DevBuf<int> conn_dof;             // P2: [6][nelem] user dof ids (SoA)
   DevBuf<int> d_d2e_ptr, d_d2e;     // P2: dof -> (element << 3 | local dof), ascending
@@ -173,7 +170,15 @@ struct Mesh {
 
 Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
                  int order, int degree = 1);
-void mesh_edges(Mesh &m);  // fills m.edges/m.nedges (MFEM numbering), m.bd_node
+// topology.hip: the MFEM edge table on the device (m.nedges, m.d_edges,
+// m.bd_node; P2: m.elem_edge, m.bd_edge), and its host copy m.edges
+void mesh_edges(Mesh &m);
+void mesh_edges_host(Mesh &m);
+// topology.hip: node order (host perm / iperm) and the device topology
+// (connectivity, incidences, pre-BC pattern, slot words, coordinates) of a
+// mesh whose elems are set and oriented; coords in user order (host)
+void mesh_topology(Mesh &m, const double *coords, hipStream_t st);
+void mesh_user_pattern(Mesh &m);
 
 // ------------------------------------------------------------------ system
 struct CGWork;
@@ -185,7 +190,10 @@ struct System {
   int nbd = 0;     // entries of bdnode as given
   int nbd_unique = 0;
   std::vector<int> row_bd;   // internal row -> index into bdval (-1 interior)
-  std::vector<int> rp, col;  // post-BC pattern, internal numbering
+  int64_t nnz = 0;           // entries of the post-BC pattern
+  // post-BC pattern, internal numbering (host copy of d_rp / d_col; for a
+  // P1 mesh system made on request, system_host_pattern)
+  std::vector<int> rp, col;
   // device: pattern + assembly maps
   DevBuf<int> d_rp, d_col, d_row_bd;
   // assembly: per block of AS_ROWS rows the distinct elements incident to
@@ -226,6 +234,12 @@ struct System {
 
 System *system_build(Mesh *m, const long long *bdnode, int nbd);
 void system_free(System *s);
+// topology.hip: solver layout from d_rp / d_col (and s->nnz); the P1
+// system's pattern, layout and assembly blocks from s->row_bd; host copy of
+// the pattern
+void solver_layout(System *s);
+void system_topology(System *s);
+void system_host_pattern(System &s);
 
 // ------------------------------------------------------------- launchers
 // geometry.hip

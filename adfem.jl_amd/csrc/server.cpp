@@ -704,7 +704,7 @@ double server_spmv_repeat(Server *sv, const int *topos, int nsys, int repeat,
       const System &s = *sv->topos[q.r.topo].sys;
       // one CG iteration of the canonical CSR form: the SpMV's
       // nnz (s + 4) + 4 (n + 1) + 2 n s plus 6 n s of vector updates (s = 4)
-      bytes += (double)s.col.size() * 8.0 + 4.0 * (s.n + 1) + 8.0 * s.n * 4.0;
+      bytes += (double)s.nnz * 8.0 + 4.0 * (s.n + 1) + 8.0 * s.n * 4.0;
     }
     sv->ensure_map(nmap, (size_t)nsys);
     std::vector<std::pair<int, int>> slots(nsys);
