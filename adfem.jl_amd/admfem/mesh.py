@@ -77,15 +77,36 @@ class Mesh:
         self.elem_type = P1 if degree == 1 else P2
         self.degree = degree
         self.nodes = coords.copy()
-        el = np.zeros((self.nelem, 3), dtype=np.int32)
-        _lib.check(L.admfem_mesh_elems(h, _lib.ptr(el)))
-        self.elems = el.astype(np.int64) + 1          # oriented, 1-based
-        dofs = np.zeros((self.nelem, self.elem_ndof), dtype=np.int32)
-        _lib.check(L.admfem_mesh_dofs(h, _lib.ptr(dofs)))
-        self.conn = dofs.astype(np.int64) + 1         # P1: the vertices; P2: + edge dofs
-        ed = np.zeros(2 * self.nedge, dtype=np.int64)
-        _lib.check(L.admfem_mesh_edges(h, _lib.ptr(ed)))
-        self.edges = ed.reshape(2, self.nedge).T.copy()
+        self._elems = self._conn = self._edges = None
+
+    # The element, dof and edge tables (the Julia struct's fields) are
+    # copied from the library on first access.
+    @property
+    def elems(self):
+        """nelem x 3, oriented, 1-based."""
+        if self._elems is None:
+            el = np.zeros((self.nelem, 3), dtype=np.int32)
+            _lib.check(_lib.lib().admfem_mesh_elems(self._h, _lib.ptr(el)))
+            self._elems = el.astype(np.int64) + 1
+        return self._elems
+
+    @property
+    def conn(self):
+        """nelem x elem_ndof, 1-based (P1: the vertices; P2: + edge dofs)."""
+        if self._conn is None:
+            dofs = np.zeros((self.nelem, self.elem_ndof), dtype=np.int32)
+            _lib.check(_lib.lib().admfem_mesh_dofs(self._h, _lib.ptr(dofs)))
+            self._conn = dofs.astype(np.int64) + 1
+        return self._conn
+
+    @property
+    def edges(self):
+        """nedge x 2, 1-based, MFEM numbering."""
+        if self._edges is None:
+            ed = np.zeros(2 * self.nedge, dtype=np.int64)
+            _lib.check(_lib.lib().admfem_mesh_edges(self._h, _lib.ptr(ed)))
+            self._edges = ed.reshape(2, self.nedge).T.copy()
+        return self._edges
 
     @property
     def handle(self):

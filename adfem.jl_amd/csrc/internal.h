@@ -57,6 +57,10 @@ struct DevBuf {
     if (count) HIPCHECK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
   }
   void upload(const std::vector<T> &v) { upload(v.data(), v.size()); }
+  void swap(DevBuf &o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
   size_t bytes() const { return n * sizeof(T); }
 };
 
@@ -240,6 +244,13 @@ void system_free(System *s);
 void solver_layout(System *s);
 void system_topology(System *s);
 void system_host_pattern(System &s);
+// topology.hip: graph helpers over a device CSR pattern
+void csr_row_ptr(const int *srow, int m, int n, int *rp, hipStream_t st);  // rows sorted
+int64_t device_bandwidth(int n, const int *rp, const int *col, const int *inv, hipStream_t st);
+int device_first_coupled(int n, const int *rp, const int *col, hipStream_t st);
+int64_t device_bfs_max_level(int n, const int *rp, const int *col, int s, hipStream_t st);
+void rcm(int n, const int *rp, const int *col, DevBuf<int> &order, DevBuf<int> &inv,
+         hipStream_t st);
 
 // ------------------------------------------------------------- launchers
 // geometry.hip

@@ -144,31 +144,6 @@ std::vector<int> rcm_order(int n, const std::vector<int> &rp, const std::vector<
   return order;
 }
 
-// Widest level of a breadth-first search of the node graph from s (its
-// component only): the yardstick for a banded order, whose bandwidth is a
-// small multiple of it (reverse Cuthill-McKee's is at most about twice the
-// widest level of its own level structure).
-int64_t bfs_max_level(int n, const std::vector<int> &rp, const std::vector<int> &col, int s) {
-  std::vector<int> level(n, -1), q;
-  q.reserve(n);
-  q.push_back(s);
-  level[s] = 0;
-  std::vector<int64_t> width(1, 1);
-  for (size_t h = 0; h < q.size(); h++) {
-    const int u = q[h];
-    for (int k = rp[u]; k < rp[u + 1]; k++) {
-      const int v = col[k];
-      if (level[v] < 0) {
-        level[v] = level[u] + 1;
-        if ((size_t)level[v] >= width.size()) width.push_back(0);
-        width[level[v]]++;
-        q.push_back(v);
-      }
-    }
-  }
-  return *std::max_element(width.begin(), width.end());
-}
-
 Mesh *mesh_build(const double *coords, int nnode, const int *elems, int nelem,
                  int order, int degree) {
   REQUIRE(nnode > 0 && nelem > 0, ADMFEM_ERR_ARG, "empty mesh");
@@ -493,20 +468,20 @@ System *system_build(Mesh *m, const long long *bdnode, int nbd) {
 }
 
 // A system with no mesh: the CSR pattern of a sparse matrix given by the
-// caller (the general K\F op), identity node order, no boundary rows.
-System *system_from_csr(int n, std::vector<int> rp, std::vector<int> col) {
-  REQUIRE(n > 0 && (int)rp.size() == n + 1, ADMFEM_ERR_ARG, "bad CSR pattern");
+// caller (the general K\F op), on the device (rp, col are taken over),
+// identity node order, no boundary rows.
+System *system_from_device_csr(int n, DevBuf<int> &rp, DevBuf<int> &col, int64_t nnz) {
+  REQUIRE(n > 0 && (int64_t)rp.n == n + 1, ADMFEM_ERR_ARG, "bad CSR pattern");
   System *s = new System();
   try {
     s->mesh = nullptr;
     s->n = n;
     s->npad = (n + 63) / 64 * 64;
     s->row_bd.assign(n, -1);
-    s->rp = std::move(rp);
-    s->col = std::move(col);
-    s->nnz = (int64_t)s->col.size();
-    s->d_rp.upload(s->rp);
-    s->d_col.upload(s->col);
+    s->nnz = nnz;
+    s->d_rp.swap(rp);
+    s->d_col.swap(col);
+    if (!s->d_col.p) s->d_col.alloc(1);
     solver_layout(s);
     s->d_row_bd.upload(s->row_bd);
   } catch (...) {

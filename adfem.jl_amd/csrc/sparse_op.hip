@@ -33,9 +33,7 @@
 
 namespace admfem {
 
-System *system_from_csr(int n, std::vector<int> rp, std::vector<int> col);
-std::vector<int> rcm_order(int n, const std::vector<int> &rp, const std::vector<int> &col);
-int64_t bfs_max_level(int n, const std::vector<int> &rp, const std::vector<int> &col, int s);
+System *system_from_device_csr(int n, DevBuf<int> &rp, DevBuf<int> &col, int64_t nnz);
 
 namespace {
 
@@ -72,6 +70,28 @@ __global__ void coo_slots(int64_t nnz, const unsigned long long *__restrict__ sk
   start[s] = (int)t;
   srow[s] = (int)(skey[t] / (unsigned long long)n);
   scol[s] = (int)(skey[t] % (unsigned long long)n);
+}
+
+// the slots' keys in the solver's row order: inv[row] * n + inv[col]
+__global__ void reorder_keys(int nslot, int n, const int *__restrict__ srow,
+                             const int *__restrict__ scol, const int *__restrict__ inv,
+                             unsigned long long *__restrict__ key, int *__restrict__ val) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslot) return;
+  key[s] = (unsigned long long)inv[srow[s]] * (unsigned long long)n + (unsigned long long)inv[scol[s]];
+  val[s] = s;
+}
+
+// sorted position i holds slot sval[i]: its new row and column, and the
+// slot's place in the reordered pattern
+__global__ void reorder_out(int nslot, int n, const unsigned long long *__restrict__ skey,
+                            const int *__restrict__ sval, int *__restrict__ nrow,
+                            int *__restrict__ ncol, int *__restrict__ pos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslot) return;
+  nrow[i] = (int)(skey[i] / (unsigned long long)n);
+  ncol[i] = (int)(skey[i] % (unsigned long long)n);
+  pos[sval[i]] = i;
 }
 
 // any index differing from the kept pattern's raises the flag (plain stores
@@ -120,58 +140,47 @@ struct CooPattern {
   System *s = nullptr;
 };
 
-int64_t bandwidth(const std::vector<int> &rp, const std::vector<int> &col,
-                  const std::vector<int> &inv) {
-  int64_t bw = 0;
-  for (int i = 0; i + 1 < (int)rp.size(); i++)
-    for (int k = rp[i]; k < rp[i + 1]; k++)
-      bw = std::max<int64_t>(bw, std::abs((int64_t)inv[i] - inv[col[k]]));
-  return bw;
-}
-
-// RCM order of the CSR pattern (rp, col) if the caller's order is not
-// banded already (bandwidth at most 5x the widest level of a breadth-first
-// search from the first row coupled to another row) and RCM has the smaller
-// bandwidth; then (rp, col) become the reordered pattern and the order, its
-// inverse and each slot's new place are uploaded to p.  The search does not
-// start at row 0: after Dirichlet elimination that is usually a boundary
-// node holding only its diagonal, whose one-node "level" would make every
-// order look unbanded.
-void reorder(CooPattern &p, int n, std::vector<int> &rp, std::vector<int> &col) {
-  std::vector<int> id(n);
-  std::iota(id.begin(), id.end(), 0);
-  const int64_t bw_id = bandwidth(rp, col, id);
-  int start = 0;
-  for (int i = 0; i < n; i++)
-    if (std::any_of(col.begin() + rp[i], col.begin() + rp[i + 1], [i](int c) { return c != i; })) {
-      start = i;
-      break;
-    }
-  if (bw_id <= 5 * bfs_max_level(n, rp, col, start)) return;
-  std::vector<int> ord = rcm_order(n, rp, col), inv(n);
-  for (int k = 0; k < n; k++) inv[ord[k]] = k;
-  if (bandwidth(rp, col, inv) >= bw_id) return;
-  std::vector<int> nrp(n + 1, 0), ncol(col.size()), pos(col.size());
-  std::vector<std::pair<int, int>> row;
-  for (int k = 0; k < n; k++) {
-    const int r = ord[k];
-    row.clear();
-    for (int j = rp[r]; j < rp[r + 1]; j++) row.emplace_back(inv[col[j]], j);
-    std::sort(row.begin(), row.end());
-    nrp[k + 1] = nrp[k] + (int)row.size();
-    for (size_t t = 0; t < row.size(); t++) {
-      ncol[nrp[k] + t] = row[t].first;
-      pos[row[t].second] = nrp[k] + (int)t;
-    }
-  }
-  rp.swap(nrp);
+// The solver's row order for the device CSR pattern (rp, col; slot s at
+// row srow[s]): reverse Cuthill-McKee if the caller's order is not banded
+// already (bandwidth at most 5x the widest level of a breadth-first search
+// from the first row coupled to another row) and RCM narrows the bandwidth;
+// then rp and col become the reordered pattern, and the order, its inverse
+// and each slot's new place are kept in p.  The search does not start at
+// row 0: after Dirichlet elimination that is usually a boundary node holding
+// only its diagonal, whose one-node "level" would make every order look
+// unbanded.
+void reorder(CooPattern &p, int n, int nslot, const int *srow, DevBuf<int> &rp,
+             DevBuf<int> &col, hipStream_t st) {
+  const int64_t bw_id = device_bandwidth(n, rp.p, col.p, nullptr, st);
+  const int start = device_first_coupled(n, rp.p, col.p, st);
+  if (bw_id <= 5 * device_bfs_max_level(n, rp.p, col.p, start, st)) return;
+  DevBuf<int> order, inv;
+  rcm(n, rp.p, col.p, order, inv, st);
+  if (device_bandwidth(n, rp.p, col.p, inv.p, st) >= bw_id) return;
+  DevBuf<unsigned long long> key(nslot), skey(nslot);
+  DevBuf<int> val(nslot), sval(nslot), nrow(nslot), ncol(nslot);
+  p.pos.alloc(nslot);
+  reorder_keys<<<nb(nslot, 256), 256, 0, st>>>(nslot, n, srow, col.p, inv.p, key.p, val.p);
+  HIPCHECK(hipGetLastError());
+  const unsigned long long nn = (unsigned long long)n * (unsigned long long)n;
+  int bits = 1;
+  while (bits < 64 && (nn - 1) >> bits) bits++;
+  size_t tmp = 0;
+  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key.p, skey.p, val.p, sval.p, nslot,
+                                              0, bits, st));
+  DevBuf<char> t(std::max<size_t>(tmp, 1));
+  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, key.p, skey.p, val.p, sval.p, nslot, 0,
+                                              bits, st));
+  reorder_out<<<nb(nslot, 256), 256, 0, st>>>(nslot, n, skey.p, sval.p, nrow.p, ncol.p, p.pos.p);
+  HIPCHECK(hipGetLastError());
+  csr_row_ptr(nrow.p, nslot, n, rp.p, st);
   col.swap(ncol);
-  p.order.upload(ord);
-  p.inv.upload(inv);
-  p.pos.upload(pos);
+  p.order.swap(order);
+  p.inv.swap(inv);
   p.b.alloc(n);
   p.x.alloc(n);
   p.reordered = true;
+  HIPCHECK(hipStreamSynchronize(st));  // temporaries go out of scope
 }
 
 std::mutex g_lock;
@@ -202,7 +211,7 @@ void build_pattern(CooPattern &p, const long long *ind, int64_t nnz, int n, int 
   p.trip.alloc(m);
   set_int<<<1, 1, 0, st>>>(err.p, 0);
   int nslot = 0;
-  std::vector<int> rp(n + 1, 0), col;
+  DevBuf<int> rp(n + 1), col;
   if (nnz) {
     coo_keys<<<nb(nnz, 256), 256, 0, st>>>(nnz, ind, n, key.p, val.p, err.p);
     HIPCHECK(hipGetLastError());
@@ -237,21 +246,18 @@ void build_pattern(CooPattern &p, const long long *ind, int64_t nnz, int n, int 
     HIPCHECK(hipGetLastError());
     const int end = (int)nnz;
     HIPCHECK(hipMemcpyAsync(p.start.p + nslot, &end, sizeof(int), hipMemcpyHostToDevice, st));
-    std::vector<int> hrow(nslot);
-    col.resize(nslot);
-    HIPCHECK(hipMemcpyAsync(hrow.data(), srow.p, sizeof(int) * nslot, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(col.data(), scol.p, sizeof(int) * nslot, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    for (int s = 0; s < nslot; s++) rp[hrow[s] + 1]++;
-    for (int i = 0; i < n; i++) rp[i + 1] += rp[i];
+    csr_row_ptr(srow.p, nslot, n, rp.p, st);
+    col.swap(scol);
+    p.nslot = nslot;
+    reorder(p, n, nslot, srow.p, rp, col, st);
   } else {
     p.start.alloc(1);
     const int zero = 0;
     HIPCHECK(hipMemcpyAsync(p.start.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemsetAsync(rp.p, 0, sizeof(int) * (n + 1), st));
+    p.nslot = 0;
   }
-  p.nslot = nslot;
-  if (nslot) reorder(p, n, rp, col);
-  p.s = system_from_csr(n, std::move(rp), std::move(col));
+  p.s = system_from_device_csr(n, rp, col, nslot);
   p.ind.alloc(std::max<int64_t>(2 * nnz, 1));
   if (nnz)
     HIPCHECK(hipMemcpyAsync(p.ind.p, ind, sizeof(long long) * 2 * nnz, hipMemcpyDeviceToDevice,

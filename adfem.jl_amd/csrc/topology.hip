@@ -11,7 +11,7 @@
 // vertices (nodes of no element last, ascending) when that order is banded
 // -- bandwidth at most 5x the widest level of a breadth-first search from
 // the first element's first vertex -- else reverse Cuthill-McKee over the
-// node graph (host, rcm_order).  Rows then run along fronts: the k-th
+// node graph (rcm: level by level on the device).  Rows then run along fronts: the k-th
 // neighbours of consecutive rows are consecutive nodes of the adjacent
 // fronts, so the solver's column gathers coalesce within a wavefront, and
 // the elements of a block of rows are consecutive in the user's element
@@ -24,6 +24,7 @@
 // level's size do not).
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
 #include <cstdlib>
 
 #include "internal.h"
@@ -196,13 +197,42 @@ __global__ void slots_kernel(int m, int ne, const int *__restrict__ srow,
   slots[i] = w;
 }
 
+// max |pos(r) - pos(c)| over the pattern's entries (pos = inv, or the
+// identity when inv is null)
 __global__ void bandwidth_kernel(int n, const int *__restrict__ rp, const int *__restrict__ col,
-                                 int *__restrict__ bw) {
+                                 const int *__restrict__ inv, int *__restrict__ bw) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int pr = inv ? inv[r] : r;
+  int d = 0;
+  for (int k = rp[r]; k < rp[r + 1]; k++) d = max(d, abs((inv ? inv[col[k]] : col[k]) - pr));
+  if (d > 0) atomicMax(bw, d);
+}
+
+// off-diagonal entries per row
+__global__ void degree_kernel(int n, const int *__restrict__ rp, const int *__restrict__ col,
+                              int *__restrict__ deg) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   int d = 0;
-  for (int k = rp[r]; k < rp[r + 1]; k++) d = max(d, abs(col[k] - r));
-  if (d > 0) atomicMax(bw, d);
+  for (int k = rp[r]; k < rp[r + 1]; k++) d += col[k] != r;
+  deg[r] = d;
+}
+
+// smallest (degree, id) over the rows with an off-diagonal entry, taken
+// from the listed rows (list null: all rows), as degree << 32 | id
+__global__ void min_degree_kernel(int m, const int *__restrict__ list, const int *__restrict__ deg,
+                                  unsigned long long *__restrict__ best) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int v = list ? list[i] : i;
+  if (deg[v] > 0)
+    atomicMin(best, (unsigned long long)deg[v] << 32 | (unsigned long long)(unsigned)v);
+}
+
+__global__ void first_coupled_kernel(int n, const int *__restrict__ deg, int *__restrict__ first) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n && deg[r] > 0) atomicMin(first, r);
 }
 
 // One level of a breadth-first search: the frontier is q[off[L], off[L] +
@@ -225,35 +255,132 @@ __global__ void __launch_bounds__(BFS_NT)
   }
 }
 
-// Widest level of a breadth-first search of the pattern's graph from s.
-int64_t device_bfs_max_level(int n, const int *rp, const int *col, int s, hipStream_t st) {
-  DevBuf<int> level(n), q(n), off(n + 2), cnt(n + 2);
-  fill_int<<<nb(n, 256), 256, 0, st>>>(level.p, n, -1);
-  fill_int<<<nb(n + 2, 256), 256, 0, st>>>(cnt.p, n + 2, 0);
-  fill_int<<<1, 1, 0, st>>>(level.p + s, 1, 0);
-  fill_int<<<1, 1, 0, st>>>(q.p, 1, s);
-  fill_int<<<1, 1, 0, st>>>(off.p, 1, 0);
-  fill_int<<<1, 1, 0, st>>>(cnt.p, 1, 1);
-  HIPCHECK(hipGetLastError());
-  // levels in batches: a level with no nodes ends the search (every later
-  // launch sees an empty frontier)
-  constexpr int BATCH = 64;
-  int L = 0;
-  for (;;) {
-    const int end = std::min(L + BATCH, n);
-    for (; L < end; L++)
-      bfs_level_kernel<<<BFS_BLOCKS, BFS_NT, 0, st>>>(L, rp, col, level.p, q.p, off.p, cnt.p);
-    HIPCHECK(hipGetLastError());
-    int last = 0;
-    HIPCHECK(hipMemcpyAsync(&last, cnt.p + L, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    if (last == 0 || L >= n) break;
+// Cuthill-McKee positions level by level, in one workgroup: a node of level
+// L + 1 follows its parent -- its neighbour of level L placed first -- and
+// the parent's children are placed in (degree, id) order, which is the
+// order a breadth-first search visiting each node's unvisited neighbours
+// sorted by (degree, id) produces.  Level L of the search occupies
+// positions [off[L], off[L] + cnt[L]).  On entry pos holds INT_MAX, ord -1
+// and ccnt zeros (left zero).  A pattern that is not symmetric raises *bad
+// (every index stays in bounds).
+constexpr int CM_NT = 1024;
+__global__ void __launch_bounds__(CM_NT)
+    cm_kernel(int nlev, int start, const int *__restrict__ rp, const int *__restrict__ col,
+              const int *__restrict__ deg, const int *__restrict__ level,
+              const int *__restrict__ q, const int *__restrict__ off,
+              const int *__restrict__ cnt, int *__restrict__ pos, int *__restrict__ ord,
+              int *__restrict__ ppos, int *__restrict__ ccnt, int *__restrict__ coff,
+              int *__restrict__ bad) {
+  __shared__ int part[CM_NT];
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    pos[start] = 0;
+    ord[0] = start;
   }
-  std::vector<int> h(L + 1);
-  HIPCHECK(hipMemcpy(h.data(), cnt.p, sizeof(int) * (L + 1), hipMemcpyDeviceToHost));
-  int64_t w = 0;
-  for (int v : h) w = std::max<int64_t>(w, v);
-  return w;
+  __threadfence();
+  __syncthreads();
+  for (int L = 0; L + 1 < nlev; L++) {
+    const int oL = off[L], cL = cnt[L], o1 = off[L + 1], c1 = cnt[L + 1];
+    for (int i = tid; i < c1; i += CM_NT) {  // parents
+      const int v = q[o1 + i];
+      int best = INT_MAX;
+      for (int k = rp[v]; k < rp[v + 1]; k++) {
+        const int u = col[k];
+        if (level[u] == L) best = min(best, pos[u]);
+      }
+      if (best < oL || best >= oL + cL) {  // the pattern is not symmetric
+        *bad = 1;
+        best = INT_MAX;
+      } else {
+        atomicAdd(&ccnt[best - oL], 1);
+      }
+      ppos[v] = best;
+    }
+    __threadfence();
+    __syncthreads();
+    // children offsets: exclusive scan of ccnt over level L's positions
+    const int per = (cL + CM_NT - 1) / CM_NT;
+    const int b = tid * per, e = min(b + per, cL);
+    int sum = 0;
+    for (int i = b; i < e; i++) sum += ccnt[i];
+    part[tid] = sum;
+    __syncthreads();
+    for (int d = 1; d < CM_NT; d <<= 1) {
+      const int v = tid >= d ? part[tid - d] : 0;
+      __syncthreads();
+      part[tid] += v;
+      __syncthreads();
+    }
+    int run = part[tid] - sum;
+    for (int i = b; i < e; i++) {
+      coff[i] = run;
+      run += ccnt[i];
+      ccnt[i] = 0;
+    }
+    __threadfence();
+    __syncthreads();
+    for (int i = tid; i < c1; i += CM_NT) {  // places
+      const int v = q[o1 + i];
+      const int pp = ppos[v];
+      if (pp == INT_MAX) continue;
+      const int par = ord[pp];
+      if (par < 0) {
+        *bad = 1;
+        continue;
+      }
+      int rank = 0;
+      for (int k = rp[par]; k < rp[par + 1]; k++) {
+        const int w = col[k];
+        if (w != v && level[w] == L + 1 && ppos[w] == pp &&
+            (deg[w] < deg[v] || (deg[w] == deg[v] && w < v)))
+          rank++;
+      }
+      const int ps = o1 + coff[pp - oL] + rank;
+      if (ps >= o1 + c1) {  // not symmetric: the siblings do not add up
+        *bad = 1;
+        continue;
+      }
+      pos[v] = ps;
+      ord[ps] = v;
+    }
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// The final order: nodes without off-diagonal entries first (ascending),
+// then the component reversed (reverse Cuthill-McKee); order[k] = node at
+// position k, inv its inverse.
+__global__ void rcm_place_kernel(int n, int ncomp, const int *__restrict__ deg,
+                                 const int *__restrict__ iso_rank, const int *__restrict__ pos,
+                                 int *__restrict__ order, int *__restrict__ inv) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int niso = n - ncomp;
+  const int k = deg[v] == 0 ? iso_rank[v] : niso + (ncomp - 1 - pos[v]);
+  order[k] = v;
+  inv[v] = k;
+}
+
+__global__ void isolated_flag_kernel(int n, const int *__restrict__ deg, int *__restrict__ f) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) f[v] = deg[v] == 0 ? 1 : 0;
+}
+
+// ip[k] = iperm[ord[k]]
+__global__ void compose_kernel(int n, const int *__restrict__ iperm, const int *__restrict__ ord,
+                               int *__restrict__ ip) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) ip[k] = iperm[ord[k]];
+}
+
+// flag when order / inv is not a permutation and its inverse
+__global__ void perm_check_kernel(int n, const int *__restrict__ order,
+                                  const int *__restrict__ inv, int *__restrict__ bad) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int k = inv[v];
+  if (k < 0 || k >= n || order[k] != v) *bad = 1;
 }
 
 // Everything that follows from the node order iperm (device, n entries).
@@ -604,6 +731,180 @@ __global__ void __launch_bounds__(SB_NT)
 
 }  // namespace
 
+void csr_row_ptr(const int *srow, int m, int n, int *rp, hipStream_t st) {
+  run_start_kernel<<<nb((int64_t)m + 1, 256), 256, 0, st>>>(m, n, srow, rp);
+  HIPCHECK(hipGetLastError());
+}
+
+int64_t device_bandwidth(int n, const int *rp, const int *col, const int *inv, hipStream_t st) {
+  DevBuf<int> bw(1);
+  fill_int<<<1, 1, 0, st>>>(bw.p, 1, 0);
+  bandwidth_kernel<<<nb(n, 256), 256, 0, st>>>(n, rp, col, inv, bw.p);
+  HIPCHECK(hipGetLastError());
+  int h = 0;
+  HIPCHECK(hipMemcpyAsync(&h, bw.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return h;
+}
+
+int device_first_coupled(int n, const int *rp, const int *col, hipStream_t st) {
+  DevBuf<int> deg(n), first(1);
+  degree_kernel<<<nb(n, 256), 256, 0, st>>>(n, rp, col, deg.p);
+  fill_int<<<1, 1, 0, st>>>(first.p, 1, INT_MAX);
+  first_coupled_kernel<<<nb(n, 256), 256, 0, st>>>(n, deg.p, first.p);
+  HIPCHECK(hipGetLastError());
+  int h = 0;
+  HIPCHECK(hipMemcpyAsync(&h, first.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return h == INT_MAX ? 0 : h;
+}
+
+namespace {
+
+// A breadth-first search from s: level of every node (-1: not reached), the
+// visit queue with level L at [off[L], off[L] + cnt[L]), the level sizes.
+struct BfsRun {
+  DevBuf<int> level, q, off, cnt;
+  std::vector<int> size;  // nodes per level, levels 0 .. nlev - 1
+};
+
+void run_bfs(int n, const int *rp, const int *col, int s, BfsRun &b, hipStream_t st) {
+  b.level.alloc(n);
+  b.q.alloc(n);
+  b.off.alloc(n + 2);
+  b.cnt.alloc(n + 2);
+  fill_int<<<nb(n, 256), 256, 0, st>>>(b.level.p, n, -1);
+  fill_int<<<nb(n + 2, 256), 256, 0, st>>>(b.cnt.p, n + 2, 0);
+  fill_int<<<1, 1, 0, st>>>(b.level.p + s, 1, 0);
+  fill_int<<<1, 1, 0, st>>>(b.q.p, 1, s);
+  fill_int<<<1, 1, 0, st>>>(b.off.p, 1, 0);
+  fill_int<<<1, 1, 0, st>>>(b.cnt.p, 1, 1);
+  HIPCHECK(hipGetLastError());
+  // levels in batches: a level with no nodes ends the search (every later
+  // launch sees an empty frontier)
+  constexpr int BATCH = 64;
+  int L = 0;
+  for (;;) {
+    const int end = std::min(L + BATCH, n);
+    for (; L < end; L++)
+      bfs_level_kernel<<<BFS_BLOCKS, BFS_NT, 0, st>>>(L, rp, col, b.level.p, b.q.p, b.off.p,
+                                                      b.cnt.p);
+    HIPCHECK(hipGetLastError());
+    int last = 0;
+    HIPCHECK(hipMemcpyAsync(&last, b.cnt.p + L, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (last == 0 || L >= n) break;
+  }
+  std::vector<int> h(L + 1);
+  HIPCHECK(hipMemcpy(h.data(), b.cnt.p, sizeof(int) * (L + 1), hipMemcpyDeviceToHost));
+  b.size.clear();
+  for (int v : h) {
+    if (v == 0) break;
+    b.size.push_back(v);
+  }
+}
+
+unsigned long long min_degree(int m, const int *list, const int *deg, hipStream_t st) {
+  DevBuf<unsigned long long> best(1);
+  HIPCHECK(hipMemsetAsync(best.p, 0xff, sizeof(unsigned long long), st));
+  if (m > 0) min_degree_kernel<<<nb(m, 256), 256, 0, st>>>(m, list, deg, best.p);
+  HIPCHECK(hipGetLastError());
+  unsigned long long h = 0;
+  HIPCHECK(hipMemcpyAsync(&h, best.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return h;
+}
+
+}  // namespace
+
+int64_t device_bfs_max_level(int n, const int *rp, const int *col, int s, hipStream_t st) {
+  BfsRun b;
+  run_bfs(n, rp, col, s, b, st);
+  int64_t w = 0;
+  for (int v : b.size) w = std::max<int64_t>(w, v);
+  return w;
+}
+
+// Reverse Cuthill-McKee order of the graph of a symmetric CSR pattern
+// (device rp / col, diagonal entries ignored): nodes without neighbours
+// first, ascending, then the rest in reverse Cuthill-McKee order from a
+// pseudo-peripheral start (George & Liu: up to four searches, each from a
+// least-degree node of the last one's deepest level, while the depth
+// grows).  On the device when the rest is one connected component; else
+// (or when the pattern turns out not to be symmetric) on the host
+// (rcm_order).  order[k] = node at position k; inv its inverse.
+void rcm(int n, const int *rp, const int *col, DevBuf<int> &order, DevBuf<int> &inv,
+         hipStream_t st) {
+  order.alloc(n);
+  inv.alloc(n);
+  DevBuf<int> deg(n), flag(n + 1), iso(n + 1), bad(1);
+  degree_kernel<<<nb(n, 256), 256, 0, st>>>(n, rp, col, deg.p);
+  isolated_flag_kernel<<<nb(n, 256), 256, 0, st>>>(n, deg.p, flag.p);
+  fill_int<<<1, 1, 0, st>>>(flag.p + n, 1, 0);
+  fill_int<<<1, 1, 0, st>>>(bad.p, 1, 0);
+  HIPCHECK(hipGetLastError());
+  exclusive_sum(flag.p, iso.p, n + 1, st);
+  int niso = 0;
+  HIPCHECK(hipMemcpy(&niso, iso.p + n, sizeof(int), hipMemcpyDeviceToHost));
+  const int ncomp = n - niso;
+  DevBuf<int> pos(n);
+  bool ok = true;
+  if (ncomp > 0) {
+    int start = (int)(min_degree(n, nullptr, deg.p, st) & 0xffffffffu);
+    BfsRun b;
+    int from = -1, ecc = -1;
+    for (int rep = 0; rep < 4; rep++) {
+      run_bfs(n, rp, col, start, b, st);
+      from = start;
+      const int maxl = (int)b.size.size() - 1;
+      int last_off = 0;
+      for (int L = 0; L < maxl; L++) last_off += b.size[L];
+      const int cand = (int)(min_degree(b.size[maxl], b.q.p + last_off, deg.p, st) & 0xffffffffu);
+      if (maxl <= ecc) break;
+      ecc = maxl;
+      start = cand;
+    }
+    if (from != start) run_bfs(n, rp, col, start, b, st);
+    int reached = 0;
+    for (int v : b.size) reached += v;
+    ok = reached == ncomp;
+    if (ok) {
+      DevBuf<int> ord(n), ppos(n), ccnt(n), coff(n);
+      fill_int<<<nb(n, 256), 256, 0, st>>>(ccnt.p, n, 0);
+      fill_int<<<nb(n, 256), 256, 0, st>>>(pos.p, n, INT_MAX);
+      fill_int<<<nb(n, 256), 256, 0, st>>>(ord.p, n, -1);
+      cm_kernel<<<1, CM_NT, 0, st>>>((int)b.size.size(), start, rp, col, deg.p, b.level.p, b.q.p,
+                                     b.off.p, b.cnt.p, pos.p, ord.p, ppos.p, ccnt.p, coff.p,
+                                     bad.p);
+      HIPCHECK(hipGetLastError());
+      int hb = 0;
+      HIPCHECK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      ok = hb == 0;
+    }
+  }
+  if (ok) {
+    rcm_place_kernel<<<nb(n, 256), 256, 0, st>>>(n, ncomp, deg.p, iso.p, pos.p, order.p, inv.p);
+    perm_check_kernel<<<nb(n, 256), 256, 0, st>>>(n, order.p, inv.p, bad.p);
+    HIPCHECK(hipGetLastError());
+    int hb = 0;
+    HIPCHECK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    ok = hb == 0;
+  }
+  if (ok) return;
+  // host: several components (or a pattern that is not symmetric)
+  int nnz = 0;
+  HIPCHECK(hipMemcpy(&nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> hrp(n + 1), hcol(nnz), hinv(n);
+  HIPCHECK(hipMemcpy(hrp.data(), rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(hcol.data(), col, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+  const std::vector<int> ho = rcm_order(n, hrp, hcol);
+  for (int k = 0; k < n; k++) hinv[ho[k]] = k;
+  order.upload(ho);
+  inv.upload(hinv);
+}
+
 void mesh_topology(Mesh &m, const double *coords, hipStream_t st) {
   const int n = m.nnode, ne = m.nelem;
   REQUIRE((int64_t)3 * ne < (int64_t)1 << 30, ADMFEM_ERR_UNSUPPORTED, "too many elements");
@@ -622,24 +923,18 @@ void mesh_topology(Mesh &m, const double *coords, hipStream_t st) {
   build_order_dependent(m, elems, cu, iperm, st);
   // banded? (pattern under the element order; the search starts at the
   // first element's first vertex)
-  DevBuf<int> bw(1);
-  fill_int<<<1, 1, 0, st>>>(bw.p, 1, 0);
-  bandwidth_kernel<<<nb(n, 256), 256, 0, st>>>(n, m.d_rp.p, m.d_col.p, bw.p);
-  HIPCHECK(hipGetLastError());
-  int hbw = 0, s = 0;
-  HIPCHECK(hipMemcpy(&hbw, bw.p, sizeof(int), hipMemcpyDeviceToHost));
+  int s = 0;
   HIPCHECK(hipMemcpy(&s, m.d_perm.p + m.elems[0], sizeof(int), hipMemcpyDeviceToHost));
-  if ((int64_t)hbw > 5 * device_bfs_max_level(n, m.d_rp.p, m.d_col.p, s, st)) {
+  const int64_t hbw = device_bandwidth(n, m.d_rp.p, m.d_col.p, nullptr, st);
+  if (hbw > 5 * device_bfs_max_level(n, m.d_rp.p, m.d_col.p, s, st)) {
     // reverse Cuthill-McKee over the pattern in the element order
-    std::vector<int> rp(n + 1), col(m.nnz_full), eo(n);
-    HIPCHECK(hipMemcpy(rp.data(), m.d_rp.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(col.data(), m.d_col.p, sizeof(int) * col.size(), hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(eo.data(), iperm.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    const std::vector<int> ord = rcm_order(n, rp, col);
-    std::vector<int> ip(n);
-    for (int k = 0; k < n; k++) ip[k] = eo[ord[k]];
-    iperm.upload(ip);
-    build_order_dependent(m, elems, cu, iperm, st);
+    DevBuf<int> ord(n), inv(n), ip(n);
+    rcm(n, m.d_rp.p, m.d_col.p, ord, inv, st);
+    compose_kernel<<<nb(n, 256), 256, 0, st>>>(n, iperm.p, ord.p, ip.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    build_order_dependent(m, elems, cu, ip, st);
+    iperm.swap(ip);
   }
   m.iperm.resize(n);
   m.perm.resize(n);

@@ -207,6 +207,30 @@ def test_sparse_solve_grid_laplacian_any_numbering(permute):
     assert admfem.ops.solve_pattern_info()["reordered"] == permute
 
 
+def test_sparse_solve_several_components_random_numbering():
+    """Two disjoint shifted grid Laplacians (80 x 80 and 60 x 60) and 50
+    decoupled unknowns, numbered at random: the graph has two components
+    besides the isolated nodes, so the reverse Cuthill-McKee order comes from
+    the host; against SciPy."""
+    blocks = []
+    for m in (80, 60):
+        T = sp.diags([-np.ones(m - 1), 2.0 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+        blocks.append(sp.kron(sp.eye(m), T) + sp.kron(T, sp.eye(m)) + 0.05 * sp.eye(m * m))
+    blocks.append(sp.eye(50) * 3.0)
+    A = sp.block_diag(blocks).tocoo()
+    n = A.shape[0]
+    rng = np.random.default_rng(13)
+    p = rng.permutation(n)
+    I, J, V = p[A.row].astype(np.int64), p[A.col].astype(np.int64), A.data
+    b = rng.standard_normal(n)
+    ind = torch.as_tensor(np.stack([I, J], 1), device="cuda")
+    u = admfem.solve(admfem.SparseTensor(ind, torch.tensor(V, device="cuda"), n),
+                     torch.tensor(b, device="cuda"), rtol=1e-13)
+    u0 = spl.spsolve(sp.csc_matrix((V, (I, J)), shape=(n, n)), b)
+    assert rel(u.cpu().numpy(), u0) < 1e-10
+    assert admfem.ops.solve_pattern_info()["reordered"]
+
+
 def test_sparse_solve_pattern_kept_between_calls():
     """The op keeps the last pattern: the same indices with new values, then
     indices of the same sizes but another pattern, then an out-of-range
