@@ -394,13 +394,14 @@ __device__ __forceinline__ double unif(uint64_t seed, uint64_t k) {  // [0, 1)
 
 // coordinates (internal order): base + U[-a, a]^2 on non-boundary nodes
 __global__ void syn_coords(int n, const double *__restrict__ bx, const double *__restrict__ by,
-                           const int *__restrict__ row_bd, const int *__restrict__ iperm,
-                           uint64_t seed, double amp, double *__restrict__ x,
-                           double *__restrict__ y, double2 *__restrict__ xy) {
+                           const int *__restrict__ row_bd, const int *__restrict__ vrow,
+                           const int *__restrict__ iperm, uint64_t seed, double amp,
+                           double *__restrict__ x, double *__restrict__ y,
+                           double2 *__restrict__ xy) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   double dx = 0, dy = 0;
-  if (row_bd[r] < 0) {
+  if (row_bd[vrow ? vrow[r] : r] < 0) {  // vertices on Dirichlet rows stay put
     const uint64_t u = (uint64_t)iperm[r];  // jitter keyed by the user node id
     dx = amp * (2.0 * unif(seed, 2 * u) - 1.0);
     dy = amp * (2.0 * unif(seed, 2 * u + 1) - 1.0);
@@ -436,13 +437,20 @@ __global__ void syn_gauss(int nelem, QuadRule rule, const int *__restrict__ conn
   }
 }
 
-// heat initial state (internal order): a GRF at the nodes
+// heat initial state (internal order): a GRF at the dofs' nodes (an edge
+// dof: its edge's midpoint, ends[r]; ends null: row r is vertex r)
 __global__ void syn_nodal(int n, const double *__restrict__ x, const double *__restrict__ y,
-                          SynField F, double *__restrict__ u) {
+                          const int2 *__restrict__ ends, SynField F, double *__restrict__ u) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
+  double px = x[r], py = y[r];
+  if (ends) {
+    const int2 e = ends[r];
+    px = 0.5 * (x[e.x] + x[e.y]);
+    py = 0.5 * (y[e.x] + y[e.y]);
+  }
   double g = 0.0;
-  for (int j = 0; j < F.nmodes; j++) g += cos(F.wx[j] * x[r] + F.wy[j] * y[r] + F.phi[j]);
+  for (int j = 0; j < F.nmodes; j++) g += cos(F.wx[j] * px + F.wy[j] * py + F.phi[j]);
   u[r] = F.scale * g;
 }
 
@@ -513,11 +521,15 @@ void synth_inputs(const System &s, const double *base_x, const double *base_y, u
                   double amp, const SynField &kf, const SynField *uf, double *x, double *y,
                   double2 *xy, double *kappa, double *f, double *u_prev_int, hipStream_t st) {
   const Mesh &m = *s.mesh;
-  syn_coords<<<nb(s.n, 256), 256, 0, st>>>(s.n, base_x, base_y, s.d_row_bd.p, m.d_iperm.p,
-                                            seed, amp, x, y, xy);
+  const bool p2 = s.degree == 2;
+  syn_coords<<<nb(m.nnode, 256), 256, 0, st>>>(m.nnode, base_x, base_y, s.d_row_bd.p,
+                                                p2 ? s.d_vrow.p : nullptr, m.d_iperm.p, seed,
+                                                amp, x, y, xy);
   syn_gauss<<<nb(m.nelem, 256), 256, 0, st>>>(m.nelem, m.rule, m.conn.p, x, y, kf,
                                                seed ^ 0x5A5A5A5A5A5A5A5Aull, kappa, f);
-  if (uf && u_prev_int) syn_nodal<<<nb(s.n, 256), 256, 0, st>>>(s.n, x, y, *uf, u_prev_int);
+  if (uf && u_prev_int)
+    syn_nodal<<<nb(s.n, 256), 256, 0, st>>>(s.n, x, y, p2 ? s.d_dof_ends.p : nullptr, *uf,
+                                             u_prev_int);
   HIPCHECK(hipGetLastError());
 }
 

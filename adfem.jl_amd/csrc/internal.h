@@ -227,6 +227,11 @@ struct System {
   std::vector<int> perm2, iperm2;  // user dof id <-> internal row
   DevBuf<int> d_perm2, d_iperm2, d_conn2, d_inc_ptr, d_blk_elem;
   DevBuf<uint64_t> d_pairs2;
+  // P2, for per-request inputs: the row of each vertex (mesh-internal vertex
+  // id -> row) and the end vertices of each row's dof (a vertex dof: twice
+  // its vertex; an edge dof: the edge's ends), mesh-internal ids
+  DevBuf<int> d_vrow;
+  DevBuf<int2> d_dof_ends;
   CGWork *work = nullptr;
   size_t device_bytes() const;
   // user node / dof id -> internal row, and back (mesh systems)

@@ -252,7 +252,8 @@ size_t System::device_bytes() const {
          d_blk_rec.bytes() + d_blk_vptr.bytes() + d_blk_vert.bytes() + d_pairs.bytes() + val.bytes() + rhs.bytes() +
          d_slice_ptr.bytes() + d_sell_col.bytes() +
          d_sell_val.bytes() + d_diag.bytes() + d_perm2.bytes() + d_iperm2.bytes() +
-         d_conn2.bytes() + d_inc_ptr.bytes() + d_blk_elem.bytes() + d_pairs2.bytes();
+         d_conn2.bytes() + d_inc_ptr.bytes() + d_blk_elem.bytes() + d_pairs2.bytes() +
+         d_vrow.bytes() + d_dof_ends.bytes();
 }
 
 // The fused system of a P2 mesh.  Rows are the ndof dofs (vertices, then
@@ -303,6 +304,15 @@ static System *system_build_p2(Mesh *m, const long long *bdnode, int nbd) {
       }
       s->perm2.assign(n, 0);
       for (int k = 0; k < n; k++) s->perm2[s->iperm2[k]] = k;
+      std::vector<int> vrow(nn);
+      std::vector<int2> ends(n);
+      for (int r = 0; r < nn; r++) vrow[r] = s->perm2[m->iperm[r]];
+      for (int k = 0; k < n; k++) {
+        const int d = s->iperm2[k];
+        ends[k] = d < nn ? make_int2(m->perm[d], m->perm[d]) : make_int2(lo[d - nn], hi[d - nn]);
+      }
+      s->d_vrow.upload(vrow);
+      s->d_dof_ends.upload(ends);
     }
     // element dofs as internal rows, row -> (element, local dof) incidences
     // in ascending element order

@@ -24,9 +24,10 @@
 // over the element's 36 triplets of FemLaplaceScalar_backward
 // (FemLaplaceScalar.h:28-54) in closed form.  One thread per element.
 //
-// Gauss weights come from the mesh's array (user element order, as kappa,
-// rho and f); geometry from the mesh's vertex coordinates (internal node
-// order) through its vertex connectivity.
+// Geometry and Gauss weights (w_k = ipw_k * |det|, Common.cpp:116 with the
+// area in closed form) come from the vertex coordinates given (the mesh's,
+// or a serving request's own, internal node order) through the mesh's
+// vertex connectivity; kappa, rho and f are in the user element order.
 #include "p2basis.h"
 
 namespace admfem {
@@ -42,8 +43,10 @@ __device__ __forceinline__ int sym6(int p, int q) {
   return (lo * (11 - lo)) / 2 + hi;
 }
 
-// Vertex coordinates of element e and the gradients of its barycentric
-// functions.
+// Vertex coordinates of element e (from xy: the mesh's own, or a serving
+// request's), the gradients of its barycentric functions, and in G.area the
+// factor |det| = 2 * area that turns a reference weight into the Gauss
+// weight (w = ipw * area / 0.5, Common.cpp:116).
 __device__ __forceinline__ ElemGeom p2_elem_geom(const int *__restrict__ conn, int nelem, int e,
                                                  const double2 *__restrict__ xy) {
   double ex[3], ey[3];
@@ -55,7 +58,7 @@ __device__ __forceinline__ ElemGeom p2_elem_geom(const int *__restrict__ conn, i
   }
   ElemGeom G;
   elem_grads(ex[0], ey[0], ex[1], ey[1], ex[2], ey[2], G.gx, G.gy);
-  G.area = 0.0;  // unused: the weights come from the mesh's array
+  G.area = fabs((ex[1] - ex[0]) * (ey[2] - ey[0]) - (ex[2] - ex[0]) * (ey[1] - ey[0]));
   return G;
 }
 
@@ -66,7 +69,6 @@ template <int NQ, bool MASS>
 __device__ __forceinline__ void p2_element(int e, int nelem, const QuadRule &rule,
                                            const int *__restrict__ conn,
                                            const double2 *__restrict__ xy,
-                                           const double *__restrict__ gw,
                                            const int *__restrict__ conn2,
                                            const int *__restrict__ row_bd,
                                            const admfem_problem &P, double (&a)[P2_SYM],
@@ -84,7 +86,7 @@ __device__ __forceinline__ void p2_element(int e, int nelem, const QuadRule &rul
 #pragma unroll
   for (int k = 0; k < NQ; k++) {
     const size_t g = (size_t)NQ * e + k;
-    const double w = gw[g];
+    const double w = rule.w[k] * G.area;
     double gx[6], gy[6], h[6];
     p2_grad(rule.xi[k], rule.eta[k], G, gx, gy);
     p2_shape(rule.xi[k], rule.eta[k], h);
@@ -145,9 +147,9 @@ __device__ __forceinline__ void p2_element(int e, int nelem, const QuadRule &rul
 template <int NQ, bool MASS>
 __global__ void __launch_bounds__(P2A_NT)
     assemble_p2_kernel(int n, int nelem, QuadRule rule, const int *__restrict__ conn,
-                       const double2 *__restrict__ xy, const double *__restrict__ gw,
-                       const int *__restrict__ conn2, const int *__restrict__ blk_eptr,
-                       const int *__restrict__ blk_elem, const int *__restrict__ inc_ptr,
+                       const double2 *__restrict__ xy, const int *__restrict__ conn2,
+                       const int *__restrict__ blk_eptr, const int *__restrict__ blk_elem,
+                       const int *__restrict__ inc_ptr,
                        const uint64_t *__restrict__ pairs, const int *__restrict__ row_bd,
                        const int *__restrict__ slice_ptr, admfem_problem P, SellOut o,
                        int max_e, int wmax) {
@@ -159,7 +161,7 @@ __global__ void __launch_bounds__(P2A_NT)
   const int e0 = blk_eptr[blk], ne = blk_eptr[blk + 1] - e0;
   for (int t = threadIdx.x; t < ne; t += P2A_NT) {
     double a[P2_SYM], r[6];
-    p2_element<NQ, MASS>(blk_elem[e0 + t], nelem, rule, conn, xy, gw, conn2, row_bd, P, a, r);
+    p2_element<NQ, MASS>(blk_elem[e0 + t], nelem, rule, conn, xy, conn2, row_bd, P, a, r);
 #pragma unroll
     for (int i = 0; i < P2_SYM; i++) ae[P2_SYM * t + i] = a[i];
 #pragma unroll
@@ -204,8 +206,7 @@ __global__ void __launch_bounds__(P2A_NT)
 template <int NQ, bool PAIR>
 __global__ void __launch_bounds__(256)
     kappa_grad_p2_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
-                         const double2 *__restrict__ xy, const double *__restrict__ gw,
-                         const int *__restrict__ conn2, const int *__restrict__ row_bd,
+                         const double2 *__restrict__ xy, const int *__restrict__ conn2, const int *__restrict__ row_bd,
                          const double *__restrict__ u, const double *__restrict__ lam,
                          const double2 *__restrict__ ul, double scale,
                          double *__restrict__ gk) {
@@ -239,7 +240,7 @@ __global__ void __launch_bounds__(256)
       ly += gy[q] * ll[q];
     }
     const size_t g = (size_t)NQ * e + k;
-    gk[g] = -scale * gw[g] * (lx * ux + ly * uy);
+    gk[g] = -scale * (rule.w[k] * G.area) * (lx * ux + ly * uy);
   }
 }
 
@@ -256,7 +257,7 @@ void launch_assemble_p2(const System &s, const double2 *xy, const admfem_problem
   REQUIRE(lds <= 160 * 1024, ADMFEM_ERR_UNSUPPORTED, "P2 assembly block too dense for LDS");
   const unsigned grid = (unsigned)(s.npad / AS_ROWS);
   auto go = [&](auto kern) {
-    kern<<<grid, P2A_NT, lds, st>>>(s.n, m.nelem, m.rule, m.conn.p, xy, m.gw.p, s.d_conn2.p,
+    kern<<<grid, P2A_NT, lds, st>>>(s.n, m.nelem, m.rule, m.conn.p, xy, s.d_conn2.p,
                                     s.d_blk_eptr.p, s.d_blk_elem.p, s.d_inc_ptr.p,
                                     s.d_pairs2.p, s.d_row_bd.p, s.d_slice_ptr.p, p, o, me, w);
   };
@@ -275,7 +276,7 @@ void launch_kappa_gradient_p2(const System &s, const double2 *xy, const double *
   const Mesh &m = *s.mesh;
   const unsigned grid = nblk(m.nelem, 256);
   auto go = [&](auto kern) {
-    kern<<<grid, 256, 0, st>>>(m.nelem, m.rule, m.conn.p, xy, m.gw.p, s.d_conn2.p,
+    kern<<<grid, 256, 0, st>>>(m.nelem, m.rule, m.conn.p, xy, s.d_conn2.p,
                                s.d_row_bd.p, u_int, lam_int, ul_int, scale, gk);
   };
   const bool pair = ul_int != nullptr;

@@ -148,6 +148,7 @@ struct Server {
     const Mesh &m = *s.mesh;
     const size_t np = s.npad, ng = m.ngauss, ns = std::max<size_t>(1, s.slice_ptr.back());
     const size_t nblk = (np + BATCH_NT - 1) / BATCH_NT;
+    const size_t nv = s.degree == 2 ? (size_t)m.nnode : np;  // vertices (coordinates)
     size_t off = 0;
     auto take = [&](size_t bytes) {
       size_t o = off;
@@ -156,11 +157,11 @@ struct Server {
     };
     auto D = [&](size_t n) { return (double *)take(n * sizeof(double)); };
     auto F = [&](size_t n) { return (float *)take(n * sizeof(float)); };
-    double *x = D(np), *y = D(np), *kappa = D(ng), *f = D(ng), *uprev = D(np), *b = D(np),
+    double *x = D(nv), *y = D(nv), *kappa = D(ng), *f = D(ng), *uprev = D(np), *b = D(np),
            *sv64 = D(ns), *diag64 = D(np), *dsq = D(np), *xa = D(np), *xb = D(np),
            *u64 = D(np), *lam64 = D(np), *grad = D(ng), *obsv = D(nobs + 1), *loss2 = D(2),
            *part = D(4 * nblk), *stage = D(2 * np);
-    double2 *xy = (double2 *)take(np * sizeof(double2));
+    double2 *xy = (double2 *)take(nv * sizeof(double2));
     float *sv32 = F(ns), *ysa = F(np), *ysb = F(np), *ra = F(np), *rb = F(np), *pa = F(np),
           *pb = F(np), *qa = F(np), *qb = F(np);
     int *obsi = (int *)take((nobs + 1) * sizeof(int));
@@ -256,19 +257,20 @@ struct Server {
         r.source == ADMFEM_SRC_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
     // (the inputs' presence was checked by server_submit)
     // user-order inputs are staged, then permuted to the internal order
+    const int nn = m.nnode;  // vertices
     if (r.coords) {
-      HIPCHECK(hipMemcpyAsync(q.stage, r.coords, sizeof(double) * 2 * s.n, kind, st));
-      launch_perm_coords(s.n, m.d_iperm.p, q.stage, q.x, q.y, q.xy, st);
+      HIPCHECK(hipMemcpyAsync(q.stage, r.coords, sizeof(double) * 2 * nn, kind, st));
+      launch_perm_coords(nn, m.d_iperm.p, q.stage, q.x, q.y, q.xy, st);
     } else {
-      HIPCHECK(hipMemcpyAsync(q.x, m.x.p, sizeof(double) * s.n, hipMemcpyDeviceToDevice, st));
-      HIPCHECK(hipMemcpyAsync(q.y, m.y.p, sizeof(double) * s.n, hipMemcpyDeviceToDevice, st));
-      HIPCHECK(hipMemcpyAsync(q.xy, m.xy.p, sizeof(double2) * s.n, hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipMemcpyAsync(q.x, m.x.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipMemcpyAsync(q.y, m.y.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipMemcpyAsync(q.xy, m.xy.p, sizeof(double2) * nn, hipMemcpyDeviceToDevice, st));
     }
     HIPCHECK(hipMemcpyAsync(q.kappa, r.kappa, sizeof(double) * m.ngauss, kind, st));
     HIPCHECK(hipMemcpyAsync(q.f, r.f, sizeof(double) * m.ngauss, kind, st));
     if (heat) {
       HIPCHECK(hipMemcpyAsync(q.stage, r.u_prev, sizeof(double) * s.n, kind, st));
-      launch_scatter(m.d_perm.p, q.stage, q.uprev, s.n, st);
+      launch_scatter(s.d_uperm(), q.stage, q.uprev, s.n, st);
     }
     if (r.nobs) {
       HIPCHECK(hipMemcpyAsync(q.obsi, r.obs_idx, sizeof(int) * r.nobs, kind, st));
@@ -368,7 +370,7 @@ struct Server {
     admfem_problem p = problem(q);
     launch_kappa_gradient_to(s, q.xy, q.u64, q.lam64, p.lap_coef, q.grad, st);
     if (q.r.u_out) {
-      launch_scatter(m.d_iperm.p, q.u64, q.stage, s.n, st);  // user order
+      launch_scatter(s.d_uiperm(), q.u64, q.stage, s.n, st);  // user order
       HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
                               st));
     }
@@ -653,7 +655,7 @@ struct Server {
     const Mesh &m = *s.mesh;
     launch_kappa_gradient_to(s, q.xy, q.u64, q.lam64, p.lap_coef, q.grad, st);
     if (q.r.u_out) {
-      launch_scatter(m.d_iperm.p, q.u64, q.stage, s.n, st);
+      launch_scatter(s.d_uiperm(), q.u64, q.stage, s.n, st);
       HIPCHECK(hipMemcpyAsync(q.r.u_out, q.stage, sizeof(double) * s.n, hipMemcpyDeviceToHost,
                               st));
     }
@@ -771,16 +773,13 @@ Server *server_create(const admfem_server_opts &o) {
 void server_free(Server *sv) { delete sv; }
 
 int server_add_topology(Server *sv, System *s, double jitter) {
-  REQUIRE(s->degree == 1, ADMFEM_ERR_UNSUPPORTED,
-          "the serving engine takes P1 topologies; solve P2 systems with "
-          "admfem_system_forward_adjoint");
   Topo T;
   T.sys = s;
   T.jitter = jitter;
   T.nblk = (s->npad + BATCH_NT - 1) / BATCH_NT;
   T.bytes = Server::layout(*s, nullptr, nullptr, std::max(1, s->n / 100));
   for (int u = 0; u < s->n; u++)
-    if (s->row_bd[s->mesh->perm[u]] < 0) T.interior.push_back(u);
+    if (s->row_bd[s->uperm()[u]] < 0) T.interior.push_back(u);
   REQUIRE(!T.interior.empty(), ADMFEM_ERR_ARG, "topology has no interior node");
   sv->topos.push_back(std::move(T));
   return (int)sv->topos.size() - 1;

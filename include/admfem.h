@@ -442,23 +442,25 @@ typedef struct {
   double dt;               /* heat step */
   double arrival;          /* seconds after admfem_server_run starts */
   int source;              /* ADMFEM_SRC_* */
-  /* HOST / DEVICE inputs (user node order; must stay valid until the
+  /* HOST / DEVICE inputs (user node / dof order; must stay valid until the
    * request is admitted): coords nnode x 2 (NULL: the topology's own),
-   * kappa/f [ngauss], u_prev [nnode] (heat), obs_idx (0-based, distinct) /
-   * obs_val.  Missing inputs and, for HOST inputs, indices outside
-   * [0, nnode) or repeated fail admfem_server_submit with ADMFEM_ERR_ARG;
-   * for DEVICE inputs the request finishes with status ADMFEM_ERR_ARG. */
+   * kappa/f [ngauss], u_prev [n] (heat; n = the system's rows: nnode for
+   * P1, ndof for P2), obs_idx (0-based dof ids, distinct) / obs_val.
+   * Missing inputs and, for HOST inputs, indices outside [0, n) or
+   * repeated fail admfem_server_submit with ADMFEM_ERR_ARG; for DEVICE
+   * inputs the request finishes with status ADMFEM_ERR_ARG. */
   const double *coords, *kappa, *f, *u_prev;
   const int *obs_idx;
   const double *obs_val;
   int nobs;
-  /* SYNTH: interior nodes jittered by U[-a, a]^2 (a from the topology),
+  /* SYNTH: interior vertices jittered by U[-a, a]^2 (a from the topology),
    * kappa = exp(GRF) (32 random Fourier modes, sigma 1, length 0.1) and
-   * f ~ U[-1, 1] at the Gauss points, u_prev a GRF (heat), 1% of the
-   * interior nodes observed with obs_val = 0; all from `seed`. */
+   * f ~ U[-1, 1] at the Gauss points, u_prev a GRF at the dofs' nodes
+   * (heat; a P2 edge dof: its edge's midpoint), 1% of the interior dofs
+   * observed with obs_val = 0; all from `seed`. */
   unsigned long long seed;
   /* host outputs (may be NULL), valid once the request is done */
-  double *u_out;           /* [nnode] */
+  double *u_out;           /* [n] */
   double *grad_out;        /* [ngauss] dL/dkappa */
 } admfem_request;
 

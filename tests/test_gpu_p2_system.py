@@ -1,5 +1,5 @@
-"""The fused P2 system (admfem_system_create on a degree-2 mesh) on the GPU
-against the oracle's P2 chain: P2 COO operators (FemLaplaceScalar /
+"""The fused P2 system (admfem_system_create on a degree-2 mesh) and P2
+requests in the serving engine, on the GPU, against the oracle's P2 chain: P2 COO operators (FemLaplaceScalar /
 ComputeFemMassMatrixMfem / FemSourceScalar, P2 branch) -> ImposeDirichlet ->
 direct solve -> the Grad-op chain for dL/dkappa.
 
@@ -160,11 +160,96 @@ def test_p2_system_gradient_entry_points():
     assert rel(got[0], om.laplace_backward(gvv)) < 1e-12
 
 
-def test_p2_system_refused_by_server():
-    gm, om = pair(6, 2)
-    S = admfem.DirichletSystem(gm, om.bcnode())
-    srv = Server(1 << 28)
-    with pytest.raises(Exception) as ei:
-        srv.add_topology(S)
-    assert "P1" in str(ei.value)
-    srv.close()
+def p2_topology(m):
+    c, e = meshgen.jittered_square(m, seed=0, jitter=0.0, permute=False)
+    gm = admfem.Mesh(c, e, -1, 2, zero_based=True)
+    return c, e, admfem.DirichletSystem(gm, admfem.bcnode(gm))
+
+
+def p2_request(c, e, seed, heat):
+    """Host inputs of one P2 request: the topology's interior vertices
+    jittered, kappa = exp(GRF) and f ~ U[-1, 1] at the Gauss points, 5% of
+    the interior dofs observed, and for a heat step u_prev at the dofs."""
+    rng = np.random.default_rng(seed)
+    m = int(round(np.sqrt(c.shape[0]))) - 1
+    cc = c.copy()
+    inner = (c[:, 0] > 0) & (c[:, 0] < 1) & (c[:, 1] > 0) & (c[:, 1] < 1)
+    cc[inner] += rng.uniform(-0.2 / m, 0.2 / m, size=(inner.sum(), 2))
+    om = O.OracleMesh(cc, e, degree=2)
+    kap = np.exp(meshgen.grf(np.stack([om.gx, om.gy], 1), seed + 1))
+    f = rng.uniform(-1, 1, om.ngauss)
+    bd = om.bcnode()
+    interior = np.setdiff1d(np.arange(om.ndof), bd - 1)
+    obs = rng.choice(interior, size=interior.size // 20, replace=False).astype(np.int32)
+    uobs = rng.standard_normal(obs.size) * 1e-3
+    up = rng.standard_normal(om.ndof) if heat else None
+    return dict(coords=cc, om=om, kappa=kap, f=f, bd=bd, obs=obs, uobs=uobs, u_prev=up,
+                dt=0.02 if heat else None)
+
+
+def p2_oracle(P):
+    kw = {} if P["dt"] is None else dict(dt=P["dt"], u_prev=P["u_prev"])
+    return O.forward_adjoint(
+        P["om"], P["kappa"], P["f"], P["bd"], np.zeros(P["bd"].size),
+        lambda u: np.bincount(P["obs"], 2 * (u[P["obs"]] - P["uobs"]), minlength=u.size), **kw)
+
+
+def p2_submit(sv, tid, P):
+    return sv.submit(tid, kind="heat" if P["dt"] else "poisson", dt=P["dt"] or 0.0,
+                     coords=P["coords"], kappa=P["kappa"], f=P["f"], u_prev=P["u_prev"],
+                     obs_idx=P["obs"], obs_val=P["uobs"])
+
+
+@pytest.mark.parametrize("mode", ["batched", "seq_fp64"])
+def test_p2_serving_matches_oracle(mode):
+    """P2 topologies in the serving engine beside a P1 one: Poisson and heat
+    requests with host inputs against the oracle's P2 chain."""
+    print(f"P2 serving vs oracle, mode={mode}", flush=True)
+    sv = Server(2 << 30, mode=mode, rtol=1e-11)
+    cases = []
+    for m, heat, seed in ((8, False, 1), (11, True, 2), (6, False, 3)):
+        c, e, S = p2_topology(m)
+        tid = sv.add_topology(S, jitter=0.2 / m)
+        P = p2_request(c, e, seed, heat)
+        cases.append((p2_submit(sv, tid, P), P))
+    c1, e1 = meshgen.jittered_square(16, seed=0, jitter=0.0, permute=False)
+    g1 = admfem.Mesh(c1, e1, zero_based=True)
+    t1 = sv.add_topology(admfem.DirichletSystem(g1, admfem.bcnode(g1)), 0.2 / 16)
+    others = [sv.submit(t1, seed=s) for s in range(3)]
+    sv.run()
+    for rid, P in cases:
+        r = sv.result(rid)
+        assert r["done"] and r["status"] == 0, r
+        ref = p2_oracle(P)
+        assert rel(r["u"], ref["u"]) < 1e-8
+        assert rel(r["grad"], ref["grad_kappa"]) < 1e-8
+        assert r["loss"] == pytest.approx(((ref["u"][P["obs"]] - P["uobs"]) ** 2).sum(), rel=1e-8)
+    assert all(sv.result(i)["status"] == 0 for i in others)
+
+
+def test_p2_serving_same_alone_and_in_mix():
+    """A P2 request's result is bit for bit the same alone and among P1 and
+    synthetic P2 requests (heat and Poisson)."""
+    print("P2 serving alone vs mix", flush=True)
+    c, e, S = p2_topology(10)
+    P = p2_request(c, e, 9, heat=True)
+    alone = Server(1 << 30, rtol=1e-10)
+    ra = p2_submit(alone, alone.add_topology(S, 0.02), P)
+    alone.run()
+    mix = Server(1 << 30, rtol=1e-10)
+    tp2 = mix.add_topology(S, 0.02)
+    c1, e1 = meshgen.jittered_square(30, seed=0, jitter=0.0, permute=False)
+    g1 = admfem.Mesh(c1, e1, zero_based=True)
+    tp1 = mix.add_topology(admfem.DirichletSystem(g1, admfem.bcnode(g1)), 0.2 / 30)
+    ids = [mix.submit(tp1 if s % 2 else tp2, kind="heat" if s % 3 == 0 else "poisson",
+                      dt=0.05, seed=50 + s) for s in range(6)]
+    rm = p2_submit(mix, tp2, P)
+    mix.run()
+    a, b = alone.result(ra), mix.result(rm)
+    assert a["status"] == 0 and b["status"] == 0
+    assert mix.stats()["peak_active"] > 1
+    np.testing.assert_array_equal(a["u"], b["u"])
+    np.testing.assert_array_equal(a["grad"], b["grad"])
+    for i in ids:
+        r = mix.result(i)
+        assert r["status"] == 0 and np.isfinite(r["grad"]).all()
