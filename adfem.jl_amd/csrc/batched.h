@@ -44,6 +44,10 @@ struct SynField {
 
 void batch_init(const BSys *D, BScal *S, int sys, int nblk, double rtol2, double delta2,
                 int maxit, hipStream_t st);
+// the block map of nsys systems (slots slot[k], first logical block
+// pref[k], nmap blocks in all) over gmap >= nmap grid blocks, XCD-aware
+void batch_map(int nsys, const int *slot, const int *pref, int nmap, int gmap, int2 *map,
+               hipStream_t st);
 void batch_iterations(const int2 *map, int nmap, const int *list, int nlist, const BSys *D,
                       BScal *S, int iters, hipStream_t st);
 void batch_spmv_repeat(const int2 *map, int nmap, const BSys *D, const BScal *S, int repeat,

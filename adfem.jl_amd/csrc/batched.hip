@@ -454,6 +454,31 @@ __global__ void syn_nodal(int n, const double *__restrict__ x, const double *__r
   u[r] = F.scale * g;
 }
 
+// The block map of a batch: grid block b -> (system slot, block of that
+// system).  The logical list is the systems' blocks in order (system k's
+// start at pref[k]); XCD x = b % 8 (blocks are dispatched round-robin over
+// the 8 XCDs) runs one contiguous eighth of it, entry j of its share at grid
+// block 8 j + x, so neighbouring blocks of a system gather from the same
+// L2.  Grid blocks past a share get system -1 (no work).
+__global__ void expand_map(int nsys, const int *__restrict__ slot, const int *__restrict__ pref,
+                           int nmap, int gmap, int2 *__restrict__ map) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= gmap) return;
+  const int x = b & 7, j = b >> 3, base = nmap >> 3, rem = nmap & 7;
+  if (j >= base + (x < rem ? 1 : 0)) {
+    map[b] = make_int2(-1, 0);
+    return;
+  }
+  const int L = x * base + min(x, rem) + j;
+  int lo = 0, hi = nsys - 1;  // the last system starting at or before L
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pref[mid] <= L) lo = mid;
+    else hi = mid - 1;
+  }
+  map[b] = make_int2(slot[lo], L - pref[lo]);
+}
+
 // user-order interleaved coordinates -> internal-order x, y and xy
 __global__ void perm_coords(int n, const int *__restrict__ iperm, const double *__restrict__ c,
                             double *__restrict__ x, double *__restrict__ y,
@@ -474,6 +499,12 @@ inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 void launch_perm_coords(int n, const int *iperm, const double *coords_user, double *x,
                         double *y, double2 *xy, hipStream_t st) {
   perm_coords<<<nb(n, 256), 256, 0, st>>>(n, iperm, coords_user, x, y, xy);
+  HIPCHECK(hipGetLastError());
+}
+
+void batch_map(int nsys, const int *slot, const int *pref, int nmap, int gmap, int2 *map,
+               hipStream_t st) {
+  expand_map<<<nb(gmap, 256), 256, 0, st>>>(nsys, slot, pref, nmap, gmap, map);
   HIPCHECK(hipGetLastError());
 }
 

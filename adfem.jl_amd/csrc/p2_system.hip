@@ -34,7 +34,7 @@ namespace admfem {
 
 namespace {
 
-constexpr int P2A_NT = 128;
+constexpr int P2A_NT = 256;
 constexpr int P2_SYM = 21;  // packed upper triangle of a 6x6 matrix
 
 // index of (p, q) in the packed upper triangle, rows 0..5
@@ -83,7 +83,7 @@ __device__ __forceinline__ void p2_element(int e, int nelem, const QuadRule &rul
   }
 #pragma unroll
   for (int p = 0; p < 6; p++) r[p] = 0.0;
-#pragma unroll
+#pragma unroll 1  // one Gauss point's basis in registers at a time (occupancy)
   for (int k = 0; k < NQ; k++) {
     const size_t g = (size_t)NQ * e + k;
     const double w = rule.w[k] * G.area;

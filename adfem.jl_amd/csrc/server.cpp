@@ -95,6 +95,7 @@ struct Req {
   float *sv32, *ysa, *ysb, *ra, *rb, *pa, *pb, *qa, *qb;
   int *obsi;
   int nobs = 0;
+  std::vector<int> h_obs;  // synthetic observation indices, kept until the request is done
 };
 
 double now_s(std::chrono::steady_clock::time_point t0) {
@@ -120,11 +121,11 @@ struct Server {
   BScal *h_scal = nullptr;       // pinned mirror
   double *h_loss = nullptr;      // pinned, per slot: loss and bad-index flag
   BSys *h_desc = nullptr;        // pinned staging of the descriptors
-  int2 *h_map = nullptr;         // pinned staging for the block map
-  int *h_list = nullptr;
+  int *h_list = nullptr;         // pinned staging: the batch's slots, then
+                                 // their first logical blocks (list_cap each)
   size_t map_cap = 0, list_cap = 0;
   DevBuf<int2> d_map;
-  DevBuf<int> d_list;
+  DevBuf<int> d_list;            // slots [list_cap], first blocks [list_cap + 1]
   bool started = false;
   bool use_graph = false;
   std::chrono::steady_clock::time_point t0;
@@ -138,7 +139,6 @@ struct Server {
     if (h_scal) (void)hipHostFree(h_scal);
     if (h_loss) (void)hipHostFree(h_loss);
     if (h_desc) (void)hipHostFree(h_desc);
-    if (h_map) (void)hipHostFree(h_map);
     if (h_list) (void)hipHostFree(h_list);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -238,19 +238,18 @@ struct Server {
       }
       synth_inputs(s, m.x.p, m.y.p, r.seed, T.jitter, kf, heat ? &uf : nullptr, q.x, q.y,
                    q.xy, q.kappa, q.f, q.uprev, st);
-      // observations: 1% of the interior nodes, obs_val = 0
-      std::vector<int> idx;
+      // observations: 1% of the interior nodes, obs_val = 0; the indices'
+      // host copy lives with the request (no wait for the copy here)
+      std::vector<int> &idx = q.h_obs;
+      idx.clear();
       std::unordered_set<int> seen;
       std::uniform_int_distribution<size_t> pick(0, T.interior.size() - 1);
       while ((int)idx.size() < q.nobs) {
         int u = T.interior[pick(rng)];
         if (seen.insert(u).second) idx.push_back(u);
       }
-      std::vector<double> zero(q.nobs, 0.0);
       HIPCHECK(hipMemcpyAsync(q.obsi, idx.data(), sizeof(int) * q.nobs, hipMemcpyHostToDevice, st));
-      HIPCHECK(hipMemcpyAsync(q.obsv, zero.data(), sizeof(double) * q.nobs, hipMemcpyHostToDevice,
-                              st));
-      HIPCHECK(hipStreamSynchronize(st));  // host staging vectors go out of scope
+      HIPCHECK(hipMemsetAsync(q.obsv, 0, sizeof(double) * q.nobs, st));
       return;
     }
     const hipMemcpyKind kind =
@@ -384,6 +383,7 @@ struct Server {
   // after the stream has drained: a finished request's loss and status, its
   // memory and slot back to the queue
   void complete(Req &q) {
+    std::vector<int>().swap(q.h_obs);
     const double *l2 = h_loss + 2 * q.slot;
     q.res.loss = l2[0];
     if (l2[1] != 0.0 && q.res.status == ADMFEM_OK) q.res.status = ADMFEM_ERR_ARG;
@@ -412,43 +412,42 @@ struct Server {
   }
 
   // ------------------------------------------------------------ batched run
-  // Fills h_map[0, gmap) with the (slot, block) pairs of `slots` (each
-  // system's blocks in order) so that each XCD runs one contiguous eighth of
-  // that list: blocks are dispatched round-robin over the 8 XCDs (grid block
-  // b to XCD b % 8), so entry j of XCD x's share goes to grid block 8 j + x.
-  // Neighbouring blocks of a system then gather from the same L2.  Grid
-  // blocks past an XCD's share get system -1 (no work).
-  void fill_map(const std::vector<std::pair<int, int>> &slots, size_t nmap, size_t gmap) {
-    size_t sys = 0, blk = 0;  // cursor into the logical list
-    for (int x = 0; x < 8; x++) {
-      const size_t share = nmap / 8 + ((size_t)x < nmap % 8 ? 1 : 0);
-      size_t b = (size_t)x;
-      for (size_t j = 0; j < share; j++, b += 8) {
-        while (blk >= (size_t)slots[sys].second) {
-          sys++;
-          blk = 0;
-        }
-        h_map[b] = make_int2(slots[sys].first, (int)blk++);
-      }
-      for (; b < gmap; b += 8) h_map[b] = make_int2(-1, 0);
-    }
-  }
-
-  void ensure_map(size_t nmap, size_t nlist) {
-    if (nmap > map_cap) {
-      if (h_map) HIPCHECK(hipHostFree(h_map));
-      map_cap = std::max(nmap, 2 * map_cap);
-      HIPCHECK(hipHostMalloc((void **)&h_map, sizeof(int2) * map_cap, hipHostMallocDefault));
+  // Device map of gmap grid blocks and host/device lists for nlist systems;
+  // growing them drops the captured graphs (they point at the old buffers).
+  void ensure_map(size_t gmap, size_t nlist) {
+    if (gmap > map_cap) {
+      map_cap = std::max(gmap, 2 * map_cap);
       d_map.alloc(map_cap);
-      drop_graphs();  // captured graphs point at the old buffers
+      drop_graphs();
     }
     if (nlist > list_cap) {
       if (h_list) HIPCHECK(hipHostFree(h_list));
       list_cap = std::max(nlist, 2 * list_cap);
-      HIPCHECK(hipHostMalloc((void **)&h_list, sizeof(int) * list_cap, hipHostMallocDefault));
-      d_list.alloc(list_cap);
+      HIPCHECK(hipHostMalloc((void **)&h_list, sizeof(int) * (2 * list_cap + 1),
+                             hipHostMallocDefault));
+      d_list.alloc(2 * list_cap + 1);
       drop_graphs();
     }
+  }
+
+  // Uploads the slots of `slots` (padded to glist with `pad`) and their
+  // first logical blocks, and expands the block map on the device.
+  void upload_batch(const std::vector<std::pair<int, int>> &slots, size_t glist, int pad,
+                    size_t nmap, size_t gmap) {
+    int *pref = h_list + list_cap;
+    int acc = 0;
+    for (size_t a = 0; a < slots.size(); a++) {
+      h_list[a] = slots[a].first;
+      pref[a] = acc;
+      acc += slots[a].second;
+    }
+    pref[slots.size()] = acc;
+    for (size_t a = slots.size(); a < glist; a++) h_list[a] = pad;
+    HIPCHECK(hipMemcpyAsync(d_list.p, h_list, sizeof(int) * glist, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(d_list.p + list_cap, pref, sizeof(int) * (slots.size() + 1),
+                            hipMemcpyHostToDevice, st));
+    batch_map((int)slots.size(), d_list.p, d_list.p + list_cap, (int)nmap, (int)gmap, d_map.p,
+              st);
   }
 
   // One CG chunk for a (map size, list size) bucket, captured once as a HIP
@@ -492,6 +491,36 @@ struct Server {
     std::vector<int> active;
     for (size_t i = 0; i < reqs.size(); i++)
       if (reqs[i].state == Q_FWD || reqs[i].state == Q_ADJ) active.push_back((int)i);
+    // requests whose last work (outputs, loss) is enqueued, and an event
+    // behind that work: they complete once it has passed -- waited for
+    // while the next chunk runs, so the host bookkeeping overlaps it
+    std::vector<int> done, failed;
+    struct Ev {
+      hipEvent_t e = nullptr;
+      ~Ev() {
+        if (e) (void)hipEventDestroy(e);
+      }
+    } ev;
+    HIPCHECK(hipEventCreateWithFlags(&ev.e, hipEventDisableTiming));
+    hipEvent_t ev_done = ev.e;
+    auto settle = [&]() {
+      for (int i : done) complete(reqs[i]);
+      for (int i : failed) {
+        Req &q = reqs[i];
+        std::vector<int>().swap(q.h_obs);
+        q.res.done = 1;
+        q.res.t_done = now_s(t0);
+        q.state = Q_DONE;
+        release(q);
+      }
+      done.clear();
+      failed.clear();
+    };
+    auto flush = [&]() {
+      if (done.empty() && failed.empty()) return;
+      HIPCHECK(hipEventSynchronize(ev_done));
+      settle();
+    };
     for (;;) {
       double t = now_s(t0);
       if (t > timeout) break;
@@ -505,6 +534,10 @@ struct Server {
       }
       peak_active = std::max(peak_active, (int)active.size());
       if (active.empty()) {
+        if (!done.empty() || !failed.empty()) {  // their slots may admit the next ones
+          flush();
+          continue;
+        }
         if (next >= order.size()) break;
         double wait = reqs[order[next]].r.arrival - now_s(t0);
         if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
@@ -522,12 +555,8 @@ struct Server {
       for (size_t a = 0; a < active.size(); a++) {
         const Req &q = reqs[active[a]];
         slots[a] = std::make_pair(q.slot, topos[q.r.topo].nblk);
-        h_list[a] = q.slot;
       }
-      fill_map(slots, nmap, gmap);
-      for (size_t a = active.size(); a < glist; a++) h_list[a] = max_slots;  // dummy slot
-      HIPCHECK(hipMemcpyAsync(d_map.p, h_map, sizeof(int2) * gmap, hipMemcpyHostToDevice, st));
-      HIPCHECK(hipMemcpyAsync(d_list.p, h_list, sizeof(int) * glist, hipMemcpyHostToDevice, st));
+      upload_batch(slots, glist, max_slots, nmap, gmap);  // padding: the dummy slot
       if (use_graph)
         launch_chunk((int)gmap, (int)glist, chunk);
       else
@@ -538,10 +567,11 @@ struct Server {
       batch_sum += (int64_t)active.size();
       HIPCHECK(hipMemcpyAsync(h_scal, d_scal.p, sizeof(BScal) * max_slots,
                               hipMemcpyDeviceToHost, st));
+      flush();  // the previous chunk's finished requests, while this one runs
       HIPCHECK(hipStreamSynchronize(st));
-      // phase changes; requests that finished or failed complete after one
-      // synchronisation
-      std::vector<int> still, done, failed;
+      // phase changes; requests that finished or failed complete once their
+      // enqueued work has passed
+      std::vector<int> still;
       for (int i : active) {
         Req &q = reqs[i];
         const BScal sc = h_scal[q.slot];
@@ -566,19 +596,10 @@ struct Server {
           done.push_back(i);
         }
       }
-      if (!done.empty() || !failed.empty()) {
-        HIPCHECK(hipStreamSynchronize(st));
-        for (int i : done) complete(reqs[i]);
-        for (int i : failed) {
-          Req &q = reqs[i];
-          q.res.done = 1;
-          q.res.t_done = now_s(t0);
-          q.state = Q_DONE;
-          release(q);
-        }
-      }
+      if (!done.empty() || !failed.empty()) HIPCHECK(hipEventRecord(ev_done, st));
       active.swap(still);
     }
+    flush();
     wall = now_s(t0);
   }
 
@@ -712,9 +733,7 @@ double server_spmv_repeat(Server *sv, const int *topos, int nsys, int repeat,
     std::vector<std::pair<int, int>> slots(nsys);
     for (int i = 0; i < nsys; i++)
       slots[i] = std::make_pair(qs[i].slot, sv->topos[qs[i].r.topo].nblk);
-    sv->fill_map(slots, nmap, nmap);
-    HIPCHECK(hipMemcpyAsync(sv->d_map.p, sv->h_map, sizeof(int2) * nmap, hipMemcpyHostToDevice,
-                            sv->st));
+    sv->upload_batch(slots, (size_t)nsys, sv->max_slots, nmap, nmap);
     HIPCHECK(hipStreamSynchronize(sv->st));
     HIPCHECK(hipEventCreate(&ev[0]));
     HIPCHECK(hipEventCreate(&ev[1]));
