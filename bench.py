@@ -488,6 +488,16 @@ def ops_roofline(m1=2048, m2=1024, solve_m=512, repeat=5):
             lambda: _lib.check(L.admfem_fem_laplace_term_grad(h, P(u), P(u), P(g), P(rhs),
                                                                P(gk), st())),
             2 * N * 8 + ng * 8 + N * 8 + ng * 8 + geo)
+        if deg == 2:  # the fused P2 system: assembly into SELL, adjoint gradient
+            S = admfem.DirichletSystem(mesh, admfem.bcnode(mesh))
+            f2 = torch.rand(ng, dtype=torch.float64, device="cuda") - 0.5
+            run(f"{tag} fused system assembly, K, F, Dirichlet -> SELL (assemble_p2_kernel)",
+                lambda: S.assemble(kappa=g, f=f2),
+                2 * ng * 8 + geo + 24 * ne + S.nnz * 8 + 2 * N * 8, nnz=S.nnz)
+            run(f"{tag} fused system dL/dkappa (permute u, lam + kappa_grad_p2_kernel)",
+                lambda: S.kappa_gradient(u, u), 2 * N * 8 + geo + 24 * ne + ng * 8)
+            S.close()
+            del f2
         mesh.close()
         del g, u, gk, gg, rhs
         torch.cuda.empty_cache()
