@@ -29,7 +29,11 @@ namespace admfem {
 
 namespace {
 
-constexpr int AS_NT = 128;
+// 256 threads, one element each per pass-1 round: 78 VGPRs, 6 waves per
+// SIMD (128 threads with three elements each in flight: 4 waves per SIMD);
+// measured the same within 2% at 2048^2 (317.7 vs 322 us)
+constexpr int AS_NT = 256;
+constexpr int AS_EPT = 1;  // elements per thread in flight in pass 1
 
 // index of (p, q) in the packed upper triangle [00 01 02 11 12 22]
 __device__ __forceinline__ int sym3(int p, int q) {
@@ -231,9 +235,9 @@ __global__ void __launch_bounds__(AS_NT)
 #pragma unroll
     for (int k = 0; k < PF; k++) wb[k] = t0 + k < t1 ? pairs[t0 + k] : 0u;
   }
-  // pass 1 (MASS: a mass term or u_prev is present), in chunks of up to
-  // three elements per thread (a 64-row block of a triangulation touches
-  // ~260 elements: one chunk).  The first chunk's records and coefficients
+  // pass 1 (MASS: a mass term or u_prev is present), in chunks of AS_EPT
+  // elements per thread (a 64-row block of a triangulation touches ~260
+  // elements: one or two chunks).  The first chunk's records and coefficients
   // are loaded before the vertices are staged, so both latencies overlap;
   // the vertices are consecutive rows in runs, so their loads coalesce.
   auto store = [&](const ElemIn<NQ> &e, int t) {
@@ -249,13 +253,12 @@ __global__ void __launch_bounds__(AS_NT)
 #pragma unroll
     for (int p = 0; p < 3; p++) rl[3 * t + p] = r[p];
   };
-  for (int c0 = 0; c0 < max(ne, 1); c0 += 3 * AS_NT) {  // uniform over the block
-    ElemIn<NQ> in[3];
+  for (int c0 = 0; c0 < max(ne, 1); c0 += AS_EPT * AS_NT) {  // uniform over the block
+    ElemIn<NQ> in[AS_EPT];
     const int t = c0 + (int)threadIdx.x;
-    const bool b0 = t < ne, b1 = t + AS_NT < ne, b2 = t + 2 * AS_NT < ne;
-    if (b0) in[0].template load<MASS>(blk_rec[e0 + t], P);
-    if (b1) in[1].template load<MASS>(blk_rec[e0 + t + AS_NT], P);
-    if (b2) in[2].template load<MASS>(blk_rec[e0 + t + 2 * AS_NT], P);
+#pragma unroll
+    for (int j = 0; j < AS_EPT; j++)
+      if (t + j * AS_NT < ne) in[j].template load<MASS>(blk_rec[e0 + t + j * AS_NT], P);
     if (c0 == 0) {
       for (int k = threadIdx.x; k < nv; k += AS_NT) {
         const int v = blk_vert[v0 + k];
@@ -268,9 +271,9 @@ __global__ void __launch_bounds__(AS_NT)
       }
       __syncthreads();
     }
-    if (b0) store(in[0], t);
-    if (b1) store(in[1], t + AS_NT);
-    if (b2) store(in[2], t + 2 * AS_NT);
+#pragma unroll
+    for (int j = 0; j < AS_EPT; j++)
+      if (t + j * AS_NT < ne) store(in[j], t + j * AS_NT);
   }
   __syncthreads();
   // pass 2: rows; each row's thread owns its slots of the accumulators and
