@@ -231,6 +231,30 @@ def test_sparse_solve_several_components_random_numbering():
     assert admfem.ops.solve_pattern_info()["reordered"]
 
 
+def test_sparse_solve_one_sided_explicit_zeros_random_numbering():
+    """A shifted grid Laplacian numbered at random plus explicit zeros on one
+    side of the diagonal only: the pattern is not symmetric, so the device
+    Cuthill-McKee pass must notice and leave the order to the host; the
+    values (and the solution) are the SPD matrix's."""
+    m = 90
+    n = m * m
+    T = sp.diags([-np.ones(m - 1), 2.0 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+    A = (sp.kron(sp.eye(m), T) + sp.kron(T, sp.eye(m)) + 0.05 * sp.eye(n)).tocoo()
+    rng = np.random.default_rng(17)
+    p = rng.permutation(n)
+    zi = rng.integers(0, n, 300)
+    zj = (zi + rng.integers(m + 2, 5 * m, 300)) % n   # no partner entry (j, i)
+    I = np.concatenate([p[A.row], zi]).astype(np.int64)
+    J = np.concatenate([p[A.col], zj]).astype(np.int64)
+    V = np.concatenate([A.data, np.zeros(zi.size)])
+    b = rng.standard_normal(n)
+    ind = torch.as_tensor(np.stack([I, J], 1), device="cuda")
+    u = admfem.solve(admfem.SparseTensor(ind, torch.tensor(V, device="cuda"), n),
+                     torch.tensor(b, device="cuda"), rtol=1e-13)
+    u0 = spl.spsolve(sp.csc_matrix((V, (I, J)), shape=(n, n)), b)
+    assert rel(u.cpu().numpy(), u0) < 1e-10
+
+
 def test_sparse_solve_pattern_kept_between_calls():
     """The op keeps the last pattern: the same indices with new values, then
     indices of the same sizes but another pattern, then an out-of-range
