@@ -59,6 +59,23 @@ __device__ __forceinline__ double gauss_weight(double ipw, double area) {
   return ipw * area / 0.5;
 }
 
+// Streamed COO output: triplet t's index pair (ind[2t], ind[2t + 1]) and
+// value, written with non-temporal stores (nothing reads them back soon),
+// the pair as one 16-byte store when `ind` is 16-byte aligned.
+typedef long long ll2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_triplet(long long *ind, double *vv, int64_t t,
+                                              long long a, long long b, double v,
+                                              bool ind16) {
+  if (ind16) {
+    const ll2_t ab = {a, b};
+    __builtin_nontemporal_store(ab, reinterpret_cast<ll2_t *>(ind) + t);
+  } else {
+    __builtin_nontemporal_store(a, ind + 2 * t);
+    __builtin_nontemporal_store(b, ind + 2 * t + 1);
+  }
+  __builtin_nontemporal_store(v, vv + t);
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b to XCD b % 8).
 // This bijection gives each XCD one contiguous range of logical blocks, so
 // data shared by neighbouring blocks is reused in that XCD's L2.

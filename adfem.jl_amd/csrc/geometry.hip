@@ -61,8 +61,9 @@ __global__ void geometry_kernel(int nelem, QuadRule rule,
 // h_p h_q (mass), w the mesh's Gauss weight (ip.weight * area / 0.5, the
 // same bits as the per-element computation).  A block owns CO_E consecutive
 // elements: one thread per element puts its 6 distinct D_pq and its user
-// vertex ids in LDS, then all threads write the block's contiguous run of
-// 9 * nq * CO_E triplets, coalesced.
+// vertex ids in LDS, the block's coef and w go to LDS too (each is used by
+// 9 triplets), then all threads write the block's contiguous run of
+// 9 * nq * CO_E triplets, coalesced, with streaming (non-temporal) stores.
 constexpr int CO_NT = 256, CO_E = 64;
 
 template <bool MASS>
@@ -71,12 +72,19 @@ __global__ void __launch_bounds__(CO_NT)
                const int *__restrict__ conn_user, const double *__restrict__ x,
                const double *__restrict__ y, const double *__restrict__ gw,
                const double *__restrict__ coef, long long *__restrict__ ind,
-               double *__restrict__ vv) {
+               double *__restrict__ vv, bool ind16) {
 #pragma clang fp contract(off)
   __shared__ double dd[CO_E][6];        // D_pq, packed upper triangle (Laplace)
   __shared__ int cu[CO_E][3];
+  __shared__ double cf[CO_E * MAXQ], cw[CO_E * MAXQ];  // coef[g], w[g] of the block
   const int e0 = blockIdx.x * CO_E;
   const int ne = min(CO_E, nelem - e0);
+  const int nq = rule.nq;
+  for (int l = threadIdx.x; l < ne * nq; l += CO_NT) {
+    const int64_t g = (int64_t)nq * e0 + l;
+    cf[l] = coef[g];
+    cw[l] = gw[g];
+  }
   if ((int)threadIdx.x < ne) {
     const int e = e0 + threadIdx.x;
 #pragma unroll
@@ -92,14 +100,13 @@ __global__ void __launch_bounds__(CO_NT)
     }
   }
   __syncthreads();
-  const int nq = rule.nq;
   const int64_t t0 = (int64_t)9 * nq * e0;
   const int nt = 9 * nq * ne;
   for (int l = threadIdx.x; l < nt; l += CO_NT) {
     const int gl = l / 9, pq = l - 9 * gl;
     const int p = pq / 3, q = pq - 3 * p;
     const int el = gl / nq, k = gl - nq * el;
-    const int64_t g = (int64_t)nq * e0 + gl, t = t0 + l;
+    const int64_t t = t0 + l;
     double d;
     if (MASS) {
       d = shape(p, rule.xi[k], rule.eta[k]) * shape(q, rule.xi[k], rule.eta[k]);
@@ -107,9 +114,7 @@ __global__ void __launch_bounds__(CO_NT)
       const int lo = min(p, q), hi = max(p, q);
       d = dd[el][lo == 0 ? hi : (lo == 1 ? 2 + hi : 5)];
     }
-    ind[2 * t] = cu[el][p];
-    ind[2 * t + 1] = cu[el][q];
-    vv[t] = d * coef[g] * gw[g];
+    store_triplet(ind, vv, t, cu[el][p], cu[el][q], d * cf[gl] * cw[gl], ind16);
   }
 }
 
@@ -222,7 +227,8 @@ void launch_laplace_coo(const Mesh &m, const double *kappa, long long *ind,
                         double *vv, hipStream_t st) {
   if (m.degree == 2) return launch_p2_coo(m, false, kappa, ind, vv, st);
   coo_kernel<false><<<nblk(m.nelem, CO_E), CO_NT, 0, st>>>(
-      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, m.gw.p, kappa, ind, vv);
+      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, m.gw.p, kappa, ind, vv,
+      ((uintptr_t)ind & 15) == 0);
   HIPCHECK(hipGetLastError());
 }
 
@@ -238,7 +244,8 @@ void launch_mass_coo(const Mesh &m, const double *rho, long long *ind,
                      double *vv, hipStream_t st) {
   if (m.degree == 2) return launch_p2_coo(m, true, rho, ind, vv, st);
   coo_kernel<true><<<nblk(m.nelem, CO_E), CO_NT, 0, st>>>(
-      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, m.gw.p, rho, ind, vv);
+      m.nelem, m.rule, m.conn.p, m.conn_user.p, m.x.p, m.y.p, m.gw.p, rho, ind, vv,
+      ((uintptr_t)ind & 15) == 0);
   HIPCHECK(hipGetLastError());
 }
 

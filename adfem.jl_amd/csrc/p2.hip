@@ -31,8 +31,9 @@ namespace {
 // (FemLaplaceScalar.h:3-26, ComputeFemMassMatrixMfem.h:4-29): triplet
 // t = 36 g + 6 p + q, value D_pq * coef[g] * w[g].  A block owns P2_E
 // consecutive elements: one thread per (element, Gauss point) puts the six
-// basis gradients in LDS (Laplace), then all threads write the block's
-// contiguous run of triplets, coalesced.
+// basis gradients in LDS (Laplace), the block's coef and w go to LDS too,
+// then all threads write the block's contiguous run of triplets, coalesced,
+// with streaming stores.
 constexpr int P2_NT = 256, P2_E = 16;
 
 template <bool MASS>
@@ -41,13 +42,19 @@ __global__ void __launch_bounds__(P2_NT)
                   const int *__restrict__ conn_dof, const double *__restrict__ x,
                   const double *__restrict__ y, const double *__restrict__ gw,
                   const double *__restrict__ coef, long long *__restrict__ ind,
-                  double *__restrict__ vv) {
+                  double *__restrict__ vv, bool ind16) {
 #pragma clang fp contract(off)
   __shared__ double lgx[P2_E][MAXQ][6], lgy[P2_E][MAXQ][6];
   __shared__ int cd[P2_E][6];
+  __shared__ double cf[P2_E * MAXQ], cw[P2_E * MAXQ];  // coef[g], w[g] of the block
   const int nq = rule.nq;
   const int e0 = blockIdx.x * P2_E;
   const int ne = min(P2_E, nelem - e0);
+  for (int l = threadIdx.x; l < ne * nq; l += P2_NT) {
+    const int64_t g = (int64_t)nq * e0 + l;
+    cf[l] = coef[g];
+    cw[l] = gw[g];
+  }
   for (int i = threadIdx.x; i < ne * 6; i += P2_NT) {
     const int el = i / 6, r = i - 6 * el;
     cd[el][r] = conn_dof[(size_t)r * nelem + e0 + el];
@@ -70,18 +77,16 @@ __global__ void __launch_bounds__(P2_NT)
     const int gl = l / 36, pq = l - 36 * gl;
     const int p = pq / 6, q = pq - 6 * p;
     const int el = gl / nq, k = gl - nq * el;
-    const int64_t g = (int64_t)nq * e0 + gl, t = t0 + l;
+    const int64_t t = t0 + l;
     double v;
     if (MASS) {
       double h[6];
       p2_shape(rule.xi[k], rule.eta[k], h);
-      v = h[p] * h[q] * coef[g] * gw[g];
+      v = h[p] * h[q] * cf[gl] * cw[gl];
     } else {
-      v = (lgx[el][k][p] * lgx[el][k][q] + lgy[el][k][p] * lgy[el][k][q]) * coef[g] * gw[g];
+      v = (lgx[el][k][p] * lgx[el][k][q] + lgy[el][k][p] * lgy[el][k][q]) * cf[gl] * cw[gl];
     }
-    ind[2 * t] = cd[el][p];
-    ind[2 * t + 1] = cd[el][q];
-    vv[t] = v;
+    store_triplet(ind, vv, t, cd[el][p], cd[el][q], v, ind16);
   }
 }
 
@@ -293,12 +298,13 @@ void dof_op(const Mesh &m, const double *in, const double *nu, double *out, bool
 void launch_p2_coo(const Mesh &m, bool mass, const double *coef, long long *ind, double *vv,
                    hipStream_t st) {
   const unsigned grid = nb(m.nelem, P2_E);
+  const bool ind16 = ((uintptr_t)ind & 15) == 0;
   if (mass)
     p2_coo_kernel<true><<<grid, P2_NT, 0, st>>>(m.nelem, m.rule, m.conn.p, m.conn_dof.p, m.x.p,
-                                                m.y.p, m.gw.p, coef, ind, vv);
+                                                m.y.p, m.gw.p, coef, ind, vv, ind16);
   else
     p2_coo_kernel<false><<<grid, P2_NT, 0, st>>>(m.nelem, m.rule, m.conn.p, m.conn_dof.p, m.x.p,
-                                                 m.y.p, m.gw.p, coef, ind, vv);
+                                                 m.y.p, m.gw.p, coef, ind, vv, ind16);
   HIPCHECK(hipGetLastError());
 }
 
