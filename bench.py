@@ -396,8 +396,11 @@ def precision_form(m, rtol, kernels_only=False):
     rl["assembly"] = roof(b_fused, avg, "assemble_kernel (fused K, F, Dirichlet -> SELL)",
                           bytes_note="kappa, f, connectivity, coordinates read; values (once, "
                                      "FP64) and rhs written: the kernel computes K and F",
-                          survey_bytes=int(b_asm),
-                          survey_frac=round(b_asm / avg / 1e9 / HBM_PEAK_GBS, 4))
+                          k_only_bytes=int(b_asm),
+                          k_only_frac=round(b_asm / avg / 1e9 / HBM_PEAK_GBS, 4),
+                          k_only_note="kappa, connectivity, coordinates read, the matrix "
+                                      "values written (ngauss*8 + 12*nelem + 16*N + nnz*8): "
+                                      "the count for assembling K alone")
     # adjoint gradient dL/dkappa at every Gauss point (the API call: u and
     # lam permuted to the internal node order, then kappa_grad_kernel)
     ug = torch.rand(N, dtype=torch.float64, device="cuda")
