@@ -56,13 +56,11 @@ __global__ void scatter_kept(const long long *__restrict__ ind,
                              const double *__restrict__ vv, int64_t sN,
                              const int *__restrict__ keep,
                              const int64_t *__restrict__ pos,
-                             long long *__restrict__ oind, double *__restrict__ ov) {
+                             long long *__restrict__ oind, double *__restrict__ ov,
+                             bool ind16) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= sN || !keep[k]) return;
-  int64_t z = pos[k];
-  oind[2 * z] = ind[2 * k];
-  oind[2 * z + 1] = ind[2 * k + 1];
-  ov[z] = vv[k];
+  store_triplet(oind, ov, pos[k], ind[2 * k], ind[2 * k + 1], vv[k], ind16);
 }
 
 // lifted triplet list: key = row (forward) or column (backward), value = k
@@ -311,7 +309,9 @@ int64_t run_impose_dirichlet(const long long *ind, const double *vv, int64_t sN,
   std::lock_guard<std::mutex> g(w.mu);
   DirPrep d{w};
   prepare(d, ind, sN, bd, bdN, N, st);
-  if (sN) scatter_kept<<<nblk(sN, 256), 256, 0, st>>>(ind, vv, sN, w.keep.p, w.pos.p, oind, ov);
+  if (sN)
+    scatter_kept<<<nblk(sN, 256), 256, 0, st>>>(ind, vv, sN, w.keep.p, w.pos.p, oind, ov,
+                                                ((uintptr_t)oind & 15) == 0);
   if (d.nu) append_diag<<<nblk(d.nu, 256), 256, 0, st>>>(w.keys.p, d.nu, d.nkept, oind, ov);
   copy_rhs<<<nblk(N, 256), 256, 0, st>>>(rhs, orhs, N);
   sorted_lift(d, ind, sN, 0, st);

@@ -466,6 +466,26 @@ def ops_roofline(m1=2048, m2=1024, solve_m=512, repeat=5):
         run(f"{tag} laplace COO grad (FemLaplaceScalarGrad)",
             lambda: _lib.check(L.admfem_fem_laplace_scalar_grad(h, P(vv), P(gk), st())),
             nt * 8 + ng * 8 + geo)
+        if deg == 1:  # the Dirichlet op on this COO, above the 256 MB Infinity Cache
+            bd1 = admfem.bcnode(mesh)
+            nb1 = bd1.size
+            bdt1 = torch.as_tensor(bd1.astype(np.int64), device="cuda")
+            bdv1 = torch.zeros(nb1, dtype=torch.float64, device="cuda")
+            rhs1 = torch.rand(N, dtype=torch.float64, device="cuda")
+            oi1 = torch.empty((nt + nb1, 2), dtype=torch.int64, device="cuda")
+            ov1 = torch.empty(nt + nb1, dtype=torch.float64, device="cuda")
+            orhs1 = torch.empty(N, dtype=torch.float64, device="cuda")
+            cnt1 = ctypes.c_int64()
+
+            def dirichlet1():
+                _lib.check(L.admfem_impose_dirichlet(P(ind), P(vv), nt, P(bdt1), P(bdv1), nb1,
+                                                     P(rhs1), N, P(oi1), P(ov1), P(orhs1),
+                                                     ctypes.byref(cnt1), st()))
+            dirichlet1()
+            run(f"{tag} impose_Dirichlet_boundary_conditions (COO)", dirichlet1,
+                nt * 24 + cnt1.value * 24 + 2 * N * 8 + nb1 * 16, triplets_in=nt,
+                triplets_out=int(cnt1.value))
+            del oi1, ov1, orhs1, rhs1
         del ind, vv
         rhs = torch.empty(N, dtype=torch.float64, device="cuda")
         run(f"{tag} source (FemSourceScalar)",
