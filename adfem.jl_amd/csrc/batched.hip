@@ -234,13 +234,16 @@ __global__ void __launch_bounds__(NT)
   const float *__restrict__ q = d.q[cur];
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < d.npad) {
-    const float pi = p[i];
-    const Step own = step_of(r[i], q[i], pi, alpha, beta);
+    const float pi = p[i], qi = q[i];
+    const Step own = step_of(r[i], qi, pi, alpha, beta);
     float *__restrict__ ys = d.ys[s.sel];
     ys[i] = (float)fma(alpha, (double)pi, (double)ys[i]);
-    const double a = own.p + sell_dot<float>(i, d.sp, d.sc, d.sv32, [&](int c) {
-                       return step_of(r[c], q[c], p[c], alpha, beta).p;
-                     });
+    // q' = Â p' = Â r' + beta Â p = Â r' + beta q: the neighbours contribute
+    // their r' = r - alpha q (two gathered values each, not three)
+    const double a = fma(beta, (double)qi,
+                         own.r + sell_dot<float>(i, d.sp, d.sc, d.sv32, [&](int c) {
+                           return fma(-alpha, (double)q[c], (double)r[c]);
+                         }));
     const float rv = (float)own.r, pv = (float)own.p, qv = (float)a;
     d.r[cur ^ 1][i] = rv;
     d.p[cur ^ 1][i] = pv;
@@ -325,6 +328,10 @@ __global__ void __launch_bounds__(NT)
     const double ri = (i < d.n ? d.b[i] : 0.0) - ax;
     const float rf = (float)(ri * dsq[i]);
     d.r[s.it & 1][i] = rf;
+    // q = Â p afresh for the kept direction (the step updates q by recurrence)
+    const float *__restrict__ p = d.p[s.it & 1];
+    d.q[s.it & 1][i] = (float)((double)p[i] + sell_dot<float>(i, d.sp, d.sc, d.sv32,
+                                                               [&](int c) { return (double)p[c]; }));
     v[0] = (double)rf * (double)rf;
     v[1] = ri * ri;
   }

@@ -417,8 +417,9 @@ __device__ __forceinline__ double dir32(float r, float q, float p, double alpha,
   return (double)(float)fma(beta, (double)p, rn);
 }
 
-// y += alpha p; r' = r - alpha q; p' = r' + beta p; q' = (S A S) p';
-// partials of r'.r', p'.q', r'.q', q'.q'
+// y += alpha p; r' = r - alpha q; p' = r' + beta p; q' = (S A S) p', formed
+// as Â r' + beta q (Â p = q), so a neighbour contributes its r' = r - alpha q
+// (two gathered values); partials of r'.r', p'.q', r'.q', q'.q'
 __global__ void __launch_bounds__(NT)
     k_mstep(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
             const float *__restrict__ sv, float *r0, float *r1, float *q0, float *q1,
@@ -432,14 +433,13 @@ __global__ void __launch_bounds__(NT)
   const int i = my_row();
   double t[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < npad) {
-    const float pi = p[i];
+    const float pi = p[i], qi = q[i];
     double ri;
-    const double pni = dir32(r[i], q[i], pi, alpha, beta, ri);
+    const double pni = dir32(r[i], qi, pi, alpha, beta, ri);
     ys[i] = (float)fma(alpha, (double)pi, (double)ys[i]);
-    const double a = pni + sell_dot<float>(i, sp, sc, sv, [&](int c) {
-                       double unused;
-                       return dir32(r[c], q[c], p[c], alpha, beta, unused);
-                     });
+    const double a = fma(beta, (double)qi, ri + sell_dot<float>(i, sp, sc, sv, [&](int c) {
+                                             return fma(-alpha, (double)q[c], (double)r[c]);
+                                           }));
     const float rv = (float)ri, pv = (float)pni, qv = (float)a;
     (odd ? r0 : r1)[i] = rv;
     (odd ? p0 : p1)[i] = pv;
@@ -510,16 +510,20 @@ __global__ void __launch_bounds__(NT)
 
 // when a replacement is due: the true residual b - A x in FP64 (FP64
 // matrix), handed to the FP32 iteration as r = S (b - A x) in the current
-// buffer (the direction is kept); partials of the scaled r.r and the true
+// buffer; the direction is kept and its product q = Â p recomputed (the step
+// carries q by recurrence); partials of the scaled r.r and the true
 // ||b - A x||^2
 __global__ void __launch_bounds__(NT)
     k_mreplace(int n, int npad, const int *__restrict__ sp, const int *__restrict__ sc,
                const double *__restrict__ sv, const double *__restrict__ diag,
-               const double *__restrict__ x, const double *__restrict__ dsq, float *r0,
-               float *r1, double *part, const CGScalars *S) {
+               const float *__restrict__ sv32, const double *__restrict__ x,
+               const double *__restrict__ dsq, float *r0, float *r1, const float *p0,
+               const float *p1, float *q0, float *q1, double *part, const CGScalars *S) {
   if (!replace_due(S)) return;
   const double *__restrict__ b = S->b;
-  float *__restrict__ r = (S->it & 1) ? r1 : r0;
+  const bool odd = S->it & 1;
+  float *__restrict__ r = odd ? r1 : r0;
+  const float *__restrict__ p = odd ? p1 : p0;
   const int i = my_row();
   double t[2] = {0.0, 0.0};
   if (i < npad) {
@@ -527,6 +531,8 @@ __global__ void __launch_bounds__(NT)
     const double ri = (i < n ? b[i] : 0.0) - ax;
     const float rf = (float)(ri * dsq[i]);
     r[i] = rf;
+    (odd ? q1 : q0)[i] =
+        (float)((double)p[i] + sell_dot<float>(i, sp, sc, sv32, [&](int c) { return (double)p[c]; }));
     t[0] = (double)rf * (double)rf;
     t[1] = ri * ri;
   }
@@ -674,8 +680,9 @@ void enqueue_iteration(System &s, CGWork *w, bool mixed, int max_iter, int it,
     if ((it & 7) == 7 || it + 1 == max_iter) {
       k_mfold<<<nb, NT, 0, st>>>(np, w->x.p, w->ys32.p, w->dsq.p, S, 0);
       k_mreplace<<<nb, NT, 0, st>>>(s.n, np, s.d_slice_ptr.p, s.d_sell_col.p, s.d_sell_val.p,
-                                    s.d_diag.p, w->x.p, w->dsq.p, w->r32[0].p, w->r32[1].p,
-                                    part, S);
+                                    s.d_diag.p, w->sv32s.p, w->x.p, w->dsq.p, w->r32[0].p,
+                                    w->r32[1].p, w->p32[0].p, w->p32[1].p, w->q32[0].p,
+                                    w->q32[1].p, part, S);
       s_replace<<<1, RT, 0, st>>>(nb, part, S, max_iter);
     }
   } else {
