@@ -152,6 +152,45 @@ __device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
   return s;
 }
 
+// Row i's products with the FP64 and the FP32 matrix of the same pattern in
+// one pass over its columns (each sum in the order sell_dot takes):
+// s64 = sum_k sv64_k f(c_k), s32 = sum_k sv32_k g(c_k).
+template <class F, class G>
+__device__ __forceinline__ void sell_dot2(int i, const int *__restrict__ sp,
+                                          const int *__restrict__ sc,
+                                          const double *__restrict__ sv64,
+                                          const float *__restrict__ sv32, F f, G g,
+                                          double &s64, double &s32) {
+  const int sl = i >> 6;
+  const int s0 = sp[sl];
+  const int off = s0 + (i & 63);
+  const int wfull = (sp[sl + 1] - s0) >> 6;
+  const int w = min(wfull, 8);
+  int c[8];
+  double v[8];
+  float u[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k < w) {
+      c[k] = sc[off + 64 * k];
+      v[k] = __builtin_nontemporal_load(sv64 + off + 64 * k);
+      u[k] = sv32[off + 64 * k];
+    }
+  s64 = 0.0;
+  s32 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k < w) {
+      s64 += v[k] * f(c[k]);
+      s32 += (double)u[k] * g(c[k]);
+    }
+  for (int k = 8; k < wfull; k++) {
+    const int ck = sc[off + 64 * k];
+    s64 += sv64[off + 64 * k] * f(ck);
+    s32 += (double)sv32[off + 64 * k] * g(ck);
+  }
+}
+
 // --------------------------------------------------------------- CG kernels
 // init of ONE system (grid = its nblk): x0 = b on identity rows, 0 elsewhere;
 // r0 = S (b - A x0) (b on the other rows, 0 on identity rows: their columns
@@ -320,18 +359,20 @@ __global__ void __launch_bounds__(NT)
   double v[2] = {0.0, 0.0};
   if (i < d.npad) {
     const double xi = x[i] + dsq[i] * (double)ys[i];
-    const double ax = d.diag64[i] * xi + sell_dot<double>(i, d.sp, d.sc, d.sv64, [&](int c) {
-                        return x[c] + dsq[c] * (double)ys[c];
-                      });
+    // the true residual's product (FP64) and q = Â p afresh for the kept
+    // direction (FP32 matrix; the step updates q by recurrence), one pass
+    const float *__restrict__ p = d.p[s.it & 1];
+    double sx, sq;
+    sell_dot2(
+        i, d.sp, d.sc, d.sv64, d.sv32, [&](int c) { return x[c] + dsq[c] * (double)ys[c]; },
+        [&](int c) { return (double)p[c]; }, sx, sq);
+    const double ax = d.diag64[i] * xi + sx;
     d.x[s.sel ^ 1][i] = xi;
     d.ys[s.sel ^ 1][i] = 0.0f;
     const double ri = (i < d.n ? d.b[i] : 0.0) - ax;
     const float rf = (float)(ri * dsq[i]);
     d.r[s.it & 1][i] = rf;
-    // q = Â p afresh for the kept direction (the step updates q by recurrence)
-    const float *__restrict__ p = d.p[s.it & 1];
-    d.q[s.it & 1][i] = (float)((double)p[i] + sell_dot<float>(i, d.sp, d.sc, d.sv32,
-                                                               [&](int c) { return (double)p[c]; }));
+    d.q[s.it & 1][i] = (float)((double)p[i] + sq);
     v[0] = (double)rf * (double)rf;
     v[1] = ri * ri;
   }
