@@ -17,9 +17,10 @@
 // One CG iteration is one wide kernel and one single-block-per-system
 // kernel (FP32 storage, FP64 arithmetic):
 //   bk_step  applies the pending step and forms the next direction,
-//              r' = r - alpha q,  y += alpha p,  p' = r' + beta p,  q' = Â p'
-//            (a neighbour's p' is formed on the fly from its r, q, p), and
-//            the partials of r'.r', p'.q', r'.q', q'.q'
+//              r' = r - alpha q,  y += alpha p,  p' = r' + beta p,
+//              q' = Â p' = Â r' + beta q
+//            (a neighbour contributes its r' = r - alpha q, formed on the fly
+//            from its r and q), and the partials of r'.r', p'.q', r'.q', q'.q'
 //   bs_step  alpha' = r'.r' / p'.q' and beta' = |r' - alpha' q'|^2 / r'.r',
 //            the numerator expanded as r'.r' - 2 alpha' r'.q' + alpha'^2 q'.q'
 //            (exact for the stored vectors up to rounding, one step ahead,
@@ -31,7 +32,9 @@
 // a chunk (a replacement point):
 //   bk_replace (frozen systems, residual fell by delta since the last
 //              replacement, or stalled): x' = x + S y in FP64, the true
-//              residual b - A x' with the FP64 matrix, r = S (b - A x'), y' = 0
+//              residual b - A x' with the FP64 matrix, r = S (b - A x'), y' = 0,
+//              and q = Â p afresh for the kept direction (the step carries q
+//              by recurrence)
 //   bs_replace true-residual convergence test, swap the x / y buffers
 // The pending alpha and beta carry over a replacement (the direction is
 // kept; reliable updates, van der Vorst & Ye 2000).  Systems start at chunk

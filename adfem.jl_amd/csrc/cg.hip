@@ -18,7 +18,9 @@
 // One iteration is one wide kernel and one single-block kernel:
 //   step:    applies the pending step and forms the next direction,
 //            r' = r - alpha q, x += alpha p, p' = M^-1 r' + beta p, q' = A p'
-//            (a neighbour's p' formed on the fly from its own vectors), and
+//            (FP64: a neighbour's p' formed on the fly from its own vectors;
+//            mixed: q' = Â r' + beta q, a neighbour contributing its
+//            r' = r - alpha q, and each replacement recomputes q = Â p), and
 //            the block partials of every dot product the next scalars need
 //   scalar:  alpha' = r'.M^-1 r' / p'.q' and beta' = the same quadratic form
 //            of r' - alpha' q' over r'.M^-1 r', its numerator expanded as
