@@ -2,12 +2,19 @@
 // (deps/MFEM/ImposeDirichlet/ImposeDirichlet.h:27-93), on the device.
 //
 // Forward: triplets with both ends off the boundary are kept in input order
-// (stable compaction by prefix sum), then (b,b,1.0) for every distinct
+// (stable compaction), then (b,b,1.0) for every distinct
 // boundary node in ascending order.  The RHS lifting rhs[i] -= v*bdval[j]
 // (i interior, j boundary) is done per row in triplet order -- the lifted
 // triplets are stably sorted by row and each row's run is summed serially --
 // so orhs is bitwise the reference's.  Duplicate bd entries: the last one
 // wins, as std::map assignment does.
+//
+// The compaction is a two-pass tiled select with no per-triplet workspace:
+// pass 1 counts each tile's kept and lifted triplets (the tile's indices
+// read once), a scan of the tile counts gives each tile its output offsets,
+// and pass 2 re-reads the tile, ranks its triplets in input order with wave
+// ballots and writes them out (the backward pass recomputes the same ranks
+// to find each kept triplet's output slot).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -34,45 +41,156 @@ __global__ void set_bdidx(const int *__restrict__ keys, const int *__restrict__ 
   if (t < nu) bdidx[keys[t]] = win[t];
 }
 
-// keep flag (both ends interior) and lift flag (i interior, j boundary)
-__global__ void classify(const long long *__restrict__ ind, int64_t sN, int N,
-                         const int *__restrict__ bdidx, int *__restrict__ keep,
-                         int *__restrict__ lift, int *__restrict__ err) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sN) return;
-  long long i = ind[2 * k], j = ind[2 * k + 1];
-  if (i < 0 || i >= N || j < 0 || j >= N) {  // never index with a bad id
-    keep[k] = 0;
-    lift[k] = 0;
-    *err = 1;
-    return;
+// Tiles of the triplet list: DIR_ITEMS rows of DIR_NT triplets, item u of
+// thread t being triplet tile * DIR_TILE + u * DIR_NT + t (coalesced loads;
+// input order is u-major, then thread).
+constexpr int DIR_NT = 256;
+constexpr int DIR_ITEMS = 8;
+constexpr int DIR_TILE = DIR_NT * DIR_ITEMS;
+constexpr int DIR_WAVES = DIR_NT / 64;
+
+__device__ __forceinline__ void load_pair(const long long *__restrict__ ind, int64_t k,
+                                          bool ind16, long long &i, long long &j) {
+  if (ind16) {
+    const ll2_t v = reinterpret_cast<const ll2_t *>(ind)[k];
+    i = v.x;
+    j = v.y;
+  } else {
+    i = ind[2 * k];
+    j = ind[2 * k + 1];
   }
-  bool bi = bdidx[i] >= 0, bj = bdidx[j] >= 0;
-  keep[k] = (!bi && !bj) ? 1 : 0;
-  lift[k] = (!bi && bj) ? 1 : 0;
 }
 
-__global__ void scatter_kept(const long long *__restrict__ ind,
-                             const double *__restrict__ vv, int64_t sN,
-                             const int *__restrict__ keep,
-                             const int64_t *__restrict__ pos,
-                             long long *__restrict__ oind, double *__restrict__ ov,
-                             bool ind16) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sN || !keep[k]) return;
-  store_triplet(oind, ov, pos[k], ind[2 * k], ind[2 * k + 1], vv[k], ind16);
+// The tile's triplets: indices (checked to lie in [0, N), so int), keep flag
+// (both ends interior) and lift flag (i interior, j boundary).  An index
+// outside [0, N) flags err and neither.
+struct TileItems {
+  int i[DIR_ITEMS], j[DIR_ITEMS];
+  bool keep[DIR_ITEMS], lift[DIR_ITEMS];
+  __device__ __forceinline__ void load(const long long *__restrict__ ind, int64_t sN, int N,
+                                       const int *__restrict__ bdidx, bool ind16,
+                                       int *__restrict__ err) {
+    const int64_t base = (int64_t)blockIdx.x * DIR_TILE + threadIdx.x;
+    long long a[DIR_ITEMS], b[DIR_ITEMS];
+#pragma unroll
+    for (int u = 0; u < DIR_ITEMS; u++) {
+      a[u] = 0;
+      b[u] = 0;
+      const int64_t k = base + (int64_t)u * DIR_NT;
+      if (k < sN) load_pair(ind, k, ind16, a[u], b[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < DIR_ITEMS; u++) {
+      keep[u] = false;
+      lift[u] = false;
+      i[u] = 0;
+      j[u] = 0;
+      const int64_t k = base + (int64_t)u * DIR_NT;
+      if (k >= sN) continue;
+      if (a[u] < 0 || a[u] >= N || b[u] < 0 || b[u] >= N) {  // never index with a bad id
+        *err = 1;
+        continue;
+      }
+      i[u] = (int)a[u];
+      j[u] = (int)b[u];
+      const bool bi = bdidx[i[u]] >= 0, bj = bdidx[j[u]] >= 0;
+      keep[u] = !bi && !bj;
+      lift[u] = !bi && bj;
+    }
+  }
+};
+
+// pass 1: the kept and lifted triplets of each tile
+__global__ void __launch_bounds__(DIR_NT)
+    tile_count(const long long *__restrict__ ind, int64_t sN, int N,
+               const int *__restrict__ bdidx, bool ind16, int *__restrict__ tkeep,
+               int *__restrict__ tlift, int *__restrict__ err) {
+  __shared__ int sk[DIR_WAVES], sl[DIR_WAVES];
+  TileItems T;
+  T.load(ind, sN, N, bdidx, ind16, err);
+  int nk = 0, nl = 0;
+#pragma unroll
+  for (int u = 0; u < DIR_ITEMS; u++) {
+    nk += __popcll(__ballot(T.keep[u]));
+    nl += __popcll(__ballot(T.lift[u]));
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sk[wave] = nk;
+    sl[wave] = nl;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0, b = 0;
+    for (int w = 0; w < DIR_WAVES; w++) {
+      a += sk[w];
+      b += sl[w];
+    }
+    tkeep[blockIdx.x] = a;
+    tlift[blockIdx.x] = b;
+  }
 }
 
-// lifted triplet list: key = row (forward) or column (backward), value = k
-__global__ void scatter_lift(const long long *__restrict__ ind, int64_t sN,
-                             const int *__restrict__ lift,
-                             const int64_t *__restrict__ lpos, int use_col,
-                             int *__restrict__ lkey, int64_t *__restrict__ lval) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sN || !lift[k]) return;
-  int64_t z = lpos[k];
-  lkey[z] = (int)ind[2 * k + use_col];
-  lval[z] = k;
+// Ranks of the tile's kept and lifted triplets in input order: rk[u] / rl[u]
+// = the tile's kept / lifted triplets before item u of this thread.
+struct TileRanks {
+  int rk[DIR_ITEMS], rl[DIR_ITEMS];
+  __device__ __forceinline__ void compute(const TileItems &T) {
+    __shared__ int wk[DIR_ITEMS][DIR_WAVES], wl[DIR_ITEMS][DIR_WAVES];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int u = 0; u < DIR_ITEMS; u++) {
+      const unsigned long long mk = __ballot(T.keep[u]), ml = __ballot(T.lift[u]);
+      rk[u] = __popcll(mk & below);
+      rl[u] = __popcll(ml & below);
+      if (lane == 0) {
+        wk[u][wave] = __popcll(mk);
+        wl[u][wave] = __popcll(ml);
+      }
+    }
+    __syncthreads();
+    int ak = 0, al = 0;  // kept / lifted before (u, wave) in input order
+#pragma unroll
+    for (int u = 0; u < DIR_ITEMS; u++) {
+      int bk = 0, bl = 0;
+      for (int w = 0; w < DIR_WAVES; w++) {
+        if (w == wave) {
+          bk = ak;
+          bl = al;
+        }
+        ak += wk[u][w];
+        al += wl[u][w];
+      }
+      rk[u] += bk;
+      rl[u] += bl;
+    }
+  }
+};
+
+// pass 2, forward: kept triplets to their output slots (tile offset + rank),
+// lifted triplets to the lift list as (row, triplet index)
+__global__ void __launch_bounds__(DIR_NT)
+    tile_place(const long long *__restrict__ ind, const double *__restrict__ vv, int64_t sN,
+               int N, const int *__restrict__ bdidx, bool ind16,
+               const int64_t *__restrict__ tkpos, const int64_t *__restrict__ tlpos,
+               long long *__restrict__ oind, double *__restrict__ ov, bool oind16,
+               int *__restrict__ lkey, int64_t *__restrict__ lval, int *__restrict__ err) {
+  TileItems T;
+  T.load(ind, sN, N, bdidx, ind16, err);
+  TileRanks R;
+  R.compute(T);
+  const int64_t base = (int64_t)blockIdx.x * DIR_TILE + threadIdx.x;
+  const int64_t k0 = tkpos[blockIdx.x], l0 = tlpos[blockIdx.x];
+#pragma unroll
+  for (int u = 0; u < DIR_ITEMS; u++) {
+    const int64_t k = base + (int64_t)u * DIR_NT;
+    if (T.keep[u]) store_triplet(oind, ov, k0 + R.rk[u], T.i[u], T.j[u], vv[k], oind16);
+    if (T.lift[u]) {
+      lkey[l0 + R.rl[u]] = T.i[u];
+      lval[l0 + R.rl[u]] = k;
+    }
+  }
 }
 
 __global__ void append_diag(const int *__restrict__ keys, int nu, int64_t base,
@@ -116,22 +234,34 @@ __global__ void set_bd_rhs(const int *__restrict__ keys, const int *__restrict__
   if (t < nu) orhs[keys[t]] = bdval[win[t]];
 }
 
-// backward: grad_vv, grad_rhs
-__global__ void grad_vv_rhs(const long long *__restrict__ ind, int64_t sN,
-                            const int *__restrict__ keep,
-                            const int64_t *__restrict__ pos,
-                            const int *__restrict__ bdidx,
-                            const double *__restrict__ bdval,
-                            const double *__restrict__ gov,
-                            const double *__restrict__ gorhs,
-                            double *__restrict__ gvv) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sN) return;
-  long long i = ind[2 * k], j = ind[2 * k + 1];
-  bool bi = bdidx[i] >= 0, bj = bdidx[j] >= 0;
-  double g = keep[k] ? gov[pos[k]] : 0.0;
-  if (!bi && bj) g -= bdval[bdidx[j]] * gorhs[i];
-  gvv[k] = g;
+// pass 2, backward: grad_vv (a kept triplet's is grad_ov at its output slot;
+// a lifted one's also gets -bdval[j] grad_orhs[i]) and the lift list as
+// (column, triplet index)
+__global__ void __launch_bounds__(DIR_NT)
+    tile_grad(const long long *__restrict__ ind, int64_t sN, int N,
+              const int *__restrict__ bdidx, bool ind16, const int64_t *__restrict__ tkpos,
+              const int64_t *__restrict__ tlpos, const double *__restrict__ bdval,
+              const double *__restrict__ gov, const double *__restrict__ gorhs,
+              double *__restrict__ gvv, int *__restrict__ lkey, int64_t *__restrict__ lval,
+              int *__restrict__ err) {
+  TileItems T;
+  T.load(ind, sN, N, bdidx, ind16, err);
+  TileRanks R;
+  R.compute(T);
+  const int64_t base = (int64_t)blockIdx.x * DIR_TILE + threadIdx.x;
+  const int64_t k0 = tkpos[blockIdx.x], l0 = tlpos[blockIdx.x];
+#pragma unroll
+  for (int u = 0; u < DIR_ITEMS; u++) {
+    const int64_t k = base + (int64_t)u * DIR_NT;
+    if (k >= sN) continue;
+    double g = T.keep[u] ? gov[k0 + R.rk[u]] : 0.0;
+    if (T.lift[u]) {
+      g -= bdval[bdidx[T.j[u]]] * gorhs[T.i[u]];
+      lkey[l0 + R.rl[u]] = T.j[u];
+      lval[l0 + R.rl[u]] = k;
+    }
+    gvv[k] = g;
+  }
 }
 
 __global__ void grad_rhs_k(const int *__restrict__ bdidx, const double *__restrict__ gorhs,
@@ -205,8 +335,8 @@ void bd_keys(const long long *bd_dev, int bdN, int N, hipStream_t st,
 // sizes.  Calls on one device are serialised by its lock.
 struct DirWork {
   std::mutex mu;
-  DevBuf<int> keys, win, bdidx, keep, lift, err, lkey, skey;
-  DevBuf<int64_t> pos, lpos, lval, sval, cnt;
+  DevBuf<int> keys, win, bdidx, tkeep, tlift, err, lkey, skey;
+  DevBuf<int64_t> tkpos, tlpos, lval, sval, cnt;
   DevBuf<char> tmp;
 };
 
@@ -228,7 +358,8 @@ DirWork &dir_work() {
 
 struct DirPrep {
   DirWork &w;
-  int nu = 0;
+  int nu = 0, ntiles = 0;
+  bool ind16 = false;
   int64_t nkept = 0, nlift = 0;
 };
 
@@ -257,39 +388,47 @@ void prepare(DirPrep &d, const long long *ind, int64_t sN, const long long *bd, 
   ensure(w.bdidx, N);
   fill_int<<<nblk(N, 256), 256, 0, st>>>(w.bdidx.p, N, -1);
   if (d.nu) set_bdidx<<<nblk(d.nu, 256), 256, 0, st>>>(w.keys.p, w.win.p, d.nu, w.bdidx.p);
-  ensure(w.keep, sN + 1);
-  ensure(w.lift, sN + 1);
-  ensure(w.pos, sN + 1);
-  ensure(w.lpos, sN + 1);
+  d.ntiles = (int)((sN + DIR_TILE - 1) / DIR_TILE);
+  d.ind16 = ((uintptr_t)ind & 15) == 0;
+  const int nt = d.ntiles;
+  ensure(w.tkeep, nt + 1);
+  ensure(w.tlift, nt + 1);
+  ensure(w.tkpos, nt + 1);
+  ensure(w.tlpos, nt + 1);
   ensure(w.err, 1);
   set_i<<<1, 1, 0, st>>>(w.err.p, 0);
-  set_i<<<1, 1, 0, st>>>(w.keep.p + sN, 0);  // the scans' last entry: their totals
-  set_i<<<1, 1, 0, st>>>(w.lift.p + sN, 0);
-  if (sN)
-    classify<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, N, w.bdidx.p, w.keep.p, w.lift.p, w.err.p);
+  set_i<<<1, 1, 0, st>>>(w.tkeep.p + nt, 0);  // the scans' last entry: their totals
+  set_i<<<1, 1, 0, st>>>(w.tlift.p + nt, 0);
+  if (nt)
+    tile_count<<<nt, DIR_NT, 0, st>>>(ind, sN, N, w.bdidx.p, d.ind16, w.tkeep.p, w.tlift.p,
+                                      w.err.p);
   HIPCHECK(hipGetLastError());
-  exclusive_scan(w, w.keep.p, w.pos.p, sN + 1, st);
-  exclusive_scan(w, w.lift.p, w.lpos.p, sN + 1, st);
+  exclusive_scan(w, w.tkeep.p, w.tkpos.p, nt + 1, st);
+  exclusive_scan(w, w.tlift.p, w.tlpos.p, nt + 1, st);
   int herr = 0;
   HIPCHECK(hipMemcpyAsync(&herr, w.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHECK(hipMemcpyAsync(&d.nkept, w.pos.p + sN, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  HIPCHECK(hipMemcpyAsync(&d.nlift, w.lpos.p + sN, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(&d.nkept, w.tkpos.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(&d.nlift, w.tlpos.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   REQUIRE(!herr, ADMFEM_ERR_ARG, "triplet index out of range [0, N)");
 }
 
-// stably sorted (key, triplet) list of the lifted triplets, in w.skey / w.sval
-void sorted_lift(const DirPrep &d, const long long *ind, int64_t sN, int use_col,
-                 hipStream_t st) {
+// room for the lift list (written by tile_place / tile_grad into w.lkey /
+// w.lval in input order)
+void lift_room(const DirPrep &d) {
   DirWork &w = d.w;
   const int64_t nl = d.nlift;
   ensure(w.lkey, std::max<int64_t>(nl, 1));
   ensure(w.lval, std::max<int64_t>(nl, 1));
   ensure(w.skey, std::max<int64_t>(nl, 1));
   ensure(w.sval, std::max<int64_t>(nl, 1));
+}
+
+// the lift list stably sorted by key, into w.skey / w.sval
+void sort_lift(const DirPrep &d, hipStream_t st) {
+  DirWork &w = d.w;
+  const int64_t nl = d.nlift;
   if (nl == 0) return;
-  scatter_lift<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, w.lift.p, w.lpos.p, use_col, w.lkey.p,
-                                              w.lval.p);
   size_t tmp = 0;
   HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, w.lkey.p, w.skey.p, w.lval.p,
                                               w.sval.p, (int)nl, 0, 32, st));
@@ -309,12 +448,14 @@ int64_t run_impose_dirichlet(const long long *ind, const double *vv, int64_t sN,
   std::lock_guard<std::mutex> g(w.mu);
   DirPrep d{w};
   prepare(d, ind, sN, bd, bdN, N, st);
-  if (sN)
-    scatter_kept<<<nblk(sN, 256), 256, 0, st>>>(ind, vv, sN, w.keep.p, w.pos.p, oind, ov,
-                                                ((uintptr_t)oind & 15) == 0);
+  lift_room(d);
+  if (d.ntiles)
+    tile_place<<<d.ntiles, DIR_NT, 0, st>>>(ind, vv, sN, N, w.bdidx.p, d.ind16, w.tkpos.p,
+                                            w.tlpos.p, oind, ov, ((uintptr_t)oind & 15) == 0,
+                                            w.lkey.p, w.lval.p, w.err.p);
   if (d.nu) append_diag<<<nblk(d.nu, 256), 256, 0, st>>>(w.keys.p, d.nu, d.nkept, oind, ov);
   copy_rhs<<<nblk(N, 256), 256, 0, st>>>(rhs, orhs, N);
-  sorted_lift(d, ind, sN, 0, st);
+  sort_lift(d, st);
   if (d.nlift)
     lift_rows<<<nblk(d.nlift, 256), 256, 0, st>>>(w.skey.p, w.sval.p, d.nlift, ind, vv,
                                                   w.bdidx.p, bdval, orhs);
@@ -333,12 +474,14 @@ void run_impose_dirichlet_grad(const double *gov, const double *gorhs,
   std::lock_guard<std::mutex> g(w.mu);
   DirPrep d{w};
   prepare(d, ind, sN, bd, bdN, N, st);
-  if (sN)
-    grad_vv_rhs<<<nblk(sN, 256), 256, 0, st>>>(ind, sN, w.keep.p, w.pos.p, w.bdidx.p, bdval,
-                                               gov, gorhs, gvv);
+  lift_room(d);
+  if (d.ntiles)
+    tile_grad<<<d.ntiles, DIR_NT, 0, st>>>(ind, sN, N, w.bdidx.p, d.ind16, w.tkpos.p,
+                                           w.tlpos.p, bdval, gov, gorhs, gvv, w.lkey.p,
+                                           w.lval.p, w.err.p);
   grad_rhs_k<<<nblk(N, 256), 256, 0, st>>>(w.bdidx.p, gorhs, grhs, N);
   if (bdN) zero_d<<<nblk(bdN, 256), 256, 0, st>>>(gbd, bdN);
-  sorted_lift(d, ind, sN, 1, st);
+  sort_lift(d, st);
   if (d.nlift)
     grad_bdval_k<<<nblk(d.nlift, 256), 256, 0, st>>>(w.skey.p, w.sval.p, d.nlift, ind, vv,
                                                      w.bdidx.p, gorhs, gbd);
