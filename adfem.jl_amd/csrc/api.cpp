@@ -292,8 +292,8 @@ int admfem_impose_dirichlet(const long long *indices, const double *vv, int64_t 
                             const double *rhs, int N, long long *oindices, double *ov,
                             double *orhs, int64_t *out_count, void *stream) {
   return guarded([&] {
-    REQUIRE(out_count && orhs && rhs && (sN == 0 || (indices && vv && oindices && ov)) &&
-                (bdN == 0 || (bd && bdval)),
+    REQUIRE(out_count && orhs && rhs && (sN == 0 || (indices && vv)) &&
+                (sN + bdN == 0 || (oindices && ov)) && (bdN == 0 || (bd && bdval)),
             ADMFEM_ERR_ARG, "null pointer");
     *out_count = run_impose_dirichlet(indices, vv, sN, bd, bdval, bdN, rhs, N, oindices, ov,
                                       orhs, S(stream));

@@ -213,6 +213,98 @@ def test_impose_dirichlet_coo_matches_oracle(dup):
     np.testing.assert_array_equal(host(gb), ogb)
 
 
+def _dirichlet_dev(I, V, bd, bdval, rhs, N, shift_in=False, shift_out=False):
+    """The op through the C-ABI; shift_in / shift_out place the index arrays
+    8 bytes off a 16-byte boundary (the kernels then take 8-byte loads and
+    stores instead of 16-byte ones)."""
+    L = _lib.lib()
+    sN = V.size
+    cap = sN + bd.size
+
+    def ind_buf(n, src=None):
+        flat = torch.zeros(2 * max(n, 1) + 2, dtype=torch.int64, device="cuda")
+        off = 1 if (shift_in if src is not None else shift_out) else 2
+        view = flat[off:off + 2 * n].view(n, 2) if n else flat[off:off]
+        if src is not None and n:
+            view.copy_(torch.as_tensor(src, dtype=torch.int64))
+        return flat, view
+
+    in_keep, dI = ind_buf(sN, I)
+    out_keep, goi = ind_buf(cap)
+    gov = torch.zeros(max(cap, 1), dtype=torch.float64, device="cuda")
+    gorhs = torch.zeros(N, dtype=torch.float64, device="cuda")
+    cnt = np.zeros(1, dtype=np.int64)
+    dV, dbd, dbv, dr = dev(V), dev(bd, torch.int64), dev(bdval), dev(rhs)
+    assert (dI.data_ptr() % 16 == 8) == (shift_in and sN > 0)
+    _lib.check(L.admfem_impose_dirichlet(ctypes.c_void_p(dI.data_ptr()), _lib.ptr(dV), sN,
+                                         _lib.ptr(dbd), _lib.ptr(dbv), bd.size, _lib.ptr(dr), N,
+                                         ctypes.c_void_p(goi.data_ptr()), _lib.ptr(gov),
+                                         _lib.ptr(gorhs), _lib.ptr(cnt), _lib.stream_ptr()))
+    n = int(cnt[0])
+    out = (host(goi.reshape(-1))[:2 * n].reshape(n, 2), host(gov)[:n], host(gorhs))
+    del in_keep, out_keep
+    return out, (dI, dV, dbd, dbv)
+
+
+@pytest.mark.parametrize("shift_in,shift_out", [(False, False), (True, False), (False, True)])
+def test_impose_dirichlet_many_tiles_shuffled(shift_in, shift_out):
+    """Triplets in random order over many 2,048-triplet tiles (kept, lifted
+    and dropped ones interleaved, the last tile partial), extra interior
+    boundary nodes with repeats, and index arrays off 16-byte alignment:
+    output, rhs and the gradient op bitwise against the oracle."""
+    gm, om = pair(40, seed=11)
+    rng = np.random.default_rng(12)
+    I, V = om.laplace_coo(rng.random(om.ngauss) + 0.2)
+    perm = rng.permutation(V.size)
+    I, V = np.ascontiguousarray(I[perm]), np.ascontiguousarray(V[perm])
+    extra = rng.choice(om.nnode, size=60, replace=False) + 1
+    bd = np.concatenate([om.bcnode(), extra, extra[:7]])
+    bdval = rng.standard_normal(bd.size)
+    rhs = rng.standard_normal(om.nnode)
+    N, sN = om.nnode, V.size
+    assert sN > 20 * 2048 and sN % 2048
+    (goi, gov, gorhs), (dI, dV, dbd, dbv) = _dirichlet_dev(I, V, bd, bdval, rhs, N,
+                                                           shift_in, shift_out)
+    oi, ov, orhs = O.dirichlet(I, V, bd, rhs, bdval)
+    np.testing.assert_array_equal(goi, oi)
+    np.testing.assert_array_equal(gov, ov)
+    np.testing.assert_array_equal(gorhs, orhs)
+    n = oi.shape[0]
+    L = _lib.lib()
+    g_ov, g_or = rng.standard_normal(n), rng.standard_normal(N)
+    gv = torch.zeros(sN, dtype=torch.float64, device="cuda")
+    gr = torch.zeros(N, dtype=torch.float64, device="cuda")
+    gb = torch.zeros(bd.size, dtype=torch.float64, device="cuda")
+    dgov, dgor = dev(g_ov), dev(g_or)
+    _lib.check(L.admfem_impose_dirichlet_grad(_lib.ptr(dgov), _lib.ptr(dgor),
+                                              ctypes.c_void_p(dI.data_ptr()), _lib.ptr(dV), sN,
+                                              _lib.ptr(dbd), _lib.ptr(dbv), bd.size, N,
+                                              _lib.ptr(gv), _lib.ptr(gr), _lib.ptr(gb),
+                                              _lib.stream_ptr()))
+    ogv, ogr, ogb = O.dirichlet_backward(g_ov, g_or, I, V, bd, bdval, N)
+    np.testing.assert_array_equal(host(gv), ogv)
+    np.testing.assert_array_equal(host(gr), ogr)
+    np.testing.assert_array_equal(host(gb), ogb)
+
+
+def test_impose_dirichlet_no_triplets():
+    """An empty triplet list: only the boundary diagonals, rhs = bdval on
+    the boundary and the input elsewhere."""
+    rng = np.random.default_rng(3)
+    N = 9
+    bd = np.array([2, 5, 5, 9])
+    bdval = rng.standard_normal(bd.size)
+    rhs = rng.standard_normal(N)
+    I = np.zeros((0, 2), dtype=np.int64)
+    V = np.zeros(0)
+    (goi, gov, gorhs), _ = _dirichlet_dev(I, V, bd, bdval, rhs, N)
+    oi, ov, orhs = O.dirichlet(I, V, bd, rhs, bdval)
+    np.testing.assert_array_equal(goi, oi)
+    np.testing.assert_array_equal(gov, ov)
+    np.testing.assert_array_equal(gorhs, orhs)
+    assert goi.shape[0] == 3
+
+
 def test_impose_dirichlet_rejects_out_of_range_index():
     L = _lib.lib()
     I = dev(np.array([[0, 0], [0, 5]]), torch.int64)
