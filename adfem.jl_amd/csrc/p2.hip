@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(P2_NT)
 // The two backward ops (FemLaplaceScalar.h:28-54, ComputeFemMassMatrixMfem.h:
 // 31-56): one thread per Gauss point, sum over its 36 triplets in order.
 template <bool MASS>
-__global__ void p2_coo_grad_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
+__global__ void __launch_bounds__(256) p2_coo_grad_kernel(int nelem, QuadRule rule, const int *__restrict__ conn,
                                    const double *__restrict__ x, const double *__restrict__ y,
                                    const double *__restrict__ gw,
                                    const double *__restrict__ gvv, double *__restrict__ out) {
@@ -101,6 +101,19 @@ __global__ void p2_coo_grad_kernel(int nelem, QuadRule rule, const int *__restri
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)rule.nq * nelem) return;
   const int e = (int)(g / rule.nq), k = (int)(g - (int64_t)rule.nq * e);
+  // the point's 36 incoming gradients, all requested before the sum (as
+  // 16-byte loads: 36 g doubles is 16-byte aligned)
+  double t[36];
+  const double2 *src = reinterpret_cast<const double2 *>(gvv + 36 * g);
+#pragma unroll
+  for (int i = 0; i < 18; i++) {
+    const double2 v2 = src[i];
+    t[2 * i] = v2.x;
+    t[2 * i + 1] = v2.y;
+  }
+  // keep the loads above: left to itself the scheduler moves them down to
+  // their uses, a few at a time, each group waiting for the last
+  __builtin_amdgcn_sched_barrier(0);
   const ElemGeom G = p2_geom(conn, nelem, e, x, y);
   const double w = gw[g];
   double a[6], b[6];
@@ -110,10 +123,12 @@ __global__ void p2_coo_grad_kernel(int nelem, QuadRule rule, const int *__restri
     p2_grad(rule.xi[k], rule.eta[k], G, a, b);
   }
   double v = 0.0;
+#pragma unroll
   for (int p = 0; p < 6; p++)
+#pragma unroll
     for (int q = 0; q < 6; q++) {
       const double n = MASS ? a[p] * a[q] * w : (a[p] * a[q] + b[p] * b[q]) * w;
-      v += gvv[36 * g + 6 * p + q] * n;
+      v += t[6 * p + q] * n;
     }
   out[g] = v;
 }
