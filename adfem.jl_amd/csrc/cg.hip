@@ -51,6 +51,7 @@
 
 #include "batched.h"
 #include "geom.h"
+#include "sell.h"
 
 namespace admfem {
 
@@ -184,42 +185,6 @@ __device__ __forceinline__ void sum_partials(const double *part, int G, double (
   }
 }
 
-// One SELL row, FP64 arithmetic: sum_k v_k * f(col_k).  The first 8 entries
-// are unrolled so every column/value load issues before the dependent
-// gathers; wider rows (high-valence nodes) finish in a loop.
-template <typename VT, class F>
-__device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
-                                           const int *__restrict__ sc,
-                                           const VT *__restrict__ sv, F f) {
-  const int sl = i >> 6;
-  const int s0 = sp[sl];
-  const int off = s0 + (i & 63);
-  const int wfull = (sp[sl + 1] - s0) >> 6;
-  const int w = min(wfull, 8);
-  int c[8];
-  VT v[8];
-  // FP64 values: the matrix is streamed once per product, so its loads are
-  // non-temporal (measured on the FP64 SpMV: 99 -> 82 us at 2048^2; FP32
-  // values lose with them, 49 -> 56 us, and keep ordinary loads)
-#pragma unroll
-  for (int k = 0; k < 8; k++)
-    if (k < w) {
-      if constexpr (sizeof(VT) == 8) {
-        c[k] = __builtin_nontemporal_load(sc + off + 64 * k);
-        v[k] = __builtin_nontemporal_load(sv + off + 64 * k);
-      } else {
-        c[k] = sc[off + 64 * k];
-        v[k] = sv[off + 64 * k];
-      }
-    }
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < 8; k++)
-    if (k < w) s += (double)v[k] * f(c[k]);
-  for (int k = 8; k < wfull; k++) s += (double)sv[off + 64 * k] * f(sc[off + 64 * k]);
-  return s;
-}
-
 // ---- FP64 Jacobi-PCG
 
 // x0 = b on identity (boundary) rows, 0 elsewhere; r0 = b - A x0 (0 on those
@@ -301,10 +266,13 @@ __global__ void __launch_bounds__(NT)
     const double pni = dir64(z[i], w[i], pi, alpha, beta, zi);
     x[i] = fma(alpha, pi, x[i]);
     const double di = diag[i];
-    const double q = di * pni + sell_dot<double>(i, sp, sc, sv, [&](int c) {
-                       double unused;
-                       return dir64(z[c], w[c], p[c], alpha, beta, unused);
-                     });
+    const double q = di * pni + sell_dot<double>(
+                                    i, sp, sc, sv,
+                                    [&](int c) { return double3{z[c], w[c], p[c]}; },
+                                    [&](double3 v) {
+                                      double unused;
+                                      return dir64(v.x, v.y, v.z, alpha, beta, unused);
+                                    });
     const double wi = di != 0.0 ? q / di : 0.0;
     (odd ? z0 : z1)[i] = zi;
     (odd ? p0 : p1)[i] = pni;
@@ -439,9 +407,10 @@ __global__ void __launch_bounds__(NT)
     double ri;
     const double pni = dir32(r[i], qi, pi, alpha, beta, ri);
     ys[i] = (float)fma(alpha, (double)pi, (double)ys[i]);
-    const double a = fma(beta, (double)qi, ri + sell_dot<float>(i, sp, sc, sv, [&](int c) {
-                                             return fma(-alpha, (double)q[c], (double)r[c]);
-                                           }));
+    const double a = fma(beta, (double)qi,
+                         ri + sell_dot<float>(
+                                  i, sp, sc, sv, [&](int c) { return make_float2(r[c], q[c]); },
+                                  [&](float2 v) { return fma(-alpha, (double)v.y, (double)v.x); }));
     const float rv = (float)ri, pv = (float)pni, qv = (float)a;
     (odd ? r0 : r1)[i] = rv;
     (odd ? p0 : p1)[i] = pv;

@@ -1,0 +1,133 @@
+// SELL-64 row products shared by the solvers' step kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace admfem {
+
+// Row i's product with its SELL row: sum_k sv_k comb(load(c_k)) in k order.
+// A wave's 64 rows are one slice, so the row width is wave-uniform and each
+// width up to 8 gets straight-line code: every column and value load, then
+// every neighbour's gathered values (load), is issued before the sums.
+// (With the loads inside per-k branches the compiler waited for each
+// neighbour's values before requesting the next one's.)
+template <int W, typename VT, class L, class F>
+__device__ __forceinline__ double sell_dot_w(int off, const int *__restrict__ sc,
+                                             const VT *__restrict__ sv, L load, F comb) {
+  int c[W];
+  VT v[W];
+  // FP64 values: the matrix is streamed once per product, so its loads are
+  // non-temporal (measured on the FP64 SpMV: 99 -> 82 us at 2048^2; FP32
+  // values lose with them, 49 -> 56 us, and keep ordinary loads)
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    if constexpr (sizeof(VT) == 8) {
+      c[k] = __builtin_nontemporal_load(sc + off + 64 * k);
+      v[k] = __builtin_nontemporal_load(sv + off + 64 * k);
+    } else {
+      c[k] = sc[off + 64 * k];
+      v[k] = sv[off + 64 * k];
+    }
+  }
+  decltype(load(0)) g[W];
+#pragma unroll
+  for (int k = 0; k < W; k++) g[k] = load(c[k]);
+  __builtin_amdgcn_sched_barrier(0);  // the gathers stay ahead of the sums
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < W; k++) s += (double)v[k] * comb(g[k]);
+  return s;
+}
+
+template <typename VT, class L, class F>
+__device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
+                                           const int *__restrict__ sc,
+                                           const VT *__restrict__ sv, L load, F comb) {
+  const int sl = i >> 6;
+  const int s0 = sp[sl];
+  const int off = s0 + (i & 63);
+  const int wfull = (sp[sl + 1] - s0) >> 6;
+  double s = 0.0;
+  switch (__builtin_amdgcn_readfirstlane(min(wfull, 8))) {  // wave-uniform
+    case 1: s = sell_dot_w<1>(off, sc, sv, load, comb); break;
+    case 2: s = sell_dot_w<2>(off, sc, sv, load, comb); break;
+    case 3: s = sell_dot_w<3>(off, sc, sv, load, comb); break;
+    case 4: s = sell_dot_w<4>(off, sc, sv, load, comb); break;
+    case 5: s = sell_dot_w<5>(off, sc, sv, load, comb); break;
+    case 6: s = sell_dot_w<6>(off, sc, sv, load, comb); break;
+    case 7: s = sell_dot_w<7>(off, sc, sv, load, comb); break;
+    case 8: s = sell_dot_w<8>(off, sc, sv, load, comb); break;
+    default: break;
+  }
+  for (int k = 8; k < wfull; k++)
+    s += (double)sv[off + 64 * k] * comb(load(sc[off + 64 * k]));
+  return s;
+}
+
+// the same with f(c) both gathering and combining
+template <typename VT, class F>
+__device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
+                                           const int *__restrict__ sc,
+                                           const VT *__restrict__ sv, F f) {
+  return sell_dot<VT>(i, sp, sc, sv, f, [](double x) { return x; });
+}
+
+// Row i's products with the FP64 and the FP32 matrix of the same pattern in
+// one pass over its columns (each sum in the order sell_dot takes):
+// s64 = sum_k sv64_k f(c_k), s32 = sum_k sv32_k g(c_k).
+template <int W, class F, class G>
+__device__ __forceinline__ void sell_dot2_w(int off, const int *__restrict__ sc,
+                                            const double *__restrict__ sv64,
+                                            const float *__restrict__ sv32, F f, G g,
+                                            double &s64, double &s32) {
+  int c[W];
+  double v[W], a[W], b[W];
+  float u[W];
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    c[k] = sc[off + 64 * k];
+    v[k] = __builtin_nontemporal_load(sv64 + off + 64 * k);
+    u[k] = sv32[off + 64 * k];
+  }
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    a[k] = f(c[k]);
+    b[k] = g(c[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    s64 += v[k] * a[k];
+    s32 += (double)u[k] * b[k];
+  }
+}
+
+template <class F, class G>
+__device__ __forceinline__ void sell_dot2(int i, const int *__restrict__ sp,
+                                          const int *__restrict__ sc,
+                                          const double *__restrict__ sv64,
+                                          const float *__restrict__ sv32, F f, G g,
+                                          double &s64, double &s32) {
+  const int sl = i >> 6;
+  const int s0 = sp[sl];
+  const int off = s0 + (i & 63);
+  const int wfull = (sp[sl + 1] - s0) >> 6;
+  s64 = 0.0;
+  s32 = 0.0;
+  switch (__builtin_amdgcn_readfirstlane(min(wfull, 8))) {  // wave-uniform
+    case 1: sell_dot2_w<1>(off, sc, sv64, sv32, f, g, s64, s32); break;
+    case 2: sell_dot2_w<2>(off, sc, sv64, sv32, f, g, s64, s32); break;
+    case 3: sell_dot2_w<3>(off, sc, sv64, sv32, f, g, s64, s32); break;
+    case 4: sell_dot2_w<4>(off, sc, sv64, sv32, f, g, s64, s32); break;
+    case 5: sell_dot2_w<5>(off, sc, sv64, sv32, f, g, s64, s32); break;
+    case 6: sell_dot2_w<6>(off, sc, sv64, sv32, f, g, s64, s32); break;
+    case 7: sell_dot2_w<7>(off, sc, sv64, sv32, f, g, s64, s32); break;
+    case 8: sell_dot2_w<8>(off, sc, sv64, sv32, f, g, s64, s32); break;
+    default: break;
+  }
+  for (int k = 8; k < wfull; k++) {
+    const int ck = sc[off + 64 * k];
+    s64 += sv64[off + 64 * k] * f(ck);
+    s32 += (double)sv32[off + 64 * k] * g(ck);
+  }
+}
+
+}  // namespace admfem
