@@ -200,14 +200,14 @@ __global__ void __launch_bounds__(NT)
   const int i = m.y * NT + threadIdx.x;
   const double alpha = s.alpha, beta = s.beta;
   const int cur = s.it & 1;
-  const float *__restrict__ r = d.r[cur];
-  const float *__restrict__ p = d.p[cur];
-  const float *__restrict__ q = d.q[cur];
+  const gptr<float> r = global_ptr(d.r[cur]);
+  const gptr<float> p = global_ptr(d.p[cur]);
+  const gptr<float> q = global_ptr(d.q[cur]);
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < d.npad) {
     const float pi = p[i], qi = q[i];
     const Step own = step_of(r[i], qi, pi, alpha, beta);
-    float *__restrict__ ys = d.ys[s.sel];
+    const gmptr<float> ys = global_mut(d.ys[s.sel]);
     ys[i] = (float)fma(alpha, (double)pi, (double)ys[i]);
     // q' = Â p' = Â r' + beta Â p = Â r' + beta q: the neighbours contribute
     // their r' = r - alpha q (two gathered values each, not three)
@@ -217,9 +217,9 @@ __global__ void __launch_bounds__(NT)
                                      [&](int c) { return make_float2(r[c], q[c]); },
                                      [&](float2 v) { return fma(-alpha, (double)v.y, (double)v.x); }));
     const float rv = (float)own.r, pv = (float)own.p, qv = (float)a;
-    d.r[cur ^ 1][i] = rv;
-    d.p[cur ^ 1][i] = pv;
-    d.q[cur ^ 1][i] = qv;
+    global_mut(d.r[cur ^ 1])[i] = rv;
+    global_mut(d.p[cur ^ 1])[i] = pv;
+    global_mut(d.q[cur ^ 1])[i] = qv;
     v[0] = (double)rv * (double)rv;
     v[1] = (double)pv * (double)qv;
     v[2] = (double)rv * (double)qv;

@@ -4,6 +4,22 @@
 
 namespace admfem {
 
+// A pointer known to address global memory: loads through it compile to
+// global (not flat) memory instructions.  For pointers read from memory
+// (descriptor tables), where the compiler cannot infer the address space.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> global_ptr(const T *p) {
+  return (gptr<T>)p;
+}
+template <class T>
+using gmptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gmptr<T> global_mut(T *p) {
+  return (gmptr<T>)p;
+}
+
 // Row i's product with its SELL row: sum_k sv_k comb(load(c_k)) in k order.
 // A wave's 64 rows are one slice, so the row width is wave-uniform and each
 // width up to 8 gets straight-line code: every column and value load, then
@@ -15,17 +31,19 @@ __device__ __forceinline__ double sell_dot_w(int off, const int *__restrict__ sc
                                              const VT *__restrict__ sv, L load, F comb) {
   int c[W];
   VT v[W];
+  const gptr<int> gc = global_ptr(sc);
+  const gptr<VT> gv = global_ptr(sv);
   // FP64 values: the matrix is streamed once per product, so its loads are
   // non-temporal (measured on the FP64 SpMV: 99 -> 82 us at 2048^2; FP32
   // values lose with them, 49 -> 56 us, and keep ordinary loads)
 #pragma unroll
   for (int k = 0; k < W; k++) {
     if constexpr (sizeof(VT) == 8) {
-      c[k] = __builtin_nontemporal_load(sc + off + 64 * k);
-      v[k] = __builtin_nontemporal_load(sv + off + 64 * k);
+      c[k] = __builtin_nontemporal_load(gc + off + 64 * k);
+      v[k] = __builtin_nontemporal_load(gv + off + 64 * k);
     } else {
-      c[k] = sc[off + 64 * k];
-      v[k] = sv[off + 64 * k];
+      c[k] = gc[off + 64 * k];
+      v[k] = gv[off + 64 * k];
     }
   }
   decltype(load(0)) g[W];
@@ -43,9 +61,9 @@ __device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
                                            const int *__restrict__ sc,
                                            const VT *__restrict__ sv, L load, F comb) {
   const int sl = i >> 6;
-  const int s0 = sp[sl];
+  const int s0 = global_ptr(sp)[sl];
   const int off = s0 + (i & 63);
-  const int wfull = (sp[sl + 1] - s0) >> 6;
+  const int wfull = (global_ptr(sp)[sl + 1] - s0) >> 6;
   double s = 0.0;
   switch (__builtin_amdgcn_readfirstlane(min(wfull, 8))) {  // wave-uniform
     case 1: s = sell_dot_w<1>(off, sc, sv, load, comb); break;
