@@ -286,26 +286,31 @@ __global__ void __launch_bounds__(NT)
   if (!replace_due(s)) return;
   const BSys &d = D[m.x];
   const int i = m.y * NT + threadIdx.x;
-  const double *__restrict__ x = d.x[s.sel];
-  const float *__restrict__ ys = d.ys[s.sel];
-  const double *__restrict__ dsq = d.dsq;
+  const gptr<double> x = global_ptr(d.x[s.sel]);
+  const gptr<float> ys = global_ptr(d.ys[s.sel]);
+  const gptr<double> dsq = global_ptr(d.dsq);
   double v[2] = {0.0, 0.0};
   if (i < d.npad) {
     const double xi = x[i] + dsq[i] * (double)ys[i];
     // the true residual's product (FP64) and q = Â p afresh for the kept
     // direction (FP32 matrix; the step updates q by recurrence), one pass
-    const float *__restrict__ p = d.p[s.it & 1];
+    const gptr<float> p = global_ptr(d.p[s.it & 1]);
+    struct XDY {
+      double x, d;
+      float y;
+    };
     double sx, sq;
     sell_dot2(
-        i, d.sp, d.sc, d.sv64, d.sv32, [&](int c) { return x[c] + dsq[c] * (double)ys[c]; },
-        [&](int c) { return (double)p[c]; }, sx, sq);
-    const double ax = d.diag64[i] * xi + sx;
-    d.x[s.sel ^ 1][i] = xi;
-    d.ys[s.sel ^ 1][i] = 0.0f;
-    const double ri = (i < d.n ? d.b[i] : 0.0) - ax;
+        i, d.sp, d.sc, d.sv64, d.sv32, [&](int c) { return XDY{x[c], dsq[c], ys[c]}; },
+        [](XDY t) { return t.x + t.d * (double)t.y; }, [&](int c) { return p[c]; },
+        [](float t) { return (double)t; }, sx, sq);
+    const double ax = global_ptr(d.diag64)[i] * xi + sx;
+    global_mut(d.x[s.sel ^ 1])[i] = xi;
+    global_mut(d.ys[s.sel ^ 1])[i] = 0.0f;
+    const double ri = (i < d.n ? global_ptr(d.b)[i] : 0.0) - ax;
     const float rf = (float)(ri * dsq[i]);
-    d.r[s.it & 1][i] = rf;
-    d.q[s.it & 1][i] = (float)((double)p[i] + sq);
+    global_mut(d.r[s.it & 1])[i] = rf;
+    global_mut(d.q[s.it & 1])[i] = (float)((double)p[i] + sq);
     v[0] = (double)rf * (double)rf;
     v[1] = ri * ri;
   }

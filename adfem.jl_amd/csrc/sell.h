@@ -91,60 +91,67 @@ __device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
 
 // Row i's products with the FP64 and the FP32 matrix of the same pattern in
 // one pass over its columns (each sum in the order sell_dot takes):
-// s64 = sum_k sv64_k f(c_k), s32 = sum_k sv32_k g(c_k).
-template <int W, class F, class G>
+// s64 = sum_k sv64_k cf(lf(c_k)), s32 = sum_k sv32_k cg(lg(c_k)), every
+// gather issued before the sums as in sell_dot.
+template <int W, class LF, class CF, class LG, class CG>
 __device__ __forceinline__ void sell_dot2_w(int off, const int *__restrict__ sc,
                                             const double *__restrict__ sv64,
-                                            const float *__restrict__ sv32, F f, G g,
-                                            double &s64, double &s32) {
+                                            const float *__restrict__ sv32, LF lf, CF cf, LG lg,
+                                            CG cg, double &s64, double &s32) {
   int c[W];
-  double v[W], a[W], b[W];
+  double v[W];
   float u[W];
+  const gptr<int> gc = global_ptr(sc);
+  const gptr<double> g64 = global_ptr(sv64);
+  const gptr<float> g32 = global_ptr(sv32);
 #pragma unroll
   for (int k = 0; k < W; k++) {
-    c[k] = sc[off + 64 * k];
-    v[k] = __builtin_nontemporal_load(sv64 + off + 64 * k);
-    u[k] = sv32[off + 64 * k];
+    c[k] = gc[off + 64 * k];
+    v[k] = __builtin_nontemporal_load(g64 + off + 64 * k);
+    u[k] = g32[off + 64 * k];
   }
+  decltype(lf(0)) a[W];
+  decltype(lg(0)) b[W];
 #pragma unroll
   for (int k = 0; k < W; k++) {
-    a[k] = f(c[k]);
-    b[k] = g(c[k]);
+    a[k] = lf(c[k]);
+    b[k] = lg(c[k]);
   }
+  __builtin_amdgcn_sched_barrier(0);  // the gathers stay ahead of the sums
 #pragma unroll
   for (int k = 0; k < W; k++) {
-    s64 += v[k] * a[k];
-    s32 += (double)u[k] * b[k];
+    s64 += v[k] * cf(a[k]);
+    s32 += (double)u[k] * cg(b[k]);
   }
 }
 
-template <class F, class G>
+template <class LF, class CF, class LG, class CG>
 __device__ __forceinline__ void sell_dot2(int i, const int *__restrict__ sp,
                                           const int *__restrict__ sc,
                                           const double *__restrict__ sv64,
-                                          const float *__restrict__ sv32, F f, G g,
-                                          double &s64, double &s32) {
+                                          const float *__restrict__ sv32, LF lf, CF cf, LG lg,
+                                          CG cg, double &s64, double &s32) {
   const int sl = i >> 6;
-  const int s0 = sp[sl];
+  const int s0 = global_ptr(sp)[sl];
   const int off = s0 + (i & 63);
-  const int wfull = (sp[sl + 1] - s0) >> 6;
+  const int wfull = (global_ptr(sp)[sl + 1] - s0) >> 6;
   s64 = 0.0;
   s32 = 0.0;
   switch (__builtin_amdgcn_readfirstlane(min(wfull, 8))) {  // wave-uniform
-    case 1: sell_dot2_w<1>(off, sc, sv64, sv32, f, g, s64, s32); break;
-    case 2: sell_dot2_w<2>(off, sc, sv64, sv32, f, g, s64, s32); break;
-    case 3: sell_dot2_w<3>(off, sc, sv64, sv32, f, g, s64, s32); break;
-    case 4: sell_dot2_w<4>(off, sc, sv64, sv32, f, g, s64, s32); break;
-    case 5: sell_dot2_w<5>(off, sc, sv64, sv32, f, g, s64, s32); break;
-    case 6: sell_dot2_w<6>(off, sc, sv64, sv32, f, g, s64, s32); break;
-    case 7: sell_dot2_w<7>(off, sc, sv64, sv32, f, g, s64, s32); break;
-    case 8: sell_dot2_w<8>(off, sc, sv64, sv32, f, g, s64, s32); break;
+    case 1: sell_dot2_w<1>(off, sc, sv64, sv32, lf, cf, lg, cg, s64, s32); break;
+    case 2: sell_dot2_w<2>(off, sc, sv64, sv32, lf, cf, lg, cg, s64, s32); break;
+    case 3: sell_dot2_w<3>(off, sc, sv64, sv32, lf, cf, lg, cg, s64, s32); break;
+    case 4: sell_dot2_w<4>(off, sc, sv64, sv32, lf, cf, lg, cg, s64, s32); break;
+    case 5: sell_dot2_w<5>(off, sc, sv64, sv32, lf, cf, lg, cg, s64, s32); break;
+    case 6: sell_dot2_w<6>(off, sc, sv64, sv32, lf, cf, lg, cg, s64, s32); break;
+    case 7: sell_dot2_w<7>(off, sc, sv64, sv32, lf, cf, lg, cg, s64, s32); break;
+    case 8: sell_dot2_w<8>(off, sc, sv64, sv32, lf, cf, lg, cg, s64, s32); break;
     default: break;
   }
   for (int k = 8; k < wfull; k++) {
     const int ck = sc[off + 64 * k];
-    s64 += sv64[off + 64 * k] * f(ck);
-    s32 += (double)sv32[off + 64 * k] * g(ck);
+    s64 += sv64[off + 64 * k] * cf(lf(ck));
+    s32 += (double)sv32[off + 64 * k] * cg(lg(ck));
   }
 }
 
