@@ -75,7 +75,7 @@ __device__ __forceinline__ void put_partials(const double (&v)[NV], double *part
       double a = 0.0;
 #pragma unroll
       for (int k = 0; k < NT / 64; k++) a += sh[j][k];
-      part[j * G + lb] = a;
+      global_mut(part)[j * G + lb] = a;
     }
   }
 }
@@ -83,8 +83,9 @@ __device__ __forceinline__ void put_partials(const double (&v)[NV], double *part
 // Sums of NV partial arrays part[j * G .. j * G + G) over one block (fixed
 // order), valid in every thread.
 template <int NV>
-__device__ __forceinline__ void sum_partials(const double *part, int G, double (&o)[NV]) {
+__device__ __forceinline__ void sum_partials(const double *part_, int G, double (&o)[NV]) {
   __shared__ double sh[NV][NT / 64];
+  const gptr<double> part = global_ptr(part_);
   double a[NV];
 #pragma unroll
   for (int j = 0; j < NV; j++) a[j] = 0.0;
