@@ -13,6 +13,7 @@ constexpr int REPLACE_EVERY = 8;  // iterations between replacement points
 // on the symmetrically scaled system (see batched.hip).
 struct BSys {
   const int *sp, *sc;       // SELL slice offsets / columns (topology)
+  const int16_t *sd;        // the columns as 16-bit offsets from the row, or null
   const int *row_bd;        // boundary index per row (topology)
   const double *sv64, *diag64;  // FP64 matrix (replacement residuals)
   const float *sv32;        // FP32 scaled off-diagonals a_ij dsq_i dsq_j

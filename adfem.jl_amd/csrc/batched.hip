@@ -212,11 +212,12 @@ __global__ void __launch_bounds__(NT)
     ys[i] = (float)fma(alpha, (double)pi, (double)ys[i]);
     // q' = Â p' = Â r' + beta Â p = Â r' + beta q: the neighbours contribute
     // their r' = r - alpha q (two gathered values each, not three)
-    const double a = fma(beta, (double)qi,
-                         own.r + sell_dot<float>(
-                                     i, d.sp, d.sc, d.sv32,
-                                     [&](int c) { return make_float2(r[c], q[c]); },
-                                     [&](float2 v) { return fma(-alpha, (double)v.y, (double)v.x); }));
+    auto nb = [&](int c) { return make_float2(r[c], q[c]); };
+    auto rn = [&](float2 v) { return fma(-alpha, (double)v.y, (double)v.x); };
+    const double a =
+        fma(beta, (double)qi,
+            own.r + (d.sd ? sell_dot16<float>(i, d.sp, d.sd, d.sv32, nb, rn)
+                          : sell_dot<float>(i, d.sp, d.sc, d.sv32, nb, rn)));
     const float rv = (float)own.r, pv = (float)own.p, qv = (float)a;
     global_mut(d.r[cur ^ 1])[i] = rv;
     global_mut(d.p[cur ^ 1])[i] = pv;

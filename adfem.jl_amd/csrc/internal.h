@@ -218,6 +218,8 @@ struct System {
   std::vector<int> slice_ptr;  // offsets (elements) of each slice
   DevBuf<int> d_slice_ptr;
   DevBuf<int> d_sell_col;      // absolute internal column ids
+  DevBuf<int16_t> d_sell_d16;  // the same as 16-bit offsets from the row (empty
+                               // when an offset does not fit)
   DevBuf<double> d_sell_val;   // FP64 values
   DevBuf<double> d_diag;
   // P2: the internal dof order (each vertex dof at the mesh's node position,

@@ -250,7 +250,7 @@ void mesh_user_pattern(Mesh &m) {
 size_t System::device_bytes() const {
   return d_rp.bytes() + d_col.bytes() + d_row_bd.bytes() + d_blk_eptr.bytes() +
          d_blk_rec.bytes() + d_blk_vptr.bytes() + d_blk_vert.bytes() + d_pairs.bytes() + val.bytes() + rhs.bytes() +
-         d_slice_ptr.bytes() + d_sell_col.bytes() +
+         d_slice_ptr.bytes() + d_sell_col.bytes() + d_sell_d16.bytes() +
          d_sell_val.bytes() + d_diag.bytes() + d_perm2.bytes() + d_iperm2.bytes() +
          d_conn2.bytes() + d_inc_ptr.bytes() + d_blk_elem.bytes() + d_pairs2.bytes() +
          d_vrow.bytes() + d_dof_ends.bytes();

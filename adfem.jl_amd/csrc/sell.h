@@ -26,25 +26,20 @@ __device__ __forceinline__ gmptr<T> global_mut(T *p) {
 // every neighbour's gathered values (load), is issued before the sums.
 // (With the loads inside per-k branches the compiler waited for each
 // neighbour's values before requesting the next one's.)
-template <int W, typename VT, class L, class F>
-__device__ __forceinline__ double sell_dot_w(int off, const int *__restrict__ sc,
-                                             const VT *__restrict__ sv, L load, F comb) {
+template <int W, typename VT, class C, class L, class F>
+__device__ __forceinline__ double sell_dot_w(int off, C col, const VT *__restrict__ sv, L load,
+                                             F comb) {
   int c[W];
   VT v[W];
-  const gptr<int> gc = global_ptr(sc);
   const gptr<VT> gv = global_ptr(sv);
   // FP64 values: the matrix is streamed once per product, so its loads are
   // non-temporal (measured on the FP64 SpMV: 99 -> 82 us at 2048^2; FP32
   // values lose with them, 49 -> 56 us, and keep ordinary loads)
 #pragma unroll
   for (int k = 0; k < W; k++) {
-    if constexpr (sizeof(VT) == 8) {
-      c[k] = __builtin_nontemporal_load(gc + off + 64 * k);
-      v[k] = __builtin_nontemporal_load(gv + off + 64 * k);
-    } else {
-      c[k] = gc[off + 64 * k];
-      v[k] = gv[off + 64 * k];
-    }
+    c[k] = col(off + 64 * k);
+    if constexpr (sizeof(VT) == 8) v[k] = __builtin_nontemporal_load(gv + off + 64 * k);
+    else v[k] = gv[off + 64 * k];
   }
   decltype(load(0)) g[W];
 #pragma unroll
@@ -56,29 +51,51 @@ __device__ __forceinline__ double sell_dot_w(int off, const int *__restrict__ sc
   return s;
 }
 
-template <typename VT, class L, class F>
-__device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
-                                           const int *__restrict__ sc,
-                                           const VT *__restrict__ sv, L load, F comb) {
+// The product given the column of each SELL position (col(e) for entry e).
+template <typename VT, class C, class L, class F>
+__device__ __forceinline__ double sell_dot_cols(int i, const int *__restrict__ sp, C col,
+                                                const VT *__restrict__ sv, L load, F comb) {
   const int sl = i >> 6;
   const int s0 = global_ptr(sp)[sl];
   const int off = s0 + (i & 63);
   const int wfull = (global_ptr(sp)[sl + 1] - s0) >> 6;
   double s = 0.0;
   switch (__builtin_amdgcn_readfirstlane(min(wfull, 8))) {  // wave-uniform
-    case 1: s = sell_dot_w<1>(off, sc, sv, load, comb); break;
-    case 2: s = sell_dot_w<2>(off, sc, sv, load, comb); break;
-    case 3: s = sell_dot_w<3>(off, sc, sv, load, comb); break;
-    case 4: s = sell_dot_w<4>(off, sc, sv, load, comb); break;
-    case 5: s = sell_dot_w<5>(off, sc, sv, load, comb); break;
-    case 6: s = sell_dot_w<6>(off, sc, sv, load, comb); break;
-    case 7: s = sell_dot_w<7>(off, sc, sv, load, comb); break;
-    case 8: s = sell_dot_w<8>(off, sc, sv, load, comb); break;
+    case 1: s = sell_dot_w<1>(off, col, sv, load, comb); break;
+    case 2: s = sell_dot_w<2>(off, col, sv, load, comb); break;
+    case 3: s = sell_dot_w<3>(off, col, sv, load, comb); break;
+    case 4: s = sell_dot_w<4>(off, col, sv, load, comb); break;
+    case 5: s = sell_dot_w<5>(off, col, sv, load, comb); break;
+    case 6: s = sell_dot_w<6>(off, col, sv, load, comb); break;
+    case 7: s = sell_dot_w<7>(off, col, sv, load, comb); break;
+    case 8: s = sell_dot_w<8>(off, col, sv, load, comb); break;
     default: break;
   }
   for (int k = 8; k < wfull; k++)
-    s += (double)sv[off + 64 * k] * comb(load(sc[off + 64 * k]));
+    s += (double)sv[off + 64 * k] * comb(load(col(off + 64 * k)));
   return s;
+}
+
+// with 32-bit column ids (non-temporal loads next to FP64 values)
+template <typename VT, class L, class F>
+__device__ __forceinline__ double sell_dot(int i, const int *__restrict__ sp,
+                                           const int *__restrict__ sc,
+                                           const VT *__restrict__ sv, L load, F comb) {
+  const gptr<int> gc = global_ptr(sc);
+  if constexpr (sizeof(VT) == 8)
+    return sell_dot_cols<VT>(
+        i, sp, [&](int e) { return __builtin_nontemporal_load(gc + e); }, sv, load, comb);
+  else
+    return sell_dot_cols<VT>(i, sp, [&](int e) { return gc[e]; }, sv, load, comb);
+}
+
+// with 16-bit column offsets from the row
+template <typename VT, class L, class F>
+__device__ __forceinline__ double sell_dot16(int i, const int *__restrict__ sp,
+                                             const int16_t *__restrict__ sd,
+                                             const VT *__restrict__ sv, L load, F comb) {
+  const gptr<int16_t> gd = global_ptr(sd);
+  return sell_dot_cols<VT>(i, sp, [&](int e) { return i + (int)gd[e]; }, sv, load, comb);
 }
 
 // the same with f(c) both gathering and combining

@@ -190,6 +190,7 @@ struct Server {
     d = BSys{};
     d.sp = s.d_slice_ptr.p;
     d.sc = s.d_sell_col.p;
+    d.sd = s.d_sell_d16.p;
     d.row_bd = s.d_row_bd.p;
     d.sv64 = q.sv64;
     d.diag64 = q.diag64;

@@ -567,6 +567,22 @@ __global__ void sell_col_kernel(int n, int npad, const int *__restrict__ rp,
   for (; kk < width; kk++) sell_col[off + 64 * kk] = r < n ? r : 0;
 }
 
+// SELL columns as 16-bit offsets from the row; *bad = 1 when one does not
+// fit (padding rows past n read their own entry: offset 0)
+__global__ void sell_d16_kernel(int n, int npad, const int *__restrict__ slice_ptr,
+                                const int *__restrict__ sell_col, int16_t *__restrict__ d16,
+                                int *__restrict__ bad) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= npad) return;
+  const int off = slice_ptr[r >> 6] + (r & 63);
+  const int width = (slice_ptr[(r >> 6) + 1] - slice_ptr[r >> 6]) >> 6;
+  for (int k = 0; k < width; k++) {
+    const int d = r < n ? sell_col[off + 64 * k] - r : 0;
+    if (d < -32768 || d > 32767) *bad = 1;
+    d16[off + 64 * k] = (int16_t)d;
+  }
+}
+
 // Block-wide sort of a[0, cnt) in LDS (bitonic, padded to a power of two
 // with INT_MAX; a holds room for it) and its distinct values: their count,
 // written to out[0, count) only when count <= cap.  part: SB_NT + 1 ints.
@@ -1036,6 +1052,20 @@ void solver_layout(System *s) {
   sell_col_kernel<<<nb(s->npad, 256), 256, 0, st>>>(n, s->npad, s->d_rp.p, s->d_col.p,
                                                      s->d_slice_ptr.p, s->d_sell_col.p);
   HIPCHECK(hipGetLastError());
+  {  // 16-bit column offsets for the FP32 step kernels, when every one fits
+    DevBuf<int> bad(1);
+    fill_int<<<1, 1, 0, st>>>(bad.p, 1, 0);
+    s->d_sell_d16.alloc(ns);
+    sell_d16_kernel<<<nb(s->npad, 256), 256, 0, st>>>(n, s->npad, s->d_slice_ptr.p,
+                                                      s->d_sell_col.p, s->d_sell_d16.p, bad.p);
+    HIPCHECK(hipGetLastError());
+    int hb = 0;
+    HIPCHECK(hipMemcpy(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (hb) {
+      DevBuf<int16_t> none;
+      s->d_sell_d16.swap(none);
+    }
+  }
   if (!s->mesh) s->val.alloc(std::max<int64_t>(1, s->nnz));
   s->rhs.alloc(s->npad);
   HIPCHECK(hipMemset(s->rhs.p, 0, s->rhs.bytes()));
