@@ -266,13 +266,14 @@ __global__ void __launch_bounds__(NT)
     const double pni = dir64(z[i], w[i], pi, alpha, beta, zi);
     x[i] = fma(alpha, pi, x[i]);
     const double di = diag[i];
-    const double q = di * pni + sell_dot<double>(
-                                    i, sp, sc, sv,
-                                    [&](int c) { return double3{z[c], w[c], p[c]}; },
-                                    [&](double3 v) {
-                                      double unused;
-                                      return dir64(v.x, v.y, v.z, alpha, beta, unused);
-                                    });
+    auto nb = [&](int c) { return double3{z[c], w[c], p[c]}; };
+    auto pn = [&](double3 v) {
+      double unused;
+      return dir64(v.x, v.y, v.z, alpha, beta, unused);
+    };
+    // 32-bit columns, non-temporal like the FP64 values (16-bit offsets
+    // measured slower here: 115 vs 97 us at 2048^2)
+    const double q = di * pni + sell_dot<double>(i, sp, sc, sv, nb, pn);
     const double wi = di != 0.0 ? q / di : 0.0;
     (odd ? z0 : z1)[i] = zi;
     (odd ? p0 : p1)[i] = pni;
@@ -392,7 +393,8 @@ __device__ __forceinline__ double dir32(float r, float q, float p, double alpha,
 // (two gathered values); partials of r'.r', p'.q', r'.q', q'.q'
 __global__ void __launch_bounds__(NT)
     k_mstep(int npad, const int *__restrict__ sp, const int *__restrict__ sc,
-            const float *__restrict__ sv, float *r0, float *r1, float *q0, float *q1,
+            const int16_t *__restrict__ sd, const float *__restrict__ sv, float *r0,
+            float *r1, float *q0, float *q1,
             float *p0, float *p1, float *__restrict__ ys, double *part, const CGScalars *S) {
   if (S->status != ST_RUN || S->replace) return;
   const double alpha = S->alpha, beta = S->beta;
@@ -407,10 +409,12 @@ __global__ void __launch_bounds__(NT)
     double ri;
     const double pni = dir32(r[i], qi, pi, alpha, beta, ri);
     ys[i] = (float)fma(alpha, (double)pi, (double)ys[i]);
+    auto nb = [&](int c) { return make_float2(r[c], q[c]); };
+    auto rn = [&](float2 v) { return fma(-alpha, (double)v.y, (double)v.x); };
+    // 16-bit column offsets when the layout has them (sd: uniform)
     const double a = fma(beta, (double)qi,
-                         ri + sell_dot<float>(
-                                  i, sp, sc, sv, [&](int c) { return make_float2(r[c], q[c]); },
-                                  [&](float2 v) { return fma(-alpha, (double)v.y, (double)v.x); }));
+                         ri + (sd ? sell_dot16<float>(i, sp, sd, sv, nb, rn)
+                                  : sell_dot<float>(i, sp, sc, sv, nb, rn)));
     const float rv = (float)ri, pv = (float)pni, qv = (float)a;
     (odd ? r0 : r1)[i] = rv;
     (odd ? p0 : p1)[i] = pv;
@@ -595,7 +599,8 @@ void launch_step(System &s, CGWork *w, bool mixed, hipStream_t st) {
   const int nb = w->nblocks, np = s.npad;
   const int *sp = s.d_slice_ptr.p, *scol = s.d_sell_col.p;
   if (mixed)
-    k_mstep<<<nb, NT, 0, st>>>(np, sp, scol, w->sv32s.p, w->r32[0].p, w->r32[1].p,
+    k_mstep<<<nb, NT, 0, st>>>(np, sp, scol, s.d_sell_d16.p, w->sv32s.p, w->r32[0].p,
+                               w->r32[1].p,
                                w->q32[0].p, w->q32[1].p, w->p32[0].p, w->p32[1].p, w->ys32.p,
                                w->part.p, w->sc.p);
   else
