@@ -176,6 +176,28 @@ def test_sparse_solve_far_coupling():
     assert np.abs(u.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-10
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_sparse_solve_star_wider_than_16_bit_offsets(precision):
+    """A star (node 0 coupled to all 69,999 others): every numbering leaves
+    column offsets beyond 16 bits, so the FP32 steps read the 32-bit
+    columns, and row 0 is far wider than the straight-line widths (its
+    entries past the eighth take the loop); against SciPy."""
+    n = 70000
+    rng = np.random.default_rng(5)
+    main = np.full(n, 2.0)
+    main[0] = float(n)
+    w = -rng.random(n - 1) * 0.5
+    I = np.concatenate([np.arange(n), np.zeros(n - 1, np.int64), np.arange(1, n)])
+    J = np.concatenate([np.arange(n), np.arange(1, n), np.zeros(n - 1, np.int64)])
+    V = np.concatenate([main, w, w])
+    b = rng.standard_normal(n)
+    ind = torch.as_tensor(np.stack([I, J], 1), device="cuda")
+    u = admfem.solve(admfem.SparseTensor(ind, torch.tensor(V, device="cuda"), n),
+                     torch.tensor(b, device="cuda"), precision=precision, rtol=1e-12)
+    ref = spl.spsolve(sp.csr_matrix((V, (I, J)), shape=(n, n)), b)
+    assert np.abs(u.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-9
+
+
 @pytest.mark.parametrize("permute", [False, True])
 def test_sparse_solve_grid_laplacian_any_numbering(permute):
     """The shifted 5-point Laplacian of a 150 x 150 grid, row-major (banded:
