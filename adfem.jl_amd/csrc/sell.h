@@ -21,7 +21,10 @@ __device__ __forceinline__ gmptr<T> global_mut(T *p) {
 }
 
 // Row i's product with its SELL row: sum_k sv_k comb(load(c_k)) in k order.
-// A wave's 64 rows are one slice, so the row width is wave-uniform and each
+// Callers run one row per lane with every lane of the wave active and the
+// wave's rows 64 consecutive rows from a multiple of 64 (the solvers' blocks
+// are 4 such waves and npad is a multiple of 64).  A wave's 64 rows are then
+// one slice, so the row width is wave-uniform and each
 // width up to 8 gets straight-line code: every column and value load, then
 // every neighbour's gathered values (load), is issued before the sums.
 // (With the loads inside per-k branches the compiler waited for each
