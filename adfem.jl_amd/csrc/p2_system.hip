@@ -204,9 +204,8 @@ __global__ void __launch_bounds__(P2A_NT)
     if (bi >= 0) {
       acc = P.bdval ? P.bdval[bi] : 0.0;  // (b,b,1.0) of ImposeDirichlet.h:46-50
     } else {
-      const int t1 = inc_ptr[row + 1];
-      for (int t = inc_ptr[row]; t < t1; t++) {
-        const uint64_t w = pairs[t];
+      const int t0 = inc_ptr[row], t1 = inc_ptr[row + 1];
+      auto add = [&](uint64_t w) {
         const int p = (int)(w & 7u), le = (int)(w >> P2_PAIR_ELEM);
         const double *arow = ae + P2_SYM * le;
 #pragma unroll
@@ -216,7 +215,17 @@ __global__ void __launch_bounds__(P2A_NT)
           my[AS_ROWS * sl] += arow[sym6(p, q)];
         }
         acc += rl[6 * le + p];
-      }
+      };
+      // the row's first pair words requested together (a vertex dof has
+      // about six incidences, an edge dof two), then added in order
+      constexpr int PW = 8;
+      uint64_t wb[PW];
+#pragma unroll
+      for (int k = 0; k < PW; k++) wb[k] = t0 + k < t1 ? pairs[t0 + k] : 0ull;
+#pragma unroll
+      for (int k = 0; k < PW; k++)
+        if (t0 + k < t1) add(wb[k]);
+      for (int t = t0 + PW; t < t1; t++) add(pairs[t]);
       d = my[AS_ROWS * wmax];
     }
   }
