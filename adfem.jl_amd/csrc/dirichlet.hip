@@ -336,7 +336,7 @@ void bd_keys(const long long *bd_dev, int bdN, int N, hipStream_t st,
 struct DirWork {
   std::mutex mu;
   DevBuf<int> keys, win, bdidx, tkeep, tlift, err, lkey, skey;
-  DevBuf<int64_t> tkpos, tlpos, lval, sval, cnt;
+  DevBuf<int64_t> tkpos, tlpos, lval, sval;
   DevBuf<char> tmp;
 };
 
